@@ -1,0 +1,154 @@
+/*
+ * mpsparse_amd -- C ABI of the MI355X sparse iterative-solver library.
+ *
+ * Plain C: opaque handles, raw pointers, sizes and status codes; no torch or
+ * HIP types. Array arguments may be host memory or device memory of the
+ * handle's GPU (the library detects which and stages host arrays itself).
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream). Calls
+ * on one handle must not overlap (the handle owns one solver workspace).
+ * On failure a function returns a nonzero msp_status and msp_last_error()
+ * (thread-local) describes it.
+ *
+ * Each entry point names the reference interface it replaces
+ * (SparseForge/mpsparse, file:line).
+ */
+#ifndef MPSPARSE_AMD_H
+#define MPSPARSE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSP_ABI_VERSION 1
+
+typedef enum msp_status {
+  MSP_OK = 0,
+  MSP_ERR_INVALID = 1,     /* bad argument (null pointer, size, dtype, ...) */
+  MSP_ERR_KEY_RANGE = 2,   /* a COO key has row >= num_rows or col >= num_cols */
+  MSP_ERR_HIP = 3,         /* HIP runtime failure */
+  MSP_ERR_NO_DEVICE = 4,   /* no GPU visible */
+  MSP_ERR_UNSUPPORTED = 5, /* option combination not implemented */
+  MSP_ERR_SINGULAR = 6     /* zero / non-finite diagonal where one is required */
+} msp_status;
+
+typedef enum msp_value_type {
+  MSP_VALUES_F32 = 0,
+  MSP_VALUES_BF16 = 1,
+  MSP_VALUES_F16 = 2
+} msp_value_type;
+
+typedef enum msp_criterion {
+  MSP_CRIT_RELRES = 0,   /* stop when ||r|| <= tol * ||b|| (checked every step) */
+  MSP_CRIT_REFERENCE = 1 /* stop when sum(r^2) < tol, tested every 50 steps
+                            (iterative/BiCGSTAB/spmv.cpp:17-18, 603-629) */
+} msp_criterion;
+
+typedef enum msp_precond {
+  MSP_PRECOND_NONE = 0,
+  MSP_PRECOND_JACOBI = 1
+} msp_precond;
+
+typedef enum msp_solve_status {
+  MSP_SOLVE_CONVERGED = 1,
+  MSP_SOLVE_BREAKDOWN = 2,
+  MSP_SOLVE_MAXITER = 3,
+  MSP_SOLVE_STAGNATED = 4 /* true residual stopped improving above tol */
+} msp_solve_status;
+
+typedef struct msp_solve_opts {
+  double tol;            /* default 1e-6 */
+  int32_t maxiter;       /* default 10000 */
+  int32_t check_every;   /* steps enqueued between host polls of the device flag; default 16 */
+  int32_t criterion;     /* msp_criterion */
+  int32_t use_x0;        /* 0: start from x = 0 (reference behaviour); 1: x is the initial guess */
+  int32_t precond;       /* msp_precond (CG only) */
+  int32_t values;        /* msp_value_type used by the SpMV inside the loop */
+  int32_t verify;        /* 1 (default): when the recurrence residual meets tol, recompute
+                            r = b - A x with compensated FP32 sums and continue from it
+                            unless it meets tol too (MSP_CRIT_RELRES only) */
+  int32_t reserved[3];
+} msp_solve_opts;
+
+typedef struct msp_solve_info {
+  int32_t iterations;
+  int32_t status;        /* msp_solve_status */
+  int32_t verifications; /* true-residual checks performed */
+  int32_t reserved;
+  double relres;         /* sqrt(rr / bb) at exit (the verified residual when verify = 1) */
+  double rr;             /* ||r||^2 at exit */
+  double bb;             /* ||b||^2 */
+  double seconds;        /* device time of the iteration loop (HIP events) */
+} msp_solve_info;
+
+typedef struct msp_csr_s *msp_csr;
+
+/* Library ----------------------------------------------------------------- */
+int32_t msp_abi_version(void);
+const char *msp_last_error(void);
+void msp_default_opts(msp_solve_opts *opts);
+/* Number of visible GPUs (0 when none); never fails. */
+int32_t msp_device_count(void);
+
+/* Standalone COO -> CSR.
+ * Replaces coo_csr.coo_csr(keys, values, row_ptr, col_ind, out_vals, num_rows, num_cols)
+ * (spmv_kernel/coo_csr/coo_csr/coo_csr/coo_to_csr.cpp:251-283; sort semantics of
+ * iterative/cgd/coo_csr.metal:17-193). keys[i] = (row << 32) | col. Writes
+ * row_ptr[num_rows+1], col_ind[nnz], out_vals[nnz] (stable: duplicate keys keep
+ * input order and are not summed). */
+msp_status msp_coo_to_csr(const uint64_t *keys, const float *vals, uint64_t nnz, uint32_t num_rows,
+                          uint32_t num_cols, uint32_t *row_ptr, uint32_t *col_ind, float *out_vals,
+                          void *stream);
+
+/* CSR matrix handle built on the GPU from packed COO keys.
+ * Replaces spmv.csr_tensor(keys, values, row_ptr, col_ind, out_vals, num_rows, num_cols)
+ * (iterative/BiCGSTAB/spmv.cpp:230-268, 62-228). */
+msp_status msp_csr_create_coo(msp_csr *out, const uint64_t *keys, const float *vals, uint64_t nnz,
+                              uint32_t num_rows, uint32_t num_cols, void *stream);
+
+/* CSR handle from sorted CSR arrays (copied). Counterpart of the array
+ * upload in bcsr.bcsr_tensor's constructor (iterative/BiCGSTAB/bcsr.cpp:357-422). */
+msp_status msp_csr_create_csr(msp_csr *out, const uint32_t *row_ptr, const uint32_t *col_ind,
+                              const float *vals, uint32_t num_rows, uint32_t num_cols, void *stream);
+
+/* Copy the handle's CSR arrays out (any pointer may be NULL to skip it). The
+ * reference constructors take these buffers but never fill them
+ * (iterative/BiCGSTAB/spmv.cpp:246-248); the standalone converter does
+ * (coo_to_csr.cpp:220-231). */
+msp_status msp_csr_export(msp_csr A, uint32_t *row_ptr, uint32_t *col_ind, float *vals, void *stream);
+
+msp_status msp_csr_shape(msp_csr A, uint32_t *num_rows, uint32_t *num_cols, uint64_t *nnz);
+
+/* Build the reduced-precision value copy used when opts.values != F32. */
+msp_status msp_csr_prepare_values(msp_csr A, int32_t value_type, void *stream);
+
+/* Release the handle (device buffers included). Replaces ~csr_tensor
+ * (iterative/BiCGSTAB/spmv.cpp:354-366). */
+msp_status msp_csr_destroy(msp_csr A);
+
+/* y = A x. Replaces csr_tensor.mv(x, b) (iterative/BiCGSTAB/spmv.cpp:436-501;
+ * kernel spmv.metal:11-49). */
+msp_status msp_csr_spmv(msp_csr A, const float *x, float *y, int32_t value_type, void *stream);
+
+/* CG on A x = b. Replaces csr_tensor.cgd(b, x) (iterative/BiCGSTAB/spmv.cpp:515-659)
+ * and the cgd module's iter_solve(b, x) (iterative/cgd/spmv.cpp:607-747). */
+msp_status msp_csr_cg(msp_csr A, const float *b, float *x, const msp_solve_opts *opts,
+                      msp_solve_info *info, void *stream);
+
+/* BiCGSTAB. Replaces csr_tensor.bicgstab(b, x) (iterative/BiCGSTAB/spmv.cpp:661-888). */
+msp_status msp_csr_bicgstab(msp_csr A, const float *b, float *x, const msp_solve_opts *opts,
+                            msp_solve_info *info, void *stream);
+
+/* `iters` steps of x <- x + omega (b - A x). Replaces the richardson module's
+ * iter_solve(b, x) (iterative/richardson/spmv.cpp:376-495: omega 0.01, 1000 steps,
+ * x0 = 0) and the CUDA reference's loop (spmv_cuda_reference.cu:227-280). */
+msp_status msp_csr_richardson(msp_csr A, const float *b, float *x, float omega, int32_t iters,
+                              int32_t use_x0, int32_t value_type, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPSPARSE_AMD_H */

@@ -1,0 +1,802 @@
+// C ABI of the library: handle management, host staging, solver drivers.
+// Everything below the ABI runs as HIP kernels on the handle's GPU; there is no
+// CPU compute path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/mpsparse_amd.h"
+#include "common.hpp"
+#include "krylov.hpp"
+#include "sort.hpp"
+#include "spmv.hpp"
+
+using namespace msp;
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local std::string g_err;
+
+static msp_status fail(msp_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return s;
+}
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
+      return fail(MSP_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define MSPCHK(expr)                    \
+  do {                                  \
+    msp_status s_ = (expr);             \
+    if (s_ != MSP_OK) return s_;        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device facts
+static int cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
+static uint32_t spmv_grid(uint32_t nblk) {
+  uint32_t cap = (uint32_t)cu_count() * 8u;
+  return std::max(1u, std::min(nblk, cap));
+}
+
+static uint32_t vec_grid(uint64_t n) {
+  uint64_t chunks = (n + 3) / 4;
+  uint64_t g = (chunks + kBlock - 1) / kBlock;
+  uint64_t cap = (uint64_t)cu_count() * 4u;
+  return (uint32_t)std::max<uint64_t>(1, std::min(g, cap));
+}
+
+static uint32_t bitlen(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v) : 0u; }
+
+// ---------------------------------------------------------------------------
+// memory helpers
+struct DevMem {
+  void* p = nullptr;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+  msp_status alloc(size_t bytes) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(MSP_ERR_HIP, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    return MSP_OK;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+static bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.isManaged;
+}
+
+// A read-only array on the device: the caller's pointer, or a staged copy.
+struct InArr {
+  const void* d = nullptr;
+  DevMem own;
+  msp_status set(const void* p, size_t bytes, hipStream_t s) {
+    if (!p && bytes) return fail(MSP_ERR_INVALID, "null input array");
+    if (bytes == 0 || is_device_ptr(p)) {
+      d = p;
+      return MSP_OK;
+    }
+    MSPCHK(own.alloc(bytes));
+    HIPCHK(hipMemcpyAsync(own.p, p, bytes, hipMemcpyHostToDevice, s));
+    d = own.p;
+    return MSP_OK;
+  }
+};
+
+// Copy `bytes` from device memory `src` to `dst` (host or device).
+static msp_status copy_out(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!dst || bytes == 0) return MSP_OK;
+  if (is_device_ptr(dst)) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+  } else {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return MSP_OK;
+}
+
+static msp_status copy_in(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return MSP_OK;
+  if (!src) return fail(MSP_ERR_INVALID, "null input array");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  return MSP_OK;
+}
+
+static msp_status launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MSP_ERR_HIP, "kernel launch (%s): %s", what, hipGetErrorString(e));
+  return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// handle
+struct Workspace {
+  size_t npad = 0;
+  DevMem vecs;            // kNumVec vectors of npad floats
+  DevMem scal;            // Scalars
+  DevMem red;             // partials + ticket
+  Scalars* hS = nullptr;  // pinned host mirror
+  int32_t* hflag = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  static constexpr int kNumVec = 11;
+  enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2 };
+  float* v(int i) const { return vecs.as<float>() + (size_t)i * npad; }
+  Scalars* S_dev() const { return scal.as<Scalars>(); }
+  GridRed gr() const {
+    GridRed g;
+    g.ticket = red.as<uint32_t>();
+    g.partials = reinterpret_cast<CSum*>(red.as<char>() + 256);
+    return g;
+  }
+  ~Workspace() {
+    if (hS) (void)hipHostFree(hS);
+    if (hflag) (void)hipHostFree(hflag);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+  }
+};
+
+struct msp_csr_s {
+  int device = 0;
+  uint32_t n = 0, m = 0;
+  uint64_t nnz = 0;
+  DevMem row_ptr, col, val, val16, blk, dinv;
+  int val16_type = -1;
+  uint32_t nblk = 0, T = 0, max_row = 0;
+  int dinv_state = 0;  // 0 not built, 1 ok, 2 singular
+  Workspace ws;
+};
+
+static msp_status check_handle(msp_csr A) {
+  if (!A) return fail(MSP_ERR_INVALID, "null matrix handle");
+  HIPCHK(hipSetDevice(A->device));
+  return MSP_OK;
+}
+
+static msp_status ensure_workspace(msp_csr A, hipStream_t s) {
+  Workspace& w = A->ws;
+  size_t need = ((size_t)std::max(A->n, A->m) + 255) / 256 * 256 + 256;
+  if (w.npad >= need && w.scal.p) return MSP_OK;
+  w.npad = need;
+  MSPCHK(w.vecs.alloc(sizeof(float) * need * Workspace::kNumVec));
+  HIPCHK(hipMemsetAsync(w.vecs.p, 0, sizeof(float) * need * Workspace::kNumVec, s));
+  if (!w.scal.p) {
+    MSPCHK(w.scal.alloc(sizeof(Scalars)));
+    HIPCHK(hipMemsetAsync(w.scal.p, 0, sizeof(Scalars), s));
+    size_t red_bytes = 256 + sizeof(CSum) * 4 * (size_t)cu_count() * 8;
+    MSPCHK(w.red.alloc(red_bytes));
+    HIPCHK(hipMemsetAsync(w.red.p, 0, red_bytes, s));
+    HIPCHK(hipHostMalloc((void**)&w.hS, sizeof(Scalars), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&w.hflag, 2 * sizeof(int32_t), hipHostMallocDefault));
+    for (auto& e : w.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreate(&w.t0));
+    HIPCHK(hipEventCreate(&w.t1));
+  }
+  return MSP_OK;
+}
+
+// Row blocks for the SpMV (and the max row length).
+static msp_status finalize_matrix(msp_csr A, hipStream_t s) {
+  const uint32_t n = A->n;
+  DevMem mx;
+  MSPCHK(mx.alloc(sizeof(uint32_t)));
+  HIPCHK(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), s));
+  if (n > 0) {
+    uint32_t g = std::min<uint32_t>((n + kBlock - 1) / kBlock, 4096);
+    k_max_row<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), n, mx.as<uint32_t>());
+    MSPCHK(launch_check("k_max_row"));
+  }
+  HIPCHK(hipMemcpyAsync(&A->max_row, mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double avg = n ? (double)A->nnz / n : 0.0;
+  uint32_t per = (uint32_t)std::max(1.0, std::floor(avg));
+  uint32_t T = std::min<uint32_t>(2048u, std::max<uint32_t>(256u, 256u * per));
+  A->T = T;
+  A->nblk = (uint32_t)std::max<uint64_t>(1, (A->nnz + T - 1) / T);
+  MSPCHK(A->blk.alloc(sizeof(uint32_t) * (A->nblk + 1)));
+  uint32_t g = (A->nblk + 1 + kBlock - 1) / kBlock;
+  k_row_blocks<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), n, T, A->nblk, A->blk.as<uint32_t>());
+  MSPCHK(launch_check("k_row_blocks"));
+  return MSP_OK;
+}
+
+static msp_status alloc_csr(msp_csr A) {
+  MSPCHK(A->row_ptr.alloc(sizeof(uint32_t) * ((size_t)A->n + 1)));
+  size_t padded = (size_t)A->nnz + 64;
+  MSPCHK(A->col.alloc(sizeof(uint32_t) * padded));
+  MSPCHK(A->val.alloc(sizeof(float) * padded));
+  return MSP_OK;
+}
+
+static msp_status zero_pad(msp_csr A, hipStream_t s) {
+  HIPCHK(hipMemsetAsync(A->col.as<uint32_t>() + A->nnz, 0, 64 * sizeof(uint32_t), s));
+  HIPCHK(hipMemsetAsync(A->val.as<float>() + A->nnz, 0, 64 * sizeof(float), s));
+  return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// COO -> CSR driver
+template <typename K>
+static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t nnz, sortk::KeyMap km, int passes,
+                              uint32_t* col_out, float* val_out, uint32_t* rows_out, int* err, hipStream_t s) {
+  using namespace sortk;
+  const uint32_t num_tiles = (uint32_t)((nnz + kTile - 1) / kTile);
+  const uint64_t N = (uint64_t)kRadix * num_tiles;
+  const uint32_t nscan = (uint32_t)((N + kScanTile - 1) / kScanTile);
+  DevMem counts, sums, ka, kb, va, vb;
+  MSPCHK(counts.alloc(sizeof(uint32_t) * N));
+  MSPCHK(sums.alloc(sizeof(uint32_t) * nscan));
+  if (passes > 1) {
+    MSPCHK(ka.alloc(sizeof(K) * nnz));
+    MSPCHK(va.alloc(sizeof(float) * nnz));
+  }
+  if (passes > 2) {
+    MSPCHK(kb.alloc(sizeof(K) * nnz));
+    MSPCHK(vb.alloc(sizeof(float) * nnz));
+  }
+  K* kbuf[2] = {ka.as<K>(), kb.as<K>()};
+  float* vbuf[2] = {va.as<float>(), vb.as<float>()};
+  const K* kin = nullptr;
+  const float* vin = vals;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kRadixBits;
+    const bool first = p == 0, last = p == passes - 1;
+    if (first) k_upsweep<uint64_t, true><<<num_tiles, kBlock, 0, s>>>(keys, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
+    else k_upsweep<K, false><<<num_tiles, kBlock, 0, s>>>(kin, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
+    MSPCHK(launch_check("k_upsweep"));
+    k_scan_reduce<<<nscan, kBlock, 0, s>>>(counts.as<uint32_t>(), N, sums.as<uint32_t>());
+    k_scan_top<<<1, kBlock, 0, s>>>(sums.as<uint32_t>(), nscan);
+    k_scan_down<<<nscan, kBlock, 0, s>>>(counts.as<uint32_t>(), N, sums.as<uint32_t>());
+    MSPCHK(launch_check("k_scan"));
+    K* kout = kbuf[p & 1];
+    float* vout = vbuf[p & 1];
+    const uint32_t* off = counts.as<uint32_t>();
+    if (first && last)
+      k_downsweep<uint64_t, K, true, true><<<num_tiles, kBlock, 0, s>>>(keys, vin, nullptr, val_out, col_out, rows_out, nnz, km, shift, num_tiles, off);
+    else if (first)
+      k_downsweep<uint64_t, K, true, false><<<num_tiles, kBlock, 0, s>>>(keys, vin, kout, vout, nullptr, nullptr, nnz, km, shift, num_tiles, off);
+    else if (last)
+      k_downsweep<K, K, false, true><<<num_tiles, kBlock, 0, s>>>(kin, vin, nullptr, val_out, col_out, rows_out, nnz, km, shift, num_tiles, off);
+    else
+      k_downsweep<K, K, false, false><<<num_tiles, kBlock, 0, s>>>(kin, vin, kout, vout, nullptr, nullptr, nnz, km, shift, num_tiles, off);
+    MSPCHK(launch_check("k_downsweep"));
+    kin = kout;
+    vin = vout;
+  }
+  // temporaries are freed on return; hipFree waits for the device
+  return MSP_OK;
+}
+
+// Sort (keys, vals) on the device into CSR arrays. `row_ptr` has n+1 entries,
+// `col_out`/`val_out` nnz entries.
+static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n, uint32_t m,
+                                    uint32_t* row_ptr, uint32_t* col_out, float* val_out, hipStream_t s) {
+  DevMem err, rows;
+  MSPCHK(err.alloc(sizeof(int)));
+  HIPCHK(hipMemsetAsync(err.p, 0, sizeof(int), s));
+  if (nnz > 0) {
+    sortk::KeyMap km;
+    km.cbits = m > 1 ? bitlen(m - 1) : 0;
+    km.rows = n;
+    km.cols = m;
+    km.cmask = km.cbits ? ((1ull << km.cbits) - 1ull) : 0ull;
+    const uint32_t rbits = n > 1 ? bitlen(n - 1) : 0;
+    const uint32_t tb = km.cbits + rbits;
+    const int passes = std::max(1, (int)((tb + sortk::kRadixBits - 1) / sortk::kRadixBits));
+    MSPCHK(rows.alloc(sizeof(uint32_t) * nnz));
+    if (tb <= 32)
+      MSPCHK(sort_passes<uint32_t>(keys, vals, nnz, km, passes, col_out, val_out, rows.as<uint32_t>(), err.as<int>(), s));
+    else
+      MSPCHK(sort_passes<uint64_t>(keys, vals, nnz, km, passes, col_out, val_out, rows.as<uint32_t>(), err.as<int>(), s));
+  }
+  uint64_t work = std::max<uint64_t>(nnz, (uint64_t)n + 1);
+  uint32_t g = (uint32_t)std::min<uint64_t>((work + kBlock - 1) / kBlock, 65536);
+  sortk::k_row_ptr<<<g, kBlock, 0, s>>>(rows.as<uint32_t>(), nnz, n, row_ptr, err.as<int>());
+  MSPCHK(launch_check("k_row_ptr"));
+  int herr = 0;
+  HIPCHK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (herr) return fail(MSP_ERR_KEY_RANGE, "a COO key lies outside [0,%u) x [0,%u)", n, m);
+  return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ABI: library
+extern "C" int32_t msp_abi_version(void) { return MSP_ABI_VERSION; }
+
+extern "C" const char* msp_last_error(void) { return g_err.c_str(); }
+
+extern "C" void msp_default_opts(msp_solve_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->tol = 1e-6;
+  o->maxiter = 10000;
+  o->check_every = 16;
+  o->criterion = MSP_CRIT_RELRES;
+  o->use_x0 = 0;
+  o->precond = MSP_PRECOND_NONE;
+  o->values = MSP_VALUES_F32;
+  o->verify = 1;
+}
+
+extern "C" int32_t msp_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return c;
+}
+
+static msp_status need_device() {
+  if (msp_device_count() <= 0) return fail(MSP_ERR_NO_DEVICE, "no HIP device visible");
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_coo_to_csr(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t num_rows,
+                                     uint32_t num_cols, uint32_t* row_ptr, uint32_t* col_ind, float* out_vals,
+                                     void* stream) {
+  MSPCHK(need_device());
+  if (!row_ptr || (nnz && (!keys || !vals || !col_ind || !out_vals)))
+    return fail(MSP_ERR_INVALID, "msp_coo_to_csr: null array");
+  if (nnz > 0xFFFFFFFFull - 64) return fail(MSP_ERR_INVALID, "nnz %llu exceeds the u32 index range", (unsigned long long)nnz);
+  hipStream_t s = (hipStream_t)stream;
+  InArr k, v;
+  MSPCHK(k.set(keys, sizeof(uint64_t) * nnz, s));
+  MSPCHK(v.set(vals, sizeof(float) * nnz, s));
+  DevMem rp, ci, ov;
+  const bool rp_dev = is_device_ptr(row_ptr), ci_dev = nnz == 0 || is_device_ptr(col_ind),
+             ov_dev = nnz == 0 || is_device_ptr(out_vals);
+  uint32_t* d_rp = row_ptr;
+  uint32_t* d_ci = col_ind;
+  float* d_ov = out_vals;
+  if (!rp_dev) {
+    MSPCHK(rp.alloc(sizeof(uint32_t) * ((size_t)num_rows + 1)));
+    d_rp = rp.as<uint32_t>();
+  }
+  if (!ci_dev) {
+    MSPCHK(ci.alloc(sizeof(uint32_t) * nnz));
+    d_ci = ci.as<uint32_t>();
+  }
+  if (!ov_dev) {
+    MSPCHK(ov.alloc(sizeof(float) * nnz));
+    d_ov = ov.as<float>();
+  }
+  MSPCHK(coo_to_csr_device(static_cast<const uint64_t*>(k.d), static_cast<const float*>(v.d), nnz, num_rows, num_cols,
+                           d_rp, d_ci, d_ov, s));
+  if (!rp_dev) MSPCHK(copy_out(row_ptr, d_rp, sizeof(uint32_t) * ((size_t)num_rows + 1), s));
+  if (!ci_dev) MSPCHK(copy_out(col_ind, d_ci, sizeof(uint32_t) * nnz, s));
+  if (!ov_dev) MSPCHK(copy_out(out_vals, d_ov, sizeof(float) * nnz, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ABI: handles
+extern "C" msp_status msp_csr_create_coo(msp_csr* out, const uint64_t* keys, const float* vals, uint64_t nnz,
+                                         uint32_t num_rows, uint32_t num_cols, void* stream) {
+  if (!out) return fail(MSP_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  MSPCHK(need_device());
+  if (nnz && (!keys || !vals)) return fail(MSP_ERR_INVALID, "msp_csr_create_coo: null keys/values");
+  if (nnz > 0xFFFFFFFFull - 64) return fail(MSP_ERR_INVALID, "nnz exceeds the u32 index range");
+  hipStream_t s = (hipStream_t)stream;
+  msp_csr A = new msp_csr_s;
+  HIPCHK(hipGetDevice(&A->device));
+  A->n = num_rows;
+  A->m = num_cols;
+  A->nnz = nnz;
+  msp_status st = alloc_csr(A);
+  InArr k, v;
+  if (st == MSP_OK) st = k.set(keys, sizeof(uint64_t) * nnz, s);
+  if (st == MSP_OK) st = v.set(vals, sizeof(float) * nnz, s);
+  if (st == MSP_OK)
+    st = coo_to_csr_device(static_cast<const uint64_t*>(k.d), static_cast<const float*>(v.d), nnz, num_rows, num_cols,
+                           A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), s);
+  if (st == MSP_OK) st = zero_pad(A, s);
+  if (st == MSP_OK) st = finalize_matrix(A, s);
+  if (st == MSP_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(MSP_ERR_HIP, "stream sync failed");
+  if (st != MSP_OK) {
+    delete A;
+    return st;
+  }
+  *out = A;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_create_csr(msp_csr* out, const uint32_t* row_ptr, const uint32_t* col_ind,
+                                         const float* vals, uint32_t num_rows, uint32_t num_cols, void* stream) {
+  if (!out) return fail(MSP_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  MSPCHK(need_device());
+  if (!row_ptr) return fail(MSP_ERR_INVALID, "null row_ptr");
+  hipStream_t s = (hipStream_t)stream;
+  // nnz = row_ptr[num_rows]
+  uint32_t nnz32 = 0;
+  if (is_device_ptr(row_ptr)) {
+    HIPCHK(hipMemcpyAsync(&nnz32, row_ptr + num_rows, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  } else {
+    nnz32 = row_ptr[num_rows];
+  }
+  if (nnz32 && (!col_ind || !vals)) return fail(MSP_ERR_INVALID, "null col_ind/values");
+  msp_csr A = new msp_csr_s;
+  HIPCHK(hipGetDevice(&A->device));
+  A->n = num_rows;
+  A->m = num_cols;
+  A->nnz = nnz32;
+  msp_status st = alloc_csr(A);
+  if (st == MSP_OK) st = copy_in(A->row_ptr.p, row_ptr, sizeof(uint32_t) * ((size_t)num_rows + 1), s);
+  if (st == MSP_OK) st = copy_in(A->col.p, col_ind, sizeof(uint32_t) * A->nnz, s);
+  if (st == MSP_OK) st = copy_in(A->val.p, vals, sizeof(float) * A->nnz, s);
+  if (st == MSP_OK) st = zero_pad(A, s);
+  if (st == MSP_OK) st = finalize_matrix(A, s);
+  if (st == MSP_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(MSP_ERR_HIP, "stream sync failed");
+  if (st != MSP_OK) {
+    delete A;
+    return st;
+  }
+  *out = A;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_export(msp_csr A, uint32_t* row_ptr, uint32_t* col_ind, float* vals, void* stream) {
+  MSPCHK(check_handle(A));
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(copy_out(row_ptr, A->row_ptr.p, sizeof(uint32_t) * ((size_t)A->n + 1), s));
+  MSPCHK(copy_out(col_ind, A->col.p, sizeof(uint32_t) * A->nnz, s));
+  MSPCHK(copy_out(vals, A->val.p, sizeof(float) * A->nnz, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_shape(msp_csr A, uint32_t* num_rows, uint32_t* num_cols, uint64_t* nnz) {
+  if (!A) return fail(MSP_ERR_INVALID, "null matrix handle");
+  if (num_rows) *num_rows = A->n;
+  if (num_cols) *num_cols = A->m;
+  if (nnz) *nnz = A->nnz;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_prepare_values(msp_csr A, int32_t value_type, void* stream) {
+  MSPCHK(check_handle(A));
+  hipStream_t s = (hipStream_t)stream;
+  if (value_type == MSP_VALUES_F32) return MSP_OK;
+  if (value_type != MSP_VALUES_BF16 && value_type != MSP_VALUES_F16)
+    return fail(MSP_ERR_INVALID, "unknown value type %d", value_type);
+  if (A->val16_type == value_type && A->val16.p) return MSP_OK;
+  size_t padded = (size_t)A->nnz + 64;
+  MSPCHK(A->val16.alloc(sizeof(uint16_t) * padded));
+  uint32_t g = (uint32_t)std::min<uint64_t>((padded + kBlock - 1) / kBlock, 65536);
+  if (value_type == MSP_VALUES_BF16)
+    k_cvt_bf16<<<g, kBlock, 0, s>>>(A->val.as<float>(), A->val16.as<uint16_t>(), padded);
+  else
+    k_cvt_f16<<<g, kBlock, 0, s>>>(A->val.as<float>(), A->val16.as<uint16_t>(), padded);
+  MSPCHK(launch_check("k_cvt"));
+  A->val16_type = value_type;
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_destroy(msp_csr A) {
+  if (!A) return MSP_OK;
+  (void)hipSetDevice(A->device);
+  (void)hipDeviceSynchronize();
+  delete A;
+  return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV dispatch helpers
+template <typename V>
+static CsrView<V> view_of(msp_csr A) {
+  CsrView<V> v;
+  v.row_ptr = A->row_ptr.as<uint32_t>();
+  v.col = A->col.as<uint32_t>();
+  if constexpr (std::is_same<V, ValF32>::value) v.val = A->val.as<float>();
+  else v.val = A->val16.as<uint16_t>();
+  v.blk = A->blk.as<uint32_t>();
+  v.nblk = A->nblk;
+  v.n = A->n;
+  return v;
+}
+
+static msp_status check_values(msp_csr A, int32_t vt) {
+  if (vt == MSP_VALUES_F32) return MSP_OK;
+  if (vt != MSP_VALUES_BF16 && vt != MSP_VALUES_F16) return fail(MSP_ERR_INVALID, "unknown value type %d", vt);
+  if (A->val16_type != vt) return fail(MSP_ERR_INVALID, "values of type %d not prepared (msp_csr_prepare_values)", vt);
+  return MSP_OK;
+}
+
+template <typename V, int ND, typename XL, typename Epi, typename Fin>
+static void launch_spmv(msp_csr A, XL xl, Epi epi, Fin fin, const int* done, hipStream_t s) {
+  GridRed rb = A->ws.scal.p ? A->ws.gr() : GridRed{nullptr, nullptr};
+  k_spmv<V, ND, XL, Epi, Fin><<<spmv_grid(A->nblk), kBlock, 0, s>>>(view_of<V>(A), xl, epi, fin, rb, done);
+}
+
+// Run `f.template operator()<V>()` for the requested value type.
+template <typename F>
+static msp_status with_values(msp_csr A, int32_t vt, F&& f) {
+  MSPCHK(check_values(A, vt));
+  if (vt == MSP_VALUES_BF16) return f(ValBF16{});
+  if (vt == MSP_VALUES_F16) return f(ValF16{});
+  return f(ValF32{});
+}
+
+extern "C" msp_status msp_csr_spmv(msp_csr A, const float* x, float* y, int32_t value_type, void* stream) {
+  MSPCHK(check_handle(A));
+  if ((!x && A->m) || (!y && A->n)) return fail(MSP_ERR_INVALID, "null x / y");
+  hipStream_t s = (hipStream_t)stream;
+  const bool fast = is_device_ptr(x) && is_device_ptr(y);
+  if (fast) {
+    return with_values(A, value_type, [&](auto vt) -> msp_status {
+      using V = decltype(vt);
+      launch_spmv<V, 0>(A, XPlain{x}, EpiStore{y}, FinNone{}, nullptr, s);
+      return launch_check("k_spmv");
+    });
+  }
+  InArr xi;
+  MSPCHK(xi.set(x, sizeof(float) * A->m, s));
+  DevMem yd;
+  MSPCHK(yd.alloc(sizeof(float) * ((size_t)A->n + 1)));
+  const float* xd = static_cast<const float*>(xi.d);
+  float* ydp = is_device_ptr(y) ? y : yd.as<float>();
+  MSPCHK(with_values(A, value_type, [&](auto vt) -> msp_status {
+    using V = decltype(vt);
+    launch_spmv<V, 0>(A, XPlain{xd}, EpiStore{ydp}, FinNone{}, nullptr, s);
+    return launch_check("k_spmv");
+  }));
+  if (ydp != y) MSPCHK(copy_out(y, ydp, sizeof(float) * A->n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// solver drivers
+static void fill_opts(msp_solve_opts& o, const msp_solve_opts* in) {
+  msp_default_opts(&o);
+  if (in) o = *in;
+  if (o.check_every <= 0) o.check_every = 16;
+}
+
+static msp_status ensure_dinv(msp_csr A, hipStream_t s) {
+  if (A->dinv_state == 1) return MSP_OK;
+  if (A->dinv_state == 2) return fail(MSP_ERR_SINGULAR, "zero or non-finite diagonal entry (Jacobi)");
+  MSPCHK(A->dinv.alloc(sizeof(float) * A->ws.npad));
+  HIPCHK(hipMemsetAsync(A->dinv.p, 0, sizeof(float) * A->ws.npad, s));
+  DevMem bad;
+  MSPCHK(bad.alloc(sizeof(int)));
+  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((A->n + kBlock - 1) / kBlock, 8192));
+  k_inv_diag<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), A->n,
+                                  A->dinv.as<float>(), bad.as<int>());
+  MSPCHK(launch_check("k_inv_diag"));
+  int hb = 0;
+  HIPCHK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  A->dinv_state = hb ? 2 : 1;
+  if (hb) return fail(MSP_ERR_SINGULAR, "zero or non-finite diagonal entry (Jacobi)");
+  return MSP_OK;
+}
+
+// Upload scalar settings, run `init`, then batches of `step` with one
+// host poll of the done flag per batch (one batch of lag).
+template <typename Init, typename Step>
+static msp_status run_solver(msp_csr A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
+                             Step step) {
+  Workspace& w = A->ws;
+  std::memset(w.hS, 0, sizeof(Scalars));
+  w.hS->tol = (float)o.tol;
+  w.hS->maxiter = o.maxiter;
+  w.hS->crit = o.criterion;
+  w.hS->use_verify = o.verify ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(w.S_dev(), w.hS, sizeof(Scalars), hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(w.t0, s));
+  MSPCHK(init());
+  int32_t* dflag = &w.S_dev()->done;
+  int enqueued = 0, slot = 0;
+  bool have_prev = false;
+  while (true) {
+    const int batch = std::min(o.check_every, std::max(1, o.maxiter - enqueued));
+    for (int k = 0; k < batch; ++k) MSPCHK(step());
+    MSPCHK(launch_check("solver step"));
+    enqueued += batch;
+    HIPCHK(hipMemcpyAsync(&w.hflag[slot], dflag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(w.ev[slot], s));
+    if (have_prev) {
+      HIPCHK(hipEventSynchronize(w.ev[slot ^ 1]));
+      if (w.hflag[slot ^ 1] != 0) break;
+    }
+    have_prev = true;
+    if (enqueued >= o.maxiter) break;
+    slot ^= 1;
+  }
+  HIPCHK(hipEventRecord(w.t1, s));
+  HIPCHK(hipMemcpyAsync(w.hS, w.S_dev(), sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (info) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, w.t0, w.t1);
+    info->iterations = w.hS->iter;
+    info->status = w.hS->done == kRunning ? MSP_SOLVE_MAXITER : w.hS->done;
+    info->verifications = w.hS->nverify;
+    info->rr = w.hS->rr;
+    info->bb = w.hS->bb;
+    info->relres = w.hS->bb > 0 ? std::sqrt((double)w.hS->rr / (double)w.hS->bb) : std::sqrt((double)w.hS->rr);
+    info->seconds = ms * 1e-3;
+  }
+  return MSP_OK;
+}
+
+static msp_status solver_prologue(msp_csr A, const float* b, const float* x, const msp_solve_opts& o, hipStream_t s) {
+  if (A->n != A->m) return fail(MSP_ERR_INVALID, "solvers need a square matrix (%u x %u)", A->n, A->m);
+  if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
+  if (o.tol < 0 || o.maxiter < 0) return fail(MSP_ERR_INVALID, "negative tol / maxiter");
+  MSPCHK(ensure_workspace(A, s));
+  MSPCHK(copy_in(A->ws.v(Workspace::B), b, sizeof(float) * A->n, s));
+  if (o.use_x0) MSPCHK(copy_in(A->ws.v(Workspace::X), x, sizeof(float) * A->n, s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_cg(msp_csr A, const float* b, float* x, const msp_solve_opts* opts,
+                                 msp_solve_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  msp_solve_opts o;
+  fill_opts(o, opts);
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(solver_prologue(A, b, x, o, s));
+  if (o.precond != MSP_PRECOND_NONE && o.precond != MSP_PRECOND_JACOBI)
+    return fail(MSP_ERR_UNSUPPORTED, "unknown preconditioner %d", o.precond);
+  if (o.precond == MSP_PRECOND_JACOBI) MSPCHK(ensure_dinv(A, s));
+  Workspace& w = A->ws;
+  float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Q = w.v(Workspace::Q);
+  const float* B = w.v(Workspace::B);
+  const float* D = A->dinv.as<float>();
+  const uint32_t* RP = A->row_ptr.as<uint32_t>();
+  const uint32_t* CI = A->col.as<uint32_t>();
+  const float* AV = A->val.as<float>();
+  Scalars* S = w.S_dev();
+  GridRed rb = w.gr();
+  const uint32_t n = A->n, gv = vec_grid(n);
+  const int pc = o.precond == MSP_PRECOND_JACOBI ? kPcJacobi : kPcNone;
+  MSPCHK(with_values(A, o.values, [&](auto vt) -> msp_status {
+    using V = decltype(vt);
+    auto init = [&]() -> msp_status {
+      if (o.use_x0) launch_spmv<V, 1>(A, XPlain{X}, EpiResidual{R, B}, FinNone{}, nullptr, s);
+      k_cg_init<<<gv, kBlock, 0, s>>>(X, R, P, B, D, n, o.use_x0, pc, S, rb);
+      return launch_check("cg init");
+    };
+    auto step = [&]() -> msp_status {
+      launch_spmv<V, 1>(A, XPlain{P}, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->done, s);
+      if (pc == kPcJacobi) {
+        k_cg_update_xr<kPcJacobi><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
+        k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
+        if (o.verify) k_verify<1><<<gv, kBlock, 0, s>>>(RP, CI, AV, X, B, R, D, n, S, rb);
+      } else {
+        k_cg_update_xr<kPcNone><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
+        k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
+        if (o.verify) k_verify<0><<<gv, kBlock, 0, s>>>(RP, CI, AV, X, B, R, D, n, S, rb);
+      }
+      return MSP_OK;
+    };
+    return run_solver(A, o, info, s, init, step);
+  }));
+  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_bicgstab(msp_csr A, const float* b, float* x, const msp_solve_opts* opts,
+                                       msp_solve_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  msp_solve_opts o;
+  fill_opts(o, opts);
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(solver_prologue(A, b, x, o, s));
+  if (o.precond != MSP_PRECOND_NONE) return fail(MSP_ERR_UNSUPPORTED, "BiCGSTAB takes no preconditioner");
+  Workspace& w = A->ws;
+  float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Vv = w.v(Workspace::Q);
+  float *R0 = w.v(Workspace::R0), *Sv = w.v(Workspace::S), *T = w.v(Workspace::T);
+  const float* B = w.v(Workspace::B);
+  const uint32_t* RP = A->row_ptr.as<uint32_t>();
+  const uint32_t* CI = A->col.as<uint32_t>();
+  const float* AV = A->val.as<float>();
+  Scalars* S = w.S_dev();
+  GridRed rb = w.gr();
+  const uint32_t n = A->n, gv = vec_grid(n);
+  MSPCHK(with_values(A, o.values, [&](auto vt) -> msp_status {
+    using V = decltype(vt);
+    auto init = [&]() -> msp_status {
+      if (o.use_x0) launch_spmv<V, 1>(A, XPlain{X}, EpiResidual{R, B}, FinNone{}, nullptr, s);
+      k_bicg_init<<<gv, kBlock, 0, s>>>(X, R, R0, P, B, n, o.use_x0, S, rb);
+      return launch_check("bicgstab init");
+    };
+    auto step = [&]() -> msp_status {
+      launch_spmv<V, 1>(A, XPlain{P}, EpiStoreDot{Vv, R0}, FinBicgAlpha{S}, &S->done, s);
+      launch_spmv<V, 2>(A, XAxpy{R, Vv, &S->alpha}, EpiBicgT{T, Sv, R, Vv, &S->alpha}, FinBicgOmega{S}, &S->done, s);
+      k_bicg_update<<<gv, kBlock, 0, s>>>(X, R, P, Sv, T, R0, n, S, rb);
+      k_bicg_p<<<gv, kBlock, 0, s>>>(P, R, Vv, n, S);
+      if (o.verify) k_verify<2><<<gv, kBlock, 0, s>>>(RP, CI, AV, X, B, R, R0, n, S, rb);
+      return MSP_OK;
+    };
+    return run_solver(A, o, info, s, init, step);
+  }));
+  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_richardson(msp_csr A, const float* b, float* x, float omega, int32_t iters,
+                                         int32_t use_x0, int32_t value_type, void* stream) {
+  MSPCHK(check_handle(A));
+  if (iters < 0) return fail(MSP_ERR_INVALID, "negative iteration count");
+  msp_solve_opts o;
+  fill_opts(o, nullptr);
+  o.use_x0 = use_x0;
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(solver_prologue(A, b, x, o, s));
+  Workspace& w = A->ws;
+  float* xa = w.v(Workspace::X);
+  float* xb = w.v(Workspace::X2);
+  const float* B = w.v(Workspace::B);
+  const uint32_t n = A->n;
+  if (!use_x0) HIPCHK(hipMemsetAsync(xa, 0, sizeof(float) * w.npad, s));
+  MSPCHK(with_values(A, value_type, [&](auto vt) -> msp_status {
+    using V = decltype(vt);
+    for (int32_t it = 0; it < iters; ++it) {
+      launch_spmv<V, 0>(A, XPlain{xa}, EpiRichardson{xb, xa, B, omega}, FinNone{}, nullptr, s);
+      std::swap(xa, xb);
+    }
+    return launch_check("richardson");
+  }));
+  MSPCHK(copy_out(x, xa, sizeof(float) * n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}

@@ -1,0 +1,195 @@
+// Device-side building blocks shared by every kernel of the library:
+// vector types, wave/block reductions with compensated FP32 sums, the
+// deterministic one-pass grid reduction (per-block partials + last-block
+// finisher), XCD-aware block numbering and value-type loaders.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msp {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;           // threads per workgroup for streaming kernels
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kNumXcd = 8;            // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// ---------------------------------------------------------------------------
+// Compensated (TwoSum) FP32 accumulation: s + c carries ~2x the precision of a
+// plain float sum. No multiplications, so FMA contraction cannot disturb it.
+struct CSum {
+  float s, c;
+};
+
+__device__ __forceinline__ CSum csum_zero() { return CSum{0.0f, 0.0f}; }
+
+__device__ __forceinline__ CSum csum_add(CSum x, float v) {
+  float s = x.s + v;
+  float bp = s - x.s;
+  float e = (x.s - (s - bp)) + (v - bp);
+  return CSum{s, x.c + e};
+}
+
+__device__ __forceinline__ CSum csum_merge(CSum x, CSum y) {
+  float s = x.s + y.s;
+  float bp = s - x.s;
+  float e = (x.s - (s - bp)) + (y.s - bp);
+  return CSum{s, (x.c + y.c) + e};
+}
+
+__device__ __forceinline__ float csum_val(CSum x) { return x.s + x.c; }
+
+__device__ __forceinline__ CSum wave_reduce(CSum v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    CSum o{__shfl_xor(v.s, off, kWave), __shfl_xor(v.c, off, kWave)};
+    v = csum_merge(v, o);
+  }
+  return v;
+}
+
+__device__ __forceinline__ float wave_reduce_sum(float v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum over groups of L consecutive lanes (L a power of two <= 64); every lane
+// of the wave must call it. Result valid in every lane of the group.
+__device__ __forceinline__ float group_reduce_sum(float v, uint32_t L) {
+  for (uint32_t off = L >> 1; off > 0; off >>= 1) v += __shfl_xor(v, (int)off, kWave);
+  return v;
+}
+
+// Block-wide reduction of ND compensated sums; result valid in thread 0.
+template <int ND>
+__device__ __forceinline__ void block_reduce(CSum (&acc)[ND]) {
+  __shared__ CSum red[kWavesPerBlock][ND > 0 ? ND : 1];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) acc[d] = wave_reduce(acc[d]);
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) red[wid][d] = acc[d];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      CSum t = red[0][d];
+      for (int w = 1; w < kWavesPerBlock; ++w) t = csum_merge(t, red[w][d]);
+      acc[d] = t;
+    }
+  }
+  __syncthreads();
+}
+
+// Scratch for a grid reduction: one CSum per (block, value) plus a ticket.
+struct GridRed {
+  CSum* partials;    // [gridDim.x][ND]
+  uint32_t* ticket;  // zero on entry; reset to zero by the finishing block
+};
+
+// Deterministic grid-wide reduction in one kernel: every block publishes its
+// partial, takes a ticket; the block that takes the last ticket sums all
+// partials in block order and gets `true` (in all its threads) with the totals
+// in `tot`. No block ever waits on another one.
+template <int ND>
+__device__ bool grid_reduce(CSum (&acc)[ND], GridRed rb, float (&tot)[ND]) {
+  __shared__ int s_last;
+  __shared__ float s_tot[ND > 0 ? ND : 1];
+  block_reduce<ND>(acc);
+  const uint32_t nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) rb.partials[blockIdx.x * ND + d] = acc[d];
+    __threadfence();
+    uint32_t t = atomicAdd(rb.ticket, 1u);
+    s_last = (t == nb - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  CSum a[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) a[d] = csum_zero();
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const volatile CSum* p = rb.partials + i * ND + d;
+      CSum v{p->s, p->c};
+      a[d] = csum_merge(a[d], v);
+    }
+  }
+  block_reduce<ND>(a);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s_tot[d] = csum_val(a[d]);
+    *rb.ticket = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < ND; ++d) tot[d] = s_tot[d];
+  return true;
+}
+
+// Workgroups are dispatched round-robin over the XCDs; renumber them so that
+// each XCD walks a contiguous range of work items (better L2 reuse of the
+// gathered vector for banded matrices). Bijective on [0, nb).
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t wg, uint32_t nb) {
+  const uint32_t xcd = wg % kNumXcd, idx = wg / kNumXcd;
+  const uint32_t per = nb / kNumXcd, rem = nb % kNumXcd;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// ---------------------------------------------------------------------------
+// Matrix value types: FP32, or 16-bit storage (BF16 / FP16) widened to FP32.
+struct ValF32 {
+  typedef float storage;
+  static __device__ __forceinline__ f4 load4(const float* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+  }
+  static __device__ __forceinline__ float load1(const float* p) { return __builtin_nontemporal_load(p); }
+};
+
+struct ValBF16 {
+  typedef uint16_t storage;
+  static __device__ __forceinline__ float cvt(uint32_t h) { return __uint_as_float(h << 16); }
+  static __device__ __forceinline__ f4 load4(const uint16_t* p) {
+    u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
+    return f4{cvt(w.x & 0xFFFFu), cvt(w.x >> 16), cvt(w.y & 0xFFFFu), cvt(w.y >> 16)};
+  }
+  static __device__ __forceinline__ float load1(const uint16_t* p) { return cvt(*p); }
+};
+
+struct ValF16 {
+  typedef uint16_t storage;
+  static __device__ __forceinline__ float cvt(uint32_t h) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+  }
+  static __device__ __forceinline__ f4 load4(const uint16_t* p) {
+    u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
+    return f4{cvt(w.x & 0xFFFFu), cvt(w.x >> 16), cvt(w.y & 0xFFFFu), cvt(w.y >> 16)};
+  }
+  static __device__ __forceinline__ float load1(const uint16_t* p) { return cvt(*p); }
+};
+
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ uint16_t f32_to_f16(float f) {
+  return __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+}  // namespace msp

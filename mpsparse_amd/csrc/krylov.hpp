@@ -1,0 +1,407 @@
+// Krylov iterations as a short chain of fused kernels per step, each with at
+// most one deterministic grid reduction, all scalars kept on the device.
+//
+// CG (reference: iterative/BiCGSTAB/cgd.metal:11-90, spmv.cpp:528-659) per step:
+//   q = A p, <p,q> -> alpha              (k_spmv + FinCgAlpha)
+//   x += alpha p, r -= alpha q, <r,r>, <r,z> -> convergence, beta  (k_cg_update_xr)
+//   p = z + beta p                       (k_cg_update_p)
+// BiCGSTAB (reference: iterative/BiCGSTAB/bicgstab.metal:12-326, spmv.cpp:699-888):
+//   v = A p, <r0,v> -> alpha             (k_spmv + FinBicgAlpha)
+//   s = r - alpha v formed inside the second SpMV's gather, t = A s,
+//   <t,s>, <t,t> -> omega                (k_spmv + XAxpy/EpiBicgT + FinBicgOmega)
+//   x += alpha p + omega s, r = s - omega t, <r0,r>, <r,r> -> beta (k_bicg_update)
+//   p = r + beta (p - omega v)           (k_bicg_p)
+// Every kernel returns at once when the device `done` flag is set, so the host
+// may enqueue whole batches of steps and poll the flag only between batches.
+#pragma once
+
+#include "common.hpp"
+
+namespace msp {
+
+enum : int32_t { kRunning = 0, kConverged = 1, kBreakdown = 2, kMaxIter = 3, kStagnated = 4 };
+enum : int32_t { kCritRelres = 0, kCritReference = 1 };
+enum : int32_t { kPcNone = 0, kPcJacobi = 1, kPcExplicit = 2 };
+
+struct Scalars {
+  float rz;      // <r, z>, z = M r (z = r without preconditioner)
+  float alpha;
+  float beta;
+  float pq;      // <p, A p>
+  float rho;     // BiCGSTAB <r0, r>
+  float sigma;   // BiCGSTAB <r0, v>
+  float omega;
+  float rr;      // ||r||^2 of the latest iterate
+  float bb;      // ||b||^2
+  float thresh;  // converged when rr <= thresh
+  float tol;
+  float rr0;     // ||r||^2 at start
+  float rr_ver;  // ||b - A x||^2 at the last verification
+  int32_t done;
+  int32_t iter;
+  int32_t maxiter;
+  int32_t crit;
+  int32_t use_verify;  // confirm recurrence convergence with a true residual
+  int32_t verify;      // a verification is pending
+  int32_t nverify;
+  int32_t pad;
+};
+
+// Recurrence says converged: either stop, or ask for a true-residual check.
+__device__ __forceinline__ void on_recurrence_converged(Scalars* S) {
+  if (S->use_verify && S->crit == kCritRelres) S->verify = 1;
+  else S->done = kConverged;
+}
+
+__device__ __forceinline__ bool finite_pos(float v) { return v > 0.0f && isfinite(v); }
+
+// Finishers (run in one thread after the grid reduction) --------------------
+struct FinCgAlpha {
+  Scalars* S;
+  __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
+    S->pq = tot[0];
+    if (!finite_pos(tot[0])) {
+      S->done = kBreakdown;
+      S->alpha = 0.0f;
+    } else {
+      S->alpha = S->rz / tot[0];
+    }
+  }
+};
+
+struct FinBicgAlpha {
+  Scalars* S;
+  __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
+    S->sigma = tot[0];
+    if (tot[0] == 0.0f || !isfinite(tot[0])) {
+      S->done = kBreakdown;
+      S->alpha = 0.0f;
+    } else {
+      S->alpha = S->rho / tot[0];
+    }
+  }
+};
+
+struct FinBicgOmega {
+  Scalars* S;
+  __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
+    S->omega = (tot[1] != 0.0f) ? tot[0] / tot[1] : 0.0f;
+  }
+};
+
+// Beta for PCG with an explicit z computed by other kernels: rz_new in tot[0].
+struct FinPcgBeta {
+  Scalars* S;
+  __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
+    float rz_new = tot[0];
+    if (!isfinite(rz_new) || S->rz == 0.0f) {
+      S->done = kBreakdown;
+      return;
+    }
+    S->beta = rz_new / S->rz;
+    S->rz = rz_new;
+  }
+};
+
+// Convergence test shared by the solvers; `rr_old` is ||r||^2 at the start of
+// the step (the reference criterion tests it every 50 steps,
+// iterative/BiCGSTAB/spmv.cpp:603-629).
+__device__ __forceinline__ bool step_converged(const Scalars* S, int it, float rr_old, float rr_new) {
+  if (S->crit == kCritReference) return (it % 50 == 0) && (rr_old < S->tol);
+  return rr_new <= S->thresh;
+}
+
+__device__ __forceinline__ uint32_t vec_chunks(uint32_t n) { return (n + 3u) / 4u; }
+
+// Initialisation: x = 0 (or kept), r = b (or precomputed b - A x0), p = z,
+// <b,b>, <r,r>, <r,z>.
+__global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float* __restrict__ r,
+                                                    float* __restrict__ p, const float* __restrict__ b,
+                                                    const float* __restrict__ dinv, uint32_t n, int use_x0,
+                                                    int pc, Scalars* S, GridRed rb) {
+  CSum acc[3] = {csum_zero(), csum_zero(), csum_zero()};
+  const uint32_t nc = vec_chunks(n);
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    f4 bv = reinterpret_cast<const f4*>(b)[c];
+    f4 rv = use_x0 ? reinterpret_cast<const f4*>(r)[c] : bv;
+    f4 zv = rv;
+    if (pc == kPcJacobi) zv = rv * reinterpret_cast<const f4*>(dinv)[c];
+    if (!use_x0) {
+      reinterpret_cast<f4*>(x)[c] = f4{0.f, 0.f, 0.f, 0.f};
+      reinterpret_cast<f4*>(r)[c] = rv;
+    }
+    if (pc != kPcExplicit) reinterpret_cast<f4*>(p)[c] = zv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[0] = csum_add(acc[0], bv[k] * bv[k]);
+      acc[1] = csum_add(acc[1], rv[k] * rv[k]);
+      acc[2] = csum_add(acc[2], rv[k] * zv[k]);
+    }
+  }
+  float tot[3];
+  if (grid_reduce<3>(acc, rb, tot) && threadIdx.x == 0) {
+    S->bb = tot[0];
+    S->rr = tot[1];
+    S->rr0 = tot[1];
+    S->rz = tot[2];
+    S->rho = tot[1];
+    S->thresh = S->tol * S->tol * tot[0];
+    S->iter = 0;
+    S->beta = 0.0f;
+    S->alpha = 0.0f;
+    S->omega = 0.0f;
+    int32_t d = kRunning;
+    if (S->crit == kCritRelres && tot[1] <= S->thresh) d = kConverged;
+    if (!isfinite(tot[0]) || !isfinite(tot[1])) d = kBreakdown;
+    if (S->maxiter <= 0 && d == kRunning) d = kMaxIter;
+    S->done = d;
+  }
+}
+
+// x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
+template <int PC>
+__global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r,
+                                                         const float* __restrict__ p, const float* __restrict__ q,
+                                                         const float* __restrict__ dinv, uint32_t n, Scalars* S,
+                                                         GridRed rb) {
+  if (S->done) return;
+  const float alpha = S->alpha;
+  CSum acc[2] = {csum_zero(), csum_zero()};
+  const uint32_t nc = vec_chunks(n);
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    f4 xv = reinterpret_cast<f4*>(x)[c];
+    f4 rv = reinterpret_cast<f4*>(r)[c];
+    f4 pv = reinterpret_cast<const f4*>(p)[c];
+    f4 qv = reinterpret_cast<const f4*>(q)[c];
+    xv += alpha * pv;
+    rv -= alpha * qv;
+    reinterpret_cast<f4*>(x)[c] = xv;
+    reinterpret_cast<f4*>(r)[c] = rv;
+    f4 zv = rv;
+    if (PC == kPcJacobi) zv = rv * reinterpret_cast<const f4*>(dinv)[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[0] = csum_add(acc[0], rv[k] * rv[k]);
+      acc[1] = csum_add(acc[1], rv[k] * zv[k]);
+    }
+  }
+  float tot[2];
+  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+    const int it = S->iter;
+    const float rr_old = S->rr;
+    S->iter = it + 1;
+    S->rr = tot[0];
+    const bool conv = step_converged(S, it, rr_old, tot[0]);
+    if (!isfinite(tot[0])) {
+      S->done = kBreakdown;
+      return;
+    }
+    if (conv) on_recurrence_converged(S);
+    if (S->done) return;
+    if (!S->verify && it + 1 >= S->maxiter) {
+      S->done = kMaxIter;
+    } else if (PC != kPcExplicit) {
+      if (S->rz == 0.0f) {
+        if (!S->verify) S->done = kBreakdown;
+      } else {
+        S->beta = tot[1] / S->rz;
+        S->rz = tot[1];
+      }
+    }
+  }
+}
+
+// p = z + beta p with z = r, dinv*r, or an explicit vector.
+template <int PC>
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(float* __restrict__ p, const float* __restrict__ r,
+                                                        const float* __restrict__ dinv, const float* __restrict__ z,
+                                                        uint32_t n, const Scalars* S) {
+  if (S->done) return;
+  const float beta = S->beta;
+  const uint32_t nc = vec_chunks(n);
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    f4 zv;
+    if (PC == kPcExplicit) zv = reinterpret_cast<const f4*>(z)[c];
+    else zv = reinterpret_cast<const f4*>(r)[c];
+    if (PC == kPcJacobi) zv *= reinterpret_cast<const f4*>(dinv)[c];
+    reinterpret_cast<f4*>(p)[c] = zv + beta * reinterpret_cast<const f4*>(p)[c];
+  }
+}
+
+// BiCGSTAB init: r0 = r, p = r (r = b, or b - A x0 precomputed), x = 0 unless kept.
+__global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, float* __restrict__ r,
+                                                      float* __restrict__ r0, float* __restrict__ p,
+                                                      const float* __restrict__ b, uint32_t n, int use_x0,
+                                                      Scalars* S, GridRed rb) {
+  CSum acc[2] = {csum_zero(), csum_zero()};
+  const uint32_t nc = vec_chunks(n);
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    f4 bv = reinterpret_cast<const f4*>(b)[c];
+    f4 rv = use_x0 ? reinterpret_cast<const f4*>(r)[c] : bv;
+    if (!use_x0) {
+      reinterpret_cast<f4*>(x)[c] = f4{0.f, 0.f, 0.f, 0.f};
+      reinterpret_cast<f4*>(r)[c] = rv;
+    }
+    reinterpret_cast<f4*>(r0)[c] = rv;
+    reinterpret_cast<f4*>(p)[c] = rv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[0] = csum_add(acc[0], bv[k] * bv[k]);
+      acc[1] = csum_add(acc[1], rv[k] * rv[k]);
+    }
+  }
+  float tot[2];
+  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+    S->bb = tot[0];
+    S->rr = tot[1];
+    S->rr0 = tot[1];
+    S->rho = tot[1];
+    S->thresh = S->tol * S->tol * tot[0];
+    S->iter = 0;
+    S->alpha = 0.0f;
+    S->omega = 0.0f;
+    S->beta = 0.0f;
+    int32_t d = kRunning;
+    if (S->crit == kCritRelres && tot[1] <= S->thresh) d = kConverged;
+    if (!isfinite(tot[0]) || !isfinite(tot[1])) d = kBreakdown;
+    if (tot[1] == 0.0f) d = kConverged;
+    if (S->maxiter <= 0 && d == kRunning) d = kMaxIter;
+    S->done = d;
+  }
+}
+
+// x += alpha p + omega s; r = s - omega t; <r0, r>, <r, r> -> bookkeeping, beta.
+__global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, float* __restrict__ r,
+                                                        const float* __restrict__ p, const float* __restrict__ s,
+                                                        const float* __restrict__ t, const float* __restrict__ r0,
+                                                        uint32_t n, Scalars* S, GridRed rb) {
+  if (S->done) return;
+  const float alpha = S->alpha, omega = S->omega;
+  CSum acc[2] = {csum_zero(), csum_zero()};
+  const uint32_t nc = vec_chunks(n);
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    f4 xv = reinterpret_cast<f4*>(x)[c];
+    f4 pv = reinterpret_cast<const f4*>(p)[c];
+    f4 sv = reinterpret_cast<const f4*>(s)[c];
+    f4 tv = reinterpret_cast<const f4*>(t)[c];
+    f4 r0v = reinterpret_cast<const f4*>(r0)[c];
+    xv += alpha * pv + omega * sv;
+    f4 rv = sv - omega * tv;
+    reinterpret_cast<f4*>(x)[c] = xv;
+    reinterpret_cast<f4*>(r)[c] = rv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[0] = csum_add(acc[0], r0v[k] * rv[k]);
+      acc[1] = csum_add(acc[1], rv[k] * rv[k]);
+    }
+  }
+  float tot[2];
+  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+    const int it = S->iter;
+    const float rr_old = S->rr;
+    const float rho_new = tot[0];
+    S->iter = it + 1;
+    S->rr = tot[1];
+    if (!isfinite(tot[1])) {
+      S->done = kBreakdown;
+      return;
+    }
+    if (step_converged(S, it, rr_old, tot[1])) on_recurrence_converged(S);
+    if (S->done) return;
+    if (!S->verify && it + 1 >= S->maxiter) {
+      S->done = kMaxIter;
+    } else if (rho_new == 0.0f || !isfinite(rho_new) || !isfinite(omega) || S->rho == 0.0f) {
+      if (!S->verify) S->done = kBreakdown;
+      else S->beta = 0.0f;
+    } else {
+      S->beta = (omega != 0.0f) ? (rho_new / S->rho) * (alpha / omega) : 0.0f;
+      S->rho = rho_new;
+    }
+  }
+}
+
+// p = r + beta (p - omega v)
+__global__ __launch_bounds__(kBlock) void k_bicg_p(float* __restrict__ p, const float* __restrict__ r,
+                                                   const float* __restrict__ v, uint32_t n, const Scalars* S) {
+  if (S->done) return;
+  const float beta = S->beta, omega = S->omega;
+  const uint32_t nc = vec_chunks(n);
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    f4 pv = reinterpret_cast<f4*>(p)[c];
+    f4 rv = reinterpret_cast<const f4*>(r)[c];
+    f4 vv = reinterpret_cast<const f4*>(v)[c];
+    reinterpret_cast<f4*>(p)[c] = rv + beta * (pv - omega * vv);
+  }
+}
+
+// True-residual check after the recurrence reports convergence:
+// r = b - A x with error-free products (FMA) and TwoSum row accumulation, so
+// the replaced residual is exact to about twice FP32 precision for the FP32 x.
+// Confirms convergence, or replaces the drifted recurrence residual and lets
+// the iteration continue (stops as stagnated when a replacement no longer
+// halves ||r||^2). MODE 0: CG (second dot <r, r>), 1: CG+Jacobi (<r, dinv r>),
+// 2: BiCGSTAB (<r0, r>).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_verify(const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                                   const float* __restrict__ x, const float* __restrict__ b,
+                                                   float* __restrict__ r, const float* __restrict__ aux, uint32_t n,
+                                                   Scalars* S, GridRed rb) {
+  if (S->done || !S->verify) return;
+  CSum acc[2] = {csum_zero(), csum_zero()};
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    CSum s{b[i], 0.0f};
+    for (uint32_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j) {
+      const float a = val[j], xv = x[col[j]];
+      const float p = a * xv;
+      const float e = __builtin_fmaf(a, xv, -p);
+      s = csum_add(s, -p);
+      s.c -= e;
+    }
+    const float ri = csum_val(s);
+    r[i] = ri;
+    acc[0] = csum_add(acc[0], ri * ri);
+    if (MODE == 0) acc[1] = csum_add(acc[1], ri * ri);
+    else if (MODE == 1) acc[1] = csum_add(acc[1], ri * ri * aux[i]);
+    else acc[1] = csum_add(acc[1], aux[i] * ri);
+  }
+  float tot[2];
+  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+    const float prev = S->rr_ver;
+    S->verify = 0;
+    S->nverify += 1;
+    S->rr = tot[0];
+    S->rr_ver = tot[0];
+    if (!isfinite(tot[0])) {
+      S->done = kBreakdown;
+    } else if (tot[0] <= S->thresh) {
+      S->done = kConverged;
+    } else if (S->nverify > 1 && tot[0] > 0.5f * prev) {
+      S->done = kStagnated;
+    } else if (S->iter >= S->maxiter) {
+      S->done = kMaxIter;
+    } else if (MODE == 2) {
+      S->rho = tot[1];
+    } else {
+      S->rz = tot[1];
+    }
+  }
+}
+
+// Plain copy of n floats (vector-typed where aligned).
+__global__ __launch_bounds__(kBlock) void k_copy(float* __restrict__ dst, const float* __restrict__ src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// Dot product <a, b> into out[0] (deterministic).
+__global__ __launch_bounds__(kBlock) void k_dot(const float* __restrict__ a, const float* __restrict__ b, uint32_t n,
+                                                float* __restrict__ out, GridRed rb) {
+  CSum acc[1] = {csum_zero()};
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    acc[0] = csum_add(acc[0], a[i] * b[i]);
+  float tot[1];
+  if (grid_reduce<1>(acc, rb, tot) && threadIdx.x == 0) out[0] = tot[0];
+}
+
+}  // namespace msp

@@ -1,0 +1,271 @@
+// CSR SpMV on row blocks of roughly equal nonzero count.
+//
+// Replaces the reference's one-simdgroup-per-row `spmv_op` / `fused_spvmv`
+// (iterative/BiCGSTAB/spmv.metal:11-132). The row range is cut into blocks
+// whose first nonzeros fall in consecutive windows of T nonzeros (T picked
+// from the mean row length). One 256-thread workgroup handles a block:
+//   1. stream the block's values/column indices with 16-byte nontemporal loads,
+//      multiply by the gathered x and park the products in LDS;
+//   2. reduce each row from LDS with L lanes per row, L the largest power of
+//      two with L * rows <= 256 (1 lane for stencil rows of 5-7 entries when
+//      rows > 256, 8-16 lanes for 144-entry rows);
+//   3. hand every row result to an epilogue (store, store + dot, Richardson
+//      update, ...), whose dot contributions go through one deterministic grid
+//      reduction.
+// Blocks with more than kCap nonzeros (a row longer than the window) take a
+// slower whole-workgroup-per-row path inside the same kernel.
+// Workgroups are persistent (grid capped near residency) and walk blocks in
+// XCD-contiguous order.
+#pragma once
+
+#include "common.hpp"
+
+namespace msp {
+
+constexpr uint32_t kCap = 4096;  // LDS product slots per workgroup
+
+template <typename V>
+struct CsrView {
+  const uint32_t* __restrict__ row_ptr;
+  const uint32_t* __restrict__ col;
+  const typename V::storage* __restrict__ val;
+  const uint32_t* __restrict__ blk;  // [nblk + 1] first row of each block
+  uint32_t nblk;
+  uint32_t n;
+};
+
+// x loaders --------------------------------------------------------------
+struct XPlain {
+  const float* __restrict__ x;
+  __device__ __forceinline__ float operator()(uint32_t c) const { return x[c]; }
+};
+
+// s = r - alpha * v formed on the fly (used to fuse a vector update into the
+// SpMV that consumes it); the owning rows also get s written by the epilogue.
+struct XAxpy {
+  const float* __restrict__ r;
+  const float* __restrict__ v;
+  const float* __restrict__ alpha;
+  __device__ __forceinline__ float operator()(uint32_t c) const { return r[c] - (*alpha) * v[c]; }
+};
+
+// One row block. Products of the block are staged in `sprod`.
+template <typename V, typename XL, typename Epi>
+__device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const XL& xl, const Epi& epi,
+                                          float* sprod, CSum* acc) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t r0 = A.blk[b], r1 = A.blk[b + 1];
+  if (r0 >= r1) return;
+  const uint32_t nz0 = A.row_ptr[r0], nz1 = A.row_ptr[r1];
+  const uint32_t cnt = nz1 - nz0;
+  if (cnt <= kCap) {
+    const uint32_t a0 = nz0 & ~3u;
+    for (uint32_t i = a0 + 4u * tid; i < nz1; i += 4u * kBlock) {
+      u4 c = __builtin_nontemporal_load(reinterpret_cast<const u4*>(A.col + i));
+      f4 v = V::load4(A.val + i);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        uint32_t j = i + k;
+        if (j >= nz0 && j < nz1) sprod[j - nz0] = v[k] * xl(c[k]);
+      }
+    }
+    __syncthreads();
+    const uint32_t nr = r1 - r0;
+    uint32_t L = 1;
+    if (nr < kBlock) {
+      uint32_t q = kBlock / nr;  // >= 1
+      L = 1u << (31 - __builtin_clz(q));
+      if (L > 64) L = 64;
+    }
+    const uint32_t G = kBlock / L, g = tid / L, lane = tid % L;
+    for (uint32_t rb = 0; rb < nr; rb += G) {
+      const uint32_t ri = rb + g;
+      float s = 0.0f;
+      if (ri < nr) {
+        const uint32_t s0 = A.row_ptr[r0 + ri] - nz0, s1 = A.row_ptr[r0 + ri + 1] - nz0;
+        for (uint32_t j = s0 + lane; j < s1; j += L) s += sprod[j];
+      }
+      s = group_reduce_sum(s, L);
+      if (lane == 0 && ri < nr) epi.row(r0 + ri, s, acc);
+    }
+    __syncthreads();
+  } else {
+    // long block: every row reduced by the whole workgroup, straight from HBM
+    __shared__ float wpart[kWavesPerBlock];
+    const int wl = tid & (kWave - 1), wid = tid / kWave;
+    for (uint32_t r = r0; r < r1; ++r) {
+      const uint32_t s0 = A.row_ptr[r], s1 = A.row_ptr[r + 1];
+      float s = 0.0f;
+      for (uint32_t j = s0 + tid; j < s1; j += kBlock) s += V::load1(A.val + j) * xl(A.col[j]);
+      s = wave_reduce_sum(s);
+      if (wl == 0) wpart[wid] = s;
+      __syncthreads();
+      if (tid == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kWavesPerBlock; ++w) t += wpart[w];
+        epi.row(r, t, acc);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// epilogues ------------------------------------------------------------------
+struct EpiStore {
+  float* __restrict__ y;
+  __device__ __forceinline__ void row(uint32_t r, float s, CSum*) const { y[r] = s; }
+};
+
+// y = A x and <y, w>
+struct EpiStoreDot {
+  float* __restrict__ y;
+  const float* __restrict__ w;
+  __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc) const {
+    y[r] = s;
+    acc[0] = csum_add(acc[0], s * w[r]);
+  }
+};
+
+// t = A s with s formed on the fly (XAxpy); also writes s for owned rows and
+// accumulates <t, s>, <t, t>.
+struct EpiBicgT {
+  float* __restrict__ t;
+  float* __restrict__ s_out;
+  const float* __restrict__ r;
+  const float* __restrict__ v;
+  const float* __restrict__ alpha;
+  __device__ __forceinline__ void row(uint32_t i, float ts, CSum* acc) const {
+    float si = r[i] - (*alpha) * v[i];
+    s_out[i] = si;
+    t[i] = ts;
+    acc[0] = csum_add(acc[0], ts * si);
+    acc[1] = csum_add(acc[1], ts * ts);
+  }
+};
+
+// x_new = x_old + omega * (b - A x_old)
+struct EpiRichardson {
+  float* __restrict__ x_new;
+  const float* __restrict__ x_old;
+  const float* __restrict__ b;
+  float omega;
+  __device__ __forceinline__ void row(uint32_t i, float ax, CSum*) const {
+    x_new[i] = x_old[i] + omega * (b[i] - ax);
+  }
+};
+
+// r = b - A x, accumulating <r, r>
+struct EpiResidual {
+  float* __restrict__ r;
+  const float* __restrict__ b;
+  __device__ __forceinline__ void row(uint32_t i, float ax, CSum* acc) const {
+    float ri = b[i] - ax;
+    r[i] = ri;
+    acc[0] = csum_add(acc[0], ri * ri);
+  }
+};
+
+// Finishers run once, in thread 0 of the last workgroup, with the grid totals.
+struct FinNone {
+  template <int ND>
+  __device__ __forceinline__ void operator()(const float (&)[ND]) const {}
+};
+
+struct FinStore {
+  float* out;  // [ND]
+  template <int ND>
+  __device__ __forceinline__ void operator()(const float (&tot)[ND]) const {
+    for (int d = 0; d < ND; ++d) out[d] = tot[d];
+  }
+};
+
+template <typename V, int ND, typename XL, typename Epi, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, Fin fin, GridRed rb,
+                                                 const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float sprod[kCap];
+  constexpr int NA = ND > 0 ? ND : 1;
+  CSum acc[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
+  const uint32_t G = gridDim.x;
+  // XCD-contiguous walk: workgroup wg owns blocks [lo, hi) of its XCD's share
+  const uint32_t wg = xcd_swizzle(blockIdx.x, G);
+  const uint32_t per = A.nblk / G, rem = A.nblk % G;
+  const uint32_t lo = wg * per + (wg < rem ? wg : rem);
+  const uint32_t hi = lo + per + (wg < rem ? 1u : 0u);
+  for (uint32_t b = lo; b < hi; ++b) csr_block<V>(A, b, xl, epi, sprod, acc);
+  if constexpr (ND > 0) {
+    float tot[ND];
+    if (grid_reduce<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb, tot)) {
+      if (threadIdx.x == 0) fin(tot);
+    }
+  }
+}
+
+// Row blocks: blk[b] = first row whose first nonzero index is >= b*T
+// (lower bound in row_ptr), blk[nblk] = n.
+__global__ __launch_bounds__(kBlock) void k_row_blocks(const uint32_t* __restrict__ row_ptr, uint32_t n,
+                                                       uint32_t T, uint32_t nblk, uint32_t* __restrict__ blk) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nblk) return;
+  if (b == nblk) {
+    blk[b] = n;
+    return;
+  }
+  uint64_t target = (uint64_t)b * T;
+  uint32_t lo = 0, hi = n;  // search in row_ptr[0..n]
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo) >> 1);
+    if ((uint64_t)row_ptr[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  blk[b] = lo;
+}
+
+// Maximum row length (for block sizing) via atomicMax.
+__global__ __launch_bounds__(kBlock) void k_max_row(const uint32_t* __restrict__ row_ptr, uint32_t n,
+                                                    uint32_t* __restrict__ out) {
+  uint32_t m = 0;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    uint32_t l = row_ptr[r + 1] - row_ptr[r];
+    m = l > m ? l : m;
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    uint32_t o = __shfl_xor(m, off, kWave);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, m);
+}
+
+// Diagonal (duplicates summed) -> inverse for Jacobi; zero diagonal -> flag.
+__global__ __launch_bounds__(kBlock) void k_inv_diag(const uint32_t* __restrict__ row_ptr,
+                                                     const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                                     uint32_t n, float* __restrict__ dinv, int* __restrict__ bad) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    float d = 0.0f;
+    for (uint32_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j)
+      if (col[j] == r) d += val[j];
+    if (d == 0.0f || !isfinite(d)) {
+      atomicOr(bad, 1);
+      dinv[r] = 0.0f;
+    } else {
+      dinv[r] = 1.0f / d;
+    }
+  }
+}
+
+// FP32 -> 16-bit value copies for the reduced-precision path.
+__global__ __launch_bounds__(kBlock) void k_cvt_bf16(const float* __restrict__ in, uint16_t* __restrict__ out,
+                                                     uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = f32_to_bf16(in[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cvt_f16(const float* __restrict__ in, uint16_t* __restrict__ out,
+                                                    uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = f32_to_f16(in[i]);
+}
+
+}  // namespace msp

@@ -1,0 +1,279 @@
+// PyTorch-ROCm extension: the reference's pybind surface on top of the C ABI.
+//
+// Mirrors PYBIND11_MODULE in iterative/BiCGSTAB/spmv.cpp:892-905
+// (csr_tensor(keys, values, row_ptr, col_ind, out_vals, num_rows, num_cols) with
+// .mv(x, b), .cgd(b, x), .bicgstab(b, x)), the Richardson module's iter_solve
+// (iterative/richardson/spmv.cpp:499-512) and the standalone converter
+// coo_csr.coo_csr(...) (spmv_kernel/coo_csr/coo_csr/coo_csr/coo_to_csr.cpp:282-283).
+// Arguments keep the reference's order and in-place meaning (the last tensor
+// argument is the output). Tensors may live on the CPU (copied) or on the
+// current ROCm device (used in place); work runs on torch's current stream.
+#include <torch/extension.h>
+#include <c10/hip/HIPStream.h>
+#include <c10/hip/HIPGuard.h>
+
+#include <string>
+
+#include "../../include/mpsparse_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void check(msp_status s, const char* what) {
+  TORCH_CHECK(s == MSP_OK, what, " failed (status ", (int)s, "): ", msp_last_error());
+}
+
+void* stream_for(int device) {
+  return (void*)c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
+}
+
+int pick_device(std::initializer_list<const torch::Tensor*> ts) {
+  int dev = -1;
+  for (const torch::Tensor* t : ts) {
+    if (t->defined() && t->is_cuda()) {
+      TORCH_CHECK(dev < 0 || dev == t->get_device(), "tensors on different GPUs");
+      dev = t->get_device();
+    }
+  }
+  if (dev < 0) dev = (int)c10::hip::current_device();
+  return dev;
+}
+
+void expect(const torch::Tensor& t, c10::ScalarType dt, int64_t numel, const char* name) {
+  TORCH_CHECK(t.scalar_type() == dt, name, ": expected dtype ", c10::toString(dt), ", got ",
+              c10::toString(t.scalar_type()));
+  TORCH_CHECK(t.is_contiguous(), name, ": must be contiguous");
+  TORCH_CHECK(numel < 0 || t.numel() == numel, name, ": expected ", numel, " elements, got ", t.numel());
+  TORCH_CHECK(t.is_cpu() || t.is_cuda(), name, ": must be a CPU or ROCm tensor");
+}
+
+int32_t value_type_of(const std::string& v) {
+  if (v == "f32" || v == "fp32" || v == "float32") return MSP_VALUES_F32;
+  if (v == "bf16" || v == "bfloat16") return MSP_VALUES_BF16;
+  if (v == "f16" || v == "fp16" || v == "float16") return MSP_VALUES_F16;
+  TORCH_CHECK(false, "unknown value type '", v, "' (f32, bf16, f16)");
+  return 0;
+}
+
+py::dict info_dict(const msp_solve_info& in) {
+  py::dict d;
+  d["iterations"] = in.iterations;
+  d["status"] = in.status == MSP_SOLVE_CONVERGED   ? "converged"
+                : in.status == MSP_SOLVE_BREAKDOWN ? "breakdown"
+                : in.status == MSP_SOLVE_STAGNATED ? "stagnated"
+                                                   : "maxiter";
+  d["verifications"] = in.verifications;
+  d["relres"] = in.relres;
+  d["rr"] = in.rr;
+  d["bb"] = in.bb;
+  d["seconds"] = in.seconds;
+  return d;
+}
+
+class CsrTensor {
+ public:
+  CsrTensor(torch::Tensor keys, torch::Tensor values, torch::Tensor row_ptr, torch::Tensor col_ind,
+            torch::Tensor out_vals, int64_t num_rows, int64_t num_cols) {
+    TORCH_CHECK(num_rows >= 0 && num_cols >= 0 && num_rows < (1ll << 32) && num_cols < (1ll << 32),
+                "num_rows / num_cols out of range");
+    expect(keys, torch::kInt64, -1, "keys");
+    expect(values, torch::kFloat32, keys.numel(), "values");
+    device_ = pick_device({&keys, &values});
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    {
+      py::gil_scoped_release nogil;
+      check(msp_csr_create_coo(&h_, reinterpret_cast<const uint64_t*>(keys.data_ptr<int64_t>()),
+                               values.data_ptr<float>(), (uint64_t)keys.numel(), (uint32_t)num_rows,
+                               (uint32_t)num_cols, s),
+            "csr_tensor");
+    }
+    // Unlike the reference (which ignores these buffers), write the CSR out
+    // when the caller passes buffers of the right size.
+    export_into(row_ptr, col_ind, out_vals);
+  }
+
+  ~CsrTensor() {
+    if (h_) msp_csr_destroy(h_);
+  }
+
+  void export_into(torch::Tensor row_ptr, torch::Tensor col_ind, torch::Tensor out_vals) {
+    uint32_t n = 0, m = 0;
+    uint64_t nnz = 0;
+    check(msp_csr_shape(h_, &n, &m, &nnz), "shape");
+    uint32_t* rp = nullptr;
+    uint32_t* ci = nullptr;
+    float* ov = nullptr;
+    if (row_ptr.defined() && row_ptr.numel() == (int64_t)n + 1) {
+      expect(row_ptr, torch::kInt32, (int64_t)n + 1, "row_ptr");
+      rp = reinterpret_cast<uint32_t*>(row_ptr.data_ptr<int32_t>());
+    }
+    if (col_ind.defined() && col_ind.numel() == (int64_t)nnz && nnz > 0) {
+      expect(col_ind, torch::kInt32, (int64_t)nnz, "col_ind");
+      ci = reinterpret_cast<uint32_t*>(col_ind.data_ptr<int32_t>());
+    }
+    if (out_vals.defined() && out_vals.numel() == (int64_t)nnz && nnz > 0) {
+      expect(out_vals, torch::kFloat32, (int64_t)nnz, "out_vals");
+      ov = out_vals.data_ptr<float>();
+    }
+    if (!rp && !ci && !ov) return;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    py::gil_scoped_release nogil;
+    check(msp_csr_export(h_, rp, ci, ov, s), "export");
+  }
+
+  void mv(torch::Tensor x, torch::Tensor b, const std::string& values) {
+    uint32_t n = 0, m = 0;
+    check(msp_csr_shape(h_, &n, &m, nullptr), "shape");
+    expect(x, torch::kFloat32, m, "x");
+    expect(b, torch::kFloat32, n, "b");
+    int32_t vt = value_type_of(values);
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    py::gil_scoped_release nogil;
+    if (vt != MSP_VALUES_F32) check(msp_csr_prepare_values(h_, vt, s), "prepare_values");
+    check(msp_csr_spmv(h_, x.data_ptr<float>(), b.data_ptr<float>(), vt, s), "mv");
+  }
+
+  msp_solve_opts make_opts(double tol, int64_t maxiter, int64_t check_every, const std::string& criterion,
+                           bool use_x0, const std::string& precond, const std::string& values, bool verify) {
+    msp_solve_opts o;
+    msp_default_opts(&o);
+    o.tol = tol;
+    o.maxiter = (int32_t)maxiter;
+    o.check_every = (int32_t)check_every;
+    if (criterion == "relres") o.criterion = MSP_CRIT_RELRES;
+    else if (criterion == "reference") o.criterion = MSP_CRIT_REFERENCE;
+    else TORCH_CHECK(false, "criterion must be 'relres' or 'reference'");
+    o.use_x0 = use_x0 ? 1 : 0;
+    if (precond == "none") o.precond = MSP_PRECOND_NONE;
+    else if (precond == "jacobi") o.precond = MSP_PRECOND_JACOBI;
+    else TORCH_CHECK(false, "precond must be 'none' or 'jacobi'");
+    o.values = value_type_of(values);
+    o.verify = verify ? 1 : 0;
+    return o;
+  }
+
+  py::dict solve(bool bicg, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
+                 const std::string& criterion, bool use_x0, const std::string& precond, const std::string& values,
+                 bool verify) {
+    uint32_t n = 0, m = 0;
+    check(msp_csr_shape(h_, &n, &m, nullptr), "shape");
+    expect(b, torch::kFloat32, n, "b");
+    expect(x, torch::kFloat32, m, "x");
+    msp_solve_opts o = make_opts(tol, maxiter, check_every, criterion, use_x0, precond, values, verify);
+    msp_solve_info info{};
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    {
+      py::gil_scoped_release nogil;
+      if (o.values != MSP_VALUES_F32) check(msp_csr_prepare_values(h_, o.values, s), "prepare_values");
+      if (bicg) check(msp_csr_bicgstab(h_, b.data_ptr<float>(), x.data_ptr<float>(), &o, &info, s), "bicgstab");
+      else check(msp_csr_cg(h_, b.data_ptr<float>(), x.data_ptr<float>(), &o, &info, s), "cgd");
+    }
+    return info_dict(info);
+  }
+
+  void richardson(torch::Tensor b, torch::Tensor x, double omega, int64_t iters, bool use_x0,
+                  const std::string& values) {
+    uint32_t n = 0, m = 0;
+    check(msp_csr_shape(h_, &n, &m, nullptr), "shape");
+    expect(b, torch::kFloat32, n, "b");
+    expect(x, torch::kFloat32, m, "x");
+    TORCH_CHECK(iters >= 0 && iters < (1ll << 31), "iters out of range");
+    int32_t vt = value_type_of(values);
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    py::gil_scoped_release nogil;
+    if (vt != MSP_VALUES_F32) check(msp_csr_prepare_values(h_, vt, s), "prepare_values");
+    check(msp_csr_richardson(h_, b.data_ptr<float>(), x.data_ptr<float>(), (float)omega, (int32_t)iters,
+                             use_x0 ? 1 : 0, vt, s),
+          "richardson");
+  }
+
+  int64_t num_rows() const {
+    uint32_t n = 0;
+    msp_csr_shape(h_, &n, nullptr, nullptr);
+    return n;
+  }
+  int64_t num_cols() const {
+    uint32_t m = 0;
+    msp_csr_shape(h_, nullptr, &m, nullptr);
+    return m;
+  }
+  int64_t nnz() const {
+    uint64_t z = 0;
+    msp_csr_shape(h_, nullptr, nullptr, &z);
+    return (int64_t)z;
+  }
+  int device() const { return device_; }
+  uintptr_t handle() const { return reinterpret_cast<uintptr_t>(h_); }
+
+ private:
+  msp_csr h_ = nullptr;
+  int device_ = 0;
+};
+
+void coo_csr(torch::Tensor keys, torch::Tensor values, torch::Tensor row_ptr, torch::Tensor col_ind,
+             torch::Tensor out_vals, int64_t num_rows, int64_t num_cols) {
+  TORCH_CHECK(num_rows >= 0 && num_cols >= 0 && num_rows < (1ll << 32) && num_cols < (1ll << 32),
+              "num_rows / num_cols out of range");
+  expect(keys, torch::kInt64, -1, "keys");
+  expect(values, torch::kFloat32, keys.numel(), "values");
+  expect(row_ptr, torch::kInt32, num_rows + 1, "row_ptr");
+  expect(col_ind, torch::kInt32, keys.numel(), "col_ind");
+  expect(out_vals, torch::kFloat32, keys.numel(), "out_vals");
+  int dev = pick_device({&keys, &values, &row_ptr, &col_ind, &out_vals});
+  c10::hip::HIPGuard guard((c10::DeviceIndex)dev);
+  void* s = stream_for(dev);
+  py::gil_scoped_release nogil;
+  check(msp_coo_to_csr(reinterpret_cast<const uint64_t*>(keys.data_ptr<int64_t>()), values.data_ptr<float>(),
+                       (uint64_t)keys.numel(), (uint32_t)num_rows, (uint32_t)num_cols,
+                       reinterpret_cast<uint32_t*>(row_ptr.data_ptr<int32_t>()),
+                       reinterpret_cast<uint32_t*>(col_ind.data_ptr<int32_t>()), out_vals.data_ptr<float>(), s),
+        "coo_csr");
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+  m.attr("abi_version") = msp_abi_version();
+  m.def("device_count", []() { return msp_device_count(); });
+  m.def("coo_csr", &coo_csr, "COO -> CSR (stable packed-key sort)", py::arg("keys"), py::arg("values"),
+        py::arg("row_ptr"), py::arg("col_ind"), py::arg("out_vals"), py::arg("num_rows"), py::arg("num_cols"));
+  py::class_<CsrTensor>(m, "csr_tensor")
+      .def(py::init<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, int64_t, int64_t>(),
+           "CSR Tensor Constructor", py::arg("keys"), py::arg("values"), py::arg("row_ptr"), py::arg("col_ind"),
+           py::arg("out_vals"), py::arg("num_rows"), py::arg("num_cols"))
+      .def("mv", &CsrTensor::mv, "b = A x", py::arg("x"), py::arg("b"), py::arg("values") = "f32")
+      .def(
+          "cgd",
+          [](CsrTensor& self, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
+             const std::string& criterion, bool use_x0, const std::string& precond, const std::string& values,
+             bool verify) {
+            return self.solve(false, b, x, tol, maxiter, check_every, criterion, use_x0, precond, values, verify);
+          },
+          "conjugate gradient", py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("maxiter") = 10000,
+          py::arg("check_every") = 16, py::arg("criterion") = "relres", py::arg("use_x0") = false,
+          py::arg("precond") = "none", py::arg("values") = "f32", py::arg("verify") = true)
+      .def(
+          "bicgstab",
+          [](CsrTensor& self, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
+             const std::string& criterion, bool use_x0, const std::string& values, bool verify) {
+            return self.solve(true, b, x, tol, maxiter, check_every, criterion, use_x0, "none", values, verify);
+          },
+          "BiCGSTAB", py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("maxiter") = 10000,
+          py::arg("check_every") = 16, py::arg("criterion") = "relres", py::arg("use_x0") = false,
+          py::arg("values") = "f32", py::arg("verify") = true)
+      .def("richardson", &CsrTensor::richardson, "x <- x + omega (b - A x), iters times", py::arg("b"), py::arg("x"),
+           py::arg("omega") = 0.01, py::arg("iters") = 1000, py::arg("use_x0") = false, py::arg("values") = "f32")
+      .def("export", &CsrTensor::export_into, "write row_ptr / col_ind / values", py::arg("row_ptr"),
+           py::arg("col_ind"), py::arg("out_vals"))
+      .def_property_readonly("num_rows", &CsrTensor::num_rows)
+      .def_property_readonly("num_cols", &CsrTensor::num_cols)
+      .def_property_readonly("nnz", &CsrTensor::nnz)
+      .def_property_readonly("device", &CsrTensor::device)
+      .def_property_readonly("handle", &CsrTensor::handle);
+}

@@ -1,0 +1,72 @@
+"""Helpers shared by the GPU tests (not collected by pytest)."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    """Name the case before a GPU step (flushed, so it survives a crash)."""
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def t_keys(keys, device="cuda"):
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(keys).view(np.int64))
+    return t.to(device) if device != "cpu" else t
+
+
+def t_f32(a, device="cuda"):
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32))
+    return t.to(device) if device != "cpu" else t
+
+
+def build_csr(keys, vals, n, m, device="cuda", export=True):
+    """spmv.csr_tensor from numpy COO; returns (A, row_ptr, col_ind, vals) as numpy."""
+    import torch
+
+    from mpsparse_amd import spmv
+
+    nnz = keys.shape[0]
+    rp = torch.zeros(n + 1, dtype=torch.int32, device=device)
+    ci = torch.zeros(nnz, dtype=torch.int32, device=device)
+    ov = torch.zeros(nnz, dtype=torch.float32, device=device)
+    A = spmv.csr_tensor(t_keys(keys, device), t_f32(vals, device), rp if export else torch.empty(0),
+                        ci if export else torch.empty(0), ov if export else torch.empty(0), n, m)
+    return A, rp.cpu().numpy().view(np.uint32), ci.cpu().numpy().view(np.uint32), ov.cpu().numpy()
+
+
+def csr_from_arrays(row_ptr, col_ind, vals, n, m):
+    """A handle built from sorted CSR arrays (via packed keys, order preserved)."""
+    from mpsparse_amd.problems import pack_keys
+
+    rows = np.repeat(np.arange(n, dtype=np.uint64), np.diff(np.asarray(row_ptr, dtype=np.int64)))
+    keys = pack_keys(rows, np.asarray(col_ind, dtype=np.uint64))
+    return build_csr(keys, vals, n, m, export=False)[0]
+
+
+def bf16_round(a):
+    """Round-to-nearest-even FP32 -> BF16, returned widened to float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+class Lib:
+    """Direct ctypes view of the C ABI (same library the extension links)."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(os.path.join(ROOT, "mpsparse_amd", "libmpsparse_amd.so"))
+        self.lib.msp_last_error.restype = ctypes.c_char_p
+
+    def stream(self):
+        import torch
+
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

@@ -1,0 +1,106 @@
+"""GPU COO->CSR parity: bit-exact against the oracle's stable sort."""
+import numpy as np
+import pytest
+
+from gpu_util import build_csr, say, t_f32, t_keys
+from mpsparse_amd.problems import block_sparse, laplacian_2d, pack_keys
+from oracle import coo_csr as oc
+
+pytestmark = pytest.mark.gpu
+
+
+def _kat_arrays(case):
+    coo = np.array(case["coo"], dtype=np.uint64).reshape(-1, 2)
+    keys = pack_keys(coo[:, 0], coo[:, 1]) if len(coo) else np.zeros(0, np.uint64)
+    return keys, np.array(case["vals"], dtype=np.float32)
+
+
+def _run_coo_csr(keys, vals, n, m, device):
+    import torch
+
+    from mpsparse_amd import coo_csr
+
+    nnz = keys.shape[0]
+    rp = torch.full((n + 1,), -7, dtype=torch.int32, device=device)
+    ci = torch.full((nnz,), -7, dtype=torch.int32, device=device)
+    ov = torch.full((nnz,), -7.0, dtype=torch.float32, device=device)
+    coo_csr.coo_csr(t_keys(keys, device), t_f32(vals, device), rp, ci, ov, n, m)
+    return rp.cpu().numpy().view(np.uint32), ci.cpu().numpy().view(np.uint32), ov.cpu().numpy()
+
+
+def _same(got, ref):
+    for g, r in zip(got, ref):
+        assert np.array_equal(np.asarray(g).view(np.uint32), np.asarray(r).view(np.uint32))
+
+
+@pytest.mark.parametrize("device", ["cpu", "cuda"])
+def test_known_answers(kat_cases, device):
+    for case in kat_cases:
+        say("kat", case["name"], device)
+        keys, vals = _kat_arrays(case)
+        got = _run_coo_csr(keys, vals, case["num_rows"], case["num_cols"], device)
+        assert got[0].tolist() == case["row_ptr"], case["name"]
+        assert got[1].tolist() == case["col_ind"], case["name"]
+        assert got[2].tolist() == case["out_vals"], case["name"]
+
+
+CASES = [
+    # n, m, nnz, distinct-key range (None = full range)
+    (1, 1, 50, None), (7, 3, 200, None), (100, 100, 3000, 8), (1000, 37, 5000, None),
+    (33, 100000, 4000, None), (5000, 5000, 1000, None), (2, 17, 300, None), (70000, 3, 2000, None),
+    (4097, 4097, 4096, None), (4096, 4096, 4097, None), (1 << 16, 1 << 16, 1 << 20, None),
+    (1 << 20, 1 << 20, 3 << 20, 64), (300000, 2 ** 31 + 5, 200000, None), (9, 2 ** 32 - 1, 100000, None),
+]
+
+
+@pytest.mark.parametrize("n,m,nnz,dup", CASES)
+def test_random_against_oracle(n, m, nnz, dup):
+    rng = np.random.default_rng(n + 31 * nnz)
+    if dup:
+        rows = rng.integers(0, min(n, dup), nnz)
+        cols = rng.integers(0, min(m, dup), nnz)
+    else:
+        rows = rng.integers(0, n, nnz)
+        cols = rng.integers(0, m, nnz)
+    keys = pack_keys(rows, cols)
+    vals = rng.standard_normal(nnz).astype(np.float32)
+    say("random", n, m, nnz, dup)
+    _same(_run_coo_csr(keys, vals, n, m, "cuda"), oc.coo_to_csr(keys, vals, n, m))
+
+
+def test_csr_tensor_exports_laplacian():
+    p = laplacian_2d(64)
+    say("csr_tensor laplacian 64")
+    A, rp, ci, v = build_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    _same((rp, ci, v), oc.coo_to_csr(p.keys, p.vals, p.num_rows, p.num_cols))
+    assert A.nnz == 20224 and A.num_rows == 4096
+
+
+def test_block_sparse_sixteenth_size():
+    q = block_sparse(4096, 16, 8, seed=0)  # 9.4M keys, same structure as the full system
+    say("block_sparse 4096 block rows", q.nnz)
+    A, rp, ci, v = build_csr(q.keys, q.vals, q.num_rows, q.num_cols)
+    _same((rp, ci, v), oc.coo_to_csr(q.keys, q.vals, q.num_rows, q.num_cols))
+
+
+def test_key_out_of_range_raises():
+    from mpsparse_amd import coo_csr
+
+    keys = pack_keys(np.array([0, 5]), np.array([1, 1]))
+    say("key range")
+    with pytest.raises(RuntimeError, match="outside"):
+        _run_coo_csr(keys, np.ones(2, np.float32), 3, 3, "cuda")
+    keys = pack_keys(np.array([0, 1]), np.array([1, 3]))
+    with pytest.raises(RuntimeError, match="outside"):
+        _run_coo_csr(keys, np.ones(2, np.float32), 3, 3, "cpu")
+    assert coo_csr is not None
+
+
+def test_repeat_is_deterministic():
+    rng = np.random.default_rng(9)
+    keys = pack_keys(rng.integers(0, 50, 100000), rng.integers(0, 50, 100000))
+    vals = rng.standard_normal(100000).astype(np.float32)
+    say("repeat")
+    a = _run_coo_csr(keys, vals, 50, 50, "cuda")
+    b = _run_coo_csr(keys, vals, 50, 50, "cuda")
+    _same(a, b)

@@ -1,0 +1,187 @@
+"""GPU CG / BiCGSTAB / Richardson against the FP64 oracle and scipy.
+
+Bars (FP32 device arithmetic):
+* converged solves: true relative residual ||b - A x|| / ||b|| (FP64, oracle)
+  <= 1.05 * tol -- the solver confirms convergence with a compensated true residual;
+* error vs the FP64 direct solution <= kappa-scaled bound stated per test;
+* iteration counts within 15% (+2) of the FP64 oracle's;
+* Richardson: |x - x64| <= 1e-4 * ||x64||_inf after the stated steps.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from gpu_util import build_csr, csr_from_arrays, say, t_f32
+from mpsparse_amd.problems import block_sparse, diffusion_2d, laplacian_2d, laplacian_2d_csr, pack_keys
+from oracle import krylov as K
+from oracle import coo_csr as oc
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(A, b, method="cgd", **kw):
+    import torch
+
+    x = torch.full((A.num_cols,), np.nan, dtype=torch.float32, device="cuda")
+    info = getattr(A, method)(t_f32(b), x, **kw)
+    return x.cpu().numpy(), info
+
+
+def _scipy(rp, ci, v, n):
+    import scipy.sparse as sp
+
+    return sp.csr_matrix((v.astype(np.float64), ci.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+
+
+def test_cg_laplacian_64():
+    p = laplacian_2d(64)
+    say("cg laplacian 64")
+    A, rp, ci, v = build_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    S = _scipy(rp, ci, v, p.num_rows)
+    x, info = _solve(A, p.b)
+    assert info["status"] == "converged", info
+    rel = K.true_relres(S, x, p.b)
+    assert rel <= 1.05e-6, (rel, info)
+    _, it64, _, _ = K.cg(S, p.b.astype(np.float64), tol=1e-6)
+    assert abs(info["iterations"] - it64) <= 0.15 * it64 + 2, (info, it64)
+    xs = spla.spsolve(S.tocsc(), p.b.astype(np.float64))
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1712 * 1.05e-6
+
+
+def test_cg_reference_criterion():
+    """sum(r^2) < 1e-5 tested every 50 steps (iterative/BiCGSTAB/spmv.cpp:603-629)."""
+    p = laplacian_2d(64)
+    A, rp, ci, v = build_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    S = _scipy(rp, ci, v, p.num_rows)
+    say("cg reference criterion")
+    x, info = _solve(A, p.b, tol=1e-5, criterion="reference", maxiter=1000)
+    it = info["iterations"]
+    assert info["status"] == "converged" and it % 50 == 1, info
+    _, _, _, hist = K.cg(S, p.b.astype(np.float64), tol=0.0, maxiter=it)
+    bb = float(np.dot(p.b.astype(np.float64), p.b.astype(np.float64)))
+    rr = [(h ** 2) * bb for h in hist]  # rr at the start of each step
+    assert rr[it - 1] < 1e-5 and rr[it - 51] >= 1e-5
+
+
+def test_cg_module_iter_solve():
+    import torch
+
+    from mpsparse_amd import cgd
+
+    p = laplacian_2d(32)
+    A = cgd.csr_tensor(torch.from_numpy(p.keys.view(np.int64)), torch.from_numpy(p.vals), torch.empty(0),
+                       torch.empty(0), torch.empty(0), p.num_rows, p.num_cols)
+    x = torch.zeros(p.num_rows)
+    say("cgd.iter_solve")
+    info = A.iter_solve(torch.from_numpy(p.b), x)
+    S = laplacian_2d_csr(32).to_scipy()
+    assert info["status"] == "converged" and K.true_relres(S, x.numpy(), p.b) <= 1.05e-6
+
+
+def test_jacobi_pcg_lognormal():
+    M = diffusion_2d(96, sigma=1.5, seed=4)
+    b = np.random.default_rng(4).standard_normal(M.num_rows).astype(np.float32)
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    S = M.to_scipy()
+    say("pcg jacobi")
+    x, info = _solve(A, b, precond="jacobi")
+    assert info["status"] == "converged", info
+    assert K.true_relres(S, x, b) <= 1.05e-6
+    dinv = 1.0 / S.diagonal()
+    _, it64, _, _ = K.cg(S, b.astype(np.float64), tol=1e-6, M_apply=lambda r: dinv * r)
+    assert abs(info["iterations"] - it64) <= 0.15 * it64 + 2, (info, it64)
+    _, info_plain = _solve(A, b)
+    assert info["iterations"] < info_plain["iterations"]
+
+
+def test_bicgstab_block_sparse():
+    q = block_sparse(1024, 16, 8, seed=2)
+    say("bicgstab block sparse 1024")
+    A, rp, ci, v = build_csr(q.keys, q.vals, q.num_rows, q.num_cols)
+    S = _scipy(rp, ci, v, q.num_rows)
+    x, info = _solve(A, q.b, method="bicgstab")
+    assert info["status"] == "converged", info
+    assert K.true_relres(S, x, q.b) <= 1.05e-6
+    _, it64, _, _ = K.bicgstab(S, q.b.astype(np.float64), tol=1e-6)
+    assert abs(info["iterations"] - it64) <= 0.15 * it64 + 2, (info, it64)
+    xs = spla.spsolve(S.tocsc(), q.b.astype(np.float64))
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-5
+
+
+def test_bicgstab_laplacian():
+    p = laplacian_2d(64)
+    say("bicgstab laplacian")
+    A, rp, ci, v = build_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    S = _scipy(rp, ci, v, p.num_rows)
+    x, info = _solve(A, p.b, method="bicgstab")
+    assert info["status"] == "converged", info
+    assert K.true_relres(S, x, p.b) <= 1.05e-6
+
+
+def test_use_x0_warm_start():
+    p = laplacian_2d(64)
+    A, rp, ci, v = build_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    S = _scipy(rp, ci, v, p.num_rows)
+    xs = spla.spsolve(S.tocsc(), p.b.astype(np.float64)).astype(np.float32)
+    import torch
+
+    x = torch.from_numpy(xs.copy()).cuda()
+    say("warm start")
+    info = A.cgd(t_f32(p.b), x, use_x0=True)
+    assert info["status"] == "converged" and info["iterations"] <= 5, info
+    assert K.true_relres(S, x.cpu().numpy(), p.b) <= 1.05e-6
+
+
+def test_richardson_matches_fp64():
+    p = laplacian_2d(64)
+    A, rp, ci, v = build_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    S = _scipy(rp, ci, v, p.num_rows)
+    import torch
+
+    for omega, iters in [(0.2, 200), (0.01, 1000)]:
+        say("richardson", omega, iters)
+        x = torch.full((p.num_rows,), np.nan, device="cuda")
+        A.richardson(t_f32(p.b), x, omega=omega, iters=iters)
+        x64 = K.richardson(S, p.b.astype(np.float64), omega, iters)
+        assert np.max(np.abs(x.cpu().numpy() - x64)) <= 1e-4 * np.max(np.abs(x64))
+
+
+def test_richardson_module_iter_solve():
+    import torch
+
+    from mpsparse_amd import richardson
+
+    p = laplacian_2d(32)
+    A = richardson.csr_tensor(torch.from_numpy(p.keys.view(np.int64)), torch.from_numpy(p.vals), torch.empty(0),
+                              torch.empty(0), torch.empty(0), p.num_rows, p.num_cols)
+    x = torch.zeros(p.num_rows)
+    say("richardson.iter_solve")
+    A.iter_solve(torch.from_numpy(p.b), x)
+    S = laplacian_2d_csr(32).to_scipy()
+    x64 = K.richardson(S, p.b.astype(np.float64), 0.01, 1000)
+    assert np.max(np.abs(x.numpy() - x64)) <= 1e-4 * np.max(np.abs(x64))
+
+
+def test_breakdown_and_limits():
+    n = 64
+    keys = pack_keys(np.arange(n), np.arange(n))
+    A0, *_ = build_csr(keys, np.zeros(n, np.float32), n, n)
+    b = np.ones(n, np.float32)
+    say("zero matrix")
+    _, info = _solve(A0, b)
+    assert info["status"] == "breakdown", info
+    _, info = _solve(A0, b, method="bicgstab")
+    assert info["status"] == "breakdown", info
+    p = laplacian_2d(64)
+    A, *_ = build_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    say("maxiter")
+    _, info = _solve(A, p.b, maxiter=5)
+    assert info["status"] == "maxiter" and info["iterations"] == 5, info
+    say("nan rhs")
+    bn = p.b.copy()
+    bn[7] = np.nan
+    _, info = _solve(A, bn)
+    assert info["status"] == "breakdown", info
+    say("zero rhs")
+    x, info = _solve(A, np.zeros_like(p.b))
+    assert info["status"] == "converged" and info["iterations"] == 0 and np.all(x == 0), info

@@ -83,7 +83,30 @@ typedef struct msp_solve_info {
   double seconds;        /* device time of the iteration loop (HIP events) */
 } msp_solve_info;
 
+/* Mixed-precision iterative refinement (msp_csr_solve_refined). */
+typedef struct msp_refine_opts {
+  double tol;            /* outer target ||b - A x|| / ||b|| in FP64; default 1e-10 */
+  double inner_tol;      /* relative tolerance of each FP32 correction solve; default 1e-3 */
+  int32_t max_outer;     /* default 30 */
+  int32_t inner_maxiter; /* default 10000 */
+  int32_t inner_solver;  /* 0: CG (default), 1: BiCGSTAB */
+  int32_t values;        /* msp_value_type of the matrix inside the correction solve */
+  int32_t precond;       /* msp_precond (CG) */
+  int32_t use_x0;        /* 1: x holds an FP64 initial guess */
+  int32_t reserved[4];
+} msp_refine_opts;
+
+typedef struct msp_refine_info {
+  int32_t outer_iterations;
+  int32_t inner_iterations; /* summed over all correction solves */
+  int32_t status;           /* msp_solve_status */
+  int32_t fallbacks;        /* correction solves repeated with FP32 values after a breakdown */
+  double relres;            /* FP64 true relative residual at exit */
+  double seconds;           /* wall time of the whole refinement */
+} msp_refine_info;
+
 typedef struct msp_csr_s *msp_csr;
+typedef struct msp_bcsr_s *msp_bcsr;
 
 /* Library ----------------------------------------------------------------- */
 int32_t msp_abi_version(void);
@@ -146,6 +169,53 @@ msp_status msp_csr_bicgstab(msp_csr A, const float *b, float *x, const msp_solve
  * x0 = 0) and the CUDA reference's loop (spmv_cuda_reference.cu:227-280). */
 msp_status msp_csr_richardson(msp_csr A, const float *b, float *x, float omega, int32_t iters,
                               int32_t use_x0, int32_t value_type, void *stream);
+
+/* Mixed-precision solve: FP64 outer residual r = b - A x and FP64 accumulation
+ * x += d; each correction d solves A d = r / ||r|| in FP32 (values optionally
+ * BF16/FP16, compensated FP32 dots). b and x are FP64 (host or device). The
+ * reference has no counterpart (it is FP32 only); this is the precision path
+ * layered on msp_csr_cg / msp_csr_bicgstab. */
+void msp_default_refine_opts(msp_refine_opts *opts);
+msp_status msp_csr_solve_refined(msp_csr A, const double *b, double *x, const msp_refine_opts *opts,
+                                 msp_refine_info *info, void *stream);
+
+/* Block CSR ---------------------------------------------------------------
+ * Square B x B blocks (B = 4, 8 or 16), row-major inside a block (the layout of
+ * scipy's BSR `data`), block rows of B rows; the last block row/column may be
+ * partial (num_rows / num_cols need not be multiples of B). */
+
+/* BCSR handle from block arrays (copied): brow_ptr[ceil(num_rows/B)+1],
+ * bcol[nblk], blocks[nblk*B*B]. Replaces bcsr.bcsr_tensor(keys, values, row_ptr,
+ * col_ind, num_rows, num_cols, block_size) (iterative/BiCGSTAB/bcsr.cpp:357-422,
+ * binding bcsr.cpp:953-965), which is fixed to B = 4. */
+msp_status msp_bcsr_create(msp_bcsr *out, const uint32_t *brow_ptr, const uint32_t *bcol, const float *blocks,
+                           uint32_t num_rows, uint32_t num_cols, uint32_t block, void *stream);
+
+/* BCSR handle from packed COO keys, converted on the GPU (COO -> CSR -> BCSR;
+ * entries with equal (row, col) are summed). Replaces the reference's CPU
+ * preprocessing coo_to_metal_bcsr (iterative/BiCGSTAB/bcsr_benchmark.py:36-66). */
+msp_status msp_bcsr_create_coo(msp_bcsr *out, const uint64_t *keys, const float *vals, uint64_t nnz,
+                               uint32_t num_rows, uint32_t num_cols, uint32_t block, void *stream);
+
+msp_status msp_bcsr_shape(msp_bcsr A, uint32_t *num_rows, uint32_t *num_cols, uint32_t *block,
+                          uint64_t *num_blocks);
+
+/* Copy the block arrays out (NULL skips an array). */
+msp_status msp_bcsr_export(msp_bcsr A, uint32_t *brow_ptr, uint32_t *bcol, float *blocks, void *stream);
+
+msp_status msp_bcsr_destroy(msp_bcsr A);
+
+/* y = A x (bcsr_spmv.metal:30-69 `spmv_op`; bcsr.cpp:438-452). */
+msp_status msp_bcsr_spmv(msp_bcsr A, const float *x, float *y, void *stream);
+
+/* CG / BiCGSTAB / Richardson on BCSR. Replace bcsr_tensor.cgd / .bicgstab
+ * (iterative/BiCGSTAB/bcsr.cpp:577-721, 723-949). */
+msp_status msp_bcsr_cg(msp_bcsr A, const float *b, float *x, const msp_solve_opts *opts, msp_solve_info *info,
+                       void *stream);
+msp_status msp_bcsr_bicgstab(msp_bcsr A, const float *b, float *x, const msp_solve_opts *opts,
+                             msp_solve_info *info, void *stream);
+msp_status msp_bcsr_richardson(msp_bcsr A, const float *b, float *x, float omega, int32_t iters, int32_t use_x0,
+                               void *stream);
 
 #ifdef __cplusplus
 }

@@ -4,14 +4,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
 
 #include "../../include/mpsparse_amd.h"
+#include "bcsr.hpp"
 #include "common.hpp"
 #include "krylov.hpp"
+#include "refine.hpp"
 #include "sort.hpp"
 #include "spmv.hpp"
 
@@ -183,24 +187,36 @@ struct Workspace {
   }
 };
 
-struct msp_csr_s {
+// State shared by every matrix kind: device, shape, solver workspace, Jacobi.
+struct MatBase {
   int device = 0;
   uint32_t n = 0, m = 0;
-  uint64_t nnz = 0;
-  DevMem row_ptr, col, val, val16, blk, dinv;
-  int val16_type = -1;
-  uint32_t nblk = 0, T = 0, max_row = 0;
-  int dinv_state = 0;  // 0 not built, 1 ok, 2 singular
   Workspace ws;
+  DevMem dinv;
+  int dinv_state = 0;  // 0 not built, 1 ok, 2 singular
+  DevMem w64;          // refinement: x64, r64, b64 (npad doubles each) + 2 sums
 };
 
-static msp_status check_handle(msp_csr A) {
+struct msp_csr_s : MatBase {
+  uint64_t nnz = 0;
+  DevMem row_ptr, col, val, val16, blk;
+  int val16_type = -1;
+  uint32_t nblk = 0, T = 0, max_row = 0;
+};
+
+struct msp_bcsr_s : MatBase {
+  uint32_t B = 0, nbr = 0, nbc = 0;
+  uint64_t nblk = 0;
+  DevMem brow_ptr, bcol, blocks;
+};
+
+static msp_status check_handle(const MatBase* A) {
   if (!A) return fail(MSP_ERR_INVALID, "null matrix handle");
   HIPCHK(hipSetDevice(A->device));
   return MSP_OK;
 }
 
-static msp_status ensure_workspace(msp_csr A, hipStream_t s) {
+static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
   Workspace& w = A->ws;
   size_t need = ((size_t)std::max(A->n, A->m) + 255) / 256 * 256 + 256;
   if (w.npad >= need && w.scal.p) return MSP_OK;
@@ -317,8 +333,9 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
 // Sort (keys, vals) on the device into CSR arrays. `row_ptr` has n+1 entries,
 // `col_out`/`val_out` nnz entries.
 static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n, uint32_t m,
-                                    uint32_t* row_ptr, uint32_t* col_out, float* val_out, hipStream_t s) {
-  DevMem err, rows;
+                                    uint32_t* row_ptr, uint32_t* col_out, float* val_out, hipStream_t s,
+                                    uint32_t* rows_out = nullptr) {
+  DevMem err, rows_own;
   MSPCHK(err.alloc(sizeof(int)));
   HIPCHK(hipMemsetAsync(err.p, 0, sizeof(int), s));
   if (nnz > 0) {
@@ -330,15 +347,18 @@ static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uin
     const uint32_t rbits = n > 1 ? bitlen(n - 1) : 0;
     const uint32_t tb = km.cbits + rbits;
     const int passes = std::max(1, (int)((tb + sortk::kRadixBits - 1) / sortk::kRadixBits));
-    MSPCHK(rows.alloc(sizeof(uint32_t) * nnz));
+    if (!rows_out) {
+      MSPCHK(rows_own.alloc(sizeof(uint32_t) * nnz));
+      rows_out = rows_own.as<uint32_t>();
+    }
     if (tb <= 32)
-      MSPCHK(sort_passes<uint32_t>(keys, vals, nnz, km, passes, col_out, val_out, rows.as<uint32_t>(), err.as<int>(), s));
+      MSPCHK(sort_passes<uint32_t>(keys, vals, nnz, km, passes, col_out, val_out, rows_out, err.as<int>(), s));
     else
-      MSPCHK(sort_passes<uint64_t>(keys, vals, nnz, km, passes, col_out, val_out, rows.as<uint32_t>(), err.as<int>(), s));
+      MSPCHK(sort_passes<uint64_t>(keys, vals, nnz, km, passes, col_out, val_out, rows_out, err.as<int>(), s));
   }
   uint64_t work = std::max<uint64_t>(nnz, (uint64_t)n + 1);
   uint32_t g = (uint32_t)std::min<uint64_t>((work + kBlock - 1) / kBlock, 65536);
-  sortk::k_row_ptr<<<g, kBlock, 0, s>>>(rows.as<uint32_t>(), nnz, n, row_ptr, err.as<int>());
+  sortk::k_row_ptr<<<g, kBlock, 0, s>>>(rows_out, nnz, n, row_ptr, err.as<int>());
   MSPCHK(launch_check("k_row_ptr"));
   int herr = 0;
   HIPCHK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -534,7 +554,8 @@ extern "C" msp_status msp_csr_destroy(msp_csr A) {
 }
 
 // ---------------------------------------------------------------------------
-// SpMV dispatch helpers
+// Operators: what a solver needs from a matrix kind (SpMV with fused epilogue,
+// the true-residual check, the inverse diagonal).
 template <typename V>
 static CsrView<V> view_of(msp_csr A) {
   CsrView<V> v;
@@ -548,6 +569,73 @@ static CsrView<V> view_of(msp_csr A) {
   return v;
 }
 
+static GridRed red_of(const MatBase* A) { return A->ws.scal.p ? A->ws.gr() : GridRed{nullptr, nullptr}; }
+
+template <typename V>
+struct OpCsr {
+  msp_csr A;
+  MatBase* base() const { return A; }
+  template <int ND, typename Epi, typename Fin>
+  void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
+    k_spmv<V, ND, XPlain, Epi, Fin><<<spmv_grid(A->nblk), kBlock, 0, s>>>(view_of<V>(A), XPlain{x}, epi, fin,
+                                                                         red_of(A), done);
+  }
+  // y = A (r - alpha v)
+  template <int ND, typename Epi, typename Fin>
+  void spmv_axpy(const float* r, const float* v, const float* alpha, Epi epi, Fin fin, const int* done,
+                 hipStream_t s) const {
+    k_spmv<V, ND, XAxpy, Epi, Fin><<<spmv_grid(A->nblk), kBlock, 0, s>>>(view_of<V>(A), XAxpy{r, v, alpha}, epi,
+                                                                        fin, red_of(A), done);
+  }
+  RowsCsr rows() const { return RowsCsr{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()}; }
+  void inv_diag(float* dinv, int* bad, hipStream_t s) const {
+    uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((A->n + kBlock - 1) / kBlock, 8192));
+    k_inv_diag<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), A->n, dinv,
+                                    bad);
+  }
+};
+
+static BcsrView bview_of(msp_bcsr A) {
+  BcsrView v;
+  v.brow_ptr = A->brow_ptr.as<uint32_t>();
+  v.bcol = A->bcol.as<uint32_t>();
+  v.blocks = A->blocks.as<float>();
+  v.nbr = A->nbr;
+  v.n = A->n;
+  return v;
+}
+
+static uint32_t bspmv_grid(const msp_bcsr A) {
+  const uint32_t rpb = (uint32_t)(kWave / (A->B * A->B / 4)) * kWavesPerBlock;
+  return spmv_grid((A->nbr + rpb - 1) / rpb);
+}
+
+struct OpBcsr {
+  msp_bcsr A;
+  MatBase* base() const { return A; }
+  template <int ND, typename XL, typename Epi, typename Fin>
+  void launch(XL xl, Epi epi, Fin fin, const int* done, hipStream_t s) const {
+    const uint32_t g = bspmv_grid(A);
+    if (A->B == 4) k_bspmv<4, ND, XL, Epi, Fin><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, fin, red_of(A), done);
+    else if (A->B == 8) k_bspmv<8, ND, XL, Epi, Fin><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, fin, red_of(A), done);
+    else k_bspmv<16, ND, XL, Epi, Fin><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, fin, red_of(A), done);
+  }
+  template <int ND, typename Epi, typename Fin>
+  void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
+    launch<ND>(X4Plain{x}, epi, fin, done, s);
+  }
+  template <int ND, typename Epi, typename Fin>
+  void spmv_axpy(const float* r, const float* v, const float* alpha, Epi epi, Fin fin, const int* done,
+                 hipStream_t s) const {
+    launch<ND>(X4Axpy{r, v, alpha}, epi, fin, done, s);
+  }
+  RowsBcsr rows() const { return RowsBcsr{bview_of(A), A->B, A->m}; }
+  void inv_diag(float* dinv, int* bad, hipStream_t s) const {
+    uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((A->n + kBlock - 1) / kBlock, 8192));
+    k_inv_diag_bcsr<<<g, kBlock, 0, s>>>(bview_of(A), A->B, A->m, dinv, bad);
+  }
+};
+
 static msp_status check_values(msp_csr A, int32_t vt) {
   if (vt == MSP_VALUES_F32) return MSP_OK;
   if (vt != MSP_VALUES_BF16 && vt != MSP_VALUES_F16) return fail(MSP_ERR_INVALID, "unknown value type %d", vt);
@@ -555,58 +643,26 @@ static msp_status check_values(msp_csr A, int32_t vt) {
   return MSP_OK;
 }
 
-template <typename V, int ND, typename XL, typename Epi, typename Fin>
-static void launch_spmv(msp_csr A, XL xl, Epi epi, Fin fin, const int* done, hipStream_t s) {
-  GridRed rb = A->ws.scal.p ? A->ws.gr() : GridRed{nullptr, nullptr};
-  k_spmv<V, ND, XL, Epi, Fin><<<spmv_grid(A->nblk), kBlock, 0, s>>>(view_of<V>(A), xl, epi, fin, rb, done);
-}
-
-// Run `f.template operator()<V>()` for the requested value type.
+// Run f(OpCsr<V>{A}) for the requested value type.
 template <typename F>
-static msp_status with_values(msp_csr A, int32_t vt, F&& f) {
+static msp_status with_csr_op(msp_csr A, int32_t vt, F&& f) {
   MSPCHK(check_values(A, vt));
-  if (vt == MSP_VALUES_BF16) return f(ValBF16{});
-  if (vt == MSP_VALUES_F16) return f(ValF16{});
-  return f(ValF32{});
-}
-
-extern "C" msp_status msp_csr_spmv(msp_csr A, const float* x, float* y, int32_t value_type, void* stream) {
-  MSPCHK(check_handle(A));
-  if ((!x && A->m) || (!y && A->n)) return fail(MSP_ERR_INVALID, "null x / y");
-  hipStream_t s = (hipStream_t)stream;
-  const bool fast = is_device_ptr(x) && is_device_ptr(y);
-  if (fast) {
-    return with_values(A, value_type, [&](auto vt) -> msp_status {
-      using V = decltype(vt);
-      launch_spmv<V, 0>(A, XPlain{x}, EpiStore{y}, FinNone{}, nullptr, s);
-      return launch_check("k_spmv");
-    });
-  }
-  InArr xi;
-  MSPCHK(xi.set(x, sizeof(float) * A->m, s));
-  DevMem yd;
-  MSPCHK(yd.alloc(sizeof(float) * ((size_t)A->n + 1)));
-  const float* xd = static_cast<const float*>(xi.d);
-  float* ydp = is_device_ptr(y) ? y : yd.as<float>();
-  MSPCHK(with_values(A, value_type, [&](auto vt) -> msp_status {
-    using V = decltype(vt);
-    launch_spmv<V, 0>(A, XPlain{xd}, EpiStore{ydp}, FinNone{}, nullptr, s);
-    return launch_check("k_spmv");
-  }));
-  if (ydp != y) MSPCHK(copy_out(y, ydp, sizeof(float) * A->n, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return MSP_OK;
+  if (vt == MSP_VALUES_BF16) return f(OpCsr<ValBF16>{A});
+  if (vt == MSP_VALUES_F16) return f(OpCsr<ValF16>{A});
+  return f(OpCsr<ValF32>{A});
 }
 
 // ---------------------------------------------------------------------------
-// solver drivers
+// solver drivers (generic over the operator)
 static void fill_opts(msp_solve_opts& o, const msp_solve_opts* in) {
   msp_default_opts(&o);
   if (in) o = *in;
   if (o.check_every <= 0) o.check_every = 16;
 }
 
-static msp_status ensure_dinv(msp_csr A, hipStream_t s) {
+template <class Op>
+static msp_status ensure_dinv(const Op& op, hipStream_t s) {
+  MatBase* A = op.base();
   if (A->dinv_state == 1) return MSP_OK;
   if (A->dinv_state == 2) return fail(MSP_ERR_SINGULAR, "zero or non-finite diagonal entry (Jacobi)");
   MSPCHK(A->dinv.alloc(sizeof(float) * A->ws.npad));
@@ -614,10 +670,8 @@ static msp_status ensure_dinv(msp_csr A, hipStream_t s) {
   DevMem bad;
   MSPCHK(bad.alloc(sizeof(int)));
   HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-  uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((A->n + kBlock - 1) / kBlock, 8192));
-  k_inv_diag<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), A->n,
-                                  A->dinv.as<float>(), bad.as<int>());
-  MSPCHK(launch_check("k_inv_diag"));
+  op.inv_diag(A->dinv.as<float>(), bad.as<int>(), s);
+  MSPCHK(launch_check("inv_diag"));
   int hb = 0;
   HIPCHK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -626,10 +680,11 @@ static msp_status ensure_dinv(msp_csr A, hipStream_t s) {
   return MSP_OK;
 }
 
-// Upload scalar settings, run `init`, then batches of `step` with one
-// host poll of the done flag per batch (one batch of lag).
+// Upload scalar settings, run `init`, then batches of `step` with one host
+// poll of the done flag per batch (the poll trails by one batch, so the GPU
+// never idles waiting for the host).
 template <typename Init, typename Step>
-static msp_status run_solver(msp_csr A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
+static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
                              Step step) {
   Workspace& w = A->ws;
   std::memset(w.hS, 0, sizeof(Scalars));
@@ -643,9 +698,11 @@ static msp_status run_solver(msp_csr A, const msp_solve_opts& o, msp_solve_info*
   int32_t* dflag = &w.S_dev()->done;
   int enqueued = 0, slot = 0;
   bool have_prev = false;
+  // a few extra steps beyond maxiter leave room for the final verification
+  const int limit = o.maxiter + (o.verify ? 2 * o.check_every : 0);
   while (true) {
-    const int batch = std::min(o.check_every, std::max(1, o.maxiter - enqueued));
-    for (int k = 0; k < batch; ++k) MSPCHK(step());
+    const int batch = std::min(o.check_every, std::max(1, limit - enqueued));
+    for (int k = 0; k < batch; ++k) step();
     MSPCHK(launch_check("solver step"));
     enqueued += batch;
     HIPCHK(hipMemcpyAsync(&w.hflag[slot], dflag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -655,7 +712,7 @@ static msp_status run_solver(msp_csr A, const msp_solve_opts& o, msp_solve_info*
       if (w.hflag[slot ^ 1] != 0) break;
     }
     have_prev = true;
-    if (enqueued >= o.maxiter) break;
+    if (enqueued >= limit) break;
     slot ^= 1;
   }
   HIPCHK(hipEventRecord(w.t1, s));
@@ -675,7 +732,8 @@ static msp_status run_solver(msp_csr A, const msp_solve_opts& o, msp_solve_info*
   return MSP_OK;
 }
 
-static msp_status solver_prologue(msp_csr A, const float* b, const float* x, const msp_solve_opts& o, hipStream_t s) {
+static msp_status solver_prologue(MatBase* A, const float* b, const float* x, const msp_solve_opts& o,
+                                  hipStream_t s) {
   if (A->n != A->m) return fail(MSP_ERR_INVALID, "solvers need a square matrix (%u x %u)", A->n, A->m);
   if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
   if (o.tol < 0 || o.maxiter < 0) return fail(MSP_ERR_INVALID, "negative tol / maxiter");
@@ -685,52 +743,141 @@ static msp_status solver_prologue(msp_csr A, const float* b, const float* x, con
   return MSP_OK;
 }
 
+template <class Op>
+static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solve_opts& o, msp_solve_info* info,
+                          hipStream_t s) {
+  MatBase* A = op.base();
+  MSPCHK(solver_prologue(A, b, x, o, s));
+  if (o.precond != MSP_PRECOND_NONE && o.precond != MSP_PRECOND_JACOBI)
+    return fail(MSP_ERR_UNSUPPORTED, "unknown preconditioner %d", o.precond);
+  if (o.precond == MSP_PRECOND_JACOBI) MSPCHK(ensure_dinv(op, s));
+  Workspace& w = A->ws;
+  float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Q = w.v(Workspace::Q);
+  const float* B = w.v(Workspace::B);
+  const float* D = A->dinv.as<float>();
+  Scalars* S = w.S_dev();
+  GridRed rb = w.gr();
+  const uint32_t n = A->n, gv = vec_grid(n);
+  const bool jac = o.precond == MSP_PRECOND_JACOBI;
+  auto init = [&]() -> msp_status {
+    if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
+    k_cg_init<<<gv, kBlock, 0, s>>>(X, R, P, B, D, n, o.use_x0, jac ? kPcJacobi : kPcNone, S, rb);
+    return launch_check("cg init");
+  };
+  auto step = [&]() {
+    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->done, s);
+    if (jac) {
+      k_cg_update_xr<kPcJacobi><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
+      k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
+      if (o.verify) k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, B, R, P, nullptr, D, n, S, rb);
+    } else {
+      k_cg_update_xr<kPcNone><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
+      k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
+      if (o.verify) k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, B, R, P, nullptr, D, n, S, rb);
+    }
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step));
+  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+template <class Op>
+static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const msp_solve_opts& o,
+                                msp_solve_info* info, hipStream_t s) {
+  MatBase* A = op.base();
+  MSPCHK(solver_prologue(A, b, x, o, s));
+  if (o.precond != MSP_PRECOND_NONE) return fail(MSP_ERR_UNSUPPORTED, "BiCGSTAB takes no preconditioner");
+  Workspace& w = A->ws;
+  float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Vv = w.v(Workspace::Q);
+  float *R0 = w.v(Workspace::R0), *Sv = w.v(Workspace::S), *T = w.v(Workspace::T);
+  const float* B = w.v(Workspace::B);
+  Scalars* S = w.S_dev();
+  GridRed rb = w.gr();
+  const uint32_t n = A->n, gv = vec_grid(n);
+  auto init = [&]() -> msp_status {
+    if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
+    k_bicg_init<<<gv, kBlock, 0, s>>>(X, R, R0, P, B, n, o.use_x0, S, rb);
+    return launch_check("bicgstab init");
+  };
+  auto step = [&]() {
+    op.template spmv<1>(P, EpiStoreDot{Vv, R0}, FinBicgAlpha{S}, &S->done, s);
+    op.template spmv_axpy<2>(R, Vv, &S->alpha, EpiBicgT{T, Sv, R, Vv, &S->alpha}, FinBicgOmega{S}, &S->done, s);
+    k_bicg_update<<<gv, kBlock, 0, s>>>(X, R, P, Sv, T, R0, n, S, rb);
+    k_bicg_p<<<gv, kBlock, 0, s>>>(P, R, Vv, n, S);
+    if (o.verify) k_verify<2><<<gv, kBlock, 0, s>>>(op.rows(), X, B, R, P, R0, nullptr, n, S, rb);
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step));
+  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+template <class Op>
+static msp_status richardson_impl(const Op& op, const float* b, float* x, float omega, int32_t iters,
+                                  int32_t use_x0, hipStream_t s) {
+  MatBase* A = op.base();
+  if (iters < 0) return fail(MSP_ERR_INVALID, "negative iteration count");
+  msp_solve_opts o;
+  fill_opts(o, nullptr);
+  o.use_x0 = use_x0;
+  MSPCHK(solver_prologue(A, b, x, o, s));
+  Workspace& w = A->ws;
+  float* xa = w.v(Workspace::X);
+  float* xb = w.v(Workspace::X2);
+  const float* B = w.v(Workspace::B);
+  if (!use_x0) HIPCHK(hipMemsetAsync(xa, 0, sizeof(float) * w.npad, s));
+  for (int32_t it = 0; it < iters; ++it) {
+    op.template spmv<0>(xa, EpiRichardson{xb, xa, B, omega}, FinNone{}, nullptr, s);
+    std::swap(xa, xb);
+  }
+  MSPCHK(launch_check("richardson"));
+  MSPCHK(copy_out(x, xa, sizeof(float) * A->n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+// y = A x for caller arrays: used in place when both are device pointers with
+// the alignment / padding the kernel needs, else staged through the workspace.
+template <class Op>
+static msp_status spmv_impl(const Op& op, const float* x, float* y, bool need_padded_x, hipStream_t s) {
+  MatBase* A = op.base();
+  if ((!x && A->m) || (!y && A->n)) return fail(MSP_ERR_INVALID, "null x / y");
+  const bool xin = is_device_ptr(x) && !need_padded_x, yin = is_device_ptr(y);
+  if (xin && yin) {
+    op.template spmv<0>(x, EpiStore{y}, FinNone{}, nullptr, s);
+    return launch_check("spmv");
+  }
+  MSPCHK(ensure_workspace(A, s));
+  float* xw = A->ws.v(Workspace::X2);
+  float* yw = A->ws.v(Workspace::W);
+  const float* xd = x;
+  if (!xin) {
+    MSPCHK(copy_in(xw, x, sizeof(float) * A->m, s));
+    xd = xw;
+  }
+  op.template spmv<0>(xd, EpiStore{yin ? y : yw}, FinNone{}, nullptr, s);
+  MSPCHK(launch_check("spmv"));
+  if (!yin) MSPCHK(copy_out(y, yw, sizeof(float) * A->n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ABI: CSR compute
+extern "C" msp_status msp_csr_spmv(msp_csr A, const float* x, float* y, int32_t value_type, void* stream) {
+  MSPCHK(check_handle(A));
+  hipStream_t s = (hipStream_t)stream;
+  return with_csr_op(A, value_type, [&](auto op) { return spmv_impl(op, x, y, false, s); });
+}
+
 extern "C" msp_status msp_csr_cg(msp_csr A, const float* b, float* x, const msp_solve_opts* opts,
                                  msp_solve_info* info, void* stream) {
   MSPCHK(check_handle(A));
   msp_solve_opts o;
   fill_opts(o, opts);
   hipStream_t s = (hipStream_t)stream;
-  MSPCHK(solver_prologue(A, b, x, o, s));
-  if (o.precond != MSP_PRECOND_NONE && o.precond != MSP_PRECOND_JACOBI)
-    return fail(MSP_ERR_UNSUPPORTED, "unknown preconditioner %d", o.precond);
-  if (o.precond == MSP_PRECOND_JACOBI) MSPCHK(ensure_dinv(A, s));
-  Workspace& w = A->ws;
-  float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Q = w.v(Workspace::Q);
-  const float* B = w.v(Workspace::B);
-  const float* D = A->dinv.as<float>();
-  const uint32_t* RP = A->row_ptr.as<uint32_t>();
-  const uint32_t* CI = A->col.as<uint32_t>();
-  const float* AV = A->val.as<float>();
-  Scalars* S = w.S_dev();
-  GridRed rb = w.gr();
-  const uint32_t n = A->n, gv = vec_grid(n);
-  const int pc = o.precond == MSP_PRECOND_JACOBI ? kPcJacobi : kPcNone;
-  MSPCHK(with_values(A, o.values, [&](auto vt) -> msp_status {
-    using V = decltype(vt);
-    auto init = [&]() -> msp_status {
-      if (o.use_x0) launch_spmv<V, 1>(A, XPlain{X}, EpiResidual{R, B}, FinNone{}, nullptr, s);
-      k_cg_init<<<gv, kBlock, 0, s>>>(X, R, P, B, D, n, o.use_x0, pc, S, rb);
-      return launch_check("cg init");
-    };
-    auto step = [&]() -> msp_status {
-      launch_spmv<V, 1>(A, XPlain{P}, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->done, s);
-      if (pc == kPcJacobi) {
-        k_cg_update_xr<kPcJacobi><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
-        k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
-        if (o.verify) k_verify<1><<<gv, kBlock, 0, s>>>(RP, CI, AV, X, B, R, D, n, S, rb);
-      } else {
-        k_cg_update_xr<kPcNone><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
-        k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
-        if (o.verify) k_verify<0><<<gv, kBlock, 0, s>>>(RP, CI, AV, X, B, R, D, n, S, rb);
-      }
-      return MSP_OK;
-    };
-    return run_solver(A, o, info, s, init, step);
-  }));
-  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return MSP_OK;
+  return with_csr_op(A, o.values, [&](auto op) { return cg_impl(op, b, x, o, info, s); });
 }
 
 extern "C" msp_status msp_csr_bicgstab(msp_csr A, const float* b, float* x, const msp_solve_opts* opts,
@@ -739,64 +886,328 @@ extern "C" msp_status msp_csr_bicgstab(msp_csr A, const float* b, float* x, cons
   msp_solve_opts o;
   fill_opts(o, opts);
   hipStream_t s = (hipStream_t)stream;
-  MSPCHK(solver_prologue(A, b, x, o, s));
-  if (o.precond != MSP_PRECOND_NONE) return fail(MSP_ERR_UNSUPPORTED, "BiCGSTAB takes no preconditioner");
-  Workspace& w = A->ws;
-  float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Vv = w.v(Workspace::Q);
-  float *R0 = w.v(Workspace::R0), *Sv = w.v(Workspace::S), *T = w.v(Workspace::T);
-  const float* B = w.v(Workspace::B);
-  const uint32_t* RP = A->row_ptr.as<uint32_t>();
-  const uint32_t* CI = A->col.as<uint32_t>();
-  const float* AV = A->val.as<float>();
-  Scalars* S = w.S_dev();
-  GridRed rb = w.gr();
-  const uint32_t n = A->n, gv = vec_grid(n);
-  MSPCHK(with_values(A, o.values, [&](auto vt) -> msp_status {
-    using V = decltype(vt);
-    auto init = [&]() -> msp_status {
-      if (o.use_x0) launch_spmv<V, 1>(A, XPlain{X}, EpiResidual{R, B}, FinNone{}, nullptr, s);
-      k_bicg_init<<<gv, kBlock, 0, s>>>(X, R, R0, P, B, n, o.use_x0, S, rb);
-      return launch_check("bicgstab init");
-    };
-    auto step = [&]() -> msp_status {
-      launch_spmv<V, 1>(A, XPlain{P}, EpiStoreDot{Vv, R0}, FinBicgAlpha{S}, &S->done, s);
-      launch_spmv<V, 2>(A, XAxpy{R, Vv, &S->alpha}, EpiBicgT{T, Sv, R, Vv, &S->alpha}, FinBicgOmega{S}, &S->done, s);
-      k_bicg_update<<<gv, kBlock, 0, s>>>(X, R, P, Sv, T, R0, n, S, rb);
-      k_bicg_p<<<gv, kBlock, 0, s>>>(P, R, Vv, n, S);
-      if (o.verify) k_verify<2><<<gv, kBlock, 0, s>>>(RP, CI, AV, X, B, R, R0, n, S, rb);
-      return MSP_OK;
-    };
-    return run_solver(A, o, info, s, init, step);
-  }));
-  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return MSP_OK;
+  return with_csr_op(A, o.values, [&](auto op) { return bicgstab_impl(op, b, x, o, info, s); });
 }
 
 extern "C" msp_status msp_csr_richardson(msp_csr A, const float* b, float* x, float omega, int32_t iters,
                                          int32_t use_x0, int32_t value_type, void* stream) {
   MSPCHK(check_handle(A));
-  if (iters < 0) return fail(MSP_ERR_INVALID, "negative iteration count");
-  msp_solve_opts o;
-  fill_opts(o, nullptr);
-  o.use_x0 = use_x0;
   hipStream_t s = (hipStream_t)stream;
-  MSPCHK(solver_prologue(A, b, x, o, s));
-  Workspace& w = A->ws;
-  float* xa = w.v(Workspace::X);
-  float* xb = w.v(Workspace::X2);
-  const float* B = w.v(Workspace::B);
-  const uint32_t n = A->n;
-  if (!use_x0) HIPCHK(hipMemsetAsync(xa, 0, sizeof(float) * w.npad, s));
-  MSPCHK(with_values(A, value_type, [&](auto vt) -> msp_status {
-    using V = decltype(vt);
-    for (int32_t it = 0; it < iters; ++it) {
-      launch_spmv<V, 0>(A, XPlain{xa}, EpiRichardson{xb, xa, B, omega}, FinNone{}, nullptr, s);
-      std::swap(xa, xb);
-    }
-    return launch_check("richardson");
-  }));
-  MSPCHK(copy_out(x, xa, sizeof(float) * n, s));
+  return with_csr_op(A, value_type,
+                     [&](auto op) { return richardson_impl(op, b, x, omega, iters, use_x0, s); });
+}
+
+// ---------------------------------------------------------------------------
+// ABI: BCSR
+static bool block_ok(uint32_t B) { return B == 4 || B == 8 || B == 16; }
+
+static msp_status bcsr_alloc(msp_bcsr A, uint32_t num_rows, uint32_t num_cols, uint32_t B, uint64_t nblk) {
+  A->n = num_rows;
+  A->m = num_cols;
+  A->B = B;
+  A->nbr = (num_rows + B - 1) / B;
+  A->nbc = (num_cols + B - 1) / B;
+  A->nblk = nblk;
+  MSPCHK(A->brow_ptr.alloc(sizeof(uint32_t) * ((size_t)A->nbr + 1)));
+  MSPCHK(A->bcol.alloc(sizeof(uint32_t) * (nblk + 1)));
+  MSPCHK(A->blocks.alloc(sizeof(float) * (nblk * B * B + 16)));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_bcsr_create(msp_bcsr* out, const uint32_t* brow_ptr, const uint32_t* bcol,
+                                      const float* blocks, uint32_t num_rows, uint32_t num_cols, uint32_t block,
+                                      void* stream) {
+  if (!out) return fail(MSP_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  MSPCHK(need_device());
+  if (!block_ok(block)) return fail(MSP_ERR_UNSUPPORTED, "block size %u (4, 8 or 16)", block);
+  if (!brow_ptr) return fail(MSP_ERR_INVALID, "null brow_ptr");
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t nbr = (num_rows + block - 1) / block;
+  uint32_t nblk = 0;
+  if (is_device_ptr(brow_ptr)) {
+    HIPCHK(hipMemcpyAsync(&nblk, brow_ptr + nbr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  } else {
+    nblk = brow_ptr[nbr];
+  }
+  if (nblk && (!bcol || !blocks)) return fail(MSP_ERR_INVALID, "null bcol / blocks");
+  msp_bcsr A = new msp_bcsr_s;
+  HIPCHK(hipGetDevice(&A->device));
+  msp_status st = bcsr_alloc(A, num_rows, num_cols, block, nblk);
+  if (st == MSP_OK) st = copy_in(A->brow_ptr.p, brow_ptr, sizeof(uint32_t) * ((size_t)nbr + 1), s);
+  if (st == MSP_OK) st = copy_in(A->bcol.p, bcol, sizeof(uint32_t) * nblk, s);
+  if (st == MSP_OK) st = copy_in(A->blocks.p, blocks, sizeof(float) * (size_t)nblk * block * block, s);
+  if (st == MSP_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(MSP_ERR_HIP, "stream sync failed");
+  if (st != MSP_OK) {
+    delete A;
+    return st;
+  }
+  *out = A;
+  return MSP_OK;
+}
+
+// CSR (device arrays) -> BCSR on the GPU.
+static msp_status csr_to_bcsr_device(msp_bcsr A, const uint32_t* row_ptr, const uint32_t* col, const float* val,
+                                     uint64_t nnz, hipStream_t s) {
+  const uint32_t B = A->B, n = A->n, nbr = A->nbr, nbc = A->nbc;
+  const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n + kBlock - 1) / kBlock, 65536));
+  const uint32_t gz = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nnz + kBlock - 1) / kBlock, 65536));
+  DevMem bkeys, payload, brp_nnz, sbcol, sbrow, spayload, rows, flag, excl, sums;
+  MSPCHK(bkeys.alloc(sizeof(uint64_t) * nnz));
+  MSPCHK(payload.alloc(sizeof(float) * nnz));
+  MSPCHK(brp_nnz.alloc(sizeof(uint32_t) * ((size_t)nbr + 1)));
+  MSPCHK(sbcol.alloc(sizeof(uint32_t) * nnz));
+  MSPCHK(sbrow.alloc(sizeof(uint32_t) * nnz));
+  MSPCHK(spayload.alloc(sizeof(float) * nnz));
+  MSPCHK(rows.alloc(sizeof(uint32_t) * nnz));
+  MSPCHK(flag.alloc(sizeof(uint32_t) * nnz));
+  MSPCHK(excl.alloc(sizeof(uint32_t) * nnz));
+  k_block_keys<<<g, kBlock, 0, s>>>(row_ptr, col, n, B, bkeys.as<uint64_t>(), payload.as<float>());
+  k_csr_rows<<<g, kBlock, 0, s>>>(row_ptr, n, rows.as<uint32_t>());
+  MSPCHK(launch_check("block keys"));
+  MSPCHK(coo_to_csr_device(bkeys.as<uint64_t>(), payload.as<float>(), nnz, nbr, nbc, brp_nnz.as<uint32_t>(),
+                           sbcol.as<uint32_t>(), spayload.as<float>(), s, sbrow.as<uint32_t>()));
+  k_block_heads<<<gz, kBlock, 0, s>>>(sbrow.as<uint32_t>(), sbcol.as<uint32_t>(), nnz, flag.as<uint32_t>());
+  // exclusive scan of the head flags
+  const uint32_t nscan = (uint32_t)((nnz + sortk::kScanTile - 1) / sortk::kScanTile);
+  MSPCHK(sums.alloc(sizeof(uint32_t) * nscan));
+  HIPCHK(hipMemcpyAsync(excl.p, flag.p, sizeof(uint32_t) * nnz, hipMemcpyDeviceToDevice, s));
+  sortk::k_scan_reduce<<<nscan, kBlock, 0, s>>>(excl.as<uint32_t>(), nnz, sums.as<uint32_t>());
+  sortk::k_scan_top<<<1, kBlock, 0, s>>>(sums.as<uint32_t>(), nscan);
+  sortk::k_scan_down<<<nscan, kBlock, 0, s>>>(excl.as<uint32_t>(), nnz, sums.as<uint32_t>());
+  MSPCHK(launch_check("block scan"));
+  uint32_t last[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(&last[0], excl.as<uint32_t>() + nnz - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&last[1], flag.as<uint32_t>() + nnz - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  const uint64_t nblk = (uint64_t)last[0] + last[1];
+  A->nblk = nblk;
+  MSPCHK(A->bcol.alloc(sizeof(uint32_t) * (nblk + 1)));
+  MSPCHK(A->blocks.alloc(sizeof(float) * (nblk * B * B + 16)));
+  HIPCHK(hipMemsetAsync(A->blocks.p, 0, sizeof(float) * (nblk * B * B + 16), s));
+  k_block_fill<<<gz, kBlock, 0, s>>>(sbcol.as<uint32_t>(), flag.as<uint32_t>(), excl.as<uint32_t>(),
+                                     spayload.as<float>(), rows.as<uint32_t>(), col, val, nnz, B,
+                                     A->bcol.as<uint32_t>(), A->blocks.as<float>());
+  const uint32_t gb = (nbr + 1 + kBlock - 1) / kBlock;
+  k_brow_ptr<<<gb, kBlock, 0, s>>>(brp_nnz.as<uint32_t>(), excl.as<uint32_t>(), flag.as<uint32_t>(), nbr, nnz,
+                                   A->brow_ptr.as<uint32_t>());
+  MSPCHK(launch_check("block fill"));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_bcsr_create_coo(msp_bcsr* out, const uint64_t* keys, const float* vals, uint64_t nnz,
+                                          uint32_t num_rows, uint32_t num_cols, uint32_t block, void* stream) {
+  if (!out) return fail(MSP_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  MSPCHK(need_device());
+  if (!block_ok(block)) return fail(MSP_ERR_UNSUPPORTED, "block size %u (4, 8 or 16)", block);
+  if (nnz && (!keys || !vals)) return fail(MSP_ERR_INVALID, "null keys / values");
+  if (nnz > 0xFFFFFFFFull - 64) return fail(MSP_ERR_INVALID, "nnz exceeds the u32 index range");
+  hipStream_t s = (hipStream_t)stream;
+  InArr k, v;
+  MSPCHK(k.set(keys, sizeof(uint64_t) * nnz, s));
+  MSPCHK(v.set(vals, sizeof(float) * nnz, s));
+  DevMem rp, ci, cv;
+  MSPCHK(rp.alloc(sizeof(uint32_t) * ((size_t)num_rows + 1)));
+  MSPCHK(ci.alloc(sizeof(uint32_t) * (nnz + 1)));
+  MSPCHK(cv.alloc(sizeof(float) * (nnz + 1)));
+  MSPCHK(coo_to_csr_device(static_cast<const uint64_t*>(k.d), static_cast<const float*>(v.d), nnz, num_rows, num_cols,
+                           rp.as<uint32_t>(), ci.as<uint32_t>(), cv.as<float>(), s));
+  msp_bcsr A = new msp_bcsr_s;
+  HIPCHK(hipGetDevice(&A->device));
+  msp_status st = bcsr_alloc(A, num_rows, num_cols, block, 0);
+  if (st == MSP_OK && nnz > 0)
+    st = csr_to_bcsr_device(A, rp.as<uint32_t>(), ci.as<uint32_t>(), cv.as<float>(), nnz, s);
+  if (st == MSP_OK && nnz == 0) {
+    if (hipMemsetAsync(A->brow_ptr.p, 0, sizeof(uint32_t) * ((size_t)A->nbr + 1), s) != hipSuccess)
+      st = fail(MSP_ERR_HIP, "memset failed");
+  }
+  if (st == MSP_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(MSP_ERR_HIP, "stream sync failed");
+  if (st != MSP_OK) {
+    delete A;
+    return st;
+  }
+  *out = A;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_bcsr_shape(msp_bcsr A, uint32_t* num_rows, uint32_t* num_cols, uint32_t* block,
+                                     uint64_t* num_blocks) {
+  if (!A) return fail(MSP_ERR_INVALID, "null matrix handle");
+  if (num_rows) *num_rows = A->n;
+  if (num_cols) *num_cols = A->m;
+  if (block) *block = A->B;
+  if (num_blocks) *num_blocks = A->nblk;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_bcsr_export(msp_bcsr A, uint32_t* brow_ptr, uint32_t* bcol, float* blocks, void* stream) {
+  MSPCHK(check_handle(A));
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(copy_out(brow_ptr, A->brow_ptr.p, sizeof(uint32_t) * ((size_t)A->nbr + 1), s));
+  MSPCHK(copy_out(bcol, A->bcol.p, sizeof(uint32_t) * A->nblk, s));
+  MSPCHK(copy_out(blocks, A->blocks.p, sizeof(float) * A->nblk * A->B * A->B, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_bcsr_destroy(msp_bcsr A) {
+  if (!A) return MSP_OK;
+  (void)hipSetDevice(A->device);
+  (void)hipDeviceSynchronize();
+  delete A;
+  return MSP_OK;
+}
+
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+extern "C" msp_status msp_bcsr_spmv(msp_bcsr A, const float* x, float* y, void* stream) {
+  MSPCHK(check_handle(A));
+  hipStream_t s = (hipStream_t)stream;
+  // the kernel reads x in 16-byte quads up to nbc * B: stage unless that is safe
+  const bool padded_ok = aligned16(x) && (A->m % A->B == 0);
+  return spmv_impl(OpBcsr{A}, x, y, !padded_ok, s);
+}
+
+extern "C" msp_status msp_bcsr_cg(msp_bcsr A, const float* b, float* x, const msp_solve_opts* opts,
+                                  msp_solve_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  msp_solve_opts o;
+  fill_opts(o, opts);
+  if (o.values != MSP_VALUES_F32) return fail(MSP_ERR_UNSUPPORTED, "BCSR stores FP32 values");
+  return cg_impl(OpBcsr{A}, b, x, o, info, (hipStream_t)stream);
+}
+
+extern "C" msp_status msp_bcsr_bicgstab(msp_bcsr A, const float* b, float* x, const msp_solve_opts* opts,
+                                        msp_solve_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  msp_solve_opts o;
+  fill_opts(o, opts);
+  if (o.values != MSP_VALUES_F32) return fail(MSP_ERR_UNSUPPORTED, "BCSR stores FP32 values");
+  return bicgstab_impl(OpBcsr{A}, b, x, o, info, (hipStream_t)stream);
+}
+
+extern "C" msp_status msp_bcsr_richardson(msp_bcsr A, const float* b, float* x, float omega, int32_t iters,
+                                          int32_t use_x0, void* stream) {
+  MSPCHK(check_handle(A));
+  return richardson_impl(OpBcsr{A}, b, x, omega, iters, use_x0, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// ABI: mixed-precision iterative refinement
+extern "C" void msp_default_refine_opts(msp_refine_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->tol = 1e-10;
+  o->inner_tol = 1e-3;
+  o->max_outer = 30;
+  o->inner_maxiter = 10000;
+  o->inner_solver = 0;
+  o->values = MSP_VALUES_F32;
+  o->precond = MSP_PRECOND_NONE;
+  o->use_x0 = 0;
+}
+
+static bool is_device_ptr_d(const double* p) { return is_device_ptr(static_cast<const void*>(p)); }
+
+extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* x, const msp_refine_opts* opts,
+                                            msp_refine_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  msp_refine_opts ro;
+  msp_default_refine_opts(&ro);
+  if (opts) ro = *opts;
+  if (A->n != A->m) return fail(MSP_ERR_INVALID, "refinement needs a square matrix");
+  if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
+  if (ro.inner_solver != 0 && ro.inner_solver != 1) return fail(MSP_ERR_INVALID, "inner_solver must be 0 or 1");
+  hipStream_t s = (hipStream_t)stream;
+  auto t_start = std::chrono::steady_clock::now();
+  MSPCHK(ensure_workspace(A, s));
+  if (ro.values != MSP_VALUES_F32) MSPCHK(msp_csr_prepare_values(A, ro.values, stream));
+  const size_t np = A->ws.npad;
+  if (!A->w64.p) MSPCHK(A->w64.alloc(sizeof(double) * (3 * np + 2)));
+  double* X64 = A->w64.as<double>();
+  double* R64 = X64 + np;
+  double* B64 = X64 + 2 * np;
+  double* SUMS = X64 + 3 * np;
+  const uint32_t n = A->n, gv = vec_grid(n);
+  HIPCHK(hipMemcpyAsync(B64, b, sizeof(double) * n, is_device_ptr_d(b) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        s));
+  if (ro.use_x0)
+    HIPCHK(hipMemcpyAsync(X64, x, sizeof(double) * n,
+                          is_device_ptr_d(x) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  else
+    k_fill64<<<gv, kBlock, 0, s>>>(X64, 0.0, n);
+  float* RHS = A->ws.v(Workspace::W);
+  float* D = A->ws.v(Workspace::Z);
+  GridRed rb = A->ws.gr();
+  const RowsCsr rows{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()};
+  msp_refine_info inf{};
+  inf.status = MSP_SOLVE_MAXITER;
+  double prev = 0.0;
+  int slow = 0;
+  for (int k = 0;; ++k) {
+    k_residual64<<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, reinterpret_cast<double*>(rb.partials), rb.ticket,
+                                       SUMS);
+    MSPCHK(launch_check("k_residual64"));
+    double sums[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(sums, SUMS, sizeof sums, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double rnorm = std::sqrt(sums[0]), bnorm = std::sqrt(sums[1]);
+    inf.relres = bnorm > 0 ? rnorm / bnorm : rnorm;
+    inf.outer_iterations = k;
+    if (!std::isfinite(inf.relres)) {
+      inf.status = MSP_SOLVE_BREAKDOWN;
+      break;
+    }
+    if (inf.relres <= ro.tol || rnorm == 0.0) {
+      inf.status = MSP_SOLVE_CONVERGED;
+      break;
+    }
+    if (k > 0 && inf.relres > 0.5 * prev) {
+      if (++slow >= 2) {
+        inf.status = MSP_SOLVE_STAGNATED;
+        break;
+      }
+    } else {
+      slow = 0;
+    }
+    if (k >= ro.max_outer) break;
+    prev = inf.relres;
+    k_scale_to_f32<<<gv, kBlock, 0, s>>>(R64, RHS, 1.0 / rnorm, n);
+    msp_solve_opts io;
+    msp_default_opts(&io);
+    io.tol = ro.inner_tol;
+    io.maxiter = ro.inner_maxiter;
+    io.precond = ro.precond;
+    io.verify = 0;
+    io.values = ro.values;
+    msp_solve_info ii{};
+    auto inner = [&](int32_t vt) -> msp_status {
+      io.values = vt;
+      return with_csr_op(A, vt, [&](auto op) {
+        return ro.inner_solver == 0 ? cg_impl(op, RHS, D, io, &ii, s) : bicgstab_impl(op, RHS, D, io, &ii, s);
+      });
+    };
+    MSPCHK(inner(ro.values));
+    inf.inner_iterations += ii.iterations;
+    if (ii.status == MSP_SOLVE_BREAKDOWN && ro.values != MSP_VALUES_F32) {
+      inf.fallbacks += 1;
+      MSPCHK(inner(MSP_VALUES_F32));
+      inf.inner_iterations += ii.iterations;
+    }
+    if (ii.status == MSP_SOLVE_BREAKDOWN && ii.iterations == 0) {
+      inf.status = MSP_SOLVE_BREAKDOWN;
+      break;
+    }
+    k_axpy64<<<gv, kBlock, 0, s>>>(X64, D, rnorm, n);
+    MSPCHK(launch_check("k_axpy64"));
+  }
+  HIPCHK(hipMemcpyAsync(x, X64, sizeof(double) * n, is_device_ptr_d(x) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));
+  inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  if (info) *info = inf;
   return MSP_OK;
 }

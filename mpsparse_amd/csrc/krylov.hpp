@@ -36,7 +36,7 @@ struct Scalars {
   float thresh;  // converged when rr <= thresh
   float tol;
   float rr0;     // ||r||^2 at start
-  float rr_ver;  // ||b - A x||^2 at the last verification
+  float rr_best;  // smallest ||b - A x||^2 seen by a verification
   int32_t done;
   int32_t iter;
   int32_t maxiter;
@@ -44,7 +44,7 @@ struct Scalars {
   int32_t use_verify;  // confirm recurrence convergence with a true residual
   int32_t verify;      // a verification is pending
   int32_t nverify;
-  int32_t pad;
+  int32_t nfail;  // consecutive verifications without a 10% gain in ||r||
 };
 
 // Recurrence says converged: either stop, or ask for a true-residual check.
@@ -334,54 +334,71 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(float* __restrict__ p, const 
   }
 }
 
+// Row access for row-per-thread kernels (true residual, diagonal).
+struct RowsCsr {
+  const uint32_t* __restrict__ row_ptr;
+  const uint32_t* __restrict__ col;
+  const float* __restrict__ val;
+  template <typename F>
+  __device__ __forceinline__ void foreach(uint32_t i, F&& f) const {
+    for (uint32_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j) f(val[j], col[j]);
+  }
+};
+
 // True-residual check after the recurrence reports convergence:
 // r = b - A x with error-free products (FMA) and TwoSum row accumulation, so
-// the replaced residual is exact to about twice FP32 precision for the FP32 x.
-// Confirms convergence, or replaces the drifted recurrence residual and lets
-// the iteration continue (stops as stagnated when a replacement no longer
-// halves ||r||^2). MODE 0: CG (second dot <r, r>), 1: CG+Jacobi (<r, dinv r>),
-// 2: BiCGSTAB (<r0, r>).
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_verify(const uint32_t* __restrict__ row_ptr,
-                                                   const uint32_t* __restrict__ col, const float* __restrict__ val,
-                                                   const float* __restrict__ x, const float* __restrict__ b,
-                                                   float* __restrict__ r, const float* __restrict__ aux, uint32_t n,
-                                                   Scalars* S, GridRed rb) {
+// the residual is exact to about twice FP32 precision for the FP32 x.
+// Confirms convergence, or restarts the recurrence from the true residual
+// (p = z; BiCGSTAB also r0 = r) -- keeping the old direction after replacing a
+// residual that drifted by more than its own size breaks conjugacy. Stops as
+// stagnated after two verifications in a row that do not improve the best true
+// residual norm by 10% (the FP32 iterate has reached its attainable accuracy),
+// or after 12 verifications.
+// MODE 0: CG, 1: CG + Jacobi (z = dinv r), 2: BiCGSTAB.
+template <int MODE, typename Rows>
+__global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __restrict__ x,
+                                                   const float* __restrict__ b, float* __restrict__ r,
+                                                   float* __restrict__ p, float* __restrict__ r0,
+                                                   const float* __restrict__ dinv, uint32_t n, Scalars* S,
+                                                   GridRed rb) {
   if (S->done || !S->verify) return;
   CSum acc[2] = {csum_zero(), csum_zero()};
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     CSum s{b[i], 0.0f};
-    for (uint32_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j) {
-      const float a = val[j], xv = x[col[j]];
-      const float p = a * xv;
-      const float e = __builtin_fmaf(a, xv, -p);
-      s = csum_add(s, -p);
+    rows.foreach(i, [&](float a, uint32_t c) {
+      const float xv = x[c];
+      const float pr = a * xv;
+      const float e = __builtin_fmaf(a, xv, -pr);
+      s = csum_add(s, -pr);
       s.c -= e;
-    }
+    });
     const float ri = csum_val(s);
+    const float zi = (MODE == 1) ? ri * dinv[i] : ri;
     r[i] = ri;
+    p[i] = zi;
+    if (MODE == 2) r0[i] = ri;
     acc[0] = csum_add(acc[0], ri * ri);
-    if (MODE == 0) acc[1] = csum_add(acc[1], ri * ri);
-    else if (MODE == 1) acc[1] = csum_add(acc[1], ri * ri * aux[i]);
-    else acc[1] = csum_add(acc[1], aux[i] * ri);
+    acc[1] = csum_add(acc[1], ri * zi);
   }
   float tot[2];
   if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
-    const float prev = S->rr_ver;
+    const float best = S->nverify > 0 ? S->rr_best : tot[0] / 0.81f;
     S->verify = 0;
     S->nverify += 1;
     S->rr = tot[0];
-    S->rr_ver = tot[0];
+    S->nfail = (tot[0] > 0.81f * best) ? S->nfail + 1 : 0;
+    S->rr_best = fminf(best, tot[0]);
+    S->beta = 0.0f;
     if (!isfinite(tot[0])) {
       S->done = kBreakdown;
     } else if (tot[0] <= S->thresh) {
       S->done = kConverged;
-    } else if (S->nverify > 1 && tot[0] > 0.5f * prev) {
+    } else if (S->nfail >= 2 || S->nverify >= 12) {
       S->done = kStagnated;
     } else if (S->iter >= S->maxiter) {
       S->done = kMaxIter;
     } else if (MODE == 2) {
-      S->rho = tot[1];
+      S->rho = tot[0];
     } else {
       S->rz = tot[1];
     }

@@ -1,0 +1,105 @@
+// Mixed-precision iterative refinement: FP64 only for the outer residual
+// r = b - A x and the accumulation x += d; the correction d comes from an FP32
+// Krylov solve whose matrix values may be stored in BF16 / FP16.
+#pragma once
+
+#include "common.hpp"
+#include "krylov.hpp"
+
+namespace msp {
+
+// Deterministic grid reduction of ND doubles (same protocol as grid_reduce).
+template <int ND>
+__device__ bool grid_reduce_d(double (&acc)[ND], double* partials, uint32_t* ticket, double (&tot)[ND]) {
+  __shared__ double red[kWavesPerBlock][ND];
+  __shared__ int s_last;
+  __shared__ double s_tot[ND];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double v = acc[d];
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (lane == 0) red[wid][d] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      double t = 0.0;
+      for (int w = 0; w < kWavesPerBlock; ++w) t += red[w][d];
+      partials[blockIdx.x * ND + d] = t;
+    }
+    __threadfence();
+    s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double a[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) a[d] = 0.0;
+  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) a[d] += *((volatile double*)(partials + i * ND + d));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double v = a[d];
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (lane == 0) red[wid][d] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      double t = 0.0;
+      for (int w = 0; w < kWavesPerBlock; ++w) t += red[w][d];
+      s_tot[d] = t;
+    }
+    *ticket = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < ND; ++d) tot[d] = s_tot[d];
+  return true;
+}
+
+// r = b - A x in FP64 (matrix values FP32, exact in FP64); out = {<r,r>, <b,b>}.
+template <typename Rows>
+__global__ __launch_bounds__(kBlock) void k_residual64(Rows rows, const double* __restrict__ x,
+                                                       const double* __restrict__ b, double* __restrict__ r,
+                                                       uint32_t n, double* partials, uint32_t* ticket,
+                                                       double* __restrict__ out) {
+  double acc[2] = {0.0, 0.0};
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double s = b[i];
+    rows.foreach(i, [&](float a, uint32_t c) { s -= (double)a * x[c]; });
+    r[i] = s;
+    acc[0] += s * s;
+    acc[1] += b[i] * b[i];
+  }
+  double tot[2];
+  if (grid_reduce_d<2>(acc, partials, ticket, tot) && threadIdx.x == 0) {
+    out[0] = tot[0];
+    out[1] = tot[1];
+  }
+}
+
+// out = float(r * scale)
+__global__ __launch_bounds__(kBlock) void k_scale_to_f32(const double* __restrict__ r, float* __restrict__ out,
+                                                         double scale, uint32_t n) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) out[i] = (float)(r[i] * scale);
+}
+
+// x += scale * d (FP64 accumulation of the FP32 correction)
+__global__ __launch_bounds__(kBlock) void k_axpy64(double* __restrict__ x, const float* __restrict__ d, double scale,
+                                                   uint32_t n) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] += scale * (double)d[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill64(double* __restrict__ x, double v, uint32_t n) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = v;
+}
+
+}  // namespace msp

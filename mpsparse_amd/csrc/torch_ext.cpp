@@ -193,6 +193,45 @@ class CsrTensor {
           "richardson");
   }
 
+  py::dict solve_refined(torch::Tensor b, torch::Tensor x, double tol, double inner_tol, int64_t max_outer,
+                         int64_t inner_maxiter, const std::string& solver, const std::string& values,
+                         const std::string& precond, bool use_x0) {
+    uint32_t n = 0, m = 0;
+    check(msp_csr_shape(h_, &n, &m, nullptr), "shape");
+    expect(b, torch::kFloat64, n, "b");
+    expect(x, torch::kFloat64, m, "x");
+    msp_refine_opts o;
+    msp_default_refine_opts(&o);
+    o.tol = tol;
+    o.inner_tol = inner_tol;
+    o.max_outer = (int32_t)max_outer;
+    o.inner_maxiter = (int32_t)inner_maxiter;
+    TORCH_CHECK(solver == "cg" || solver == "bicgstab", "solver must be 'cg' or 'bicgstab'");
+    o.inner_solver = solver == "cg" ? 0 : 1;
+    o.values = value_type_of(values);
+    TORCH_CHECK(precond == "none" || precond == "jacobi", "precond must be 'none' or 'jacobi'");
+    o.precond = precond == "jacobi" ? MSP_PRECOND_JACOBI : MSP_PRECOND_NONE;
+    o.use_x0 = use_x0 ? 1 : 0;
+    msp_refine_info info{};
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    {
+      py::gil_scoped_release nogil;
+      check(msp_csr_solve_refined(h_, b.data_ptr<double>(), x.data_ptr<double>(), &o, &info, s), "solve_refined");
+    }
+    py::dict d;
+    d["outer_iterations"] = info.outer_iterations;
+    d["inner_iterations"] = info.inner_iterations;
+    d["status"] = info.status == MSP_SOLVE_CONVERGED   ? "converged"
+                  : info.status == MSP_SOLVE_BREAKDOWN ? "breakdown"
+                  : info.status == MSP_SOLVE_STAGNATED ? "stagnated"
+                                                       : "maxiter";
+    d["fallbacks"] = info.fallbacks;
+    d["relres"] = info.relres;
+    d["seconds"] = info.seconds;
+    return d;
+  }
+
   int64_t num_rows() const {
     uint32_t n = 0;
     msp_csr_shape(h_, &n, nullptr, nullptr);
@@ -213,6 +252,144 @@ class CsrTensor {
 
  private:
   msp_csr h_ = nullptr;
+  int device_ = 0;
+};
+
+// bcsr.bcsr_tensor(keys, values, row_ptr, col_ind, num_rows, num_cols, block_size)
+// (iterative/BiCGSTAB/bcsr.cpp:357-422, 953-965): with block arrays in
+// values/row_ptr/col_ind (keys unused, as in the reference) or, when row_ptr is
+// empty, from packed COO keys converted on the GPU.
+class BcsrTensor {
+ public:
+  BcsrTensor(torch::Tensor keys, torch::Tensor values, torch::Tensor row_ptr, torch::Tensor col_ind,
+             int64_t num_rows, int64_t num_cols, int64_t block_size) {
+    TORCH_CHECK(num_rows >= 0 && num_cols >= 0 && num_rows < (1ll << 32) && num_cols < (1ll << 32),
+                "num_rows / num_cols out of range");
+    TORCH_CHECK(block_size == 4 || block_size == 8 || block_size == 16, "block_size must be 4, 8 or 16");
+    device_ = pick_device({&keys, &values, &row_ptr, &col_ind});
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    const int64_t B = block_size;
+    if (row_ptr.defined() && row_ptr.numel() > 0) {
+      const int64_t nbr = (num_rows + B - 1) / B;
+      expect(row_ptr, torch::kInt32, nbr + 1, "row_ptr (block rows)");
+      expect(col_ind, torch::kInt32, -1, "col_ind (block columns)");
+      expect(values, torch::kFloat32, col_ind.numel() * B * B, "values (blocks)");
+      py::gil_scoped_release nogil;
+      check(msp_bcsr_create(&h_, reinterpret_cast<const uint32_t*>(row_ptr.data_ptr<int32_t>()),
+                            reinterpret_cast<const uint32_t*>(col_ind.data_ptr<int32_t>()), values.data_ptr<float>(),
+                            (uint32_t)num_rows, (uint32_t)num_cols, (uint32_t)B, s),
+            "bcsr_tensor");
+    } else {
+      expect(keys, torch::kInt64, -1, "keys");
+      expect(values, torch::kFloat32, keys.numel(), "values");
+      py::gil_scoped_release nogil;
+      check(msp_bcsr_create_coo(&h_, reinterpret_cast<const uint64_t*>(keys.data_ptr<int64_t>()),
+                                values.data_ptr<float>(), (uint64_t)keys.numel(), (uint32_t)num_rows,
+                                (uint32_t)num_cols, (uint32_t)B, s),
+            "bcsr_tensor (from COO)");
+    }
+  }
+  ~BcsrTensor() {
+    if (h_) msp_bcsr_destroy(h_);
+  }
+
+  void shape(uint32_t& n, uint32_t& m, uint32_t& B, uint64_t& nb) const {
+    check(msp_bcsr_shape(h_, &n, &m, &B, &nb), "shape");
+  }
+
+  // vectors may be logical (n / m) or padded to whole blocks
+  static void expect_vec(const torch::Tensor& t, int64_t logical, int64_t B, const char* name) {
+    const int64_t padded = (logical + B - 1) / B * B;
+    expect(t, torch::kFloat32, -1, name);
+    TORCH_CHECK(t.numel() == logical || t.numel() == padded, name, ": expected ", logical, " or ", padded,
+                " elements, got ", t.numel());
+  }
+
+  void mv(torch::Tensor x, torch::Tensor b) {
+    uint32_t n, m, B;
+    uint64_t nb;
+    shape(n, m, B, nb);
+    expect_vec(x, m, B, "x");
+    expect_vec(b, n, B, "b");
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    py::gil_scoped_release nogil;
+    check(msp_bcsr_spmv(h_, x.data_ptr<float>(), b.data_ptr<float>(), s), "mv");
+  }
+
+  py::dict solve(bool bicg, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
+                 const std::string& criterion, bool use_x0, const std::string& precond, bool verify) {
+    uint32_t n, m, B;
+    uint64_t nb;
+    shape(n, m, B, nb);
+    expect_vec(b, n, B, "b");
+    expect_vec(x, m, B, "x");
+    msp_solve_opts o;
+    msp_default_opts(&o);
+    o.tol = tol;
+    o.maxiter = (int32_t)maxiter;
+    o.check_every = (int32_t)check_every;
+    TORCH_CHECK(criterion == "relres" || criterion == "reference", "criterion must be 'relres' or 'reference'");
+    o.criterion = criterion == "relres" ? MSP_CRIT_RELRES : MSP_CRIT_REFERENCE;
+    o.use_x0 = use_x0 ? 1 : 0;
+    TORCH_CHECK(precond == "none" || precond == "jacobi", "precond must be 'none' or 'jacobi'");
+    o.precond = precond == "jacobi" ? MSP_PRECOND_JACOBI : MSP_PRECOND_NONE;
+    o.verify = verify ? 1 : 0;
+    msp_solve_info info{};
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    {
+      py::gil_scoped_release nogil;
+      if (bicg) check(msp_bcsr_bicgstab(h_, b.data_ptr<float>(), x.data_ptr<float>(), &o, &info, s), "bicgstab");
+      else check(msp_bcsr_cg(h_, b.data_ptr<float>(), x.data_ptr<float>(), &o, &info, s), "cgd");
+    }
+    return info_dict(info);
+  }
+
+  void richardson(torch::Tensor b, torch::Tensor x, double omega, int64_t iters, bool use_x0) {
+    uint32_t n, m, B;
+    uint64_t nb;
+    shape(n, m, B, nb);
+    expect_vec(b, n, B, "b");
+    expect_vec(x, m, B, "x");
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    py::gil_scoped_release nogil;
+    check(msp_bcsr_richardson(h_, b.data_ptr<float>(), x.data_ptr<float>(), (float)omega, (int32_t)iters,
+                              use_x0 ? 1 : 0, s),
+          "richardson");
+  }
+
+  py::tuple export_arrays() {
+    uint32_t n, m, B;
+    uint64_t nb;
+    shape(n, m, B, nb);
+    const int64_t nbr = (n + B - 1) / B;
+    auto opt = torch::TensorOptions().device(torch::kCUDA, device_);
+    torch::Tensor rp = torch::empty({nbr + 1}, opt.dtype(torch::kInt32));
+    torch::Tensor bc = torch::empty({(int64_t)nb}, opt.dtype(torch::kInt32));
+    torch::Tensor bl = torch::empty({(int64_t)nb, (int64_t)B, (int64_t)B}, opt.dtype(torch::kFloat32));
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    {
+      py::gil_scoped_release nogil;
+      check(msp_bcsr_export(h_, reinterpret_cast<uint32_t*>(rp.data_ptr<int32_t>()),
+                            nb ? reinterpret_cast<uint32_t*>(bc.data_ptr<int32_t>()) : nullptr,
+                            nb ? bl.data_ptr<float>() : nullptr, s),
+            "export");
+    }
+    return py::make_tuple(rp, bc, bl);
+  }
+
+  int64_t num_rows() const { uint32_t n, m, B; uint64_t nb; shape(n, m, B, nb); return n; }
+  int64_t num_cols() const { uint32_t n, m, B; uint64_t nb; shape(n, m, B, nb); return m; }
+  int64_t block_size() const { uint32_t n, m, B; uint64_t nb; shape(n, m, B, nb); return B; }
+  int64_t num_blocks() const { uint32_t n, m, B; uint64_t nb; shape(n, m, B, nb); return (int64_t)nb; }
+  uintptr_t handle() const { return reinterpret_cast<uintptr_t>(h_); }
+
+ private:
+  msp_bcsr h_ = nullptr;
   int device_ = 0;
 };
 
@@ -269,6 +446,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           py::arg("values") = "f32", py::arg("verify") = true)
       .def("richardson", &CsrTensor::richardson, "x <- x + omega (b - A x), iters times", py::arg("b"), py::arg("x"),
            py::arg("omega") = 0.01, py::arg("iters") = 1000, py::arg("use_x0") = false, py::arg("values") = "f32")
+      .def("solve_refined", &CsrTensor::solve_refined,
+           "FP64 iterative refinement around FP32 (BF16/FP16-valued) Krylov corrections; b, x float64",
+           py::arg("b"), py::arg("x"), py::arg("tol") = 1e-10, py::arg("inner_tol") = 1e-3,
+           py::arg("max_outer") = 30, py::arg("inner_maxiter") = 10000, py::arg("solver") = "cg",
+           py::arg("values") = "f32", py::arg("precond") = "none", py::arg("use_x0") = false)
       .def("export", &CsrTensor::export_into, "write row_ptr / col_ind / values", py::arg("row_ptr"),
            py::arg("col_ind"), py::arg("out_vals"))
       .def_property_readonly("num_rows", &CsrTensor::num_rows)
@@ -276,4 +458,35 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def_property_readonly("nnz", &CsrTensor::nnz)
       .def_property_readonly("device", &CsrTensor::device)
       .def_property_readonly("handle", &CsrTensor::handle);
+  py::class_<BcsrTensor>(m, "bcsr_tensor")
+      .def(py::init<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, int64_t, int64_t, int64_t>(),
+           "BCSR Tensor Constructor", py::arg("keys"), py::arg("values"), py::arg("row_ptr"), py::arg("col_ind"),
+           py::arg("num_rows"), py::arg("num_cols"), py::arg("block_size") = 4)
+      .def("mv", &BcsrTensor::mv, "b = A x", py::arg("x"), py::arg("b"))
+      .def(
+          "cgd",
+          [](BcsrTensor& self, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
+             const std::string& criterion, bool use_x0, const std::string& precond, bool verify) {
+            return self.solve(false, b, x, tol, maxiter, check_every, criterion, use_x0, precond, verify);
+          },
+          "conjugate gradient", py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("maxiter") = 10000,
+          py::arg("check_every") = 16, py::arg("criterion") = "relres", py::arg("use_x0") = false,
+          py::arg("precond") = "none", py::arg("verify") = true)
+      .def(
+          "bicgstab",
+          [](BcsrTensor& self, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
+             const std::string& criterion, bool use_x0, bool verify) {
+            return self.solve(true, b, x, tol, maxiter, check_every, criterion, use_x0, "none", verify);
+          },
+          "BiCGSTAB", py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("maxiter") = 10000,
+          py::arg("check_every") = 16, py::arg("criterion") = "relres", py::arg("use_x0") = false,
+          py::arg("verify") = true)
+      .def("richardson", &BcsrTensor::richardson, "x <- x + omega (b - A x), iters times", py::arg("b"),
+           py::arg("x"), py::arg("omega") = 0.01, py::arg("iters") = 1000, py::arg("use_x0") = false)
+      .def("export", &BcsrTensor::export_arrays, "(block row_ptr, block col_ind, blocks) as device tensors")
+      .def_property_readonly("num_rows", &BcsrTensor::num_rows)
+      .def_property_readonly("num_cols", &BcsrTensor::num_cols)
+      .def_property_readonly("block_size", &BcsrTensor::block_size)
+      .def_property_readonly("num_blocks", &BcsrTensor::num_blocks)
+      .def_property_readonly("handle", &BcsrTensor::handle);
 }

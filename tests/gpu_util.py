@@ -70,3 +70,29 @@ class Lib:
         import torch
 
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def fp32_floor(A, b):
+    """Relative residual of the exact solution rounded to FP32: no FP32-stored x can do better."""
+    import scipy.sparse.linalg as spla
+
+    from oracle import krylov as K
+
+    xs = spla.spsolve(A.tocsc(), np.asarray(b, dtype=np.float64))
+    return K.true_relres(A, xs.astype(np.float32), b)
+
+
+def check_fp32_solution(info, A, x, b, tol):
+    """A converged FP32 solve meets tol in the FP64 true residual; a stagnated one
+    is accepted only when the FP32 rounding floor is within 3x of tol, and then
+    must lie within 4x of that floor."""
+    from oracle import krylov as K
+
+    rel = K.true_relres(A, x, b)
+    if info["status"] == "converged":
+        assert rel <= 1.05 * tol, (rel, info)
+        return rel
+    floor = fp32_floor(A, b)
+    assert info["status"] == "stagnated" and floor > tol / 3, (info, floor, rel)
+    assert rel <= max(1.05 * tol, 4 * floor), (info, floor, rel)
+    return rel

@@ -167,8 +167,8 @@ struct Workspace {
   int32_t* hflag = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
-  static constexpr int kNumVec = 11;
-  enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2 };
+  static constexpr int kNumVec = 12;
+  enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL };
   float* v(int i) const { return vecs.as<float>() + (size_t)i * npad; }
   Scalars* S_dev() const { return scal.as<Scalars>(); }
   GridRed gr() const {
@@ -753,6 +753,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   if (o.precond == MSP_PRECOND_JACOBI) MSPCHK(ensure_dinv(op, s));
   Workspace& w = A->ws;
   float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Q = w.v(Workspace::Q);
+  float* XLo = w.v(Workspace::XL);
   const float* B = w.v(Workspace::B);
   const float* D = A->dinv.as<float>();
   Scalars* S = w.S_dev();
@@ -761,22 +762,23 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   const bool jac = o.precond == MSP_PRECOND_JACOBI;
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
-    k_cg_init<<<gv, kBlock, 0, s>>>(X, R, P, B, D, n, o.use_x0, jac ? kPcJacobi : kPcNone, S, rb);
+    k_cg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, P, B, D, n, o.use_x0, jac ? kPcJacobi : kPcNone, S, rb);
     return launch_check("cg init");
   };
   auto step = [&]() {
     op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->done, s);
     if (jac) {
-      k_cg_update_xr<kPcJacobi><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
+      k_cg_update_xr<kPcJacobi><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, D, n, S, rb);
       k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
-      if (o.verify) k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, B, R, P, nullptr, D, n, S, rb);
+      if (o.verify) k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
     } else {
-      k_cg_update_xr<kPcNone><<<gv, kBlock, 0, s>>>(X, R, P, Q, D, n, S, rb);
+      k_cg_update_xr<kPcNone><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, D, n, S, rb);
       k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
-      if (o.verify) k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, B, R, P, nullptr, D, n, S, rb);
+      if (o.verify) k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
     }
   };
   MSPCHK(run_solver(A, o, info, s, init, step));
+  k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
@@ -791,23 +793,25 @@ static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const ms
   Workspace& w = A->ws;
   float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Vv = w.v(Workspace::Q);
   float *R0 = w.v(Workspace::R0), *Sv = w.v(Workspace::S), *T = w.v(Workspace::T);
+  float* XLo = w.v(Workspace::XL);
   const float* B = w.v(Workspace::B);
   Scalars* S = w.S_dev();
   GridRed rb = w.gr();
   const uint32_t n = A->n, gv = vec_grid(n);
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
-    k_bicg_init<<<gv, kBlock, 0, s>>>(X, R, R0, P, B, n, o.use_x0, S, rb);
+    k_bicg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, R0, P, B, n, o.use_x0, S, rb);
     return launch_check("bicgstab init");
   };
   auto step = [&]() {
     op.template spmv<1>(P, EpiStoreDot{Vv, R0}, FinBicgAlpha{S}, &S->done, s);
     op.template spmv_axpy<2>(R, Vv, &S->alpha, EpiBicgT{T, Sv, R, Vv, &S->alpha}, FinBicgOmega{S}, &S->done, s);
-    k_bicg_update<<<gv, kBlock, 0, s>>>(X, R, P, Sv, T, R0, n, S, rb);
+    k_bicg_update<<<gv, kBlock, 0, s>>>(X, XLo, R, P, Sv, T, R0, n, S, rb);
     k_bicg_p<<<gv, kBlock, 0, s>>>(P, R, Vv, n, S);
-    if (o.verify) k_verify<2><<<gv, kBlock, 0, s>>>(op.rows(), X, B, R, P, R0, nullptr, n, S, rb);
+    if (o.verify) k_verify<2><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, nullptr, n, S, rb);
   };
   MSPCHK(run_solver(A, o, info, s, init, step));
+  k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;

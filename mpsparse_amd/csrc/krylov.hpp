@@ -33,7 +33,8 @@ struct Scalars {
   float omega;
   float rr;      // ||r||^2 of the latest iterate
   float bb;      // ||b||^2
-  float thresh;  // converged when rr <= thresh
+  float thresh;  // converged when the true ||r||^2 <= thresh
+  float thresh_rec;  // recurrence target (tightened after a failed verification)
   float tol;
   float rr0;     // ||r||^2 at start
   float rr_best;  // smallest ||b - A x||^2 seen by a verification
@@ -44,7 +45,9 @@ struct Scalars {
   int32_t use_verify;  // confirm recurrence convergence with a true residual
   int32_t verify;      // a verification is pending
   int32_t nverify;
-  int32_t nfail;  // consecutive verifications without a 10% gain in ||r||
+  int32_t nfail;     // consecutive verifications without a 10% gain in ||r||
+  int32_t collapse;  // fold xl into x at the next update (after a restart)
+  int32_t pad[3];
 };
 
 // Recurrence says converged: either stop, or ask for a true-residual check.
@@ -108,17 +111,29 @@ struct FinPcgBeta {
 // iterative/BiCGSTAB/spmv.cpp:603-629).
 __device__ __forceinline__ bool step_converged(const Scalars* S, int it, float rr_old, float rr_new) {
   if (S->crit == kCritReference) return (it % 50 == 0) && (rr_old < S->tol);
-  return rr_new <= S->thresh;
+  return rr_new <= S->thresh_rec;
 }
 
 __device__ __forceinline__ uint32_t vec_chunks(uint32_t n) { return (n + 3u) / 4u; }
 
+// (x, xl) += a * p with the rounding errors of the product (FMA) and of the sum
+// (TwoSum) carried in xl: x + xl holds the iterate to about twice FP32 precision.
+__device__ __forceinline__ void comp_axpy(float& x, float& xl, float a, float p) {
+  const float t = a * p;
+  const float te = __builtin_fmaf(a, p, -t);
+  const float s = x + t;
+  const float bp = s - x;
+  const float e = (x - (s - bp)) + (t - bp);
+  x = s;
+  xl += e + te;
+}
+
 // Initialisation: x = 0 (or kept), r = b (or precomputed b - A x0), p = z,
 // <b,b>, <r,r>, <r,z>.
-__global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float* __restrict__ r,
-                                                    float* __restrict__ p, const float* __restrict__ b,
-                                                    const float* __restrict__ dinv, uint32_t n, int use_x0,
-                                                    int pc, Scalars* S, GridRed rb) {
+__global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float* __restrict__ xl,
+                                                    float* __restrict__ r, float* __restrict__ p,
+                                                    const float* __restrict__ b, const float* __restrict__ dinv,
+                                                    uint32_t n, int use_x0, int pc, Scalars* S, GridRed rb) {
   CSum acc[3] = {csum_zero(), csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
@@ -130,6 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float
       reinterpret_cast<f4*>(x)[c] = f4{0.f, 0.f, 0.f, 0.f};
       reinterpret_cast<f4*>(r)[c] = rv;
     }
+    reinterpret_cast<f4*>(xl)[c] = f4{0.f, 0.f, 0.f, 0.f};
     if (pc != kPcExplicit) reinterpret_cast<f4*>(p)[c] = zv;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -146,6 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float
     S->rz = tot[2];
     S->rho = tot[1];
     S->thresh = S->tol * S->tol * tot[0];
+    S->thresh_rec = S->thresh;
     S->iter = 0;
     S->beta = 0.0f;
     S->alpha = 0.0f;
@@ -160,22 +177,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float
 
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
 template <int PC>
-__global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r,
-                                                         const float* __restrict__ p, const float* __restrict__ q,
-                                                         const float* __restrict__ dinv, uint32_t n, Scalars* S,
-                                                         GridRed rb) {
+__global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ xl,
+                                                         float* __restrict__ r, const float* __restrict__ p,
+                                                         const float* __restrict__ q, const float* __restrict__ dinv,
+                                                         uint32_t n, Scalars* S, GridRed rb) {
   if (S->done) return;
   const float alpha = S->alpha;
+  const bool fold = S->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
     f4 xv = reinterpret_cast<f4*>(x)[c];
+    f4 xlv = reinterpret_cast<f4*>(xl)[c];
+    if (fold) {
+      xv += xlv;
+      xlv = f4{0.f, 0.f, 0.f, 0.f};
+    }
     f4 rv = reinterpret_cast<f4*>(r)[c];
     f4 pv = reinterpret_cast<const f4*>(p)[c];
     f4 qv = reinterpret_cast<const f4*>(q)[c];
-    xv += alpha * pv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float xk = xv[k], lk = xlv[k];
+      comp_axpy(xk, lk, alpha, pv[k]);
+      xv[k] = xk;
+      xlv[k] = lk;
+    }
     rv -= alpha * qv;
     reinterpret_cast<f4*>(x)[c] = xv;
+    reinterpret_cast<f4*>(xl)[c] = xlv;
     reinterpret_cast<f4*>(r)[c] = rv;
     f4 zv = rv;
     if (PC == kPcJacobi) zv = rv * reinterpret_cast<const f4*>(dinv)[c];
@@ -191,6 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
     const float rr_old = S->rr;
     S->iter = it + 1;
     S->rr = tot[0];
+    S->collapse = 0;
     const bool conv = step_converged(S, it, rr_old, tot[0]);
     if (!isfinite(tot[0])) {
       S->done = kBreakdown;
@@ -229,10 +260,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(float* __restrict__ p, c
 }
 
 // BiCGSTAB init: r0 = r, p = r (r = b, or b - A x0 precomputed), x = 0 unless kept.
-__global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, float* __restrict__ r,
-                                                      float* __restrict__ r0, float* __restrict__ p,
-                                                      const float* __restrict__ b, uint32_t n, int use_x0,
-                                                      Scalars* S, GridRed rb) {
+__global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, float* __restrict__ xl,
+                                                      float* __restrict__ r, float* __restrict__ r0,
+                                                      float* __restrict__ p, const float* __restrict__ b, uint32_t n,
+                                                      int use_x0, Scalars* S, GridRed rb) {
   CSum acc[2] = {csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
@@ -244,6 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, flo
     }
     reinterpret_cast<f4*>(r0)[c] = rv;
     reinterpret_cast<f4*>(p)[c] = rv;
+    reinterpret_cast<f4*>(xl)[c] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       acc[0] = csum_add(acc[0], bv[k] * bv[k]);
@@ -257,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, flo
     S->rr0 = tot[1];
     S->rho = tot[1];
     S->thresh = S->tol * S->tol * tot[0];
+    S->thresh_rec = S->thresh;
     S->iter = 0;
     S->alpha = 0.0f;
     S->omega = 0.0f;
@@ -271,23 +304,38 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, flo
 }
 
 // x += alpha p + omega s; r = s - omega t; <r0, r>, <r, r> -> bookkeeping, beta.
-__global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, float* __restrict__ r,
-                                                        const float* __restrict__ p, const float* __restrict__ s,
-                                                        const float* __restrict__ t, const float* __restrict__ r0,
-                                                        uint32_t n, Scalars* S, GridRed rb) {
+__global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, float* __restrict__ xl,
+                                                        float* __restrict__ r, const float* __restrict__ p,
+                                                        const float* __restrict__ s, const float* __restrict__ t,
+                                                        const float* __restrict__ r0, uint32_t n, Scalars* S,
+                                                        GridRed rb) {
   if (S->done) return;
   const float alpha = S->alpha, omega = S->omega;
+  const bool fold = S->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
     f4 xv = reinterpret_cast<f4*>(x)[c];
+    f4 xlv = reinterpret_cast<f4*>(xl)[c];
+    if (fold) {
+      xv += xlv;
+      xlv = f4{0.f, 0.f, 0.f, 0.f};
+    }
     f4 pv = reinterpret_cast<const f4*>(p)[c];
     f4 sv = reinterpret_cast<const f4*>(s)[c];
     f4 tv = reinterpret_cast<const f4*>(t)[c];
     f4 r0v = reinterpret_cast<const f4*>(r0)[c];
-    xv += alpha * pv + omega * sv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float xk = xv[k], lk = xlv[k];
+      comp_axpy(xk, lk, alpha, pv[k]);
+      comp_axpy(xk, lk, omega, sv[k]);
+      xv[k] = xk;
+      xlv[k] = lk;
+    }
     f4 rv = sv - omega * tv;
     reinterpret_cast<f4*>(x)[c] = xv;
+    reinterpret_cast<f4*>(xl)[c] = xlv;
     reinterpret_cast<f4*>(r)[c] = rv;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -302,6 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, f
     const float rho_new = tot[0];
     S->iter = it + 1;
     S->rr = tot[1];
+    S->collapse = 0;
     if (!isfinite(tot[1])) {
       S->done = kBreakdown;
       return;
@@ -346,8 +395,10 @@ struct RowsCsr {
 };
 
 // True-residual check after the recurrence reports convergence:
-// r = b - A x with error-free products (FMA) and TwoSum row accumulation, so
-// the residual is exact to about twice FP32 precision for the FP32 x.
+// r = b - A x for the FP32 iterate x = fl(x_hi + x_lo) that the solver returns,
+// with error-free products (FMA) and TwoSum row accumulation (the residual is
+// exact to about twice FP32 precision). On a restart the next update folds
+// x_lo into x_hi so that x and the restarted residual describe the same point.
 // Confirms convergence, or restarts the recurrence from the true residual
 // (p = z; BiCGSTAB also r0 = r) -- keeping the old direction after replacing a
 // residual that drifted by more than its own size breaks conjugacy. Stops as
@@ -357,7 +408,8 @@ struct RowsCsr {
 // MODE 0: CG, 1: CG + Jacobi (z = dinv r), 2: BiCGSTAB.
 template <int MODE, typename Rows>
 __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __restrict__ x,
-                                                   const float* __restrict__ b, float* __restrict__ r,
+                                                   const float* __restrict__ xl, const float* __restrict__ b,
+                                                   float* __restrict__ r,
                                                    float* __restrict__ p, float* __restrict__ r0,
                                                    const float* __restrict__ dinv, uint32_t n, Scalars* S,
                                                    GridRed rb) {
@@ -366,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __res
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     CSum s{b[i], 0.0f};
     rows.foreach(i, [&](float a, uint32_t c) {
-      const float xv = x[c];
+      const float xv = x[c] + xl[c];  // the FP32 iterate the solver would return
       const float pr = a * xv;
       const float e = __builtin_fmaf(a, xv, -pr);
       s = csum_add(s, -pr);
@@ -397,12 +449,20 @@ __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __res
       S->done = kStagnated;
     } else if (S->iter >= S->maxiter) {
       S->done = kMaxIter;
-    } else if (MODE == 2) {
-      S->rho = tot[0];
     } else {
-      S->rz = tot[1];
+      // the recurrence drifted from the true residual: aim its next run at half
+      // the target norm so that the returned iterate lands below tol
+      S->thresh_rec = 0.25f * fminf(S->thresh_rec, S->thresh);
+      S->collapse = 1;
+      if (MODE == 2) S->rho = tot[0];
+      else S->rz = tot[1];
     }
   }
+}
+
+// x = x + xl (collapse the compensated iterate to FP32)
+__global__ __launch_bounds__(kBlock) void k_collapse(float* __restrict__ x, const float* __restrict__ xl, uint32_t n) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = x[i] + xl[i];
 }
 
 // Plain copy of n floats (vector-typed where aligned).

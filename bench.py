@@ -76,8 +76,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--sections", default="spmv,solve,cpu",
-                    help="comma list of spmv, solve, cpu, copy (copy: 1 GiB device copy for counter calibration)")
+    ap.add_argument("--sections", default="spmv,solve,bcsr,cpu,precision",
+                    help="comma list of spmv, solve, bcsr, cpu, precision, copy "
+                         "(copy: 1 GiB device copy for counter calibration)")
     ap.add_argument("--block-rows", type=int, default=65536, help="block-rows of the block-sparse system")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -190,6 +191,40 @@ def main():
                                   "status": cinfo["status"], "relres": cinfo["relres"]}
         log("cg laplacian", out["cg_laplacian_64"])
 
+    if "bcsr" in sections:
+        from mpsparse_amd import bcsr
+
+        # the same matrix, handed over as packed keys rebuilt from the CSR
+        bkeys = (rows.to(torch.int64) << 32) | cols
+        Bm = bcsr.bcsr_tensor(bkeys, cv, torch.empty(0, dtype=torch.int32), torch.empty(0, dtype=torch.int32),
+                              n, n, 16)
+        del bkeys
+        nblk = Bm.num_blocks
+        balg = nblk * (16 * 16 * 4 + 4) + 4 * (n // 16 + 1) + 8 * n
+        xb = torch.randn(n, device=dev)
+        yb = torch.empty(n, device=dev)
+        bms = time_launches(lambda: Bm.mv(xb, yb), args.steps, args.warmup)
+        yref = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * xb.double()[cols])
+        berr = float((yb.double() - yref).abs().max() / yref.abs().max())
+        xs = torch.zeros(n, device=dev)
+        Bm.bicgstab(b, xs)
+        bt = []
+        for _ in range(3):
+            xs.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            binfo = Bm.bicgstab(b, xs)
+            torch.cuda.synchronize()
+            bt.append(time.perf_counter() - t)
+        out["bcsr16"] = {"num_blocks": nblk, "spmv_ms": bms, "algorithmic_bytes": balg,
+                         "GBps": balg / (bms * 1e-3) / 1e9, "frac_of_peak": balg / (bms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                         "max_rel_err_vs_fp64": berr,
+                         "bicgstab": {"wall_s": float(np.median(bt)), "iterations": binfo["iterations"],
+                                      "status": binfo["status"],
+                                      "true_relres_fp64": true_relres_gpu(rows, cols, cv, xs, b)}}
+        log("bcsr16", out["bcsr16"])
+        del Bm
+
     if "cpu" in sections:
         import scipy.sparse as sp
 
@@ -217,7 +252,129 @@ def main():
                                                "cores_available": cores}
             log("cpu bicgstab", tc, itc)
 
+    if "precision" in sections:
+        out["precision_path"] = precision_section(dev, empty)
+
     print(json.dumps(out), flush=True)
+
+
+def stencil_operator64(rp, ci, v, n, nx):
+    """FP64 y = A x for a 7-point grid matrix, rebuilt as 7 coefficient planes and
+    shifted vector products in plain torch (the GPU FP64 oracle; no library code)."""
+    import torch
+
+    dev = v.device
+    rows = torch.repeat_interleave(torch.arange(n, device=dev), (rp[1:] - rp[:-1]).long())
+    off = ci.long() - rows
+    shifts = [-nx * nx, -nx, -1, 0, 1, nx, nx * nx]
+    coef = torch.zeros(7, n, dtype=torch.float64, device=dev)
+    for k, sft in enumerate(shifts):
+        m = off == sft
+        coef[k].index_put_((rows[m],), v[m].double())
+    assert int(sum(int((off == sft).sum()) for sft in shifts)) == ci.numel(), "not a 7-point matrix"
+
+    def apply(x):
+        y = coef[3] * x
+        for k, sft in enumerate(shifts):
+            if sft > 0:
+                y[:-sft] += coef[k][:-sft] * x[sft:]
+            elif sft < 0:
+                y[-sft:] += coef[k][-sft:] * x[:sft]
+        return y
+
+    return apply
+
+
+def fp64_cg_torch(apply, b, tol, maxiter):
+    """Textbook FP64 CG on `apply` (torch only). Returns (x, iterations)."""
+    import torch
+
+    bd = b.double()
+    x = torch.zeros_like(bd)
+    r = bd.clone()
+    p = r.clone()
+    rr = torch.dot(r, r)
+    thr = (tol ** 2) * float(torch.dot(bd, bd))
+    it = 0
+    while it < maxiter:
+        q = apply(p)
+        a = rr / torch.dot(p, q)
+        x += a * p
+        r -= a * q
+        rr_new = torch.dot(r, r)
+        it += 1
+        if it % 50 == 0:
+            if float(rr_new) <= thr:
+                break
+            if it % 1000 == 0:
+                log(f"fp64 oracle cg it {it} relres {float(rr_new / torch.dot(bd, bd)) ** 0.5:.2e}")
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+    return x, it
+
+
+def precision_section(dev, empty):
+    """256^3 7-point log-normal diffusion: plain FP32 CG at 1e-6 next to FP64
+    iterative refinement (FP32 / BF16 / FP16 values) to 1e-10."""
+    import numpy as np
+    import torch
+
+    from mpsparse_amd import problems, spmv
+
+    t0 = time.time()
+    M = problems.diffusion_3d(256, sigma=1.0, seed=1)
+    n = M.num_rows
+    log(f"diffusion 256^3 n={n} nnz={M.nnz} generated in {time.time() - t0:.1f}s")
+    rows_np = np.repeat(np.arange(n, dtype=np.uint64), np.diff(M.row_ptr.astype(np.int64)))
+    keys = torch.from_numpy(problems.pack_keys(rows_np, M.col_ind).view(np.int64)).to(dev)
+    del rows_np
+    vals = torch.from_numpy(M.vals).to(dev)
+    rp = torch.from_numpy(M.row_ptr.view(np.int32)).to(dev)
+    ci = torch.from_numpy(M.col_ind.view(np.int32)).to(dev)
+    A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
+    del keys
+    b64 = torch.from_numpy(problems.rhs(n, seed=1).astype(np.float64)).to(dev)
+    b32 = b64.float()
+    rows = torch.repeat_interleave(torch.arange(n, device=dev), (rp[1:] - rp[:-1]).long())
+    cols = ci.long()
+
+    def relres64(x):
+        ax = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, vals.double() * x.double()[cols])
+        return float(torch.linalg.vector_norm(b64 - ax) / torch.linalg.vector_norm(b64))
+
+    res = {"workload": "diffusion_3d 256^3 sigma=1 seed=1 (7-point, SpMV alg bytes FP32 %d, BF16 %d)" % (
+        M.nnz * 8 + 4 * (n + 1) + 8 * n, M.nnz * 6 + 4 * (n + 1) + 8 * n)}
+    x32 = torch.zeros(n, device=dev)
+    A.cgd(b32, x32, maxiter=20000)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    info = A.cgd(b32, x32, maxiter=20000)
+    torch.cuda.synchronize()
+    res["fp32_cg_1e-6"] = {"wall_s": time.perf_counter() - t, "iterations": info["iterations"],
+                           "status": info["status"], "true_relres_fp64": relres64(x32)}
+    log("fp32 cg", res["fp32_cg_1e-6"])
+    t = time.perf_counter()
+    op64 = stencil_operator64(rp, ci, vals, n, 256)
+    xo, ito = fp64_cg_torch(op64, b64, 1e-11, 40000)
+    torch.cuda.synchronize()
+    res["fp64_oracle"] = {"solver": "FP64 CG, torch 7-plane stencil operator", "iterations": ito,
+                          "wall_s": time.perf_counter() - t, "true_relres_fp64": relres64(xo)}
+    log("fp64 oracle", res["fp64_oracle"])
+    xnorm = float(torch.linalg.vector_norm(xo))
+    res["fp32_cg_1e-6"]["rel_err_vs_fp64_oracle"] = float(torch.linalg.vector_norm(x32.double() - xo)) / xnorm
+    for vt in ("f32", "bf16", "f16"):
+        x64 = torch.zeros(n, dtype=torch.float64, device=dev)
+        A.solve_refined(b64, x64, tol=1e-10, values=vt)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rinfo = A.solve_refined(b64, x64, tol=1e-10, values=vt)
+        torch.cuda.synchronize()
+        res[f"refined_{vt}_1e-10"] = {"wall_s": time.perf_counter() - t, "outer": rinfo["outer_iterations"],
+                                      "inner_iterations": rinfo["inner_iterations"], "status": rinfo["status"],
+                                      "fallbacks": rinfo["fallbacks"], "true_relres_fp64": relres64(x64),
+                                      "rel_err_vs_fp64_oracle": float(torch.linalg.vector_norm(x64 - xo)) / xnorm}
+        log("refined", vt, res[f"refined_{vt}_1e-10"])
+    return res
 
 
 if __name__ == "__main__":

@@ -139,14 +139,19 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag_bcsr(BcsrView A, uint32_t B
 // which leaves the nonzeros of each block adjacent and in CSR order (row-major
 // within the block, duplicates of one (row, col) next to each other).
 
-// keys[i] = (row/B << 32) | col/B, payload[i] = i (bit pattern carried as float)
+// keys[i] = (row/B << 32) | col/B, payload[i] = i (bit pattern carried as
+// float), rows[i] = row. L lanes walk one row (coalesced for long rows).
 __global__ __launch_bounds__(kBlock) void k_block_keys(const uint32_t* __restrict__ row_ptr,
                                                        const uint32_t* __restrict__ col, uint32_t n, uint32_t B,
-                                                       uint64_t* __restrict__ keys, float* __restrict__ payload) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    for (uint32_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j) {
+                                                       uint32_t L, uint64_t* __restrict__ keys,
+                                                       float* __restrict__ payload, uint32_t* __restrict__ rows) {
+  const uint32_t lane = threadIdx.x % L;
+  const uint32_t groups = gridDim.x * (kBlock / L);
+  for (uint32_t r = blockIdx.x * (kBlock / L) + threadIdx.x / L; r < n; r += groups) {
+    for (uint32_t j = row_ptr[r] + lane; j < row_ptr[r + 1]; j += L) {
       keys[j] = ((uint64_t)(r / B) << 32) | (col[j] / B);
       payload[j] = __uint_as_float(j);
+      rows[j] = r;
     }
   }
 }
@@ -198,13 +203,6 @@ __global__ __launch_bounds__(kBlock) void k_brow_ptr(const uint32_t* __restrict_
     const uint32_t pos = brp_nnz[br];
     brow_ptr[br] = pos < nnz ? excl[pos] : (nnz ? excl[nnz - 1] + flag[nnz - 1] : 0u);
   }
-}
-
-// Row index of every CSR nonzero.
-__global__ __launch_bounds__(kBlock) void k_csr_rows(const uint32_t* __restrict__ row_ptr, uint32_t n,
-                                                     uint32_t* __restrict__ rows) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
-    for (uint32_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j) rows[j] = r;
 }
 
 }  // namespace msp

@@ -251,9 +251,10 @@ static msp_status finalize_matrix(msp_csr A, hipStream_t s) {
   }
   HIPCHK(hipMemcpyAsync(&A->max_row, mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  double avg = n ? (double)A->nnz / n : 0.0;
-  uint32_t per = (uint32_t)std::max(1.0, std::floor(avg));
-  uint32_t T = std::min<uint32_t>(2048u, std::max<uint32_t>(256u, 256u * per));
+  // nonzeros per row block: as large as the LDS window allows with room for
+  // the longest row (measured best near 3584 for 7- and 144-entry rows)
+  const uint32_t room = A->max_row + 4 < kCap ? kCap - 4 - A->max_row : 0;
+  const uint32_t T = std::min<uint32_t>(3584u, std::max<uint32_t>(512u, room));
   A->T = T;
   A->nblk = (uint32_t)std::max<uint64_t>(1, (A->nnz + T - 1) / T);
   MSPCHK(A->blk.alloc(sizeof(uint32_t) * (A->nblk + 1)));
@@ -869,10 +870,13 @@ static msp_status spmv_impl(const Op& op, const float* x, float* y, bool need_pa
 
 // ---------------------------------------------------------------------------
 // ABI: CSR compute
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 extern "C" msp_status msp_csr_spmv(msp_csr A, const float* x, float* y, int32_t value_type, void* stream) {
   MSPCHK(check_handle(A));
   hipStream_t s = (hipStream_t)stream;
-  return with_csr_op(A, value_type, [&](auto op) { return spmv_impl(op, x, y, false, s); });
+  // the kernel may read x in aligned 16-byte quads: stage a misaligned x
+  return with_csr_op(A, value_type, [&](auto op) { return spmv_impl(op, x, y, !aligned16(x), s); });
 }
 
 extern "C" msp_status msp_csr_cg(msp_csr A, const float* b, float* x, const msp_solve_opts* opts,
@@ -967,8 +971,10 @@ static msp_status csr_to_bcsr_device(msp_bcsr A, const uint32_t* row_ptr, const 
   MSPCHK(rows.alloc(sizeof(uint32_t) * nnz));
   MSPCHK(flag.alloc(sizeof(uint32_t) * nnz));
   MSPCHK(excl.alloc(sizeof(uint32_t) * nnz));
-  k_block_keys<<<g, kBlock, 0, s>>>(row_ptr, col, n, B, bkeys.as<uint64_t>(), payload.as<float>());
-  k_csr_rows<<<g, kBlock, 0, s>>>(row_ptr, n, rows.as<uint32_t>());
+  uint32_t L = 1;
+  while (L < 64 && L * n < nnz) L <<= 1;  // lanes per row ~ mean row length
+  k_block_keys<<<g, kBlock, 0, s>>>(row_ptr, col, n, B, L, bkeys.as<uint64_t>(), payload.as<float>(),
+                                    rows.as<uint32_t>());
   MSPCHK(launch_check("block keys"));
   MSPCHK(coo_to_csr_device(bkeys.as<uint64_t>(), payload.as<float>(), nnz, nbr, nbc, brp_nnz.as<uint32_t>(),
                            sbcol.as<uint32_t>(), spayload.as<float>(), s, sbrow.as<uint32_t>()));
@@ -1064,8 +1070,6 @@ extern "C" msp_status msp_bcsr_destroy(msp_bcsr A) {
   delete A;
   return MSP_OK;
 }
-
-static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 extern "C" msp_status msp_bcsr_spmv(msp_bcsr A, const float* x, float* y, void* stream) {
   MSPCHK(check_handle(A));

@@ -35,9 +35,20 @@ struct CsrView {
 };
 
 // x loaders --------------------------------------------------------------
+// Four columns at once: one 16-byte load when they are consecutive and
+// 4-aligned (block-structured rows), else four gathers. Needs a 16-byte
+// aligned vector.
+__device__ __forceinline__ bool quad_run(u4 c) {
+  return c.y == c.x + 1 && c.z == c.x + 2 && c.w == c.x + 3 && (c.x & 3u) == 0;
+}
+
 struct XPlain {
   const float* __restrict__ x;
   __device__ __forceinline__ float operator()(uint32_t c) const { return x[c]; }
+  __device__ __forceinline__ f4 load4(u4 c) const {
+    if (quad_run(c)) return *reinterpret_cast<const f4*>(x + c.x);
+    return f4{x[c.x], x[c.y], x[c.z], x[c.w]};
+  }
 };
 
 // s = r - alpha * v formed on the fly (used to fuse a vector update into the
@@ -47,9 +58,16 @@ struct XAxpy {
   const float* __restrict__ v;
   const float* __restrict__ alpha;
   __device__ __forceinline__ float operator()(uint32_t c) const { return r[c] - (*alpha) * v[c]; }
+  __device__ __forceinline__ f4 load4(u4 c) const {
+    const float a = *alpha;
+    if (quad_run(c))
+      return *reinterpret_cast<const f4*>(r + c.x) - a * *reinterpret_cast<const f4*>(v + c.x);
+    return f4{r[c.x] - a * v[c.x], r[c.y] - a * v[c.y], r[c.z] - a * v[c.z], r[c.w] - a * v[c.w]};
+  }
 };
 
-// One row block. Products of the block are staged in `sprod`.
+// One row block. Products of the block are staged in `sprod`. Column index
+// padding past nnz must be valid columns (the library pads with zeros).
 template <typename V, typename XL, typename Epi>
 __device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const XL& xl, const Epi& epi,
                                           float* sprod, CSum* acc) {
@@ -63,10 +81,11 @@ __device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const
     for (uint32_t i = a0 + 4u * tid; i < nz1; i += 4u * kBlock) {
       u4 c = __builtin_nontemporal_load(reinterpret_cast<const u4*>(A.col + i));
       f4 v = V::load4(A.val + i);
+      f4 p = v * xl.load4(c);  // quads at the block edges read valid columns of neighbours / padding
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         uint32_t j = i + k;
-        if (j >= nz0 && j < nz1) sprod[j - nz0] = v[k] * xl(c[k]);
+        if (j >= nz0 && j < nz1) sprod[j - nz0] = p[k];
       }
     }
     __syncthreads();

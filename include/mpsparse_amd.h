@@ -107,6 +107,7 @@ typedef struct msp_refine_info {
 
 typedef struct msp_csr_s *msp_csr;
 typedef struct msp_bcsr_s *msp_bcsr;
+typedef struct msp_fsai_s *msp_fsai;
 
 /* Library ----------------------------------------------------------------- */
 int32_t msp_abi_version(void);
@@ -178,6 +179,55 @@ msp_status msp_csr_richardson(msp_csr A, const float *b, float *x, float omega, 
 void msp_default_refine_opts(msp_refine_opts *opts);
 msp_status msp_csr_solve_refined(msp_csr A, const double *b, double *x, const msp_refine_opts *opts,
                                  msp_refine_info *info, void *stream);
+
+/* FSAI preconditioner ------------------------------------------------------
+ * M = G^T G with G lower triangular on the lower-triangle pattern of A
+ * (diagonal included); applied inside PCG as two SpMVs. No counterpart in the
+ * reference (which has no preconditioner): this is the learned-preconditioner
+ * addition. G's values come from classical FSAI, from the caller, or from an
+ * edge network's output o_e as g_e = o_e / sqrt(a_jj). A handle borrows A: A
+ * must outlive it. */
+msp_status msp_fsai_create(msp_fsai *out, msp_csr A, void *stream);
+msp_status msp_fsai_destroy(msp_fsai F);
+msp_status msp_fsai_num_edges(msp_fsai F, uint64_t *num_edges);
+/* Edge list of the pattern (row, col), in G's CSR order. */
+msp_status msp_fsai_edges(msp_fsai F, uint32_t *rows, uint32_t *cols, void *stream);
+/* 8 features per edge: a_ij/sqrt(a_ii a_jj), log(a_jj/a_ii), nnz(row i)/8,
+ * sum_j|a_ij|/a_ii, [i == j], 0, 0, 0; and the column scaling 1/sqrt(a_jj). */
+msp_status msp_fsai_features(msp_fsai F, float *feat, float *colscale, void *stream);
+/* Classical FSAI: per row, A[J,J] y = e_last, g = y/sqrt(y_last); rows that
+ * cannot be solved become Jacobi rows (counted in fallback_rows). */
+msp_status msp_fsai_classical(msp_fsai F, int32_t *fallback_rows, void *stream);
+msp_status msp_fsai_set_values(msp_fsai F, const float *g, int32_t *nonpositive_diag, void *stream);
+msp_status msp_fsai_set_network_output(msp_fsai F, const float *out, int32_t *nonpositive_diag, void *stream);
+msp_status msp_fsai_export_values(msp_fsai F, float *g, void *stream);
+/* z = G^T G r */
+msp_status msp_fsai_apply(msp_fsai F, const float *r, float *z, void *stream);
+/* loss = sum_k ||G A G^T w_k - w_k||^2 over k probe vectors (W: k rows of n);
+ * adds dloss/dg_e into grad[num_edges] (device). */
+msp_status msp_fsai_probe_loss_grad(msp_fsai F, const float *W, int32_t k, float *grad, double *loss,
+                                    void *stream);
+/* FSAI-preconditioned CG; falls back to plain CG (info->reserved = 1) when G
+ * has a non-positive diagonal entry. */
+msp_status msp_csr_pcg_fsai(msp_csr A, msp_fsai F, const float *b, float *x, const msp_solve_opts *opts,
+                            msp_solve_info *info, void *stream);
+
+/* Edge MLP of the learned FSAI: 8 inputs (msp_fsai_features) -> 64 -> 64 -> 1,
+ * ReLU, BF16 MFMA with FP32 accumulation. Parameters are msp_mlp_num_params()
+ * FP32 values: W1[64][8] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3. All
+ * arrays are device memory; feat is 16-byte aligned. No reference counterpart. */
+int32_t msp_mlp_num_params(void);
+msp_status msp_mlp_forward(const float *params, const float *feat, uint64_t num_edges, float *out, void *stream);
+/* grad_params = d(sum_e grad_out[e] * out[e]) / d params (overwritten). */
+msp_status msp_mlp_backward(const float *params, const float *feat, const float *grad_out, uint64_t num_edges,
+                            float *grad_params, void *stream);
+msp_status msp_mlp_adam(float *params, const float *grad, float *m, float *v, float lr, float beta1, float beta2,
+                        float eps, int32_t step, float weight_decay, void *stream);
+/* G from the network applied to F's features. */
+msp_status msp_fsai_apply_network(msp_fsai F, const float *params, int32_t *nonpositive_diag, void *stream);
+/* forward + probe loss (msp_fsai_probe_loss_grad) + backward: grad_params overwritten. */
+msp_status msp_fsai_train_step(msp_fsai F, const float *params, const float *W, int32_t k, float *grad_params,
+                               double *loss, void *stream);
 
 /* Block CSR ---------------------------------------------------------------
  * Square B x B blocks (B = 4, 8 or 16), row-major inside a block (the layout of

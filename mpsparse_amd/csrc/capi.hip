@@ -14,7 +14,9 @@
 #include "../../include/mpsparse_amd.h"
 #include "bcsr.hpp"
 #include "common.hpp"
+#include "fsai.hpp"
 #include "krylov.hpp"
+#include "mlp.hpp"
 #include "refine.hpp"
 #include "sort.hpp"
 #include "spmv.hpp"
@@ -1218,4 +1220,443 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
   inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   if (info) *info = inf;
   return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ABI: FSAI preconditioner M = G^T G (G on the lower-triangle pattern of A)
+struct msp_fsai_s {
+  int device = 0;
+  msp_csr A = nullptr;  // borrowed
+  msp_csr_s G, GT;      // G and its transpose as CSR matrices
+  uint64_t ne = 0;
+  DevMem Lrow, Lsrc, GTmap, feat, colscale, diag, vec, sums;
+  int ready = 0;    // values set
+  int nonpos = 0;   // diagonal entries of G that are not positive
+};
+
+static msp_status exclusive_scan_u32(uint32_t* data, uint64_t N, hipStream_t s) {
+  const uint32_t nscan = (uint32_t)((N + sortk::kScanTile - 1) / sortk::kScanTile);
+  DevMem sums;
+  MSPCHK(sums.alloc(sizeof(uint32_t) * std::max<uint32_t>(1, nscan)));
+  sortk::k_scan_reduce<<<nscan, kBlock, 0, s>>>(data, N, sums.as<uint32_t>());
+  sortk::k_scan_top<<<1, kBlock, 0, s>>>(sums.as<uint32_t>(), nscan);
+  sortk::k_scan_down<<<nscan, kBlock, 0, s>>>(data, N, sums.as<uint32_t>());
+  MSPCHK(launch_check("scan"));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+static uint32_t grid_for(uint64_t work) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + kBlock - 1) / kBlock, 65536));
+}
+
+extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
+  if (!out) return fail(MSP_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  MSPCHK(check_handle(A));
+  if (A->n != A->m) return fail(MSP_ERR_INVALID, "FSAI needs a square matrix");
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t n = A->n;
+  msp_fsai F = new msp_fsai_s;
+  F->device = A->device;
+  F->A = A;
+  auto body = [&]() -> msp_status {
+    // lower pattern of A -> G's CSR
+    F->G.device = F->GT.device = A->device;
+    F->G.n = F->G.m = F->GT.n = F->GT.m = n;
+    MSPCHK(F->G.row_ptr.alloc(sizeof(uint32_t) * ((size_t)n + 1)));
+    k_lower_count<<<grid_for((uint64_t)n + 1), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), n,
+                                                               F->G.row_ptr.as<uint32_t>());
+    MSPCHK(launch_check("lower count"));
+    MSPCHK(exclusive_scan_u32(F->G.row_ptr.as<uint32_t>(), (uint64_t)n + 1, s));
+    uint32_t ne32 = 0;
+    HIPCHK(hipMemcpy(&ne32, F->G.row_ptr.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    F->ne = F->G.nnz = F->GT.nnz = ne32;
+    const uint64_t ne = F->ne;
+    MSPCHK(F->G.col.alloc(sizeof(uint32_t) * (ne + 64)));
+    MSPCHK(F->G.val.alloc(sizeof(float) * (ne + 64)));
+    MSPCHK(F->Lrow.alloc(sizeof(uint32_t) * (ne + 1)));
+    MSPCHK(F->Lsrc.alloc(sizeof(uint32_t) * (ne + 1)));
+    k_lower_fill<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), n,
+                                                F->G.row_ptr.as<uint32_t>(), F->G.col.as<uint32_t>(),
+                                                F->Lrow.as<uint32_t>(), F->Lsrc.as<uint32_t>());
+    MSPCHK(launch_check("lower fill"));
+    HIPCHK(hipMemsetAsync(F->G.val.p, 0, sizeof(float) * (ne + 64), s));
+    MSPCHK(zero_pad(&F->G, s));
+    MSPCHK(finalize_matrix(&F->G, s));
+    // transpose pattern by sorting (col << 32 | row) with the edge index as payload
+    DevMem keys, payload;
+    MSPCHK(keys.alloc(sizeof(uint64_t) * (ne + 1)));
+    MSPCHK(payload.alloc(sizeof(float) * (ne + 1)));
+    k_transpose_keys<<<grid_for(ne), kBlock, 0, s>>>(F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), ne,
+                                                     keys.as<uint64_t>(), payload.as<float>());
+    MSPCHK(launch_check("transpose keys"));
+    MSPCHK(alloc_csr(&F->GT));
+    MSPCHK(F->GTmap.alloc(sizeof(float) * (ne + 64)));
+    MSPCHK(coo_to_csr_device(keys.as<uint64_t>(), payload.as<float>(), ne, n, n, F->GT.row_ptr.as<uint32_t>(),
+                             F->GT.col.as<uint32_t>(), F->GTmap.as<float>(), s));
+    HIPCHK(hipMemsetAsync(F->GT.val.p, 0, sizeof(float) * (ne + 64), s));
+    MSPCHK(zero_pad(&F->GT, s));
+    MSPCHK(finalize_matrix(&F->GT, s));
+    // features and column scaling
+    MSPCHK(F->diag.alloc(sizeof(float) * ((size_t)n + 1)));
+    MSPCHK(F->feat.alloc(sizeof(float) * 8 * (ne + 64)));
+    MSPCHK(F->colscale.alloc(sizeof(float) * (ne + 64)));
+    HIPCHK(hipMemsetAsync(F->feat.p, 0, sizeof(float) * 8 * (ne + 64), s));
+    k_diag<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n,
+                                          F->diag.as<float>());
+    k_fsai_features<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
+                                                   A->val.as<float>(), n, F->G.row_ptr.as<uint32_t>(),
+                                                   F->G.col.as<uint32_t>(), F->Lsrc.as<uint32_t>(),
+                                                   F->diag.as<float>(), F->feat.as<float>(),
+                                                   F->colscale.as<float>());
+    MSPCHK(launch_check("features"));
+    MSPCHK(ensure_workspace(A, s));
+    MSPCHK(F->vec.alloc(sizeof(float) * 6 * A->ws.npad));
+    HIPCHK(hipMemsetAsync(F->vec.p, 0, sizeof(float) * 6 * A->ws.npad, s));
+    MSPCHK(F->sums.alloc(sizeof(float) * 4));
+    HIPCHK(hipStreamSynchronize(s));
+    return MSP_OK;
+  };
+  msp_status st = body();
+  if (st != MSP_OK) {
+    delete F;
+    return st;
+  }
+  *out = F;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_destroy(msp_fsai F) {
+  if (!F) return MSP_OK;
+  (void)hipSetDevice(F->device);
+  (void)hipDeviceSynchronize();
+  delete F;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_num_edges(msp_fsai F, uint64_t* ne) {
+  if (!F || !ne) return fail(MSP_ERR_INVALID, "null argument");
+  *ne = F->ne;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_edges(msp_fsai F, uint32_t* rows, uint32_t* cols, void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(copy_out(rows, F->Lrow.p, sizeof(uint32_t) * F->ne, s));
+  MSPCHK(copy_out(cols, F->G.col.p, sizeof(uint32_t) * F->ne, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_features(msp_fsai F, float* feat, float* colscale, void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(copy_out(feat, F->feat.p, sizeof(float) * 8 * F->ne, s));
+  MSPCHK(copy_out(colscale, F->colscale.p, sizeof(float) * F->ne, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+// Refresh G^T from G and count non-positive diagonal entries of G.
+static msp_status fsai_commit(msp_fsai F, hipStream_t s) {
+  k_gather_by_payload<<<grid_for(F->ne), kBlock, 0, s>>>(F->GT.val.as<float>(), F->G.val.as<float>(),
+                                                         F->GTmap.as<float>(), F->ne);
+  MSPCHK(launch_check("gather G^T"));
+  F->ready = 1;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_set_values(msp_fsai F, const float* g, int32_t* nonpositive_diag, void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  MSPCHK(copy_in(F->G.val.p, g, sizeof(float) * F->ne, s));
+  // count non-positive diagonal entries with the scaling kernel (unit scaling)
+  DevMem cnt, ones;
+  MSPCHK(cnt.alloc(sizeof(uint32_t)));
+  HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
+  MSPCHK(ones.alloc(sizeof(float) * (F->ne + 1)));
+  k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(F->G.val.as<float>(), F->colscale.as<float>(), F->feat.as<float>(),
+                                                  F->ne, ones.as<float>(), cnt.as<uint32_t>());
+  MSPCHK(fsai_commit(F, s));
+  uint32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  F->nonpos = (int)h;
+  if (nonpositive_diag) *nonpositive_diag = (int32_t)h;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_set_network_output(msp_fsai F, const float* out, int32_t* nonpositive_diag,
+                                                  void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  InArr o;
+  MSPCHK(o.set(out, sizeof(float) * F->ne, s));
+  DevMem cnt;
+  MSPCHK(cnt.alloc(sizeof(uint32_t)));
+  HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
+  k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(static_cast<const float*>(o.d), F->colscale.as<float>(),
+                                                  F->feat.as<float>(), F->ne, F->G.val.as<float>(),
+                                                  cnt.as<uint32_t>());
+  MSPCHK(launch_check("fsai scale"));
+  MSPCHK(fsai_commit(F, s));
+  uint32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  F->nonpos = (int)h;
+  if (nonpositive_diag) *nonpositive_diag = (int32_t)h;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_classical(msp_fsai F, int32_t* fallback_rows, void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  msp_csr A = F->A;
+  DevMem cnt;
+  MSPCHK(cnt.alloc(sizeof(uint32_t)));
+  HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
+  k_fsai_classical<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
+                                                     A->val.as<float>(), A->n, F->G.row_ptr.as<uint32_t>(),
+                                                     F->G.col.as<uint32_t>(), F->G.val.as<float>(),
+                                                     cnt.as<uint32_t>());
+  MSPCHK(launch_check("fsai classical"));
+  MSPCHK(fsai_commit(F, s));
+  uint32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  F->nonpos = 0;
+  if (fallback_rows) *fallback_rows = (int32_t)h;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_export_values(msp_fsai F, float* g, void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  MSPCHK(copy_out(g, F->G.val.p, sizeof(float) * F->ne, (hipStream_t)stream));
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return MSP_OK;
+}
+
+template <int ND, typename XL, typename Epi, typename Fin>
+static void spmv_with(msp_csr M, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
+  k_spmv<ValF32, ND, XL, Epi, Fin><<<spmv_grid(M->nblk), kBlock, 0, s>>>(view_of<ValF32>(M), xl, epi, fin, rb, done);
+}
+
+extern "C" msp_status msp_fsai_apply(msp_fsai F, const float* r, float* z, void* stream) {
+  if (!F || !F->ready) return fail(MSP_ERR_INVALID, "FSAI values not set");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  const size_t np = F->A->ws.npad;
+  float* rin = F->vec.as<float>();
+  float* w = rin + np;
+  float* zo = rin + 2 * np;
+  MSPCHK(copy_in(rin, r, sizeof(float) * F->A->n, s));
+  spmv_with<0>(&F->G, XPlain{rin}, EpiStore{w}, FinNone{}, GridRed{nullptr, nullptr}, nullptr, s);
+  spmv_with<0>(&F->GT, XPlain{w}, EpiStore{zo}, FinNone{}, GridRed{nullptr, nullptr}, nullptr, s);
+  MSPCHK(launch_check("fsai apply"));
+  MSPCHK(copy_out(z, zo, sizeof(float) * F->A->n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+// L = sum_k ||G A G^T w_k - w_k||^2 over k probes (rows of W, n floats each,
+// device or host); adds dL/dG per edge into grad (ne floats, device).
+extern "C" msp_status msp_fsai_probe_loss_grad(msp_fsai F, const float* W, int32_t k, float* grad, double* loss,
+                                               void* stream) {
+  if (!F || !F->ready) return fail(MSP_ERR_INVALID, "FSAI values not set");
+  if (k < 0 || (k > 0 && (!W || !grad))) return fail(MSP_ERR_INVALID, "bad probes / gradient");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  msp_csr A = F->A;
+  const uint32_t n = A->n;
+  const size_t np = A->ws.npad;
+  float* w = F->vec.as<float>();
+  float* u = w + np;
+  float* y = w + 2 * np;
+  float* z = w + 3 * np;
+  float* e = w + 4 * np;
+  float* h = w + 5 * np;
+  GridRed rb = A->ws.gr();
+  double total = 0.0;
+  const bool wdev = is_device_ptr(W);
+  for (int32_t q = 0; q < k; ++q) {
+    HIPCHK(hipMemcpyAsync(w, W + (size_t)q * n, sizeof(float) * n,
+                          wdev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    spmv_with<0>(&F->GT, XPlain{w}, EpiStore{u}, FinNone{}, rb, nullptr, s);
+    spmv_with<0>(A, XPlain{u}, EpiStore{y}, FinNone{}, rb, nullptr, s);
+    spmv_with<0>(&F->G, XPlain{y}, EpiStore{z}, FinNone{}, rb, nullptr, s);
+    k_probe_resid<<<vec_grid(n), kBlock, 0, s>>>(z, w, n, e, F->sums.as<float>(), rb);
+    spmv_with<0>(&F->GT, XPlain{e}, EpiStore{u}, FinNone{}, rb, nullptr, s);
+    spmv_with<0>(A, XPlain{u}, EpiStore{h}, FinNone{}, rb, nullptr, s);
+    k_fsai_edge_grad<<<grid_for(F->ne), kBlock, 0, s>>>(F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), F->ne, e, y,
+                                                        w, h, 1.0f, grad);
+    MSPCHK(launch_check("probe grad"));
+    float sums[2];
+    HIPCHK(hipMemcpyAsync(sums, F->sums.p, sizeof sums, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    total += sums[0];
+  }
+  if (loss) *loss = total;
+  return MSP_OK;
+}
+
+// FSAI-preconditioned CG. Step order puts the preconditioner first, so that a
+// restart after a failed true-residual check is just beta = 0:
+//   w = G r; z = G^T w, <r,z> -> beta; p = z + beta p; q = A p, <p,q> -> alpha;
+//   x += alpha p, r -= alpha q, <r,r> -> convergence; [true residual check].
+template <typename V>
+static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, float* x, const msp_solve_opts& o,
+                                msp_solve_info* info, hipStream_t s) {
+  MatBase* A = op.base();
+  MSPCHK(solver_prologue(A, b, x, o, s));
+  Workspace& w = A->ws;
+  float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Q = w.v(Workspace::Q);
+  float *Z = w.v(Workspace::Z), *Wv = w.v(Workspace::W), *XLo = w.v(Workspace::XL);
+  const float* B = w.v(Workspace::B);
+  Scalars* S = w.S_dev();
+  GridRed rb = w.gr();
+  const uint32_t n = A->n, gv = vec_grid(n);
+  auto init = [&]() -> msp_status {
+    if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
+    k_cg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, P, B, nullptr, n, o.use_x0, kPcExplicit, S, rb);
+    k_set_restart<<<1, 1, 0, s>>>(S);
+    return launch_check("pcg init");
+  };
+  auto step = [&]() {
+    spmv_with<0>(&F->G, XPlain{R}, EpiStore{Wv}, FinNone{}, rb, &S->done, s);
+    spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinPcgBeta{S}, rb, &S->done, s);
+    k_cg_update_p<kPcExplicit><<<gv, kBlock, 0, s>>>(P, R, nullptr, Z, n, S);
+    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->done, s);
+    k_cg_update_xr<kPcExplicit><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb);
+    if (o.verify) k_verify<3><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, nullptr, n, S, rb);
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step));
+  k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
+  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_pcg_fsai(msp_csr A, msp_fsai F, const float* b, float* x, const msp_solve_opts* opts,
+                                       msp_solve_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  if (!F || F->A != A) return fail(MSP_ERR_INVALID, "FSAI handle does not belong to this matrix");
+  if (!F->ready) return fail(MSP_ERR_INVALID, "FSAI values not set");
+  msp_solve_opts o;
+  fill_opts(o, opts);
+  hipStream_t s = (hipStream_t)stream;
+  if (F->nonpos > 0) {
+    // G has a non-positive diagonal: M = G^T G may be singular -> plain CG
+    o.precond = MSP_PRECOND_NONE;
+    msp_status st = with_csr_op(A, o.values, [&](auto op) { return cg_impl(op, b, x, o, info, s); });
+    if (info) info->reserved = 1;
+    return st;
+  }
+  msp_status st = with_csr_op(A, o.values, [&](auto op) { return pcg_fsai_impl(op, F, b, x, o, info, s); });
+  if (info) info->reserved = 0;
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// ABI: edge MLP of the learned FSAI (BF16 MFMA, FP32 accumulate)
+extern "C" int32_t msp_mlp_num_params(void) { return mlp::kParams; }
+
+static uint32_t mlp_grid(uint64_t ne) {
+  const uint64_t tiles = (ne + 31) / 32;
+  const uint64_t g = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)cu_count() * 2));
+}
+
+static msp_status need_dev(const void* p, const char* what) {
+  if (!p || !is_device_ptr(p)) return fail(MSP_ERR_INVALID, "%s must be device memory", what);
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_mlp_forward(const float* params, const float* feat, uint64_t ne, float* out, void* stream) {
+  MSPCHK(need_device());
+  MSPCHK(need_dev(params, "params"));
+  if (ne == 0) return MSP_OK;
+  MSPCHK(need_dev(feat, "feat"));
+  MSPCHK(need_dev(out, "out"));
+  if ((reinterpret_cast<uintptr_t>(feat) & 15u) != 0) return fail(MSP_ERR_INVALID, "feat must be 16-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  mlp::k_mlp_forward<<<mlp_grid(ne), kBlock, 0, s>>>(params, feat, ne, out);
+  return launch_check("mlp forward");
+}
+
+extern "C" msp_status msp_mlp_backward(const float* params, const float* feat, const float* grad_out, uint64_t ne,
+                                       float* grad_params, void* stream) {
+  MSPCHK(need_device());
+  MSPCHK(need_dev(params, "params"));
+  MSPCHK(need_dev(grad_params, "grad_params"));
+  hipStream_t s = (hipStream_t)stream;
+  if (ne == 0) {
+    HIPCHK(hipMemsetAsync(grad_params, 0, sizeof(float) * mlp::kParams, s));
+    return MSP_OK;
+  }
+  MSPCHK(need_dev(feat, "feat"));
+  MSPCHK(need_dev(grad_out, "grad_out"));
+  if ((reinterpret_cast<uintptr_t>(feat) & 15u) != 0) return fail(MSP_ERR_INVALID, "feat must be 16-byte aligned");
+  const uint32_t g = mlp_grid(ne);
+  DevMem partial;
+  MSPCHK(partial.alloc(sizeof(float) * (size_t)g * mlp::kParams));
+  mlp::k_mlp_backward<<<g, kBlock, 0, s>>>(params, feat, grad_out, ne, partial.as<float>());
+  mlp::k_mlp_reduce<<<(mlp::kParams + kBlock - 1) / kBlock, kBlock, 0, s>>>(partial.as<float>(), g, grad_params);
+  MSPCHK(launch_check("mlp backward"));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_mlp_adam(float* params, const float* grad, float* m, float* v, float lr, float beta1,
+                                   float beta2, float eps, int32_t step, float weight_decay, void* stream) {
+  MSPCHK(need_device());
+  MSPCHK(need_dev(params, "params"));
+  MSPCHK(need_dev(grad, "grad"));
+  MSPCHK(need_dev(m, "m"));
+  MSPCHK(need_dev(v, "v"));
+  if (step < 1) return fail(MSP_ERR_INVALID, "adam step counts from 1");
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  mlp::k_adam<<<(mlp::kParams + kBlock - 1) / kBlock, kBlock, 0, (hipStream_t)stream>>>(params, grad, m, v, lr, beta1,
+                                                                                          beta2, eps, c1, c2,
+                                                                                          weight_decay);
+  return launch_check("adam");
+}
+
+// G = network(features of F) scaled by 1/sqrt(a_jj); counts non-positive diagonal.
+extern "C" msp_status msp_fsai_apply_network(msp_fsai F, const float* params, int32_t* nonpositive_diag,
+                                             void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  DevMem out;
+  MSPCHK(out.alloc(sizeof(float) * (F->ne + 1)));
+  MSPCHK(msp_mlp_forward(params, F->feat.as<float>(), F->ne, out.as<float>(), stream));
+  return msp_fsai_set_network_output(F, out.as<float>(), nonpositive_diag, stream);
+}
+
+// One training evaluation on one matrix: forward, G from the output, probe loss
+// and dL/dG, chain rule through the column scaling, backward to the parameters.
+extern "C" msp_status msp_fsai_train_step(msp_fsai F, const float* params, const float* W, int32_t k,
+                                          float* grad_params, double* loss, void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  int32_t nonpos = 0;
+  MSPCHK(msp_fsai_apply_network(F, params, &nonpos, stream));
+  DevMem gg, go;
+  MSPCHK(gg.alloc(sizeof(float) * (F->ne + 1)));
+  MSPCHK(go.alloc(sizeof(float) * (F->ne + 1)));
+  HIPCHK(hipMemsetAsync(gg.p, 0, sizeof(float) * (F->ne + 1), s));
+  MSPCHK(msp_fsai_probe_loss_grad(F, W, k, gg.as<float>(), loss, stream));
+  mlp::k_scale_grad<<<grid_for(F->ne), kBlock, 0, s>>>(gg.as<float>(), F->colscale.as<float>(), F->ne,
+                                                        go.as<float>());
+  MSPCHK(launch_check("scale grad"));
+  return msp_mlp_backward(params, F->feat.as<float>(), go.as<float>(), F->ne, grad_params, stream);
 }

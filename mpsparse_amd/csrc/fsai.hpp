@@ -1,0 +1,236 @@
+// Factorized sparse approximate inverse (FSAI) support: M = G^T G with G lower
+// triangular on the lower-triangle pattern of A (diagonal included).
+//
+// The reference has no preconditioner; this is the learned-preconditioner
+// addition. G and G^T are kept as two CSR matrices so that applying M is two
+// row-block SpMVs; G^T's values are gathered from G's through a permutation
+// computed once by the radix sort.
+#pragma once
+
+#include "common.hpp"
+
+namespace msp {
+
+constexpr int kFsaiMaxRow = 8;  // classical FSAI: largest lower row solved densely
+
+// Lower-triangle entries per row (columns sorted within a row).
+__global__ __launch_bounds__(kBlock) void k_lower_count(const uint32_t* __restrict__ row_ptr,
+                                                        const uint32_t* __restrict__ col, uint32_t n,
+                                                        uint32_t* __restrict__ cnt) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= n; r += gridDim.x * blockDim.x) {
+    if (r == n) {
+      cnt[r] = 0;
+      continue;
+    }
+    uint32_t lo = row_ptr[r], hi = row_ptr[r + 1];
+    while (lo < hi) {  // first entry with col > r
+      uint32_t mid = lo + ((hi - lo) >> 1);
+      if (col[mid] <= r) lo = mid + 1;
+      else hi = mid;
+    }
+    cnt[r] = lo - row_ptr[r];
+  }
+}
+
+// Lower pattern: Lcol, Lrow (row of each edge), Lsrc (position in A).
+__global__ __launch_bounds__(kBlock) void k_lower_fill(const uint32_t* __restrict__ row_ptr,
+                                                       const uint32_t* __restrict__ col, uint32_t n,
+                                                       const uint32_t* __restrict__ Lrp, uint32_t* __restrict__ Lcol,
+                                                       uint32_t* __restrict__ Lrow, uint32_t* __restrict__ Lsrc) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint32_t a = row_ptr[r], o = Lrp[r], k = Lrp[r + 1] - o;
+    for (uint32_t t = 0; t < k; ++t) {
+      Lcol[o + t] = col[a + t];
+      Lrow[o + t] = r;
+      Lsrc[o + t] = a + t;
+    }
+  }
+}
+
+// Transpose keys: (col << 32) | row, payload = edge index (as float bits).
+__global__ __launch_bounds__(kBlock) void k_transpose_keys(const uint32_t* __restrict__ Lrow,
+                                                           const uint32_t* __restrict__ Lcol, uint64_t ne,
+                                                           uint64_t* __restrict__ keys, float* __restrict__ payload) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
+    keys[e] = ((uint64_t)Lcol[e] << 32) | Lrow[e];
+    payload[e] = __uint_as_float((uint32_t)e);
+  }
+}
+
+// dst[k] = src[map[k]] (map carried as float bits from the sort payload)
+__global__ __launch_bounds__(kBlock) void k_gather_by_payload(float* __restrict__ dst, const float* __restrict__ src,
+                                                              const float* __restrict__ map, uint64_t cnt) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += (uint64_t)gridDim.x * blockDim.x)
+    dst[k] = src[__float_as_uint(map[k])];
+}
+
+// a(r, c) by binary search in row r of A (0 if absent; duplicates summed).
+__device__ __forceinline__ float csr_lookup(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                                            const float* __restrict__ val, uint32_t r, uint32_t c) {
+  uint32_t lo = row_ptr[r], hi = row_ptr[r + 1];
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo) >> 1);
+    if (col[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  float s = 0.0f;
+  for (uint32_t j = lo; j < row_ptr[r + 1] && col[j] == c; ++j) s += val[j];
+  return s;
+}
+
+// Classical FSAI row by row: S = A[J, J] (J = lower pattern of row i, i last),
+// solve S y = e_last by Cholesky, g = y / sqrt(y_last). Rows longer than
+// kFsaiMaxRow, or with a non-positive pivot, fall back to the Jacobi row
+// g = e_i / sqrt(a_ii) and are counted in `fallback`.
+__global__ __launch_bounds__(kBlock) void k_fsai_classical(const uint32_t* __restrict__ row_ptr,
+                                                           const uint32_t* __restrict__ col,
+                                                           const float* __restrict__ val, uint32_t n,
+                                                           const uint32_t* __restrict__ Lrp,
+                                                           const uint32_t* __restrict__ Lcol, float* __restrict__ G,
+                                                           uint32_t* __restrict__ fallback) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t o = Lrp[i], k = Lrp[i + 1] - o;
+    bool ok = k >= 1 && k <= (uint32_t)kFsaiMaxRow && Lcol[o + k - 1] == i;
+    double S[kFsaiMaxRow][kFsaiMaxRow];
+    double y[kFsaiMaxRow];
+    if (ok) {
+      for (uint32_t p = 0; p < k; ++p)
+        for (uint32_t q = 0; q <= p; ++q) {
+          const float a = csr_lookup(row_ptr, col, val, Lcol[o + p], Lcol[o + q]);
+          S[p][q] = a;
+        }
+      // Cholesky S = L L^T in place (lower triangle)
+      for (uint32_t p = 0; p < k && ok; ++p) {
+        double d = S[p][p];
+        for (uint32_t t = 0; t < p; ++t) d -= S[p][t] * S[p][t];
+        if (!(d > 0.0)) {
+          ok = false;
+          break;
+        }
+        d = sqrt(d);
+        S[p][p] = d;
+        for (uint32_t q = p + 1; q < k; ++q) {
+          double s = S[q][p];
+          for (uint32_t t = 0; t < p; ++t) s -= S[q][t] * S[p][t];
+          S[q][p] = s / d;
+        }
+      }
+      if (ok) {
+        // L z = e_last, then L^T y = z
+        for (uint32_t p = 0; p < k; ++p) {
+          double s = (p == k - 1) ? 1.0 : 0.0;
+          for (uint32_t t = 0; t < p; ++t) s -= S[p][t] * y[t];
+          y[p] = s / S[p][p];
+        }
+        for (int p = (int)k - 1; p >= 0; --p) {
+          double s = y[p];
+          for (uint32_t t = p + 1; t < k; ++t) s -= S[t][p] * y[t];
+          y[p] = s / S[p][p];
+        }
+        if (!(y[k - 1] > 0.0)) ok = false;
+      }
+    }
+    if (ok) {
+      const double sc = 1.0 / sqrt(y[k - 1]);
+      for (uint32_t p = 0; p < k; ++p) G[o + p] = (float)(y[p] * sc);
+    } else {
+      atomicAdd(fallback, 1u);
+      for (uint32_t p = 0; p < k; ++p) {
+        const uint32_t c = Lcol[o + p];
+        const float d = csr_lookup(row_ptr, col, val, i, i);
+        G[o + p] = (c == i && d > 0.0f) ? 1.0f / sqrtf(d) : 0.0f;
+      }
+    }
+  }
+}
+
+// Edge features of the lower pattern, 8 floats per edge:
+// [a_ij / sqrt(a_ii a_jj), log(a_jj / a_ii), nnz(row i) / 8, sum_j |a_ij| / a_ii,
+//  [j == i], 0, 0, 0]; and s[e] = 1 / sqrt(a_jj) (the column scaling of G).
+__global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __restrict__ row_ptr,
+                                                          const uint32_t* __restrict__ col,
+                                                          const float* __restrict__ val, uint32_t n,
+                                                          const uint32_t* __restrict__ Lrp,
+                                                          const uint32_t* __restrict__ Lcol,
+                                                          const uint32_t* __restrict__ Lsrc, const float* __restrict__ diag,
+                                                          float* __restrict__ feat, float* __restrict__ colscale) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float aii = diag[i];
+    float asum = 0.0f;
+    for (uint32_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j) asum += fabsf(val[j]);
+    const float rn = (float)(row_ptr[i + 1] - row_ptr[i]) * 0.125f;
+    for (uint32_t e = Lrp[i]; e < Lrp[i + 1]; ++e) {
+      const uint32_t c = Lcol[e];
+      const float ajj = diag[c];
+      const float aij = val[Lsrc[e]];
+      float* f = feat + (size_t)e * 8;
+      const bool ok = aii > 0.0f && ajj > 0.0f;
+      f[0] = ok ? aij * rsqrtf(aii * ajj) : 0.0f;
+      f[1] = ok ? logf(ajj / aii) : 0.0f;
+      f[2] = rn;
+      f[3] = ok ? asum / aii : 0.0f;
+      f[4] = (c == i) ? 1.0f : 0.0f;
+      f[5] = 0.0f;
+      f[6] = 0.0f;
+      f[7] = 0.0f;
+      colscale[e] = ajj > 0.0f ? rsqrtf(ajj) : 0.0f;
+    }
+  }
+}
+
+// Diagonal of A (duplicates summed).
+__global__ __launch_bounds__(kBlock) void k_diag(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                                                 const float* __restrict__ val, uint32_t n, float* __restrict__ d) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (uint32_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j)
+      if (col[j] == r) s += val[j];
+    d[r] = s;
+  }
+}
+
+// G values from the network output: g_e = out_e * colscale_e; counts diagonal
+// entries that are not positive (the solver then falls back to plain CG).
+__global__ __launch_bounds__(kBlock) void k_fsai_scale(const float* __restrict__ out, const float* __restrict__ colscale,
+                                                       const float* __restrict__ feat, uint64_t ne,
+                                                       float* __restrict__ G, uint32_t* __restrict__ nonpos) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
+    const float g = out[e] * colscale[e];
+    G[e] = g;
+    if (feat[e * 8 + 4] != 0.0f && !(g > 0.0f)) atomicAdd(nonpos, 1u);
+  }
+}
+
+// Probe-loss gradient per edge of G for L = sum_w ||G A G^T w - w||^2:
+// with u = G^T w, y = A u, e = G y - w, h = A G^T e:
+// dL/dG_ij = 2 (e_i y_j + w_i h_j).
+__global__ __launch_bounds__(kBlock) void k_fsai_edge_grad(const uint32_t* __restrict__ Lrow,
+                                                           const uint32_t* __restrict__ Lcol, uint64_t ne,
+                                                           const float* __restrict__ e, const float* __restrict__ y,
+                                                           const float* __restrict__ w, const float* __restrict__ h,
+                                                           float scale, float* __restrict__ grad) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ne; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t i = Lrow[k], j = Lcol[k];
+    grad[k] += scale * 2.0f * (e[i] * y[j] + w[i] * h[j]);
+  }
+}
+
+// e = z - w and ||e||^2, ||w||^2 (deterministic)
+__global__ __launch_bounds__(kBlock) void k_probe_resid(const float* __restrict__ z, const float* __restrict__ w,
+                                                        uint32_t n, float* __restrict__ e, float* __restrict__ out,
+                                                        GridRed rb) {
+  CSum acc[2] = {csum_zero(), csum_zero()};
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float d = z[i] - w[i];
+    e[i] = d;
+    acc[0] = csum_add(acc[0], d * d);
+    acc[1] = csum_add(acc[1], w[i] * w[i]);
+  }
+  float tot[2];
+  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+    out[0] = tot[0];
+    out[1] = tot[1];
+  }
+}
+
+}  // namespace msp

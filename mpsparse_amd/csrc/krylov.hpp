@@ -47,7 +47,8 @@ struct Scalars {
   int32_t nverify;
   int32_t nfail;     // consecutive verifications without a 10% gain in ||r||
   int32_t collapse;  // fold xl into x at the next update (after a restart)
-  int32_t pad[3];
+  int32_t restart;   // next preconditioned direction update uses beta = 0
+  int32_t pad[2];
 };
 
 // Recurrence says converged: either stop, or ask for a true-residual check.
@@ -92,19 +93,23 @@ struct FinBicgOmega {
   }
 };
 
-// Beta for PCG with an explicit z computed by other kernels: rz_new in tot[0].
+// Beta for PCG with an explicit z computed by other kernels: rz_new in tot[0]
+// (beta = 0 on the first step and after a restart).
 struct FinPcgBeta {
   Scalars* S;
   __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
-    float rz_new = tot[0];
-    if (!isfinite(rz_new) || S->rz == 0.0f) {
+    const float rz_new = tot[0];
+    if (!isfinite(rz_new) || !(rz_new > 0.0f)) {
       S->done = kBreakdown;
       return;
     }
-    S->beta = rz_new / S->rz;
+    S->beta = (S->restart || S->rz == 0.0f) ? 0.0f : rz_new / S->rz;
+    S->restart = 0;
     S->rz = rz_new;
   }
 };
+
+__global__ void k_set_restart(Scalars* S) { S->restart = 1; }
 
 // Convergence test shared by the solvers; `rr_old` is ||r||^2 at the start of
 // the step (the reference criterion tests it every 50 steps,
@@ -405,7 +410,9 @@ struct RowsCsr {
 // stagnated after two verifications in a row that do not improve the best true
 // residual norm by 10% (the FP32 iterate has reached its attainable accuracy),
 // or after 12 verifications.
-// MODE 0: CG, 1: CG + Jacobi (z = dinv r), 2: BiCGSTAB.
+// MODE 0: CG, 1: CG + Jacobi (z = dinv r), 2: BiCGSTAB, 3: PCG with an
+// explicit preconditioner applied at the start of the next step (p is left
+// alone; the restart flag makes that step use beta = 0).
 template <int MODE, typename Rows>
 __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __restrict__ x,
                                                    const float* __restrict__ xl, const float* __restrict__ b,
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __res
     const float ri = csum_val(s);
     const float zi = (MODE == 1) ? ri * dinv[i] : ri;
     r[i] = ri;
-    p[i] = zi;
+    if (MODE != 3) p[i] = zi;
     if (MODE == 2) r0[i] = ri;
     acc[0] = csum_add(acc[0], ri * ri);
     acc[1] = csum_add(acc[1], ri * zi);
@@ -455,6 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __res
       S->thresh_rec = 0.25f * fminf(S->thresh_rec, S->thresh);
       S->collapse = 1;
       if (MODE == 2) S->rho = tot[0];
+      else if (MODE == 3) S->restart = 1;
       else S->rz = tot[1];
     }
   }

@@ -249,11 +249,145 @@ class CsrTensor {
   }
   int device() const { return device_; }
   uintptr_t handle() const { return reinterpret_cast<uintptr_t>(h_); }
+  msp_csr raw() const { return h_; }
 
  private:
   msp_csr h_ = nullptr;
   int device_ = 0;
 };
+
+void* stream_of(int device) { return stream_for(device); }
+
+// Learned / classical FSAI preconditioner bound to one csr_tensor (kept alive
+// by a reference held on the Python side).
+class FsaiTensor {
+ public:
+  explicit FsaiTensor(CsrTensor& A) : A_(A.raw()), device_(A.device()) {
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_of(device_);
+    py::gil_scoped_release nogil;
+    check(msp_fsai_create(&h_, A_, s), "fsai");
+    check(msp_fsai_num_edges(h_, &ne_), "fsai edges");
+  }
+  ~FsaiTensor() {
+    if (h_) msp_fsai_destroy(h_);
+  }
+  int64_t num_edges() const { return (int64_t)ne_; }
+  torch::TensorOptions dopt(c10::ScalarType t) const {
+    return torch::TensorOptions().device(torch::kCUDA, device_).dtype(t);
+  }
+  py::tuple edges() {
+    auto r = torch::empty({(int64_t)ne_}, dopt(torch::kInt32));
+    auto cc = torch::empty({(int64_t)ne_}, dopt(torch::kInt32));
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_edges(h_, reinterpret_cast<uint32_t*>(r.data_ptr<int32_t>()),
+                         reinterpret_cast<uint32_t*>(cc.data_ptr<int32_t>()), stream_of(device_)),
+          "edges");
+    return py::make_tuple(r, cc);
+  }
+  py::tuple features() {
+    auto f = torch::empty({(int64_t)ne_, 8}, dopt(torch::kFloat32));
+    auto sc = torch::empty({(int64_t)ne_}, dopt(torch::kFloat32));
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_features(h_, f.data_ptr<float>(), sc.data_ptr<float>(), stream_of(device_)), "features");
+    return py::make_tuple(f, sc);
+  }
+  int classical() {
+    int32_t fb = 0;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_classical(h_, &fb, stream_of(device_)), "classical");
+    return fb;
+  }
+  int set_values(torch::Tensor g) {
+    expect(g, torch::kFloat32, (int64_t)ne_, "g");
+    int32_t np = 0;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_set_values(h_, g.data_ptr<float>(), &np, stream_of(device_)), "set_values");
+    return np;
+  }
+  int set_network(torch::Tensor params) {
+    expect(params, torch::kFloat32, msp_mlp_num_params(), "params");
+    TORCH_CHECK(params.is_cuda(), "params must be a ROCm tensor");
+    int32_t np = 0;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_apply_network(h_, params.data_ptr<float>(), &np, stream_of(device_)), "set_network");
+    return np;
+  }
+  torch::Tensor values() {
+    auto g = torch::empty({(int64_t)ne_}, dopt(torch::kFloat32));
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_export_values(h_, g.data_ptr<float>(), stream_of(device_)), "values");
+    return g;
+  }
+  void apply(torch::Tensor r, torch::Tensor z) {
+    expect(r, torch::kFloat32, -1, "r");
+    expect(z, torch::kFloat32, r.numel(), "z");
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_apply(h_, r.data_ptr<float>(), z.data_ptr<float>(), stream_of(device_)), "apply");
+  }
+  double probe_loss_grad(torch::Tensor W, torch::Tensor grad) {
+    expect(W, torch::kFloat32, -1, "W");
+    TORCH_CHECK(W.dim() == 2, "W must be (k, n)");
+    expect(grad, torch::kFloat32, (int64_t)ne_, "grad");
+    TORCH_CHECK(grad.is_cuda(), "grad must be a ROCm tensor");
+    double loss = 0.0;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_of(device_);
+    py::gil_scoped_release nogil;
+    check(msp_fsai_probe_loss_grad(h_, W.data_ptr<float>(), (int32_t)W.size(0), grad.data_ptr<float>(), &loss, s),
+          "probe_loss_grad");
+    return loss;
+  }
+  double train_step(torch::Tensor params, torch::Tensor W, torch::Tensor grad_params) {
+    expect(params, torch::kFloat32, msp_mlp_num_params(), "params");
+    expect(grad_params, torch::kFloat32, msp_mlp_num_params(), "grad_params");
+    expect(W, torch::kFloat32, -1, "W");
+    TORCH_CHECK(W.dim() == 2, "W must be (k, n)");
+    double loss = 0.0;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_of(device_);
+    py::gil_scoped_release nogil;
+    check(msp_fsai_train_step(h_, params.data_ptr<float>(), W.data_ptr<float>(), (int32_t)W.size(0),
+                              grad_params.data_ptr<float>(), &loss, s),
+          "train_step");
+    return loss;
+  }
+  msp_fsai raw() const { return h_; }
+  msp_csr matrix() const { return A_; }
+
+ private:
+  msp_csr A_ = nullptr;
+  int device_ = 0;
+  msp_fsai h_ = nullptr;
+  uint64_t ne_ = 0;
+};
+
+py::dict pcg_fsai(CsrTensor& A, FsaiTensor& F, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter,
+                  int64_t check_every, bool use_x0, const std::string& values, bool verify) {
+  TORCH_CHECK(F.matrix() == A.raw(), "the FSAI preconditioner was built for another matrix");
+  const int64_t n = A.num_rows();
+  expect(b, torch::kFloat32, n, "b");
+  expect(x, torch::kFloat32, n, "x");
+  msp_solve_opts o;
+  msp_default_opts(&o);
+  o.tol = tol;
+  o.maxiter = (int32_t)maxiter;
+  o.check_every = (int32_t)check_every;
+  o.use_x0 = use_x0 ? 1 : 0;
+  o.values = value_type_of(values);
+  o.verify = verify ? 1 : 0;
+  msp_solve_info info{};
+  c10::hip::HIPGuard guard((c10::DeviceIndex)A.device());
+  void* s = stream_of(A.device());
+  {
+    py::gil_scoped_release nogil;
+    if (o.values != MSP_VALUES_F32) check(msp_csr_prepare_values(A.raw(), o.values, s), "prepare_values");
+    check(msp_csr_pcg_fsai(A.raw(), F.raw(), b.data_ptr<float>(), x.data_ptr<float>(), &o, &info, s), "pcg_fsai");
+  }
+  py::dict d = info_dict(info);
+  d["fell_back_to_cg"] = info.reserved == 1;
+  return d;
+}
 
 // bcsr.bcsr_tensor(keys, values, row_ptr, col_ind, num_rows, num_cols, block_size)
 // (iterative/BiCGSTAB/bcsr.cpp:357-422, 953-965): with block arrays in
@@ -415,8 +549,52 @@ void coo_csr(torch::Tensor keys, torch::Tensor values, torch::Tensor row_ptr, to
 
 }  // namespace
 
+void mlp_forward(torch::Tensor params, torch::Tensor feat, torch::Tensor out) {
+  expect(params, torch::kFloat32, msp_mlp_num_params(), "params");
+  expect(feat, torch::kFloat32, -1, "feat");
+  TORCH_CHECK(feat.dim() == 2 && feat.size(1) == 8, "feat must be (num_edges, 8)");
+  expect(out, torch::kFloat32, feat.size(0), "out");
+  c10::hip::HIPGuard guard(params.device().index());
+  check(msp_mlp_forward(params.data_ptr<float>(), feat.data_ptr<float>(), (uint64_t)feat.size(0), out.data_ptr<float>(),
+                        stream_of(params.device().index())),
+        "mlp_forward");
+}
+
+void mlp_backward(torch::Tensor params, torch::Tensor feat, torch::Tensor go, torch::Tensor grad) {
+  expect(params, torch::kFloat32, msp_mlp_num_params(), "params");
+  expect(feat, torch::kFloat32, -1, "feat");
+  TORCH_CHECK(feat.dim() == 2 && feat.size(1) == 8, "feat must be (num_edges, 8)");
+  expect(go, torch::kFloat32, feat.size(0), "grad_out");
+  expect(grad, torch::kFloat32, msp_mlp_num_params(), "grad_params");
+  c10::hip::HIPGuard guard(params.device().index());
+  check(msp_mlp_backward(params.data_ptr<float>(), feat.data_ptr<float>(), go.data_ptr<float>(), (uint64_t)feat.size(0),
+                         grad.data_ptr<float>(), stream_of(params.device().index())),
+        "mlp_backward");
+}
+
+void mlp_adam(torch::Tensor params, torch::Tensor grad, torch::Tensor m, torch::Tensor v, double lr, double b1,
+              double b2, double eps, int64_t step, double wd) {
+  for (auto* t : {&params, &grad, &m, &v}) expect(*t, torch::kFloat32, msp_mlp_num_params(), "adam tensor");
+  c10::hip::HIPGuard guard(params.device().index());
+  check(msp_mlp_adam(params.data_ptr<float>(), grad.data_ptr<float>(), m.data_ptr<float>(), v.data_ptr<float>(),
+                     (float)lr, (float)b1, (float)b2, (float)eps, (int32_t)step, (float)wd,
+                     stream_of(params.device().index())),
+        "mlp_adam");
+}
+
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.attr("abi_version") = msp_abi_version();
+  m.attr("mlp_num_params") = msp_mlp_num_params();
+  m.def("mlp_forward", &mlp_forward, "edge MLP forward (BF16 MFMA)", py::arg("params"), py::arg("feat"),
+        py::arg("out"));
+  m.def("mlp_backward", &mlp_backward, "edge MLP backward: grad = d(sum grad_out*out)/d params", py::arg("params"),
+        py::arg("feat"), py::arg("grad_out"), py::arg("grad"));
+  m.def("mlp_adam", &mlp_adam, "Adam step", py::arg("params"), py::arg("grad"), py::arg("m"), py::arg("v"),
+        py::arg("lr"), py::arg("beta1") = 0.9, py::arg("beta2") = 0.999, py::arg("eps") = 1e-8, py::arg("step") = 1,
+        py::arg("weight_decay") = 0.0);
+  m.def("pcg_fsai", &pcg_fsai, "FSAI-preconditioned CG", py::arg("A"), py::arg("F"), py::arg("b"), py::arg("x"),
+        py::arg("tol") = 1e-6, py::arg("maxiter") = 10000, py::arg("check_every") = 16, py::arg("use_x0") = false,
+        py::arg("values") = "f32", py::arg("verify") = true);
   m.def("device_count", []() { return msp_device_count(); });
   m.def("coo_csr", &coo_csr, "COO -> CSR (stable packed-key sort)", py::arg("keys"), py::arg("values"),
         py::arg("row_ptr"), py::arg("col_ind"), py::arg("out_vals"), py::arg("num_rows"), py::arg("num_cols"));
@@ -458,6 +636,18 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def_property_readonly("nnz", &CsrTensor::nnz)
       .def_property_readonly("device", &CsrTensor::device)
       .def_property_readonly("handle", &CsrTensor::handle);
+  py::class_<FsaiTensor>(m, "fsai_tensor")
+      .def(py::init<CsrTensor&>(), py::arg("A"), py::keep_alive<1, 2>())
+      .def_property_readonly("num_edges", &FsaiTensor::num_edges)
+      .def("edges", &FsaiTensor::edges)
+      .def("features", &FsaiTensor::features)
+      .def("classical", &FsaiTensor::classical, "classical FSAI values; returns the number of Jacobi fallback rows")
+      .def("set_values", &FsaiTensor::set_values, py::arg("g"))
+      .def("set_network", &FsaiTensor::set_network, py::arg("params"))
+      .def("values", &FsaiTensor::values)
+      .def("apply", &FsaiTensor::apply, py::arg("r"), py::arg("z"))
+      .def("probe_loss_grad", &FsaiTensor::probe_loss_grad, py::arg("W"), py::arg("grad"))
+      .def("train_step", &FsaiTensor::train_step, py::arg("params"), py::arg("W"), py::arg("grad_params"));
   py::class_<BcsrTensor>(m, "bcsr_tensor")
       .def(py::init<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, int64_t, int64_t, int64_t>(),
            "BCSR Tensor Constructor", py::arg("keys"), py::arg("values"), py::arg("row_ptr"), py::arg("col_ind"),

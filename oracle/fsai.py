@@ -1,0 +1,147 @@
+"""FSAI / edge-MLP oracle (test infrastructure only).
+
+The reference has no preconditioner; these restate the library's learned-FSAI
+additions from their definitions so the GPU kernels can be checked:
+
+* :func:`lower_pattern`, :func:`features` -- the lower-triangle pattern of A in
+  CSR order and the 8 per-edge features;
+* :func:`classical_fsai` -- per row i: A[J,J] y = e_i, g = y / sqrt(y_i)
+  (Kolotilina-Yeremin FSAI); pinned by its defining property (G A)[i, J] =
+  e_i / sqrt(y_i) and by the diagonal case G = D^{-1/2};
+* :func:`probe_loss_grad` -- L = sum_w ||G A G^T w - w||^2 and dL/dG on the
+  pattern; pinned by central finite differences;
+* :func:`mlp_forward` / :func:`mlp_backward` -- 8 -> 64 -> 64 -> 1 ReLU network
+  in float64, optionally rounding the matrix-core operands to BF16 exactly where
+  the kernels do; pinned against torch autograd (float64) with rounding off.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+KIN, KH = 8, 64
+OW1, OB1 = 0, 512
+OW2 = OB1 + KH
+OB2 = OW2 + KH * KH
+OW3 = OB2 + KH
+OB3 = OW3 + KH
+NPARAMS = OB3 + 1
+
+
+def lower_pattern(row_ptr, col_ind):
+    """Edges (i, j <= i) of A's pattern in CSR order: (rows, cols, src positions)."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    col = np.asarray(col_ind, dtype=np.int64)
+    n = row_ptr.size - 1
+    rows = np.repeat(np.arange(n), np.diff(row_ptr))
+    keep = col <= rows
+    return rows[keep], col[keep], np.nonzero(keep)[0]
+
+
+def diag_of(row_ptr, col_ind, vals):
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    n = row_ptr.size - 1
+    rows = np.repeat(np.arange(n), np.diff(row_ptr))
+    d = np.zeros(n)
+    m = np.asarray(col_ind, dtype=np.int64) == rows
+    np.add.at(d, rows[m], np.asarray(vals, dtype=np.float64)[m])
+    return d
+
+
+def features(row_ptr, col_ind, vals):
+    """(feat[ne, 8], colscale[ne]) for the lower pattern (float64)."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    vals = np.asarray(vals, dtype=np.float64)
+    n = row_ptr.size - 1
+    r, c, src = lower_pattern(row_ptr, col_ind)
+    d = diag_of(row_ptr, col_ind, vals)
+    rows_all = np.repeat(np.arange(n), np.diff(row_ptr))
+    asum = np.zeros(n)
+    np.add.at(asum, rows_all, np.abs(vals))
+    f = np.zeros((r.size, 8))
+    f[:, 0] = vals[src] / np.sqrt(d[r] * d[c])
+    f[:, 1] = np.log(d[c] / d[r])
+    f[:, 2] = np.diff(row_ptr)[r] / 8.0
+    f[:, 3] = asum[r] / d[r]
+    f[:, 4] = (r == c).astype(np.float64)
+    return f, 1.0 / np.sqrt(d[c])
+
+
+def classical_fsai(A: sp.csr_matrix):
+    """Classical FSAI values on the lower pattern of A (float64)."""
+    A = sp.csr_matrix(A, dtype=np.float64)
+    A.sort_indices()
+    r, c, _ = lower_pattern(A.indptr, A.indices)
+    g = np.zeros(r.size)
+    n = A.shape[0]
+    starts = np.searchsorted(r, np.arange(n + 1))
+    Ad = A.tolil()
+    for i in range(n):
+        J = c[starts[i]:starts[i + 1]]
+        S = np.array(Ad[J][:, J].todense())
+        e = np.zeros(J.size)
+        e[-1] = 1.0
+        y = np.linalg.solve(S, e)
+        g[starts[i]:starts[i + 1]] = y / np.sqrt(y[-1])
+    return g
+
+
+def G_matrix(n, rows, cols, g):
+    return sp.csr_matrix((np.asarray(g, dtype=np.float64), (rows, cols)), shape=(n, n))
+
+
+def probe_loss_grad(A, rows, cols, g, W):
+    """L = sum_k ||G A G^T w_k - w_k||^2 and dL/dg (per pattern edge)."""
+    n = A.shape[0]
+    G = G_matrix(n, rows, cols, g)
+    loss = 0.0
+    grad = np.zeros(len(g))
+    for w in np.atleast_2d(W):
+        w = np.asarray(w, dtype=np.float64)
+        u = G.T @ w
+        y = A @ u
+        e = G @ y - w
+        h = A @ (G.T @ e)
+        loss += float(e @ e)
+        grad += 2.0 * (e[rows] * y[cols] + w[rows] * h[cols])
+    return loss, grad
+
+
+def _bf16(a, on):
+    if not on:
+        return np.asarray(a, dtype=np.float64)
+    a32 = np.ascontiguousarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def unpack(P):
+    P = np.asarray(P, dtype=np.float64)
+    return (P[OW1:OB1].reshape(KH, KIN), P[OB1:OW2], P[OW2:OB2].reshape(KH, KH), P[OB2:OW3], P[OW3:OB3], P[OB3])
+
+
+def mlp_forward(P, F, bf16=True):
+    """out[e] and the activations (H1, H2) -- matrix-core operands rounded to BF16 when asked."""
+    W1, b1, W2, b2, w3, b3 = unpack(P)
+    H1 = np.maximum(_bf16(F, bf16) @ _bf16(W1, bf16).T + b1, 0.0)
+    H2 = np.maximum(_bf16(H1, bf16) @ _bf16(W2, bf16).T + b2, 0.0)
+    out = H2 @ w3 + b3
+    return out, H1, H2
+
+
+def mlp_backward(P, F, go, bf16=True):
+    """d(sum_e go[e] out[e]) / dP, with the kernels' BF16 operand roundings."""
+    W1, b1, W2, b2, w3, b3 = unpack(P)
+    _, H1, H2 = mlp_forward(P, F, bf16)
+    go = np.asarray(go, dtype=np.float64)
+    dH2 = np.where(H2 > 0, go[:, None] * w3[None, :], 0.0)
+    g = np.zeros(NPARAMS)
+    g[OW3:OB3] = go @ H2
+    g[OB3] = go.sum()
+    g[OB2:OW3] = dH2.sum(0)
+    g[OW2:OB2] = (_bf16(dH2, bf16).T @ _bf16(H1, bf16)).ravel()
+    dH1 = np.where(H1 > 0, _bf16(dH2, bf16) @ _bf16(W2, bf16), 0.0)
+    g[OB1:OW2] = dH1.sum(0)
+    g[OW1:OB1] = (_bf16(dH1, bf16).T @ _bf16(F, bf16)).ravel()
+    return g

@@ -1,0 +1,86 @@
+"""Pins the FSAI and edge-MLP oracle (CPU only)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from mpsparse_amd.problems import diffusion_2d
+from oracle import fsai as OF
+
+
+def _small():
+    M = diffusion_2d(7, sigma=1.0, seed=3)
+    A = M.to_scipy()
+    A.sort_indices()
+    return A
+
+
+def test_classical_fsai_defining_property():
+    A = _small()
+    r, c, _ = OF.lower_pattern(A.indptr, A.indices)
+    g = OF.classical_fsai(A)
+    G = OF.G_matrix(A.shape[0], r, c, g)
+    GA = (G @ A).toarray()
+    for i in range(A.shape[0]):
+        J = c[r == i]
+        row = GA[i, J]
+        gii = g[(r == i) & (c == i)][0]
+        assert np.allclose(row[:-1], 0.0, atol=1e-12)
+        assert abs(row[-1] * gii - 1.0) < 1e-12
+    # GAG^T has unit diagonal (the FSAI normalisation)
+    assert np.allclose((G @ A @ G.T).diagonal(), 1.0)
+
+
+def test_classical_fsai_of_a_diagonal_is_inverse_sqrt():
+    d = np.array([4.0, 1.0, 9.0, 2.0])
+    A = sp.diags(d).tocsr()
+    g = OF.classical_fsai(A)
+    assert np.allclose(g, 1 / np.sqrt(d))
+
+
+def test_probe_gradient_matches_finite_differences():
+    A = _small()
+    r, c, _ = OF.lower_pattern(A.indptr, A.indices)
+    rng = np.random.default_rng(0)
+    g = OF.classical_fsai(A) * (1 + 0.1 * rng.standard_normal(r.size))
+    W = rng.standard_normal((2, A.shape[0]))
+    L, grad = OF.probe_loss_grad(A, r, c, g, W)
+    for k in rng.choice(r.size, 12, replace=False):
+        h = 1e-6
+        gp, gm = g.copy(), g.copy()
+        gp[k] += h
+        gm[k] -= h
+        fd = (OF.probe_loss_grad(A, r, c, gp, W)[0] - OF.probe_loss_grad(A, r, c, gm, W)[0]) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-5 * max(1.0, abs(fd))
+
+
+def test_mlp_backward_matches_autograd():
+    import torch
+
+    rng = np.random.default_rng(1)
+    P = rng.standard_normal(OF.NPARAMS) * 0.3
+    F = rng.standard_normal((50, 8))
+    go = rng.standard_normal(50)
+    g = OF.mlp_backward(P, F, go, bf16=False)
+    Pt = torch.tensor(P, dtype=torch.float64, requires_grad=True)
+    W1 = Pt[OF.OW1:OF.OB1].reshape(64, 8)
+    b1 = Pt[OF.OB1:OF.OW2]
+    W2 = Pt[OF.OW2:OF.OB2].reshape(64, 64)
+    b2 = Pt[OF.OB2:OF.OW3]
+    w3 = Pt[OF.OW3:OF.OB3]
+    b3 = Pt[OF.OB3]
+    Ft = torch.tensor(F)
+    out = torch.relu(torch.relu(Ft @ W1.T + b1) @ W2.T + b2) @ w3 + b3
+    (out * torch.tensor(go)).sum().backward()
+    assert np.allclose(g, Pt.grad.numpy(), rtol=1e-10, atol=1e-10)
+    o, _, _ = OF.mlp_forward(P, F, bf16=False)
+    assert np.allclose(o, out.detach().numpy())
+
+
+def test_features_on_the_diagonal():
+    A = _small()
+    f, cs = OF.features(A.indptr, A.indices, A.data)
+    r, c, _ = OF.lower_pattern(A.indptr, A.indices)
+    dg = r == c
+    assert np.allclose(f[dg, 0], 1.0) and np.allclose(f[dg, 1], 0.0) and np.all(f[dg, 4] == 1)
+    assert np.all(f[~dg, 4] == 0) and np.all(f[~dg, 0] < 0)
+    assert np.allclose(cs, 1 / np.sqrt(A.diagonal()[c]))

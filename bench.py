@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the hot path on one MI355X.
 
-Workload (the block-sparse system, n = 1,048,576; 65,536 block-rows of 16x16
-blocks, diagonal block + 8 random block-columns; nnz = 150,994,944; seed 0;
-synthetic, generated here):
+Headline (`value`): time to true relative residual 1e-6 of learned-FSAI PCG
+(BF16 matrix values, FP32 Krylov, FP64 refinement, network set-up included)
+against plain FP32 CG on the same GPU, as the ratio plain-CG seconds /
+learned-PCG seconds, on a held-out 128^3 7-point log-normal diffusion system
+(sigma 2, n = 2,097,152; synthetic, generated on the GPU). Plain CG is the
+faster of FP32 CG and FP64-refined CG. Jacobi-PCG and classical FSAI-PCG are
+reported beside it.
 
-1. COO -> CSR on the GPU from the shuffled packed u64 keys (timed, reported);
-2. CSR SpMV y = A x, FP32 -- the timed steps: `--steps` launches after
-   `--warmup`, bracketed by HIP events on the launch stream. `value` is the
-   SpMV's HBM throughput from algorithmic bytes nnz*(4+4) + 4(n+1) + 4n + 4n;
-3. BiCGSTAB to relative residual 1e-6 (time-to-solution; true residual
-   checked in FP64 on the GPU), and CG on the 64x64 Laplacian;
-4. CPU baseline: scipy CSR matvec / BiCGSTAB on the host cores (oracle leg).
+SpMV part (`spmv`, `roofline`): the block-sparse system, n = 1,048,576 (65,536
+block-rows of 16x16 blocks, diagonal block + 8 random block-columns;
+nnz = 150,994,944; seed 0): COO -> CSR on the GPU from shuffled packed u64 keys,
+then `--steps` CSR SpMV launches after `--warmup`, bracketed by HIP events on
+the launch stream; HBM throughput from algorithmic bytes
+nnz*(4+4) + 4(n+1) + 4n + 4n. Also BiCGSTAB to 1e-6 on it, CG on the 64x64
+Laplacian, BCSR-16, and the 256^3 precision path (FP64 refinement to 1e-10).
+
+CPU baseline: scipy CG on the 128^3 system, a bounded sample of iterations
+on the host, extrapolated to plain CG's iteration count; scipy CSR matvec.
 
 Prints one JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 """
@@ -76,10 +83,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--sections", default="spmv,solve,bcsr,cpu,precision",
-                    help="comma list of spmv, solve, bcsr, cpu, precision, copy "
+    ap.add_argument("--sections", default="fsai,spmv,solve,bcsr,cpu,precision",
+                    help="comma list of fsai, spmv, solve, bcsr, cpu, precision, copy "
                          "(copy: 1 GiB device copy for counter calibration)")
     ap.add_argument("--block-rows", type=int, default=65536, help="block-rows of the block-sparse system")
+    ap.add_argument("--fsai-weights", default=None, help="edge-MLP weights (.npz); default: the packaged ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -97,6 +105,10 @@ def main():
     out = {"metric": METRIC, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded generator, random-init values)"}
+    fsai = None
+    if "fsai" in sections:
+        fsai = fsai_section(dev, args.steps, args.fsai_weights)
+        out["learned_pcg"] = {k: v for k, v in fsai.items() if not k.startswith("_")}
 
     if "copy" in sections:
         nbytes = 1 << 30
@@ -148,6 +160,7 @@ def main():
         # spot check of this y against an FP64 product computed by torch
         y64 = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * x.double()[cols])
         err = float((y.double() - y64).abs().max() / y64.abs().max())
+        out["spmv"] = {"GBps": gbs, "ms": ms, "workload": "block_sparse n=1048576 nnz=150994944"}
         out["value"] = gbs
         out["ms_per_step"] = ms
         traffic = measured_traffic("csr_spmv_block_sparse")
@@ -255,6 +268,42 @@ def main():
     if "precision" in sections:
         out["precision_path"] = precision_section(dev, empty)
 
+    if fsai is not None:
+        lp = fsai["learned_fsai_refined_bf16"]
+        out["value"] = fsai["speedup_learned_bf16_vs_plain_cg"]
+        out["unit"] = "x (plain FP32 CG seconds / learned-FSAI PCG seconds, both to true relres 1e-6)"
+        out["ms_per_step"] = lp["wall_s"] * 1e3
+        out["dtype"] = "bf16 matrix values, f32 Krylov, f64 refinement"
+        out["config"] = {"workload": "learned_fsai_pcg_vs_plain_cg diffusion 128^3 sigma=2",
+                         "n": fsai["n"], "nnz": fsai["nnz"], "tol": 1e-6, "seed": 2024,
+                         "spmv_workload": config}
+        out["time_to_solution_s"] = {"learned_fsai_pcg_bf16": lp["wall_s"],
+                                     "plain_cg": min(fsai["plain_cg_fp32"]["wall_s"],
+                                                     fsai["plain_cg_refined"]["wall_s"])}
+        if "cpu" in sections:
+            import scipy.sparse as sp
+
+            from oracle import krylov as K
+
+            rp5, ci5, cv5, b5 = (t.cpu().numpy() for t in fsai["_csr"])
+            n5 = rp5.size - 1
+            S5 = sp.csr_matrix((cv5, ci5, rp5), shape=(n5, n5))
+            sample_its = 300
+            from threadpoolctl import threadpool_limits
+
+            with threadpool_limits(limits=1):
+                t = time.perf_counter()
+                K.scipy_cg(S5, b5, tol=1e-30, maxiter=sample_its)
+                per_it = (time.perf_counter() - t) / sample_its
+            its_gpu = fsai["plain_cg_fp32"]["iterations"]
+            spmv_cpu = out.get("cpu_baseline", {})
+            out["cpu_baseline"] = {"value": per_it * its_gpu, "unit": "s (plain CG time to relres 1e-6)",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"scipy.sparse.linalg.cg (the reference's CPU path) on the 128^3 "
+                                             f"system, {sample_its} iterations timed ({per_it * 1e3:.1f} ms each), "
+                                             f"extrapolated to the {its_gpu} iterations plain CG needs on the GPU",
+                                   "spmv": spmv_cpu}
+            log("cpu cg", out["cpu_baseline"])
     print(json.dumps(out), flush=True)
 
 
@@ -285,32 +334,164 @@ def stencil_operator64(rp, ci, v, n, nx):
     return apply
 
 
-def fp64_cg_torch(apply, b, tol, maxiter):
-    """Textbook FP64 CG on `apply` (torch only). Returns (x, iterations)."""
+def fp64_cg_torch(apply, b, tol, maxiter, dinv=None):
+    """Textbook FP64 CG (Jacobi-preconditioned when `dinv` = 1/diag is given) on
+    `apply` (torch only); stops on the true-recurrence residual ||r|| <= tol ||b||.
+    Returns (x, iterations)."""
     import torch
 
     bd = b.double()
     x = torch.zeros_like(bd)
     r = bd.clone()
-    p = r.clone()
-    rr = torch.dot(r, r)
+    z = r * dinv if dinv is not None else r
+    p = z.clone()
+    rz = torch.dot(r, z)
     thr = (tol ** 2) * float(torch.dot(bd, bd))
     it = 0
     while it < maxiter:
         q = apply(p)
-        a = rr / torch.dot(p, q)
+        a = rz / torch.dot(p, q)
         x += a * p
         r -= a * q
-        rr_new = torch.dot(r, r)
         it += 1
         if it % 50 == 0:
-            if float(rr_new) <= thr:
+            rr = float(torch.dot(r, r))
+            if rr <= thr:
                 break
             if it % 1000 == 0:
-                log(f"fp64 oracle cg it {it} relres {float(rr_new / torch.dot(bd, bd)) ** 0.5:.2e}")
-        p = r + (rr_new / rr) * p
-        rr = rr_new
+                log(f"fp64 oracle cg it {it} relres {(rr / float(torch.dot(bd, bd))) ** 0.5:.2e}")
+        z = r * dinv if dinv is not None else r
+        rz_new = torch.dot(r, z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
     return x, it
+
+
+def fsai_section(dev, steps, weights=None):
+    """Held-out 128^3 7-point log-normal diffusion (sigma 2): time to true relres
+    1e-6 of plain FP32 CG and of FP64-refined CG / Jacobi-PCG / classical FSAI-PCG
+    / learned FSAI-PCG (FP32 and BF16 matrix values), preconditioner set-up
+    included; every error against a Jacobi-PCG FP64 oracle (torch only)."""
+    import numpy as np
+    import torch
+
+    from mpsparse_amd import learned_fsai as LF
+    from mpsparse_amd import spmv
+
+    side, sigma, seed = 128, 2.0, 2024
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    keys, vals, n = LF.diffusion_keys_torch((side, side, side), sigma, gen, dev)
+    nnz = keys.numel()
+    rp = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    ci = torch.empty(nnz, dtype=torch.int32, device=dev)
+    cv = torch.empty(nnz, dtype=torch.float32, device=dev)
+    A = spmv.csr_tensor(keys, vals, rp, ci, cv, n, n)
+    del keys, vals
+    b64 = torch.randn(n, generator=gen, dtype=torch.float64, device=dev)
+    b32 = b64.float()
+    rows = torch.repeat_interleave(torch.arange(n, device=dev), (rp[1:] - rp[:-1]).long())
+    cols = ci.long()
+    bnorm = float(torch.linalg.vector_norm(b64))
+
+    def relres64(x):
+        ax = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * x.double()[cols])
+        return float(torch.linalg.vector_norm(b64 - ax)) / bnorm
+
+    res = {"workload": f"diffusion 3D 7-point {side}^3 sigma={sigma} (held out: seed {seed}, side outside the "
+                       f"training range)", "n": n, "nnz": nnz, "tol": 1e-6}
+    t = time.perf_counter()
+    op64 = stencil_operator64(rp, ci, cv, n, side)
+    d64 = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(
+        0, rows[rows == cols], cv[rows == cols].double())
+    xo, ito = fp64_cg_torch(op64, b64, 1e-12, 100000, dinv=1.0 / d64)
+    torch.cuda.synchronize()
+    xo_norm = float(torch.linalg.vector_norm(xo))
+    res["fp64_oracle"] = {"solver": "FP64 Jacobi-PCG, torch 7-plane stencil operator", "iterations": ito,
+                          "wall_s": time.perf_counter() - t, "true_relres_fp64": relres64(xo)}
+    log("fsai fp64 oracle", res["fp64_oracle"])
+
+    def err(x):
+        return float(torch.linalg.vector_norm(x.double() - xo)) / xo_norm
+
+    def timed(fn, reps=2):
+        best, out = None, None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best, out
+
+    # plain FP32 CG (the baseline)
+    x32 = torch.zeros(n, device=dev)
+    t_cg, info = timed(lambda: A.cgd(b32, x32, tol=1e-6, maxiter=100000))
+    res["plain_cg_fp32"] = {"wall_s": t_cg, "iterations": info["iterations"], "status": info["status"],
+                            "true_relres_fp64": relres64(x32), "rel_err_vs_fp64_oracle": err(x32)}
+    log("plain cg", res["plain_cg_fp32"])
+    x64 = torch.zeros(n, dtype=torch.float64, device=dev)
+
+    def refined(precond):
+        x64.zero_()
+        return A.solve_refined(b64, x64, tol=1e-6, precond=precond)
+
+    for name, pc in (("plain_cg_refined", "none"), ("jacobi_pcg_refined", "jacobi")):
+        t_r, rinfo = timed(lambda: refined(pc))
+        res[name] = {"wall_s": t_r, "outer": rinfo["outer_iterations"], "iterations": rinfo["inner_iterations"],
+                     "status": rinfo["status"], "true_relres_fp64": relres64(x64),
+                     "rel_err_vs_fp64_oracle": err(x64)}
+        log(name, res[name])
+
+    params = LF.load_params(weights, device=dev) if weights else LF.load_params(device=dev)
+    res["weights"] = os.path.relpath(weights or LF.DEFAULT_WEIGHTS, HERE)
+    feat_ne = None
+    for name, kind, vt in (("classical_fsai_refined", "classical", "f32"),
+                           ("learned_fsai_refined_f32", "learned", "f32"),
+                           ("learned_fsai_refined_bf16", "learned", "bf16")):
+        holder = {}
+
+        def setup():
+            F = LF.fsai_tensor(A)
+            if kind == "classical":
+                holder["fallback_rows"] = F.classical()
+            else:
+                holder["nonpositive_diag"] = F.set_network(params)
+            holder["F"] = F
+            return F
+
+        def solve():
+            F = setup()
+            x64.zero_()
+            return LF.solve_refined(A, F, b64, x64, tol=1e-6, values=vt)
+
+        t_s, _ = timed(setup)
+        t_all, rinfo = timed(solve)
+        F = holder["F"]
+        feat_ne = F.num_edges
+        res[name] = {"wall_s": t_all, "setup_s": t_s, "outer": rinfo["outer_iterations"],
+                     "iterations": rinfo["inner_iterations"], "fallbacks": rinfo["fallbacks"],
+                     "status": rinfo["status"], "true_relres_fp64": relres64(x64),
+                     "rel_err_vs_fp64_oracle": err(x64),
+                     **{k: v for k, v in holder.items() if k != "F"}}
+        log(name, res[name])
+        del F, holder
+    # the edge MLP on this matrix's lower pattern
+    F = LF.fsai_tensor(A)
+    feat, _ = F.features()
+    o = torch.empty(feat.shape[0], device=dev)
+    ms = time_launches(lambda: LF._C.mlp_forward(params, feat, o), steps, 3)
+    flops = feat.shape[0] * 2 * (8 * 64 + 64 * 64 + 64)
+    res["mlp_forward"] = {"edges": int(feat.shape[0]), "ms": ms, "flops_per_edge": 2 * (8 * 64 + 64 * 64 + 64),
+                          "TFLOPs": flops / (ms * 1e-3) / 1e12}
+    log("mlp forward", res["mlp_forward"])
+    base = min(res["plain_cg_fp32"]["wall_s"], res["plain_cg_refined"]["wall_s"])
+    res["speedup_learned_bf16_vs_plain_cg"] = base / res["learned_fsai_refined_bf16"]["wall_s"]
+    res["speedup_learned_bf16_vs_jacobi_pcg"] = (res["jacobi_pcg_refined"]["wall_s"]
+                                                  / res["learned_fsai_refined_bf16"]["wall_s"])
+    res["_csr"] = (rp, ci, cv, b32)  # for the CPU baseline; removed before printing
+    return res
 
 
 def precision_section(dev, empty):

@@ -193,7 +193,9 @@ msp_status msp_fsai_num_edges(msp_fsai F, uint64_t *num_edges);
 /* Edge list of the pattern (row, col), in G's CSR order. */
 msp_status msp_fsai_edges(msp_fsai F, uint32_t *rows, uint32_t *cols, void *stream);
 /* 8 features per edge: a_ij/sqrt(a_ii a_jj), log(a_jj/a_ii), nnz(row i)/8,
- * sum_j|a_ij|/a_ii, [i == j], 0, 0, 0; and the column scaling 1/sqrt(a_jj). */
+ * sum_j|a_ij|/a_ii, [i == j], s_low(i), s_all(i), 0 with
+ * s_low(i) = sum_{j<i} a_ij^2/(a_ii a_jj) and s_all(i) the same over j != i;
+ * and the column scaling 1/sqrt(a_jj). */
 msp_status msp_fsai_features(msp_fsai F, float *feat, float *colscale, void *stream);
 /* Classical FSAI: per row, A[J,J] y = e_last, g = y/sqrt(y_last); rows that
  * cannot be solved become Jacobi rows (counted in fallback_rows). */
@@ -211,6 +213,11 @@ msp_status msp_fsai_probe_loss_grad(msp_fsai F, const float *W, int32_t k, float
  * has a non-positive diagonal entry. */
 msp_status msp_csr_pcg_fsai(msp_csr A, msp_fsai F, const float *b, float *x, const msp_solve_opts *opts,
                             msp_solve_info *info, void *stream);
+/* msp_csr_solve_refined with FSAI-PCG corrections (opts->values selects A's
+ * values inside them; inner_solver and precond are ignored). A correction that
+ * ends without reaching inner_tol is redone with plain CG (info->fallbacks). */
+msp_status msp_csr_solve_refined_fsai(msp_csr A, msp_fsai F, const double *b, double *x, const msp_refine_opts *opts,
+                                      msp_refine_info *info, void *stream);
 
 /* Edge MLP of the learned FSAI: 8 inputs (msp_fsai_features) -> 64 -> 64 -> 1,
  * ReLU, BF16 MFMA with FP32 accumulation. Parameters are msp_mlp_num_params()

@@ -169,8 +169,10 @@ struct Workspace {
   int32_t* hflag = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
-  static constexpr int kNumVec = 12;
-  enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL };
+  static constexpr int kNumVec = 13;
+  // RH and X2 are reserved for the refinement's right-hand side and correction
+  // (no inner solver touches them)
+  enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL, RH };
   float* v(int i) const { return vecs.as<float>() + (size_t)i * npad; }
   Scalars* S_dev() const { return scal.as<Scalars>(); }
   GridRed gr() const {
@@ -1122,19 +1124,15 @@ extern "C" void msp_default_refine_opts(msp_refine_opts* o) {
 
 static bool is_device_ptr_d(const double* p) { return is_device_ptr(static_cast<const void*>(p)); }
 
-extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* x, const msp_refine_opts* opts,
-                                            msp_refine_info* info, void* stream) {
-  MSPCHK(check_handle(A));
-  msp_refine_opts ro;
-  msp_default_refine_opts(&ro);
-  if (opts) ro = *opts;
-  if (A->n != A->m) return fail(MSP_ERR_INVALID, "refinement needs a square matrix");
-  if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
-  if (ro.inner_solver != 0 && ro.inner_solver != 1) return fail(MSP_ERR_INVALID, "inner_solver must be 0 or 1");
-  hipStream_t s = (hipStream_t)stream;
+// FP64 outer loop: r = b - A x in FP64 (FP32 matrix values), correction
+// A d = r / ||r|| by `inner(rhs, d, &info)` in FP32, x += ||r|| d in FP64.
+// Stops at relres <= tol, after two outer steps that fail to halve the
+// residual (stagnated), or after max_outer corrections.
+template <class Inner>
+static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_refine_opts& ro, msp_refine_info* info,
+                              hipStream_t s, Inner&& inner) {
   auto t_start = std::chrono::steady_clock::now();
   MSPCHK(ensure_workspace(A, s));
-  if (ro.values != MSP_VALUES_F32) MSPCHK(msp_csr_prepare_values(A, ro.values, stream));
   const size_t np = A->ws.npad;
   if (!A->w64.p) MSPCHK(A->w64.alloc(sizeof(double) * (3 * np + 2)));
   double* X64 = A->w64.as<double>();
@@ -1149,8 +1147,8 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
                           is_device_ptr_d(x) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
   else
     k_fill64<<<gv, kBlock, 0, s>>>(X64, 0.0, n);
-  float* RHS = A->ws.v(Workspace::W);
-  float* D = A->ws.v(Workspace::Z);
+  float* RHS = A->ws.v(Workspace::RH);
+  float* D = A->ws.v(Workspace::X2);
   GridRed rb = A->ws.gr();
   const RowsCsr rows{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()};
   msp_refine_info inf{};
@@ -1186,28 +1184,9 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
     if (k >= ro.max_outer) break;
     prev = inf.relres;
     k_scale_to_f32<<<gv, kBlock, 0, s>>>(R64, RHS, 1.0 / rnorm, n);
-    msp_solve_opts io;
-    msp_default_opts(&io);
-    io.tol = ro.inner_tol;
-    io.maxiter = ro.inner_maxiter;
-    io.precond = ro.precond;
-    io.verify = 0;
-    io.values = ro.values;
-    msp_solve_info ii{};
-    auto inner = [&](int32_t vt) -> msp_status {
-      io.values = vt;
-      return with_csr_op(A, vt, [&](auto op) {
-        return ro.inner_solver == 0 ? cg_impl(op, RHS, D, io, &ii, s) : bicgstab_impl(op, RHS, D, io, &ii, s);
-      });
-    };
-    MSPCHK(inner(ro.values));
-    inf.inner_iterations += ii.iterations;
-    if (ii.status == MSP_SOLVE_BREAKDOWN && ro.values != MSP_VALUES_F32) {
-      inf.fallbacks += 1;
-      MSPCHK(inner(MSP_VALUES_F32));
-      inf.inner_iterations += ii.iterations;
-    }
-    if (ii.status == MSP_SOLVE_BREAKDOWN && ii.iterations == 0) {
+    bool stop = false;
+    MSPCHK(inner(RHS, D, &inf, &stop));
+    if (stop) {
       inf.status = MSP_SOLVE_BREAKDOWN;
       break;
     }
@@ -1220,6 +1199,50 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
   inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   if (info) *info = inf;
   return MSP_OK;
+}
+
+static msp_solve_opts inner_opts(const msp_refine_opts& ro) {
+  msp_solve_opts io;
+  msp_default_opts(&io);
+  io.tol = ro.inner_tol;
+  io.maxiter = ro.inner_maxiter;
+  io.precond = ro.precond;
+  io.verify = 0;
+  io.values = ro.values;
+  return io;
+}
+
+extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* x, const msp_refine_opts* opts,
+                                            msp_refine_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  msp_refine_opts ro;
+  msp_default_refine_opts(&ro);
+  if (opts) ro = *opts;
+  if (A->n != A->m) return fail(MSP_ERR_INVALID, "refinement needs a square matrix");
+  if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
+  if (ro.inner_solver != 0 && ro.inner_solver != 1) return fail(MSP_ERR_INVALID, "inner_solver must be 0 or 1");
+  hipStream_t s = (hipStream_t)stream;
+  if (ro.values != MSP_VALUES_F32) MSPCHK(msp_csr_prepare_values(A, ro.values, stream));
+  msp_solve_opts io = inner_opts(ro);
+  auto inner = [&](const float* rhs, float* d, msp_refine_info* inf, bool* stop) -> msp_status {
+    msp_solve_info ii{};
+    auto run = [&](int32_t vt) -> msp_status {
+      io.values = vt;
+      return with_csr_op(A, vt, [&](auto op) {
+        return ro.inner_solver == 0 ? cg_impl(op, rhs, d, io, &ii, s) : bicgstab_impl(op, rhs, d, io, &ii, s);
+      });
+    };
+    MSPCHK(run(ro.values));
+    inf->inner_iterations += ii.iterations;
+    if (ii.status == MSP_SOLVE_BREAKDOWN && ro.values != MSP_VALUES_F32) {
+      inf->fallbacks += 1;
+      MSPCHK(run(MSP_VALUES_F32));
+      inf->inner_iterations += ii.iterations;
+    }
+    *stop = ii.status == MSP_SOLVE_BREAKDOWN && ii.iterations == 0;
+    return MSP_OK;
+  };
+  return refine_impl(A, b, x, ro, info, s, inner);
 }
 
 // ---------------------------------------------------------------------------
@@ -1375,13 +1398,14 @@ extern "C" msp_status msp_fsai_set_values(msp_fsai F, const float* g, int32_t* n
   HIPCHK(hipSetDevice(F->device));
   hipStream_t s = (hipStream_t)stream;
   MSPCHK(copy_in(F->G.val.p, g, sizeof(float) * F->ne, s));
-  // count non-positive diagonal entries with the scaling kernel (unit scaling)
-  DevMem cnt, ones;
+  // count non-positive diagonal entries with the scaling kernel (its scaled
+  // output goes to a throw-away buffer; colscale > 0 keeps the signs)
+  DevMem cnt, scaled;
   MSPCHK(cnt.alloc(sizeof(uint32_t)));
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
-  MSPCHK(ones.alloc(sizeof(float) * (F->ne + 1)));
+  MSPCHK(scaled.alloc(sizeof(float) * (F->ne + 1)));
   k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(F->G.val.as<float>(), F->colscale.as<float>(), F->feat.as<float>(),
-                                                  F->ne, ones.as<float>(), cnt.as<uint32_t>());
+                                                  F->ne, scaled.as<float>(), cnt.as<uint32_t>());
   MSPCHK(fsai_commit(F, s));
   uint32_t h = 0;
   HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1562,6 +1586,41 @@ extern "C" msp_status msp_csr_pcg_fsai(msp_csr A, msp_fsai F, const float* b, fl
   msp_status st = with_csr_op(A, o.values, [&](auto op) { return pcg_fsai_impl(op, F, b, x, o, info, s); });
   if (info) info->reserved = 0;
   return st;
+}
+
+// Refinement whose corrections are FSAI-preconditioned CG solves. A correction
+// that breaks down, or that ends without reaching inner_tol, is repeated with
+// plain CG (counted in fallbacks); a G with a non-positive diagonal uses plain
+// CG throughout.
+extern "C" msp_status msp_csr_solve_refined_fsai(msp_csr A, msp_fsai F, const double* b, double* x,
+                                                 const msp_refine_opts* opts, msp_refine_info* info, void* stream) {
+  MSPCHK(check_handle(A));
+  if (!F || F->A != A) return fail(MSP_ERR_INVALID, "FSAI handle does not belong to this matrix");
+  if (!F->ready) return fail(MSP_ERR_INVALID, "FSAI values not set");
+  msp_refine_opts ro;
+  msp_default_refine_opts(&ro);
+  if (opts) ro = *opts;
+  if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
+  hipStream_t s = (hipStream_t)stream;
+  if (ro.values != MSP_VALUES_F32) MSPCHK(msp_csr_prepare_values(A, ro.values, stream));
+  msp_solve_opts io = inner_opts(ro);
+  io.precond = MSP_PRECOND_NONE;
+  auto inner = [&](const float* rhs, float* d, msp_refine_info* inf, bool* stop) -> msp_status {
+    msp_solve_info ii{};
+    const bool use_fsai = F->nonpos == 0;
+    if (use_fsai)
+      MSPCHK(with_csr_op(A, io.values, [&](auto op) { return pcg_fsai_impl(op, F, rhs, d, io, &ii, s); }));
+    inf->inner_iterations += ii.iterations;
+    const bool reached = ii.status == MSP_SOLVE_CONVERGED;
+    if (!use_fsai || !reached) {
+      if (use_fsai) inf->fallbacks += 1;
+      MSPCHK(with_csr_op(A, io.values, [&](auto op) { return cg_impl(op, rhs, d, io, &ii, s); }));
+      inf->inner_iterations += ii.iterations;
+    }
+    *stop = ii.status == MSP_SOLVE_BREAKDOWN && ii.iterations == 0;
+    return MSP_OK;
+  };
+  return refine_impl(A, b, x, ro, info, s, inner);
 }
 
 // ---------------------------------------------------------------------------

@@ -146,7 +146,11 @@ __global__ __launch_bounds__(kBlock) void k_fsai_classical(const uint32_t* __res
 
 // Edge features of the lower pattern, 8 floats per edge:
 // [a_ij / sqrt(a_ii a_jj), log(a_jj / a_ii), nnz(row i) / 8, sum_j |a_ij| / a_ii,
-//  [j == i], 0, 0, 0]; and s[e] = 1 / sqrt(a_jj) (the column scaling of G).
+//  [j == i], s_low(i), s_all(i), 0] with the row couplings
+//  s_low(i) = sum_{j < i} a_ij^2 / (a_ii a_jj), s_all(i) = sum_{j != i} a_ij^2 / (a_ii a_jj)
+// (for an arrow-shaped A[J,J] classical FSAI is g_ii sqrt(a_ii) = 1/sqrt(1 - s_low),
+// g_ij sqrt(a_jj) = -f0 / sqrt(1 - s_low)); and s[e] = 1 / sqrt(a_jj) (the column
+// scaling of G).
 __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __restrict__ row_ptr,
                                                           const uint32_t* __restrict__ col,
                                                           const float* __restrict__ val, uint32_t n,
@@ -156,8 +160,18 @@ __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __rest
                                                           float* __restrict__ feat, float* __restrict__ colscale) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float aii = diag[i];
-    float asum = 0.0f;
-    for (uint32_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j) asum += fabsf(val[j]);
+    float asum = 0.0f, s_low = 0.0f, s_all = 0.0f;
+    for (uint32_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j) {
+      const float a = val[j];
+      const uint32_t c = col[j];
+      asum += fabsf(a);
+      const float dc = diag[c];
+      if (c != i && aii > 0.0f && dc > 0.0f) {
+        const float q = a * a / (aii * dc);
+        s_all += q;
+        if (c < i) s_low += q;
+      }
+    }
     const float rn = (float)(row_ptr[i + 1] - row_ptr[i]) * 0.125f;
     for (uint32_t e = Lrp[i]; e < Lrp[i + 1]; ++e) {
       const uint32_t c = Lcol[e];
@@ -170,8 +184,8 @@ __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __rest
       f[2] = rn;
       f[3] = ok ? asum / aii : 0.0f;
       f[4] = (c == i) ? 1.0f : 0.0f;
-      f[5] = 0.0f;
-      f[6] = 0.0f;
+      f[5] = s_low;
+      f[6] = s_all;
       f[7] = 0.0f;
       colscale[e] = ajj > 0.0f ? rsqrtf(ajj) : 0.0f;
     }

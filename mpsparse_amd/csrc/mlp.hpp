@@ -100,7 +100,7 @@ __device__ __forceinline__ void forward_tile(const float* __restrict__ feat, uin
 #pragma unroll
       for (int i = 0; i < 4; ++i) sh[(8 * j + 4 * h + i) * 68 + 32 * t + c] = bf16_bits(H1[t][4 * j + i]);
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
   for (int t = 0; t < 2; ++t) H2[t] = zero16();
 #pragma unroll
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_backward(const float* __restrict
 #pragma unroll
         for (int i = 0; i < 4; ++i) sh[(8 * j + 4 * h + i) * 68 + 32 * t + c] = bf16_bits(dH2[t][4 * j + i]);
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     f16 dH1[2] = {zero16(), zero16()};
 #pragma unroll
     for (int q = 0; q < 8; ++q) {

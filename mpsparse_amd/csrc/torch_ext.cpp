@@ -71,6 +71,20 @@ py::dict info_dict(const msp_solve_info& in) {
   return d;
 }
 
+py::dict refine_dict(const msp_refine_info& info) {
+  py::dict d;
+  d["outer_iterations"] = info.outer_iterations;
+  d["inner_iterations"] = info.inner_iterations;
+  d["status"] = info.status == MSP_SOLVE_CONVERGED   ? "converged"
+                : info.status == MSP_SOLVE_BREAKDOWN ? "breakdown"
+                : info.status == MSP_SOLVE_STAGNATED ? "stagnated"
+                                                     : "maxiter";
+  d["fallbacks"] = info.fallbacks;
+  d["relres"] = info.relres;
+  d["seconds"] = info.seconds;
+  return d;
+}
+
 class CsrTensor {
  public:
   CsrTensor(torch::Tensor keys, torch::Tensor values, torch::Tensor row_ptr, torch::Tensor col_ind,
@@ -219,17 +233,7 @@ class CsrTensor {
       py::gil_scoped_release nogil;
       check(msp_csr_solve_refined(h_, b.data_ptr<double>(), x.data_ptr<double>(), &o, &info, s), "solve_refined");
     }
-    py::dict d;
-    d["outer_iterations"] = info.outer_iterations;
-    d["inner_iterations"] = info.inner_iterations;
-    d["status"] = info.status == MSP_SOLVE_CONVERGED   ? "converged"
-                  : info.status == MSP_SOLVE_BREAKDOWN ? "breakdown"
-                  : info.status == MSP_SOLVE_STAGNATED ? "stagnated"
-                                                       : "maxiter";
-    d["fallbacks"] = info.fallbacks;
-    d["relres"] = info.relres;
-    d["seconds"] = info.seconds;
-    return d;
+    return refine_dict(info);
   }
 
   int64_t num_rows() const {
@@ -387,6 +391,31 @@ py::dict pcg_fsai(CsrTensor& A, FsaiTensor& F, torch::Tensor b, torch::Tensor x,
   py::dict d = info_dict(info);
   d["fell_back_to_cg"] = info.reserved == 1;
   return d;
+}
+
+py::dict pcg_fsai_refined(CsrTensor& A, FsaiTensor& F, torch::Tensor b, torch::Tensor x, double tol, double inner_tol,
+                          int64_t max_outer, int64_t inner_maxiter, const std::string& values, bool use_x0) {
+  TORCH_CHECK(F.matrix() == A.raw(), "the FSAI preconditioner was built for another matrix");
+  const int64_t n = A.num_rows();
+  expect(b, torch::kFloat64, n, "b");
+  expect(x, torch::kFloat64, n, "x");
+  msp_refine_opts o;
+  msp_default_refine_opts(&o);
+  o.tol = tol;
+  o.inner_tol = inner_tol;
+  o.max_outer = (int32_t)max_outer;
+  o.inner_maxiter = (int32_t)inner_maxiter;
+  o.values = value_type_of(values);
+  o.use_x0 = use_x0 ? 1 : 0;
+  msp_refine_info info{};
+  c10::hip::HIPGuard guard((c10::DeviceIndex)A.device());
+  void* s = stream_of(A.device());
+  {
+    py::gil_scoped_release nogil;
+    check(msp_csr_solve_refined_fsai(A.raw(), F.raw(), b.data_ptr<double>(), x.data_ptr<double>(), &o, &info, s),
+          "pcg_fsai_refined");
+  }
+  return refine_dict(info);
 }
 
 // bcsr.bcsr_tensor(keys, values, row_ptr, col_ind, num_rows, num_cols, block_size)
@@ -595,6 +624,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("pcg_fsai", &pcg_fsai, "FSAI-preconditioned CG", py::arg("A"), py::arg("F"), py::arg("b"), py::arg("x"),
         py::arg("tol") = 1e-6, py::arg("maxiter") = 10000, py::arg("check_every") = 16, py::arg("use_x0") = false,
         py::arg("values") = "f32", py::arg("verify") = true);
+  m.def("pcg_fsai_refined", &pcg_fsai_refined,
+        "FP64 iterative refinement around FSAI-PCG corrections (b, x float64)", py::arg("A"), py::arg("F"),
+        py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("inner_tol") = 1e-3, py::arg("max_outer") = 30,
+        py::arg("inner_maxiter") = 10000, py::arg("values") = "f32", py::arg("use_x0") = false);
   m.def("device_count", []() { return msp_device_count(); });
   m.def("coo_csr", &coo_csr, "COO -> CSR (stable packed-key sort)", py::arg("keys"), py::arg("values"),
         py::arg("row_ptr"), py::arg("col_ind"), py::arg("out_vals"), py::arg("num_rows"), py::arg("num_cols"));

@@ -17,11 +17,13 @@ from __future__ import annotations
 
 import math
 import os
+import sys
 import time
 
 import numpy as np
 
 from . import load_ext
+from .problems import diffusion_stencil
 
 _C = load_ext()
 fsai_tensor = _C.fsai_tensor
@@ -75,58 +77,36 @@ def solve(A, F, b, x, **kw):
     return _C.pcg_fsai(A, F, b, x, **kw)
 
 
+def solve_refined(A, F, b, x, **kw):
+    """FP64 iterative refinement (b, x float64) around FSAI-PCG corrections in
+    FP32 Krylov arithmetic; ``values`` = "f32" / "bf16" / "f16" selects A's
+    stored values inside the corrections. Default target: relres 1e-6."""
+    return _C.pcg_fsai_refined(A, F, b, x, **kw)
+
+
 # ---------------------------------------------------------------------------
 # training data on the GPU
 def diffusion_keys_torch(shape, sigma, gen, device="cuda"):
-    """Packed keys + FP32 values of a log-normal diffusion matrix (the 2D / 3D
-    finite-volume operator of problems.diffusion_3d), generated with torch on
-    the device. shape = (nz, ny, nx); nz = 1 gives the 5-point 2D operator."""
+    """Packed keys + FP32 values of a log-normal diffusion matrix
+    (problems.diffusion_stencil), generated with torch on ``device``.
+    shape = (nz, ny, nx); nz = 1 gives the 5-point 2D operator."""
     import torch
 
-    nz, ny, nx = shape
-    n = nx * ny * nz
-    k = torch.exp(sigma * torch.randn((nz, ny, nx), generator=gen, device=device, dtype=torch.float64))
-    idx = torch.arange(n, device=device, dtype=torch.int64).reshape(nz, ny, nx)
-    cols = torch.full((nz, ny, nx, 7), -1, dtype=torch.int64, device=device)
-    vals = torch.zeros((nz, ny, nx, 7), dtype=torch.float64, device=device)
-    diag = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+    k = torch.exp(sigma * torch.randn(shape, generator=gen, device=device, dtype=torch.float64))
+    return diffusion_keys_from_k(k)
 
-    def harm(a, b):
-        return 2.0 * a * b / (a + b)
 
-    cx = harm(k[:, :, 1:], k[:, :, :-1])
-    diag[:, :, 1:] += cx
-    diag[:, :, :-1] += cx
-    diag[:, :, 0] += 2.0 * k[:, :, 0]
-    diag[:, :, -1] += 2.0 * k[:, :, -1]
-    cols[:, :, 1:, 2] = idx[:, :, :-1]
-    vals[:, :, 1:, 2] = -cx
-    cols[:, :, :-1, 4] = idx[:, :, 1:]
-    vals[:, :, :-1, 4] = -cx
-    cy = harm(k[:, 1:, :], k[:, :-1, :])
-    diag[:, 1:, :] += cy
-    diag[:, :-1, :] += cy
-    diag[:, 0, :] += 2.0 * k[:, 0, :]
-    diag[:, -1, :] += 2.0 * k[:, -1, :]
-    cols[:, 1:, :, 1] = idx[:, :-1, :]
-    vals[:, 1:, :, 1] = -cy
-    cols[:, :-1, :, 5] = idx[:, 1:, :]
-    vals[:, :-1, :, 5] = -cy
-    if nz > 1:
-        cz = harm(k[1:, :, :], k[:-1, :, :])
-        diag[1:, :, :] += cz
-        diag[:-1, :, :] += cz
-        cols[1:, :, :, 0] = idx[:-1, :, :]
-        vals[1:, :, :, 0] = -cz
-        cols[:-1, :, :, 6] = idx[1:, :, :]
-        vals[:-1, :, :, 6] = -cz
-        diag[0, :, :] += 2.0 * k[0, :, :]
-        diag[-1, :, :] += 2.0 * k[-1, :, :]
-    cols[..., 3] = idx
-    vals[..., 3] = diag
-    cols = cols.reshape(n, 7)
-    vals = vals.reshape(n, 7)
-    rows = torch.arange(n, device=device, dtype=torch.int64)[:, None].expand(n, 7)
+def diffusion_keys_from_k(k):
+    """(keys int64[nnz], vals float32[nnz], n) of the diffusion operator of a
+    torch conductivity array k (nz, ny, nx), in row-major stencil order."""
+    import torch
+
+    dev = k.device
+    cols, vals = diffusion_stencil(k, lambda s, f: torch.full(s, f, dtype=torch.int64, device=dev),
+                                   lambda s: torch.zeros(s, dtype=torch.float64, device=dev),
+                                   lambda m: torch.arange(m, dtype=torch.int64, device=dev))
+    n = cols.shape[0]
+    rows = torch.arange(n, device=dev, dtype=torch.int64)[:, None].expand(n, 7)
     m = cols >= 0
     keys = (rows[m] << 32) | cols[m]
     return keys.contiguous(), vals[m].float().contiguous(), n
@@ -159,9 +139,55 @@ def harvest_probes(A, n, gen, device="cuda", iters=(5, 20, 80)):
     return (W / torch.linalg.vector_norm(W, dim=1, keepdim=True)).contiguous()
 
 
-def train(params=None, matrices=2000, updates_per_matrix=2, lr=2e-3, seed=0, device="cuda", log=print,
-          sigma_range=(0.5, 2.5)):
-    """Train the edge MLP on freshly generated matrices; returns (params, history)."""
+def imitation_target(F):
+    """Network output that reproduces classical FSAI: o_e = g_e * sqrt(a_jj)."""
+    F.classical()
+    g = F.values()
+    _, cs = F.features()
+    return g / cs
+
+
+class Validator:
+    """FSAI-PCG iterations to relres 1e-6 on a fixed matrix (3D, sigma 2)."""
+
+    def __init__(self, side=40, sigma=2.0, seed=999, device="cuda"):
+        import torch
+
+        from . import spmv
+
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed)
+        keys, vals, n = diffusion_keys_torch((side, side, side), sigma, gen, device)
+        empty = torch.empty(0)
+        self.A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
+        self.b = torch.randn(n, generator=gen, device=device)
+        self.F = fsai_tensor(self.A)
+        self.n = n
+        x = torch.zeros(n, device=device)
+        self.jacobi = self.A.cgd(self.b, x, tol=1e-6, precond="jacobi")["iterations"]
+        self.F.classical()
+        x.zero_()
+        self.classical = _C.pcg_fsai(self.A, self.F, self.b, x, tol=1e-6)["iterations"]
+
+    def __call__(self, params):
+        import torch
+
+        nonpos = self.F.set_network(params)
+        x = torch.zeros(self.n, device=self.b.device)
+        info = _C.pcg_fsai(self.A, self.F, self.b, x, tol=1e-6, maxiter=5000)
+        return info["iterations"] if (nonpos == 0 and info["status"] == "converged") else 10 ** 9
+
+
+def train(params=None, matrices=2000, imitate=300, updates_per_matrix=2, lr=2e-3, seed=0, device="cuda", log=print,
+          sigma_range=(0.5, 2.5), validate_every=100):
+    """Train the edge MLP on freshly generated matrices.
+
+    Matrices 0 .. imitate-1: regression of the output onto classical FSAI
+    (mean squared error over the edges); the rest: the probe loss
+    sum_w ||G A G^T w - w||^2 / |W| with CG-residual probes. Every
+    `validate_every` matrices the FSAI-PCG iteration count on a fixed held-out
+    matrix is measured; the parameters with the fewest iterations are returned.
+    Returns (params, history dict)."""
     import torch
 
     from . import spmv
@@ -174,7 +200,10 @@ def train(params=None, matrices=2000, updates_per_matrix=2, lr=2e-3, seed=0, dev
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
-    hist = []
+    val = Validator(device=device)
+    hist = {"loss": [], "validation": [], "jacobi_its": val.jacobi, "classical_its": val.classical}
+    log(f"[train] validation matrix n={val.n}: jacobi {val.jacobi} its, classical FSAI {val.classical} its")
+    best = (val(params), params.clone())
     step = 0
     t0 = time.time()
     empty = torch.empty(0)
@@ -185,16 +214,76 @@ def train(params=None, matrices=2000, updates_per_matrix=2, lr=2e-3, seed=0, dev
         A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
         del keys, vals
         F = fsai_tensor(A)
-        W = harvest_probes(A, n, gen, device)
-        for _ in range(updates_per_matrix):
-            loss = F.train_step(params, W, grad) / W.shape[0]
-            step += 1
-            lr_t = lr * 0.5 * (1 + math.cos(math.pi * min(1.0, k / max(1, matrices))))
-            _C.mlp_adam(params, grad, m, v, lr=lr_t, step=step)
-            hist.append((k, n, sigma, float(loss)))
-        if k % 50 == 0 or k == matrices - 1:
-            recent = [h[3] for h in hist[-100:]]
-            log(f"[train] matrix {k}/{matrices} n={n} sigma={sigma:.2f} loss={np.mean(recent):.4f} "
+        if k < imitate:
+            feat, _ = F.features()
+            target = imitation_target(F)
+            out = torch.empty_like(target)
+            ne = target.numel()
+            for _ in range(updates_per_matrix):
+                _C.mlp_forward(params, feat, out)
+                d = out - target
+                loss = float(torch.dot(d, d)) / ne
+                _C.mlp_backward(params, feat, (2.0 / ne) * d, grad)
+                step += 1
+                _C.mlp_adam(params, grad, m, v, lr=lr, step=step)
+                hist["loss"].append(("imitate", k, n, sigma, loss))
+            del feat, target, out
+        else:
+            W = harvest_probes(A, n, gen, device)
+            kk = k - imitate
+            lr_t = lr * 0.5 * (1 + math.cos(math.pi * kk / max(1, matrices - imitate)))
+            for _ in range(updates_per_matrix):
+                loss = F.train_step(params, W, grad) / W.shape[0]
+                step += 1
+                _C.mlp_adam(params, grad, m, v, lr=lr_t, step=step)
+                hist["loss"].append(("probe", k, n, sigma, float(loss)))
+            del W
+        del F, A
+        if (k + 1) % validate_every == 0 or k == matrices - 1:
+            its = val(params)
+            hist["validation"].append((k + 1, its))
+            if its <= best[0]:
+                best = (its, params.clone())
+            recent = [h[4] for h in hist["loss"][-100:]]
+            log(f"[train] matrix {k + 1}/{matrices} ({hist['loss'][-1][0]}) loss={np.mean(recent):.4g} "
+                f"validation its={its} (best {best[0]}, classical {val.classical}, jacobi {val.jacobi}) "
                 f"elapsed {time.time() - t0:.0f}s")
-        del F, A, W
-    return params, hist
+    hist["best_validation_its"] = best[0]
+    del val
+    return best[1], hist
+
+
+def main(argv=None):
+    import argparse
+
+    ap = argparse.ArgumentParser(description="train the learned-FSAI edge MLP on generated diffusion matrices")
+    ap.add_argument("--matrices", type=int, default=2000)
+    ap.add_argument("--imitate", type=int, default=300, help="matrices of regression onto classical FSAI first")
+    ap.add_argument("--updates", type=int, default=2, help="Adam steps per matrix")
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--init", default=None, help="start from these weights")
+    a = ap.parse_args(argv)
+    import torch
+
+    P = load_params(a.init) if a.init else None
+    P, hist = train(P, matrices=a.matrices, imitate=a.imitate, updates_per_matrix=a.updates, lr=a.lr, seed=a.seed,
+                    log=lambda s: print(s, flush=True))
+    torch.cuda.synchronize()
+    save_params(P, a.out, {"matrices": a.matrices, "imitate": a.imitate, "updates": a.updates, "lr": a.lr,
+                           "seed": a.seed, "validation": hist["validation"],
+                           "best_validation_its": hist["best_validation_its"],
+                           "classical_its": hist["classical_its"], "jacobi_its": hist["jacobi_its"]})
+    print(f"saved {a.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
+    # After a training run (thousands of matrix and preconditioner handles
+    # created and destroyed) the process has ended with SIGSEGV inside native
+    # library teardown, after Python finalisation and after the weights were
+    # written; skip that teardown once everything is flushed.
+    sys.stdout.flush()
+    sys.stderr.flush()
+    os._exit(0)

@@ -243,25 +243,20 @@ def _harmonic(a, b):
     return 2.0 * a * b / (a + b)
 
 
-def diffusion_3d(nx: int, ny: int | None = None, nz: int | None = None, sigma: float = 1.0,
-                 seed: int = 1) -> CsrMatrix:
-    """7-point variable-coefficient diffusion (SPD), sorted CSR.
-
-    Cell (z, y, x) has row ``(z*ny + y)*nx + x``; conductivities
-    ``k = exp(sigma * N(0,1))`` drawn as a (nz, ny, nx) array from
-    ``default_rng(seed)``. Interior face coefficient: harmonic mean of the two
-    cells. A Dirichlet boundary face adds ``2*k`` to the diagonal. Within a row
-    the columns are in increasing order (-z, -y, -x, diag, +x, +y, +z).
-    """
-    ny = nx if ny is None else ny
-    nz = nx if nz is None else nz
-    rng = np.random.default_rng(seed)
-    k = _lognormal_k((nz, ny, nx), sigma, rng)
+def diffusion_stencil(k, new_int, new_float, arange):
+    """(cols[n, 7], vals[n, 7]) of the finite-volume diffusion operator for a
+    (nz, ny, nx) conductivity array ``k`` (FP64). Interior face coefficient:
+    harmonic mean of the two cells; a Dirichlet boundary face adds ``2*k`` to
+    the diagonal; a single plane (nz = 1) gives the 2D 5-point operator. Slots
+    are (-z, -y, -x, diag, +x, +y, +z); absent neighbours have column -1.
+    Works on numpy arrays or torch tensors: ``new_int(shape, fill)``,
+    ``new_float(shape)`` and ``arange(n)`` create arrays of the same kind."""
+    nz, ny, nx = k.shape
     n = nx * ny * nz
-    idx = np.arange(n, dtype=np.int64).reshape(nz, ny, nx)
-    cols = np.full((nz, ny, nx, 7), -1, dtype=np.int64)
-    vals = np.zeros((nz, ny, nx, 7), dtype=np.float64)
-    diag = np.zeros((nz, ny, nx), dtype=np.float64)
+    idx = arange(n).reshape(nz, ny, nx)
+    cols = new_int((nz, ny, nx, 7), -1)
+    vals = new_float((nz, ny, nx, 7))
+    diag = new_float((nz, ny, nx))
     # x faces
     cx = _harmonic(k[:, :, 1:], k[:, :, :-1])
     diag[:, :, 1:] += cx
@@ -298,9 +293,29 @@ def diffusion_3d(nx: int, ny: int | None = None, nz: int | None = None, sigma: f
         diag[-1, :, :] += 2.0 * k[-1, :, :]
     cols[..., 3] = idx
     vals[..., 3] = diag
+    return cols.reshape(n, 7), vals.reshape(n, 7)
+
+
+def diffusion_3d(nx: int, ny: int | None = None, nz: int | None = None, sigma: float = 1.0,
+                 seed: int = 1) -> CsrMatrix:
+    """7-point variable-coefficient diffusion (SPD), sorted CSR.
+
+    Cell (z, y, x) has row ``(z*ny + y)*nx + x``; conductivities
+    ``k = exp(sigma * N(0,1))`` drawn as a (nz, ny, nx) array from
+    ``default_rng(seed)``; the operator is :func:`diffusion_stencil`. Within a
+    row the columns are in increasing order (-z, -y, -x, diag, +x, +y, +z).
+    """
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    rng = np.random.default_rng(seed)
+    k = _lognormal_k((nz, ny, nx), sigma, rng)
+    n = nx * ny * nz
+    cols, vals = diffusion_stencil(k, lambda s, f: np.full(s, f, dtype=np.int64),
+                                   lambda s: np.zeros(s, dtype=np.float64),
+                                   lambda m: np.arange(m, dtype=np.int64))
     meta = {"kind": "diffusion_3d" if nz > 1 else "diffusion_2d", "nx": nx, "ny": ny, "nz": nz,
             "sigma": sigma, "seed": seed}
-    return _stencil_to_csr(cols.reshape(n, 7), vals.reshape(n, 7), n, meta)
+    return _stencil_to_csr(cols, vals, n, meta)
 
 
 def diffusion_2d(nx: int, ny: int | None = None, sigma: float = 1.5, seed: int = 0) -> CsrMatrix:

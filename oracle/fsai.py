@@ -4,7 +4,9 @@ The reference has no preconditioner; these restate the library's learned-FSAI
 additions from their definitions so the GPU kernels can be checked:
 
 * :func:`lower_pattern`, :func:`features` -- the lower-triangle pattern of A in
-  CSR order and the 8 per-edge features;
+  CSR order and the 8 per-edge features (the row couplings s_low / s_all are pinned
+  by the closed form of classical FSAI on stencil matrices, whose lower neighbours
+  are not coupled to each other: g_ii sqrt(a_ii) = 1 / sqrt(1 - s_low));
 * :func:`classical_fsai` -- per row i: A[J,J] y = e_i, g = y / sqrt(y_i)
   (Kolotilina-Yeremin FSAI); pinned by its defining property (G A)[i, J] =
   e_i / sqrt(y_i) and by the diagonal case G = D^{-1/2};
@@ -64,6 +66,15 @@ def features(row_ptr, col_ind, vals):
     f[:, 2] = np.diff(row_ptr)[r] / 8.0
     f[:, 3] = asum[r] / d[r]
     f[:, 4] = (r == c).astype(np.float64)
+    cols_all = np.asarray(col_ind, dtype=np.int64)
+    off = cols_all != rows_all
+    q = np.where(off, vals ** 2 / (d[rows_all] * d[cols_all]), 0.0)
+    s_all = np.zeros(n)
+    np.add.at(s_all, rows_all, q)
+    s_low = np.zeros(n)
+    np.add.at(s_low, rows_all, np.where(cols_all < rows_all, q, 0.0))
+    f[:, 5] = s_low[r]
+    f[:, 6] = s_all[r]
     return f, 1.0 / np.sqrt(d[c])
 
 

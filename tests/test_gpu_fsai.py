@@ -1,0 +1,320 @@
+"""GPU FSAI preconditioner and BF16-MFMA edge MLP against the oracle (oracle/fsai.py).
+
+Bars:
+* lower pattern / edges: exact; features: |d| <= 2e-6 * (1 + |ref|) (FP32 device
+  arithmetic on FP32 matrix values vs FP64 on the same values);
+* classical FSAI (FP64 per-row Cholesky on the device, stored FP32): |d| <= 1e-5 * max|g|
+  per matrix; rows with more than 8 lower entries are Jacobi rows, counted;
+* M = G^T G apply and the probe loss / gradient: relative 2-norm error <= 1e-5 (apply)
+  and 1e-4 (loss, gradient; three chained FP32 SpMVs each way);
+* MLP forward vs the oracle with the same BF16 operand roundings:
+  |d| <= 1e-3 * max(1, |ref|); backward: per parameter block relative 2-norm <= 2e-3;
+* FSAI-PCG: converges to relres 1e-6 (FP64 true residual) and its iteration count is
+  within 15% + 2 of FP64 PCG with the same G; the fallback runs plain CG.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from gpu_util import csr_from_arrays, say, t_f32
+from mpsparse_amd.problems import diffusion_2d, diffusion_3d
+from oracle import fsai as OF
+from oracle import krylov as K
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _handles(M):
+    from mpsparse_amd.learned_fsai import fsai_tensor
+
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    return A, fsai_tensor(A)
+
+
+def _irregular_spd(n=300, seed=5):
+    """Symmetric, diagonally dominant, with some rows holding more than 8 lower entries."""
+    rng = np.random.default_rng(seed)
+    R = sp.random(n, n, density=0.02, random_state=rng, data_rvs=lambda k: -rng.uniform(0.1, 1.0, k))
+    R = sp.tril(R, -1)
+    hub = rng.choice(n // 2, 12, replace=False) + n // 2  # rows with many lower entries
+    for i in hub:
+        cols = rng.choice(i, 10, replace=False)
+        R = R + sp.csr_matrix((-rng.uniform(0.1, 1.0, 10), (np.full(10, i), cols)), shape=(n, n))
+    S = (R + R.T).tocsr()
+    S.sum_duplicates()
+    d = np.asarray(abs(S).sum(axis=1)).ravel() + 1.0
+    S = (S + sp.diags(d)).tocsr().astype(np.float32)
+    S.sort_indices()
+
+    class M:
+        pass
+
+    M.row_ptr, M.col_ind, M.vals = S.indptr.astype(np.uint32), S.indices.astype(np.uint32), S.data
+    M.num_rows = M.num_cols = n
+    return M
+
+
+def _scipy(M):
+    return sp.csr_matrix((M.vals.astype(np.float64), M.col_ind.astype(np.int64), M.row_ptr.astype(np.int64)),
+                         shape=(M.num_rows, M.num_cols))
+
+
+CASES = {
+    "diffusion3d_12_s1.5": lambda: diffusion_3d(12, sigma=1.5, seed=2),
+    "diffusion2d_30_s2": lambda: diffusion_2d(30, sigma=2.0, seed=4),
+    "irregular_300": _irregular_spd,
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_edges_and_features(name):
+    M = CASES[name]()
+    say("fsai edges/features", name)
+    A, F = _handles(M)
+    r_ref, c_ref, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    r, c = (t.cpu().numpy() for t in F.edges())
+    assert F.num_edges == r_ref.size
+    assert np.array_equal(r, r_ref) and np.array_equal(c, c_ref)
+    f, cs = (t.cpu().numpy().astype(np.float64) for t in F.features())
+    f_ref, cs_ref = OF.features(M.row_ptr, M.col_ind, M.vals)
+    assert np.all(np.abs(f - f_ref) <= 2e-6 * (1 + np.abs(f_ref))), np.abs(f - f_ref).max()
+    assert np.all(np.abs(cs - cs_ref) <= 2e-6 * cs_ref)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_classical_fsai(name):
+    M = CASES[name]()
+    say("fsai classical", name)
+    A, F = _handles(M)
+    fallback = F.classical()
+    g = F.values().cpu().numpy().astype(np.float64)
+    S = _scipy(M)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    per_row = np.bincount(r, minlength=M.num_rows)
+    big = per_row > 8
+    assert fallback == int(big.sum())
+    g_ref = OF.classical_fsai(S)
+    d = S.diagonal()
+    jac = np.where(r == c, 1.0 / np.sqrt(d[r]), 0.0)
+    g_ref = np.where(big[r], jac, g_ref)
+    assert np.max(np.abs(g - g_ref)) <= 1e-5 * np.max(np.abs(g_ref)), np.max(np.abs(g - g_ref))
+    # defining property on the non-fallback rows: (G A)[i, J_i] = e_i / sqrt(y_i), i.e. zero off the diagonal
+    G = OF.G_matrix(M.num_rows, r, c, g)
+    GA = (G @ S).tocsr()
+    for i in np.nonzero(~big)[0][:50]:
+        J = c[r == i]
+        row = GA[i, J].toarray().ravel()
+        assert np.all(np.abs(row[:-1]) <= 1e-5 * abs(row[-1])), (i, row)
+
+
+def test_apply_is_gt_g():
+    torch = _torch()
+    M = diffusion_3d(12, sigma=1.5, seed=2)
+    say("fsai apply")
+    A, F = _handles(M)
+    F.classical()
+    g = F.values().cpu().numpy().astype(np.float64)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    G = OF.G_matrix(M.num_rows, r, c, g)
+    v = np.random.default_rng(1).standard_normal(M.num_rows).astype(np.float32)
+    z = torch.empty(M.num_rows, device="cuda")
+    F.apply(t_f32(v), z)
+    ref = G.T @ (G @ v.astype(np.float64))
+    assert np.linalg.norm(z.cpu().numpy() - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+def test_probe_loss_grad_matches_oracle():
+    torch = _torch()
+    M = diffusion_2d(20, sigma=1.0, seed=7)
+    say("fsai probe loss/grad")
+    A, F = _handles(M)
+    F.classical()
+    rng = np.random.default_rng(3)
+    g = F.values().cpu().numpy() * (1 + 0.1 * rng.standard_normal(F.num_edges)).astype(np.float32)
+    assert F.set_values(t_f32(g)) == 0
+    W = rng.standard_normal((3, M.num_rows)).astype(np.float32)
+    grad = torch.zeros(F.num_edges, device="cuda")
+    loss = F.probe_loss_grad(t_f32(W), grad)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    loss_ref, grad_ref = OF.probe_loss_grad(_scipy(M), r, c, g.astype(np.float64), W)
+    assert abs(loss - loss_ref) <= 1e-4 * loss_ref, (loss, loss_ref)
+    gg = grad.cpu().numpy()
+    assert np.linalg.norm(gg - grad_ref) <= 1e-4 * np.linalg.norm(grad_ref)
+
+
+def _params(seed, scale=1.0):
+    from mpsparse_amd.learned_fsai import init_params
+
+    P = init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 100)
+    P[OF.OB1:OF.OW2] = 0.1 * rng.standard_normal(64)
+    P[OF.OB2:OF.OW3] = 0.1 * rng.standard_normal(64)
+    P[OF.OW3:OF.OB3] = scale * rng.standard_normal(64) / 8
+    return P.astype(np.float32)
+
+
+def _feat_cases():
+    M = diffusion_3d(12, sigma=1.5, seed=2)
+    f, _ = OF.features(M.row_ptr, M.col_ind, M.vals)
+    rng = np.random.default_rng(9)
+    return {
+        "matrix": f.astype(np.float32),
+        "random_ragged": rng.standard_normal((1000 * 3 + 37, 8)).astype(np.float32),
+        "one_edge": rng.standard_normal((1, 8)).astype(np.float32),
+    }
+
+
+@pytest.mark.parametrize("case", ["matrix", "random_ragged", "one_edge"])
+def test_mlp_forward_matches_oracle(case):
+    torch = _torch()
+    from mpsparse_amd.learned_fsai import _C
+
+    feat = _feat_cases()[case]
+    P = _params(1)
+    say("mlp forward", case, feat.shape)
+    out = torch.full((feat.shape[0],), np.nan, device="cuda")
+    _C.mlp_forward(t_f32(P), t_f32(feat), out)
+    got = out.cpu().numpy().astype(np.float64)
+    ref, _, _ = OF.mlp_forward(P, feat, bf16=True)
+    assert np.all(np.abs(got - ref) <= 1e-3 * np.maximum(1, np.abs(ref))), np.abs(got - ref).max()
+    # the BF16 emulation is the right model: without it the error is larger but bounded
+    ref32, _, _ = OF.mlp_forward(P, feat, bf16=False)
+    assert np.max(np.abs(got - ref32)) <= 0.05 * max(1.0, np.max(np.abs(ref32)))
+
+
+@pytest.mark.parametrize("case", ["matrix", "random_ragged", "one_edge"])
+def test_mlp_backward_matches_oracle(case):
+    torch = _torch()
+    from mpsparse_amd.learned_fsai import _C
+
+    feat = _feat_cases()[case]
+    P = _params(2)
+    go = np.random.default_rng(4).standard_normal(feat.shape[0]).astype(np.float32)
+    say("mlp backward", case, feat.shape)
+    grad = torch.full((OF.NPARAMS,), np.nan, device="cuda")
+    _C.mlp_backward(t_f32(P), t_f32(feat), t_f32(go), grad)
+    got = grad.cpu().numpy().astype(np.float64)
+    ref = OF.mlp_backward(P, feat, go, bf16=True)
+    for lo, hi, nm in ((OF.OW1, OF.OB1, "W1"), (OF.OB1, OF.OW2, "b1"), (OF.OW2, OF.OB2, "W2"),
+                       (OF.OB2, OF.OW3, "b2"), (OF.OW3, OF.OB3, "w3"), (OF.OB3, OF.NPARAMS, "b3")):
+        num = np.linalg.norm(got[lo:hi] - ref[lo:hi])
+        assert num <= 2e-3 * np.linalg.norm(ref[lo:hi]) + 1e-6, (nm, num, np.linalg.norm(ref[lo:hi]))
+
+
+def test_train_step_is_the_composition():
+    """train_step = forward -> G = out * colscale -> probe loss/grad -> backward (oracle composition)."""
+    torch = _torch()
+    M = diffusion_2d(24, sigma=1.5, seed=11)
+    say("fsai train_step")
+    A, F = _handles(M)
+    P = _params(3, scale=0.1)
+    W = np.random.default_rng(12).standard_normal((2, M.num_rows)).astype(np.float32)
+    grad = torch.zeros(OF.NPARAMS, device="cuda")
+    loss = F.train_step(t_f32(P), t_f32(W), grad)
+    feat, cs = (t.cpu().numpy() for t in F.features())
+    out, _, _ = OF.mlp_forward(P, feat, bf16=True)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    loss_ref, gg = OF.probe_loss_grad(_scipy(M), r, c, out * cs, W)
+    ref = OF.mlp_backward(P, feat, gg * cs, bf16=True)
+    assert abs(loss - loss_ref) <= 1e-3 * loss_ref, (loss, loss_ref)
+    got = grad.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-2 * np.linalg.norm(ref)
+
+
+def test_adam_step():
+    torch = _torch()
+    from mpsparse_amd.learned_fsai import _C
+
+    rng = np.random.default_rng(0)
+    P, g, m, v = (rng.standard_normal(OF.NPARAMS).astype(np.float32) for _ in range(4))
+    v = np.abs(v)
+    say("mlp adam")
+    tP, tg, tm, tv = (t_f32(a) for a in (P, g, m, v))
+    _C.mlp_adam(tP, tg, tm, tv, lr=1e-2, step=3, weight_decay=1e-3)
+    # the ABI takes FP32 hyper-parameters: 1 - float32(0.999) is 0.0009999871
+    f = np.float32
+    b1, b2, eps, lr, wd = float(f(0.9)), float(f(0.999)), float(f(1e-8)), float(f(1e-2)), float(f(1e-3))
+    c1, c2 = float(f(1) - f(b1) ** f(3)), float(f(1) - f(b2) ** f(3))
+    gi = g.astype(np.float64) + wd * P
+    m1 = b1 * m + (1 - b1) * gi
+    v1 = b2 * v + (1 - b2) * gi * gi
+    P1 = P - lr * (m1 / c1) / (np.sqrt(v1 / c2) + eps)
+    assert np.allclose(tm.cpu().numpy(), m1, rtol=1e-6, atol=1e-7)
+    assert np.allclose(tv.cpu().numpy(), v1, rtol=1e-6, atol=1e-7)
+    assert np.allclose(tP.cpu().numpy(), P1, rtol=1e-5, atol=1e-6)
+
+
+def test_pcg_fsai_classical_converges_faster_than_jacobi():
+    torch = _torch()
+    from mpsparse_amd.learned_fsai import solve
+
+    M = diffusion_3d(24, sigma=2.0, seed=3)
+    S = _scipy(M)
+    b = np.random.default_rng(0).standard_normal(M.num_rows).astype(np.float32)
+    say("pcg fsai classical 24^3 sigma 2")
+    A, F = _handles(M)
+    assert F.classical() == 0
+    x = torch.full((M.num_rows,), np.nan, device="cuda")
+    info = solve(A, F, t_f32(b), x, tol=1e-6)
+    assert info["status"] == "converged" and not info["fell_back_to_cg"], info
+    rel = K.true_relres(S, x.cpu().numpy(), b)
+    assert rel <= 1.05e-6, (rel, info)
+    # FP64 PCG with the same G
+    g = F.values().cpu().numpy().astype(np.float64)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    G = OF.G_matrix(M.num_rows, r, c, g)
+    _, it64, _, _ = K.cg(S, b.astype(np.float64), tol=1e-6, M_apply=lambda v: G.T @ (G @ v))
+    assert abs(info["iterations"] - it64) <= 0.15 * it64 + 2, (info, it64)
+    xj = torch.zeros(M.num_rows, device="cuda")
+    ij = A.cgd(t_f32(b), xj, tol=1e-6, precond="jacobi")
+    assert ij["status"] == "converged"
+    assert info["iterations"] < 0.8 * ij["iterations"], (info["iterations"], ij["iterations"])
+
+
+def test_nonpositive_diagonal_falls_back_to_cg():
+    torch = _torch()
+    from mpsparse_amd.learned_fsai import build, solve
+
+    M = diffusion_2d(24, sigma=1.0, seed=5)
+    b = np.random.default_rng(2).standard_normal(M.num_rows).astype(np.float32)
+    P = _params(4)
+    P[OF.OW3:OF.OB3] = 0.0
+    P[OF.OB3] = -1.0  # every output -1: G has a negative diagonal
+    say("fsai fallback")
+    A, _ = _handles(M)
+    F, nonpos = build(A, t_f32(P))
+    assert nonpos == M.num_rows
+    x = torch.zeros(M.num_rows, device="cuda")
+    info = solve(A, F, t_f32(b), x, tol=1e-6)
+    assert info["fell_back_to_cg"] and info["status"] == "converged", info
+    xc = torch.zeros(M.num_rows, device="cuda")
+    ic = A.cgd(t_f32(b), xc, tol=1e-6)
+    assert info["iterations"] == ic["iterations"]
+    assert K.true_relres(_scipy(M), x.cpu().numpy(), b) <= 1.05e-6
+
+
+@pytest.mark.parametrize("values", ["f32", "bf16"])
+def test_refined_fsai_reaches_fp64_tolerance(values):
+    """FP64 refinement around FSAI-PCG corrections reaches 1e-9 in the FP64 true
+    residual (oracle), with BF16 or FP32 matrix values inside the corrections."""
+    torch = _torch()
+    from mpsparse_amd.learned_fsai import solve_refined
+
+    M = diffusion_3d(20, sigma=2.0, seed=8)
+    S = _scipy(M)
+    b = np.random.default_rng(6).standard_normal(M.num_rows)
+    say("refined fsai", values)
+    A, F = _handles(M)
+    F.classical()
+    x = torch.zeros(M.num_rows, dtype=torch.float64, device="cuda")
+    info = solve_refined(A, F, torch.from_numpy(b).cuda(), x, tol=1e-9, values=values)
+    assert info["status"] == "converged" and info["fallbacks"] == 0, info
+    xs = x.cpu().numpy()
+    rel = np.linalg.norm(b - S @ xs) / np.linalg.norm(b)
+    assert rel <= 1.05e-9 and abs(rel - info["relres"]) <= 1e-3 * rel + 1e-15, (rel, info)

@@ -84,3 +84,27 @@ def test_features_on_the_diagonal():
     assert np.allclose(f[dg, 0], 1.0) and np.allclose(f[dg, 1], 0.0) and np.all(f[dg, 4] == 1)
     assert np.all(f[~dg, 4] == 0) and np.all(f[~dg, 0] < 0)
     assert np.allclose(cs, 1 / np.sqrt(A.diagonal()[c]))
+
+
+def test_row_coupling_features_give_classical_fsai_on_stencils():
+    """7-point stencil: lower neighbours are mutually uncoupled, so classical FSAI
+    has the closed form g_ii sqrt(a_ii) = 1/sqrt(1 - s_low), g_ij sqrt(a_jj) =
+    -f0 / sqrt(1 - s_low) in the features f0 and s_low (f[5]); s_all (f[6]) is
+    the same sum over the whole row."""
+    from mpsparse_amd.problems import diffusion_3d
+
+    M = diffusion_3d(5, 4, 3, sigma=1.5, seed=2)
+    S = M.to_scipy()
+    f, cs = OF.features(M.row_ptr, M.col_ind, M.vals)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    g = OF.classical_fsai(S)
+    out = g / cs  # g_ij sqrt(a_jj)
+    closed = np.where(r == c, 1.0, -f[:, 0]) / np.sqrt(1.0 - f[:, 5])
+    assert np.allclose(out, closed, rtol=1e-12, atol=1e-14)
+    d = S.diagonal()
+    Sd = S.tocoo()
+    off = Sd.row != Sd.col
+    s_all = np.bincount(Sd.row[off], weights=Sd.data[off] ** 2 / (d[Sd.row[off]] * d[Sd.col[off]]),
+                        minlength=S.shape[0])
+    assert np.allclose(f[:, 6], s_all[r], rtol=1e-12)
+    assert np.all(f[:, 6] >= f[:, 5])

@@ -63,3 +63,21 @@ def test_block_sparse_structure():
 
 def test_full_size_block_sparse_counts():
     assert 65536 * 9 * 256 == 150994944
+
+
+def test_torch_diffusion_generator_matches_numpy():
+    """The torch (device) generator used for training builds the same matrix as
+    diffusion_3d from the same conductivities (2D and 3D)."""
+    import torch
+
+    from mpsparse_amd.learned_fsai import diffusion_keys_from_k
+
+    for shape in ((4, 5, 6), (1, 7, 9)):
+        nz, ny, nx = shape
+        A = P.diffusion_3d(nx, ny, nz, sigma=1.3, seed=3)
+        k = P._lognormal_k(shape, 1.3, np.random.default_rng(3))
+        keys, vals, n = diffusion_keys_from_k(torch.from_numpy(k))
+        rows = np.repeat(np.arange(n, dtype=np.uint64), np.diff(A.row_ptr.astype(np.int64)))
+        assert n == A.num_rows
+        assert np.array_equal(keys.numpy().view(np.uint64), P.pack_keys(rows, A.col_ind.astype(np.uint64)))
+        assert np.array_equal(vals.numpy(), A.vals)

@@ -193,9 +193,9 @@ msp_status msp_fsai_num_edges(msp_fsai F, uint64_t *num_edges);
 /* Edge list of the pattern (row, col), in G's CSR order. */
 msp_status msp_fsai_edges(msp_fsai F, uint32_t *rows, uint32_t *cols, void *stream);
 /* 8 features per edge: a_ij/sqrt(a_ii a_jj), log(a_jj/a_ii), nnz(row i)/8,
- * sum_j|a_ij|/a_ii, [i == j], s_low(i), s_all(i), 0 with
- * s_low(i) = sum_{j<i} a_ij^2/(a_ii a_jj) and s_all(i) the same over j != i;
- * and the column scaling 1/sqrt(a_jj). */
+ * sum_j|a_ij|/a_ii, [i == j], s_low(i), s_all(i), 1/sqrt(max(1 - s_low(i), 1/64))
+ * with s_low(i) = sum_{j<i} a_ij^2/(a_ii a_jj) and s_all(i) the same over
+ * j != i; and the column scaling 1/sqrt(a_jj). */
 msp_status msp_fsai_features(msp_fsai F, float *feat, float *colscale, void *stream);
 /* Classical FSAI: per row, A[J,J] y = e_last, g = y/sqrt(y_last); rows that
  * cannot be solved become Jacobi rows (counted in fallback_rows). */

@@ -38,8 +38,8 @@ struct X4Axpy {  // r - alpha v
   }
 };
 
-template <int B, int ND, typename XL, typename Epi, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_bspmv(BcsrView A, XL xl, Epi epi, Fin fin, GridRed rb,
+template <int B, int ND, typename XL, typename Epi>
+__global__ __launch_bounds__(kBlock) void k_bspmv(BcsrView A, XL xl, Epi epi, GridRed rb,
                                                   const int* __restrict__ done) {
   static_assert(B == 4 || B == 8 || B == 16, "block size");
   if (done && *done) return;
@@ -88,12 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(BcsrView A, XL xl, Epi epi, Fi
       if (row < A.n) epi.row(row, sum, acc);
     }
   }
-  if constexpr (ND > 0) {
-    float tot[ND];
-    if (grid_reduce<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb, tot)) {
-      if (threadIdx.x == 0) fin(tot);
-    }
-  }
+  if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
 }
 
 // Row r of a BCSR matrix as (value, column) pairs, for row-per-thread kernels.

@@ -164,7 +164,7 @@ struct Workspace {
   size_t npad = 0;
   DevMem vecs;            // kNumVec vectors of npad floats
   DevMem scal;            // Scalars
-  DevMem red;             // partials + ticket
+  DevMem red;             // grid-reduction partials
   Scalars* hS = nullptr;  // pinned host mirror
   int32_t* hflag = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -175,12 +175,7 @@ struct Workspace {
   enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL, RH };
   float* v(int i) const { return vecs.as<float>() + (size_t)i * npad; }
   Scalars* S_dev() const { return scal.as<Scalars>(); }
-  GridRed gr() const {
-    GridRed g;
-    g.ticket = red.as<uint32_t>();
-    g.partials = reinterpret_cast<CSum*>(red.as<char>() + 256);
-    return g;
-  }
+  GridRed gr() const { return GridRed{red.as<CSum>()}; }
   ~Workspace() {
     if (hS) (void)hipHostFree(hS);
     if (hflag) (void)hipHostFree(hflag);
@@ -203,7 +198,7 @@ struct MatBase {
 
 struct msp_csr_s : MatBase {
   uint64_t nnz = 0;
-  DevMem row_ptr, col, val, val16, blk;
+  DevMem row_ptr, col, val, val16, dlo, blk;  // val16 + dlo: 16-bit values, FP32 diagonal remainder
   int val16_type = -1;
   uint32_t nblk = 0, T = 0, max_row = 0;
 };
@@ -230,7 +225,8 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
   if (!w.scal.p) {
     MSPCHK(w.scal.alloc(sizeof(Scalars)));
     HIPCHK(hipMemsetAsync(w.scal.p, 0, sizeof(Scalars), s));
-    size_t red_bytes = 256 + sizeof(CSum) * 4 * (size_t)cu_count() * 8;
+    // partials of up to 4 values for grids of up to 8 workgroups per CU
+    size_t red_bytes = sizeof(CSum) * 4 * (size_t)cu_count() * 8;
     MSPCHK(w.red.alloc(red_bytes));
     HIPCHK(hipMemsetAsync(w.red.p, 0, red_bytes, s));
     HIPCHK(hipHostMalloc((void**)&w.hS, sizeof(Scalars), hipHostMallocDefault));
@@ -539,12 +535,15 @@ extern "C" msp_status msp_csr_prepare_values(msp_csr A, int32_t value_type, void
   if (A->val16_type == value_type && A->val16.p) return MSP_OK;
   size_t padded = (size_t)A->nnz + 64;
   MSPCHK(A->val16.alloc(sizeof(uint16_t) * padded));
-  uint32_t g = (uint32_t)std::min<uint64_t>((padded + kBlock - 1) / kBlock, 65536);
+  MSPCHK(A->dlo.alloc(sizeof(float) * ((size_t)A->n + 1)));
+  const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(((uint64_t)A->n + kBlock - 1) / kBlock, 65536));
   if (value_type == MSP_VALUES_BF16)
-    k_cvt_bf16<<<g, kBlock, 0, s>>>(A->val.as<float>(), A->val16.as<uint16_t>(), padded);
+    k_cvt16_rowsum<true><<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
+                                              A->n, A->nnz, A->val16.as<uint16_t>(), A->dlo.as<float>());
   else
-    k_cvt_f16<<<g, kBlock, 0, s>>>(A->val.as<float>(), A->val16.as<uint16_t>(), padded);
-  MSPCHK(launch_check("k_cvt"));
+    k_cvt16_rowsum<false><<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
+                                               A->n, A->nnz, A->val16.as<uint16_t>(), A->dlo.as<float>());
+  MSPCHK(launch_check("k_cvt16_rowsum"));
   A->val16_type = value_type;
   HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
@@ -566,15 +565,27 @@ static CsrView<V> view_of(msp_csr A) {
   CsrView<V> v;
   v.row_ptr = A->row_ptr.as<uint32_t>();
   v.col = A->col.as<uint32_t>();
-  if constexpr (std::is_same<V, ValF32>::value) v.val = A->val.as<float>();
-  else v.val = A->val16.as<uint16_t>();
+  if constexpr (std::is_same<V, ValF32>::value) {
+    v.val = A->val.as<float>();
+    v.dlo = nullptr;
+  } else {
+    v.val = A->val16.as<uint16_t>();
+    v.dlo = A->dlo.as<float>();
+  }
   v.blk = A->blk.as<uint32_t>();
   v.nblk = A->nblk;
   v.n = A->n;
   return v;
 }
 
-static GridRed red_of(const MatBase* A) { return A->ws.scal.p ? A->ws.gr() : GridRed{nullptr, nullptr}; }
+static GridRed red_of(const MatBase* A) { return A->ws.scal.p ? A->ws.gr() : GridRed{nullptr}; }
+
+// Second half of a grid reduction: one workgroup folds the nb block partials
+// of the kernel just launched into the finisher (nothing for FinNone).
+template <int ND, typename Fin>
+static void finish(const Fin& fin, GridRed rb, uint32_t nb, hipStream_t s) {
+  if constexpr (ND > 0 && !std::is_same<Fin, FinNone>::value) k_finish<ND, Fin><<<1, kBlock, 0, s>>>(rb, nb, fin);
+}
 
 template <typename V>
 struct OpCsr {
@@ -582,15 +593,17 @@ struct OpCsr {
   MatBase* base() const { return A; }
   template <int ND, typename Epi, typename Fin>
   void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
-    k_spmv<V, ND, XPlain, Epi, Fin><<<spmv_grid(A->nblk), kBlock, 0, s>>>(view_of<V>(A), XPlain{x}, epi, fin,
-                                                                         red_of(A), done);
+    const uint32_t g = spmv_grid(A->nblk);
+    k_spmv<V, ND, XPlain, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), XPlain{x}, epi, red_of(A), done);
+    finish<ND>(fin, red_of(A), g, s);
   }
   // y = A (r - alpha v)
   template <int ND, typename Epi, typename Fin>
   void spmv_axpy(const float* r, const float* v, const float* alpha, Epi epi, Fin fin, const int* done,
                  hipStream_t s) const {
-    k_spmv<V, ND, XAxpy, Epi, Fin><<<spmv_grid(A->nblk), kBlock, 0, s>>>(view_of<V>(A), XAxpy{r, v, alpha}, epi,
-                                                                        fin, red_of(A), done);
+    const uint32_t g = spmv_grid(A->nblk);
+    k_spmv<V, ND, XAxpy, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), XAxpy{r, v, alpha}, epi, red_of(A), done);
+    finish<ND>(fin, red_of(A), g, s);
   }
   RowsCsr rows() const { return RowsCsr{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()}; }
   void inv_diag(float* dinv, int* bad, hipStream_t s) const {
@@ -621,9 +634,10 @@ struct OpBcsr {
   template <int ND, typename XL, typename Epi, typename Fin>
   void launch(XL xl, Epi epi, Fin fin, const int* done, hipStream_t s) const {
     const uint32_t g = bspmv_grid(A);
-    if (A->B == 4) k_bspmv<4, ND, XL, Epi, Fin><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, fin, red_of(A), done);
-    else if (A->B == 8) k_bspmv<8, ND, XL, Epi, Fin><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, fin, red_of(A), done);
-    else k_bspmv<16, ND, XL, Epi, Fin><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, fin, red_of(A), done);
+    if (A->B == 4) k_bspmv<4, ND, XL, Epi><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, red_of(A), done);
+    else if (A->B == 8) k_bspmv<8, ND, XL, Epi><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, red_of(A), done);
+    else k_bspmv<16, ND, XL, Epi><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, red_of(A), done);
+    finish<ND>(fin, red_of(A), g, s);
   }
   template <int ND, typename Epi, typename Fin>
   void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
@@ -687,10 +701,12 @@ static msp_status ensure_dinv(const Op& op, hipStream_t s) {
 
 // Upload scalar settings, run `init`, then batches of `step` with one host
 // poll of the done flag per batch (the poll trails by one batch, so the GPU
-// never idles waiting for the host).
-template <typename Init, typename Step>
+// never idles waiting for the host). A recurrence that reports convergence
+// halts the steps until the true-residual check `verify` at the end of the
+// batch (launched only when o.verify).
+template <typename Init, typename Step, typename Verify>
 static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
-                             Step step) {
+                             Step step, Verify verify) {
   Workspace& w = A->ws;
   std::memset(w.hS, 0, sizeof(Scalars));
   w.hS->tol = (float)o.tol;
@@ -708,6 +724,7 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
   while (true) {
     const int batch = std::min(o.check_every, std::max(1, limit - enqueued));
     for (int k = 0; k < batch; ++k) step();
+    if (o.verify) verify();
     MSPCHK(launch_check("solver step"));
     enqueued += batch;
     HIPCHK(hipMemcpyAsync(&w.hflag[slot], dflag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -768,21 +785,31 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
     k_cg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, P, B, D, n, o.use_x0, jac ? kPcJacobi : kPcNone, S, rb);
+    finish<3>(FinCgInit{S}, rb, gv, s);
     return launch_check("cg init");
   };
   auto step = [&]() {
-    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->done, s);
+    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->halt, s);
     if (jac) {
       k_cg_update_xr<kPcJacobi><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, D, n, S, rb);
+      finish<2>(FinCgStep<kPcJacobi>{S}, rb, gv, s);
       k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
-      if (o.verify) k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
     } else {
       k_cg_update_xr<kPcNone><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, D, n, S, rb);
+      finish<2>(FinCgStep<kPcNone>{S}, rb, gv, s);
       k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
-      if (o.verify) k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
     }
   };
-  MSPCHK(run_solver(A, o, info, s, init, step));
+  auto verify = [&]() {
+    if (jac) {
+      k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
+      finish<2>(FinVerify<1>{S}, rb, gv, s);
+    } else {
+      k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
+      finish<2>(FinVerify<0>{S}, rb, gv, s);
+    }
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step, verify));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -806,16 +833,21 @@ static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const ms
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
     k_bicg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, R0, P, B, n, o.use_x0, S, rb);
+    finish<2>(FinBicgInit{S}, rb, gv, s);
     return launch_check("bicgstab init");
   };
   auto step = [&]() {
-    op.template spmv<1>(P, EpiStoreDot{Vv, R0}, FinBicgAlpha{S}, &S->done, s);
-    op.template spmv_axpy<2>(R, Vv, &S->alpha, EpiBicgT{T, Sv, R, Vv, &S->alpha}, FinBicgOmega{S}, &S->done, s);
+    op.template spmv<1>(P, EpiStoreDot{Vv, R0}, FinBicgAlpha{S}, &S->halt, s);
+    op.template spmv_axpy<2>(R, Vv, &S->alpha, EpiBicgT{T, Sv, R, Vv, &S->alpha}, FinBicgOmega{S}, &S->halt, s);
     k_bicg_update<<<gv, kBlock, 0, s>>>(X, XLo, R, P, Sv, T, R0, n, S, rb);
+    finish<2>(FinBicgStep{S}, rb, gv, s);
     k_bicg_p<<<gv, kBlock, 0, s>>>(P, R, Vv, n, S);
-    if (o.verify) k_verify<2><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, nullptr, n, S, rb);
   };
-  MSPCHK(run_solver(A, o, info, s, init, step));
+  auto verify = [&]() {
+    k_verify<2><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, nullptr, n, S, rb);
+    finish<2>(FinVerify<2>{S}, rb, gv, s);
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step, verify));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1125,9 +1157,11 @@ extern "C" void msp_default_refine_opts(msp_refine_opts* o) {
 static bool is_device_ptr_d(const double* p) { return is_device_ptr(static_cast<const void*>(p)); }
 
 // FP64 outer loop: r = b - A x in FP64 (FP32 matrix values), correction
-// A d = r / ||r|| by `inner(rhs, d, &info)` in FP32, x += ||r|| d in FP64.
-// Stops at relres <= tol, after two outer steps that fail to halve the
-// residual (stagnated), or after max_outer corrections.
+// A d = r / ||r|| by `inner(rhs, d, inner_tol, &info, &stop)` in FP32,
+// x += ||r|| d in FP64. A correction is solved only as far as the target
+// needs: to max(inner_tol, 0.3 tol / relres) (at most 0.5), so the last one
+// does not over-solve. Stops at relres <= tol, after two outer steps that fail
+// to halve the residual (stagnated), or after max_outer corrections.
 template <class Inner>
 static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_refine_opts& ro, msp_refine_info* info,
                               hipStream_t s, Inner&& inner) {
@@ -1156,8 +1190,8 @@ static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_r
   double prev = 0.0;
   int slow = 0;
   for (int k = 0;; ++k) {
-    k_residual64<<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, reinterpret_cast<double*>(rb.partials), rb.ticket,
-                                       SUMS);
+    k_residual64<<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, reinterpret_cast<double*>(rb.partials));
+    k_finish_d<2><<<1, kBlock, 0, s>>>(reinterpret_cast<const double*>(rb.partials), gv, SUMS);
     MSPCHK(launch_check("k_residual64"));
     double sums[2] = {0.0, 0.0};
     HIPCHK(hipMemcpyAsync(sums, SUMS, sizeof sums, hipMemcpyDeviceToHost, s));
@@ -1185,7 +1219,8 @@ static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_r
     prev = inf.relres;
     k_scale_to_f32<<<gv, kBlock, 0, s>>>(R64, RHS, 1.0 / rnorm, n);
     bool stop = false;
-    MSPCHK(inner(RHS, D, &inf, &stop));
+    const double itol = std::max(ro.inner_tol, std::min(0.5, 0.3 * ro.tol / inf.relres));
+    MSPCHK(inner(RHS, D, itol, &inf, &stop));
     if (stop) {
       inf.status = MSP_SOLVE_BREAKDOWN;
       break;
@@ -1224,8 +1259,9 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
   hipStream_t s = (hipStream_t)stream;
   if (ro.values != MSP_VALUES_F32) MSPCHK(msp_csr_prepare_values(A, ro.values, stream));
   msp_solve_opts io = inner_opts(ro);
-  auto inner = [&](const float* rhs, float* d, msp_refine_info* inf, bool* stop) -> msp_status {
+  auto inner = [&](const float* rhs, float* d, double itol, msp_refine_info* inf, bool* stop) -> msp_status {
     msp_solve_info ii{};
+    io.tol = itol;
     auto run = [&](int32_t vt) -> msp_status {
       io.values = vt;
       return with_csr_op(A, vt, [&](auto op) {
@@ -1470,7 +1506,9 @@ extern "C" msp_status msp_fsai_export_values(msp_fsai F, float* g, void* stream)
 
 template <int ND, typename XL, typename Epi, typename Fin>
 static void spmv_with(msp_csr M, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
-  k_spmv<ValF32, ND, XL, Epi, Fin><<<spmv_grid(M->nblk), kBlock, 0, s>>>(view_of<ValF32>(M), xl, epi, fin, rb, done);
+  const uint32_t g = spmv_grid(M->nblk);
+  k_spmv<ValF32, ND, XL, Epi><<<g, kBlock, 0, s>>>(view_of<ValF32>(M), xl, epi, rb, done);
+  finish<ND>(fin, rb, g, s);
 }
 
 extern "C" msp_status msp_fsai_apply(msp_fsai F, const float* r, float* z, void* stream) {
@@ -1482,8 +1520,8 @@ extern "C" msp_status msp_fsai_apply(msp_fsai F, const float* r, float* z, void*
   float* w = rin + np;
   float* zo = rin + 2 * np;
   MSPCHK(copy_in(rin, r, sizeof(float) * F->A->n, s));
-  spmv_with<0>(&F->G, XPlain{rin}, EpiStore{w}, FinNone{}, GridRed{nullptr, nullptr}, nullptr, s);
-  spmv_with<0>(&F->GT, XPlain{w}, EpiStore{zo}, FinNone{}, GridRed{nullptr, nullptr}, nullptr, s);
+  spmv_with<0>(&F->G, XPlain{rin}, EpiStore{w}, FinNone{}, GridRed{nullptr}, nullptr, s);
+  spmv_with<0>(&F->GT, XPlain{w}, EpiStore{zo}, FinNone{}, GridRed{nullptr}, nullptr, s);
   MSPCHK(launch_check("fsai apply"));
   MSPCHK(copy_out(z, zo, sizeof(float) * F->A->n, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1516,7 +1554,8 @@ extern "C" msp_status msp_fsai_probe_loss_grad(msp_fsai F, const float* W, int32
     spmv_with<0>(&F->GT, XPlain{w}, EpiStore{u}, FinNone{}, rb, nullptr, s);
     spmv_with<0>(A, XPlain{u}, EpiStore{y}, FinNone{}, rb, nullptr, s);
     spmv_with<0>(&F->G, XPlain{y}, EpiStore{z}, FinNone{}, rb, nullptr, s);
-    k_probe_resid<<<vec_grid(n), kBlock, 0, s>>>(z, w, n, e, F->sums.as<float>(), rb);
+    k_probe_resid<<<vec_grid(n), kBlock, 0, s>>>(z, w, n, e, rb);
+    finish<2>(FinStore{F->sums.as<float>()}, rb, vec_grid(n), s);
     spmv_with<0>(&F->GT, XPlain{e}, EpiStore{u}, FinNone{}, rb, nullptr, s);
     spmv_with<0>(A, XPlain{u}, EpiStore{h}, FinNone{}, rb, nullptr, s);
     k_fsai_edge_grad<<<grid_for(F->ne), kBlock, 0, s>>>(F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), F->ne, e, y,
@@ -1550,18 +1589,23 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
     k_cg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, P, B, nullptr, n, o.use_x0, kPcExplicit, S, rb);
+    finish<3>(FinCgInit{S}, rb, gv, s);
     k_set_restart<<<1, 1, 0, s>>>(S);
     return launch_check("pcg init");
   };
   auto step = [&]() {
-    spmv_with<0>(&F->G, XPlain{R}, EpiStore{Wv}, FinNone{}, rb, &S->done, s);
-    spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinPcgBeta{S}, rb, &S->done, s);
+    spmv_with<0>(&F->G, XPlain{R}, EpiStore{Wv}, FinNone{}, rb, &S->halt, s);
+    spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinPcgBeta{S}, rb, &S->halt, s);
     k_cg_update_p<kPcExplicit><<<gv, kBlock, 0, s>>>(P, R, nullptr, Z, n, S);
-    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->done, s);
+    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->halt, s);
     k_cg_update_xr<kPcExplicit><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb);
-    if (o.verify) k_verify<3><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, nullptr, n, S, rb);
+    finish<2>(FinCgStep<kPcExplicit>{S}, rb, gv, s);
   };
-  MSPCHK(run_solver(A, o, info, s, init, step));
+  auto verify = [&]() {
+    k_verify<3><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, nullptr, n, S, rb);
+    finish<2>(FinVerify<3>{S}, rb, gv, s);
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step, verify));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1605,8 +1649,9 @@ extern "C" msp_status msp_csr_solve_refined_fsai(msp_csr A, msp_fsai F, const do
   if (ro.values != MSP_VALUES_F32) MSPCHK(msp_csr_prepare_values(A, ro.values, stream));
   msp_solve_opts io = inner_opts(ro);
   io.precond = MSP_PRECOND_NONE;
-  auto inner = [&](const float* rhs, float* d, msp_refine_info* inf, bool* stop) -> msp_status {
+  auto inner = [&](const float* rhs, float* d, double itol, msp_refine_info* inf, bool* stop) -> msp_status {
     msp_solve_info ii{};
+    io.tol = itol;
     const bool use_fsai = F->nonpos == 0;
     if (use_fsai)
       MSPCHK(with_csr_op(A, io.values, [&](auto op) { return pcg_fsai_impl(op, F, rhs, d, io, &ii, s); }));
@@ -1693,7 +1738,6 @@ extern "C" msp_status msp_fsai_apply_network(msp_fsai F, const float* params, in
                                              void* stream) {
   if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
   HIPCHK(hipSetDevice(F->device));
-  hipStream_t s = (hipStream_t)stream;
   DevMem out;
   MSPCHK(out.alloc(sizeof(float) * (F->ne + 1)));
   MSPCHK(msp_mlp_forward(params, F->feat.as<float>(), F->ne, out.as<float>(), stream));

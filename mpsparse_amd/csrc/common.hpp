@@ -90,53 +90,49 @@ __device__ __forceinline__ void block_reduce(CSum (&acc)[ND]) {
   __syncthreads();
 }
 
-// Scratch for a grid reduction: one CSum per (block, value) plus a ticket.
+// Scratch of a grid reduction: one CSum per (block, value).
 struct GridRed {
-  CSum* partials;    // [gridDim.x][ND]
-  uint32_t* ticket;  // zero on entry; reset to zero by the finishing block
+  CSum* partials;  // [gridDim.x][ND] of the producing kernel
 };
 
-// Deterministic grid-wide reduction in one kernel: every block publishes its
-// partial, takes a ticket; the block that takes the last ticket sums all
-// partials in block order and gets `true` (in all its threads) with the totals
-// in `tot`. No block ever waits on another one.
+// Deterministic grid-wide reduction in two kernels. The producing kernel ends
+// with block_partials (plain stores of each block's sums, no fences or
+// atomics); the host then launches k_finish on the same stream, whose single
+// workgroup sums the partials in block order and hands the totals to the
+// finisher. The kernel boundary makes the partials visible. (The one-kernel
+// last-block-done variant needs a device-scope fence and a same-address atomic
+// in every workgroup; on this chip that costs ~50 ns per workgroup, 100 us for
+// a 2048-workgroup launch.)
 template <int ND>
-__device__ bool grid_reduce(CSum (&acc)[ND], GridRed rb, float (&tot)[ND]) {
-  __shared__ int s_last;
-  __shared__ float s_tot[ND > 0 ? ND : 1];
+__device__ __forceinline__ void block_partials(CSum (&acc)[ND], GridRed rb) {
   block_reduce<ND>(acc);
-  const uint32_t nb = gridDim.x;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int d = 0; d < ND; ++d) rb.partials[blockIdx.x * ND + d] = acc[d];
-    __threadfence();
-    uint32_t t = atomicAdd(rb.ticket, 1u);
-    s_last = (t == nb - 1);
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
+}
+
+// Finisher protocol: `active()` repeats the producer's skip test (a producer
+// that returned early wrote no partials), `operator()(tot)` then `after()` run
+// in thread 0.
+template <int ND, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_finish(GridRed rb, uint32_t nb, Fin fin) {
+  if (!fin.active()) return;
   CSum a[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) a[d] = csum_zero();
-  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
+  for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      const volatile CSum* p = rb.partials + i * ND + d;
-      CSum v{p->s, p->c};
-      a[d] = csum_merge(a[d], v);
-    }
+    for (int d = 0; d < ND; ++d) a[d] = csum_merge(a[d], rb.partials[i * ND + d]);
   }
   block_reduce<ND>(a);
   if (threadIdx.x == 0) {
+    float tot[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) s_tot[d] = csum_val(a[d]);
-    *rb.ticket = 0u;
+    for (int d = 0; d < ND; ++d) tot[d] = csum_val(a[d]);
+    fin(tot);
+    fin.after();
   }
-  __syncthreads();
-#pragma unroll
-  for (int d = 0; d < ND; ++d) tot[d] = s_tot[d];
-  return true;
 }
 
 // Workgroups are dispatched round-robin over the XCDs; renumber them so that
@@ -150,8 +146,11 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t wg, uint32_t nb) {
 
 // ---------------------------------------------------------------------------
 // Matrix value types: FP32, or 16-bit storage (BF16 / FP16) widened to FP32.
+// Value formats of the CSR SpMV. The 16-bit formats keep the diagonal exact to
+// FP32 through a per-row remainder (kLowDiag): y_i gains dlo_i * x_i.
 struct ValF32 {
   typedef float storage;
+  static constexpr bool kLowDiag = false;
   static __device__ __forceinline__ f4 load4(const float* p) {
     return __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
   }
@@ -160,6 +159,7 @@ struct ValF32 {
 
 struct ValBF16 {
   typedef uint16_t storage;
+  static constexpr bool kLowDiag = true;
   static __device__ __forceinline__ float cvt(uint32_t h) { return __uint_as_float(h << 16); }
   static __device__ __forceinline__ f4 load4(const uint16_t* p) {
     u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
@@ -170,6 +170,7 @@ struct ValBF16 {
 
 struct ValF16 {
   typedef uint16_t storage;
+  static constexpr bool kLowDiag = true;
   static __device__ __forceinline__ float cvt(uint32_t h) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
   }

@@ -146,11 +146,11 @@ __global__ __launch_bounds__(kBlock) void k_fsai_classical(const uint32_t* __res
 
 // Edge features of the lower pattern, 8 floats per edge:
 // [a_ij / sqrt(a_ii a_jj), log(a_jj / a_ii), nnz(row i) / 8, sum_j |a_ij| / a_ii,
-//  [j == i], s_low(i), s_all(i), 0] with the row couplings
+//  [j == i], s_low(i), s_all(i), t(i)] with the row couplings
 //  s_low(i) = sum_{j < i} a_ij^2 / (a_ii a_jj), s_all(i) = sum_{j != i} a_ij^2 / (a_ii a_jj)
-// (for an arrow-shaped A[J,J] classical FSAI is g_ii sqrt(a_ii) = 1/sqrt(1 - s_low),
-// g_ij sqrt(a_jj) = -f0 / sqrt(1 - s_low)); and s[e] = 1 / sqrt(a_jj) (the column
-// scaling of G).
+// and t(i) = 1 / sqrt(max(1 - s_low(i), 1/64)) (for an arrow-shaped A[J,J]
+// classical FSAI is g_ii sqrt(a_ii) = t, g_ij sqrt(a_jj) = -f0 t); and
+// s[e] = 1 / sqrt(a_jj) (the column scaling of G).
 __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __restrict__ row_ptr,
                                                           const uint32_t* __restrict__ col,
                                                           const float* __restrict__ val, uint32_t n,
@@ -173,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __rest
       }
     }
     const float rn = (float)(row_ptr[i + 1] - row_ptr[i]) * 0.125f;
+    const float t = 1.0f / sqrtf(fmaxf(1.0f - s_low, 1.0f / 64.0f));
     for (uint32_t e = Lrp[i]; e < Lrp[i + 1]; ++e) {
       const uint32_t c = Lcol[e];
       const float ajj = diag[c];
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __rest
       f[4] = (c == i) ? 1.0f : 0.0f;
       f[5] = s_low;
       f[6] = s_all;
-      f[7] = 0.0f;
+      f[7] = t;
       colscale[e] = ajj > 0.0f ? rsqrtf(ajj) : 0.0f;
     }
   }
@@ -229,10 +230,9 @@ __global__ __launch_bounds__(kBlock) void k_fsai_edge_grad(const uint32_t* __res
   }
 }
 
-// e = z - w and ||e||^2, ||w||^2 (deterministic)
+// e = z - w; block partials of ||e||^2, ||w||^2 (finish with k_finish<2, FinStore>)
 __global__ __launch_bounds__(kBlock) void k_probe_resid(const float* __restrict__ z, const float* __restrict__ w,
-                                                        uint32_t n, float* __restrict__ e, float* __restrict__ out,
-                                                        GridRed rb) {
+                                                        uint32_t n, float* __restrict__ e, GridRed rb) {
   CSum acc[2] = {csum_zero(), csum_zero()};
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const float d = z[i] - w[i];
@@ -240,11 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_resid(const float* __restrict_
     acc[0] = csum_add(acc[0], d * d);
     acc[1] = csum_add(acc[1], w[i] * w[i]);
   }
-  float tot[2];
-  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
-    out[0] = tot[0];
-    out[1] = tot[1];
-  }
+  block_partials<2>(acc, rb);
 }
 
 }  // namespace msp

@@ -1,5 +1,7 @@
 // Krylov iterations as a short chain of fused kernels per step, each with at
-// most one deterministic grid reduction, all scalars kept on the device.
+// most one deterministic grid reduction (block partials in the kernel, totals
+// and scalar bookkeeping in the k_finish launched after it), all scalars kept
+// on the device.
 //
 // CG (reference: iterative/BiCGSTAB/cgd.metal:11-90, spmv.cpp:528-659) per step:
 //   q = A p, <p,q> -> alpha              (k_spmv + FinCgAlpha)
@@ -48,8 +50,13 @@ struct Scalars {
   int32_t nfail;     // consecutive verifications without a 10% gain in ||r||
   int32_t collapse;  // fold xl into x at the next update (after a restart)
   int32_t restart;   // next preconditioned direction update uses beta = 0
-  int32_t pad[2];
+  int32_t halt;      // done || verify: the step kernels do nothing (kept by k_finish)
+  int32_t pad;
 };
+
+// k_finish hook: after a finisher that owns the solver state, refresh the
+// step kernels' gate.
+__device__ __forceinline__ void refresh_halt(Scalars* S) { S->halt = (S->done != 0 || S->verify != 0) ? 1 : 0; }
 
 // Recurrence says converged: either stop, or ask for a true-residual check.
 __device__ __forceinline__ void on_recurrence_converged(Scalars* S) {
@@ -62,6 +69,8 @@ __device__ __forceinline__ bool finite_pos(float v) { return v > 0.0f && isfinit
 // Finishers (run in one thread after the grid reduction) --------------------
 struct FinCgAlpha {
   Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return S->halt == 0; }
   __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
     S->pq = tot[0];
     if (!finite_pos(tot[0])) {
@@ -75,6 +84,8 @@ struct FinCgAlpha {
 
 struct FinBicgAlpha {
   Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return S->halt == 0; }
   __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
     S->sigma = tot[0];
     if (tot[0] == 0.0f || !isfinite(tot[0])) {
@@ -88,6 +99,8 @@ struct FinBicgAlpha {
 
 struct FinBicgOmega {
   Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return S->halt == 0; }
   __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
     S->omega = (tot[1] != 0.0f) ? tot[0] / tot[1] : 0.0f;
   }
@@ -97,6 +110,8 @@ struct FinBicgOmega {
 // (beta = 0 on the first step and after a restart).
 struct FinPcgBeta {
   Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return S->halt == 0; }
   __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
     const float rz_new = tot[0];
     if (!isfinite(rz_new) || !(rz_new > 0.0f)) {
@@ -159,8 +174,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float
       acc[2] = csum_add(acc[2], rv[k] * zv[k]);
     }
   }
-  float tot[3];
-  if (grid_reduce<3>(acc, rb, tot) && threadIdx.x == 0) {
+  block_partials<3>(acc, rb);
+}
+
+// Totals of k_cg_init: <b,b>, <r,r>, <r,z>.
+struct FinCgInit {
+  Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return true; }
+  __device__ __forceinline__ void operator()(const float (&tot)[3]) const {
     S->bb = tot[0];
     S->rr = tot[1];
     S->rr0 = tot[1];
@@ -178,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float
     if (S->maxiter <= 0 && d == kRunning) d = kMaxIter;
     S->done = d;
   }
-}
+};
 
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
 template <int PC>
@@ -186,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
                                                          float* __restrict__ r, const float* __restrict__ p,
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
                                                          uint32_t n, Scalars* S, GridRed rb) {
-  if (S->done) return;
+  if (S->halt) return;
   const float alpha = S->alpha;
   const bool fold = S->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
@@ -220,8 +242,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
       acc[1] = csum_add(acc[1], rv[k] * zv[k]);
     }
   }
-  float tot[2];
-  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+  block_partials<2>(acc, rb);
+}
+
+// Totals of k_cg_update_xr: <r,r> and <r,z> -> iteration count, convergence
+// (or a pending verification), maxiter, beta.
+template <int PC>
+struct FinCgStep {
+  Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return S->halt == 0; }
+  __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
     const int it = S->iter;
     const float rr_old = S->rr;
     S->iter = it + 1;
@@ -245,14 +276,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
       }
     }
   }
-}
+};
 
 // p = z + beta p with z = r, dinv*r, or an explicit vector.
 template <int PC>
 __global__ __launch_bounds__(kBlock) void k_cg_update_p(float* __restrict__ p, const float* __restrict__ r,
                                                         const float* __restrict__ dinv, const float* __restrict__ z,
                                                         uint32_t n, const Scalars* S) {
-  if (S->done) return;
+  if (S->halt) return;
   const float beta = S->beta;
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
@@ -287,8 +318,15 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, flo
       acc[1] = csum_add(acc[1], rv[k] * rv[k]);
     }
   }
-  float tot[2];
-  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+  block_partials<2>(acc, rb);
+}
+
+// Totals of k_bicg_init: <b,b>, <r,r>.
+struct FinBicgInit {
+  Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return true; }
+  __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
     S->bb = tot[0];
     S->rr = tot[1];
     S->rr0 = tot[1];
@@ -306,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(float* __restrict__ x, flo
     if (S->maxiter <= 0 && d == kRunning) d = kMaxIter;
     S->done = d;
   }
-}
+};
 
 // x += alpha p + omega s; r = s - omega t; <r0, r>, <r, r> -> bookkeeping, beta.
 __global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, float* __restrict__ xl,
@@ -314,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, f
                                                         const float* __restrict__ s, const float* __restrict__ t,
                                                         const float* __restrict__ r0, uint32_t n, Scalars* S,
                                                         GridRed rb) {
-  if (S->done) return;
+  if (S->halt) return;
   const float alpha = S->alpha, omega = S->omega;
   const bool fold = S->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
@@ -348,8 +386,16 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, f
       acc[1] = csum_add(acc[1], rv[k] * rv[k]);
     }
   }
-  float tot[2];
-  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+  block_partials<2>(acc, rb);
+}
+
+// Totals of k_bicg_update: <r0,r>, <r,r> -> bookkeeping, beta.
+struct FinBicgStep {
+  Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return S->halt == 0; }
+  __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
+    const float alpha = S->alpha, omega = S->omega;
     const int it = S->iter;
     const float rr_old = S->rr;
     const float rho_new = tot[0];
@@ -372,12 +418,12 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(float* __restrict__ x, f
       S->rho = rho_new;
     }
   }
-}
+};
 
 // p = r + beta (p - omega v)
 __global__ __launch_bounds__(kBlock) void k_bicg_p(float* __restrict__ p, const float* __restrict__ r,
                                                    const float* __restrict__ v, uint32_t n, const Scalars* S) {
-  if (S->done) return;
+  if (S->halt) return;
   const float beta = S->beta, omega = S->omega;
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
@@ -439,8 +485,16 @@ __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __res
     acc[0] = csum_add(acc[0], ri * ri);
     acc[1] = csum_add(acc[1], ri * zi);
   }
-  float tot[2];
-  if (grid_reduce<2>(acc, rb, tot) && threadIdx.x == 0) {
+  block_partials<2>(acc, rb);
+}
+
+// Totals of k_verify: true <r,r>, <r,z> -> converged / stagnated / restart.
+template <int MODE>
+struct FinVerify {
+  Scalars* S;
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  __device__ __forceinline__ bool active() const { return S->done == 0 && S->verify != 0; }
+  __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
     const float best = S->nverify > 0 ? S->rr_best : tot[0] / 0.81f;
     S->verify = 0;
     S->nverify += 1;
@@ -466,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_verify(Rows rows, const float* __res
       else S->rz = tot[1];
     }
   }
-}
+};
 
 // x = x + xl (collapse the compensated iterate to FP32)
 __global__ __launch_bounds__(kBlock) void k_collapse(float* __restrict__ x, const float* __restrict__ xl, uint32_t n) {
@@ -477,16 +531,6 @@ __global__ __launch_bounds__(kBlock) void k_collapse(float* __restrict__ x, cons
 __global__ __launch_bounds__(kBlock) void k_copy(float* __restrict__ dst, const float* __restrict__ src, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
-}
-
-// Dot product <a, b> into out[0] (deterministic).
-__global__ __launch_bounds__(kBlock) void k_dot(const float* __restrict__ a, const float* __restrict__ b, uint32_t n,
-                                                float* __restrict__ out, GridRed rb) {
-  CSum acc[1] = {csum_zero()};
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-    acc[0] = csum_add(acc[0], a[i] * b[i]);
-  float tot[1];
-  if (grid_reduce<1>(acc, rb, tot) && threadIdx.x == 0) out[0] = tot[0];
 }
 
 }  // namespace msp

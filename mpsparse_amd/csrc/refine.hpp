@@ -8,12 +8,10 @@
 
 namespace msp {
 
-// Deterministic grid reduction of ND doubles (same protocol as grid_reduce).
+// Block-wide sums of ND doubles; result valid in thread 0.
 template <int ND>
-__device__ bool grid_reduce_d(double (&acc)[ND], double* partials, uint32_t* ticket, double (&tot)[ND]) {
+__device__ __forceinline__ void block_reduce_d(double (&acc)[ND]) {
   __shared__ double red[kWavesPerBlock][ND];
-  __shared__ int s_last;
-  __shared__ double s_tot[ND];
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
@@ -27,50 +25,37 @@ __device__ bool grid_reduce_d(double (&acc)[ND], double* partials, uint32_t* tic
     for (int d = 0; d < ND; ++d) {
       double t = 0.0;
       for (int w = 0; w < kWavesPerBlock; ++w) t += red[w][d];
-      partials[blockIdx.x * ND + d] = t;
+      acc[d] = t;
     }
-    __threadfence();
-    s_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
   }
   __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
+}
+
+// One workgroup: out[d] = sum over nb blocks of partials[b][d], in block order
+// of a fixed tree (deterministic).
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_finish_d(const double* __restrict__ partials, uint32_t nb,
+                                                     double* __restrict__ out) {
   double a[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) a[d] = 0.0;
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+  for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
 #pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] += *((volatile double*)(partials + i * ND + d));
+    for (int d = 0; d < ND; ++d) a[d] += partials[i * ND + d];
   }
-  __syncthreads();
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    double v = a[d];
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    if (lane == 0) red[wid][d] = v;
-  }
-  __syncthreads();
+  block_reduce_d<ND>(a);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      double t = 0.0;
-      for (int w = 0; w < kWavesPerBlock; ++w) t += red[w][d];
-      s_tot[d] = t;
-    }
-    *ticket = 0u;
+    for (int d = 0; d < ND; ++d) out[d] = a[d];
   }
-  __syncthreads();
-#pragma unroll
-  for (int d = 0; d < ND; ++d) tot[d] = s_tot[d];
-  return true;
 }
 
-// r = b - A x in FP64 (matrix values FP32, exact in FP64); out = {<r,r>, <b,b>}.
+// r = b - A x in FP64 (matrix values FP32, exact in FP64); block partials of
+// <r,r>, <b,b> (finish with k_finish_d<2>).
 template <typename Rows>
 __global__ __launch_bounds__(kBlock) void k_residual64(Rows rows, const double* __restrict__ x,
                                                        const double* __restrict__ b, double* __restrict__ r,
-                                                       uint32_t n, double* partials, uint32_t* ticket,
-                                                       double* __restrict__ out) {
+                                                       uint32_t n, double* __restrict__ partials) {
   double acc[2] = {0.0, 0.0};
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     double s = b[i];
@@ -79,10 +64,10 @@ __global__ __launch_bounds__(kBlock) void k_residual64(Rows rows, const double* 
     acc[0] += s * s;
     acc[1] += b[i] * b[i];
   }
-  double tot[2];
-  if (grid_reduce_d<2>(acc, partials, ticket, tot) && threadIdx.x == 0) {
-    out[0] = tot[0];
-    out[1] = tot[1];
+  block_reduce_d<2>(acc);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 2] = acc[0];
+    partials[blockIdx.x * 2 + 1] = acc[1];
   }
 }
 

@@ -30,6 +30,7 @@ struct CsrView {
   const uint32_t* __restrict__ col;
   const typename V::storage* __restrict__ val;
   const uint32_t* __restrict__ blk;  // [nblk + 1] first row of each block
+  const float* __restrict__ dlo;     // 16-bit formats: FP32 remainder of the diagonal
   uint32_t nblk;
   uint32_t n;
 };
@@ -105,7 +106,10 @@ __device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const
         for (uint32_t j = s0 + lane; j < s1; j += L) s += sprod[j];
       }
       s = group_reduce_sum(s, L);
-      if (lane == 0 && ri < nr) epi.row(r0 + ri, s, acc);
+      if (lane == 0 && ri < nr) {
+        if constexpr (V::kLowDiag) s += A.dlo[r0 + ri] * xl(r0 + ri);
+        epi.row(r0 + ri, s, acc);
+      }
     }
     __syncthreads();
   } else {
@@ -122,6 +126,7 @@ __device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const
       if (tid == 0) {
         float t = 0.0f;
         for (int w = 0; w < kWavesPerBlock; ++w) t += wpart[w];
+        if constexpr (V::kLowDiag) t += A.dlo[r] * xl(r);
         epi.row(r, t, acc);
       }
       __syncthreads();
@@ -184,22 +189,28 @@ struct EpiResidual {
   }
 };
 
-// Finishers run once, in thread 0 of the last workgroup, with the grid totals.
+// Finishers run once, in thread 0 of k_finish, with the grid totals.
 struct FinNone {
+  __device__ __forceinline__ bool active() const { return false; }
+  __device__ __forceinline__ void after() const {}
   template <int ND>
   __device__ __forceinline__ void operator()(const float (&)[ND]) const {}
 };
 
 struct FinStore {
   float* out;  // [ND]
+  __device__ __forceinline__ bool active() const { return true; }
+  __device__ __forceinline__ void after() const {}
   template <int ND>
   __device__ __forceinline__ void operator()(const float (&tot)[ND]) const {
     for (int d = 0; d < ND; ++d) out[d] = tot[d];
   }
 };
 
-template <typename V, int ND, typename XL, typename Epi, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, Fin fin, GridRed rb,
+// Block partials of the epilogue sums go to rb (ND > 0); the caller launches
+// k_finish<ND> after this kernel.
+template <typename V, int ND, typename XL, typename Epi>
+__global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, GridRed rb,
                                                  const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ float sprod[kCap];
@@ -214,12 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, F
   const uint32_t lo = wg * per + (wg < rem ? wg : rem);
   const uint32_t hi = lo + per + (wg < rem ? 1u : 0u);
   for (uint32_t b = lo; b < hi; ++b) csr_block<V>(A, b, xl, epi, sprod, acc);
-  if constexpr (ND > 0) {
-    float tot[ND];
-    if (grid_reduce<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb, tot)) {
-      if (threadIdx.x == 0) fin(tot);
-    }
-  }
+  if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
 }
 
 // Row blocks: blk[b] = first row whose first nonzero index is >= b*T
@@ -275,16 +281,46 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag(const uint32_t* __restrict_
 }
 
 // FP32 -> 16-bit value copies for the reduced-precision path.
-__global__ __launch_bounds__(kBlock) void k_cvt_bf16(const float* __restrict__ in, uint16_t* __restrict__ out,
-                                                     uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = f32_to_bf16(in[i]);
-}
-
-__global__ __launch_bounds__(kBlock) void k_cvt_f16(const float* __restrict__ in, uint16_t* __restrict__ out,
-                                                    uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = f32_to_f16(in[i]);
+// 16-bit copy of the values that keeps every row sum: off-diagonal a_ij are
+// rounded, the diagonal absorbs their rounding defects,
+//   d_i = a_ii + sum_{j != i} (a_ij - round(a_ij)),
+// and is stored as round(d_i) in the diagonal slot plus the FP32 remainder
+// dlo_i = d_i - round(d_i). For an M-matrix (diffusion) the 16-bit operator is
+// then the same operator with every coupling perturbed by at most half an ulp
+// of the format, so its energy norm is within that factor of A's. Duplicate
+// diagonal entries: the first slot carries round(d_i), the others 0. One
+// thread per row; padding past nnz is written as 0.
+template <bool BF16>
+__global__ __launch_bounds__(kBlock) void k_cvt16_rowsum(const uint32_t* __restrict__ row_ptr,
+                                                          const uint32_t* __restrict__ col,
+                                                          const float* __restrict__ in, uint32_t n, uint64_t nnz,
+                                                          uint16_t* __restrict__ out, float* __restrict__ dlo) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = nnz + tid; e < nnz + 64; e += stride) out[e] = 0;
+  for (uint64_t i = tid; i < n; i += stride) {
+    double d = 0.0;
+    int64_t dslot = -1;
+    for (uint32_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+      const float a = in[k];
+      if (col[k] == (uint32_t)i) {
+        d += (double)a;
+        if (dslot < 0) dslot = k;
+        else out[k] = 0;
+      } else {
+        const uint16_t h = BF16 ? f32_to_bf16(a) : f32_to_f16(a);
+        out[k] = h;
+        d += (double)a - (double)(BF16 ? ValBF16::cvt(h) : ValF16::cvt(h));
+      }
+    }
+    float hi = 0.0f;
+    if (dslot >= 0) {
+      const uint16_t h = BF16 ? f32_to_bf16((float)d) : f32_to_f16((float)d);
+      out[dslot] = h;
+      hi = BF16 ? ValBF16::cvt(h) : ValF16::cvt(h);
+    }
+    dlo[i] = (float)(d - (double)hi);
+  }
 }
 
 }  // namespace msp

@@ -178,8 +178,8 @@ class Validator:
         return info["iterations"] if (nonpos == 0 and info["status"] == "converged") else 10 ** 9
 
 
-def train(params=None, matrices=2000, imitate=300, updates_per_matrix=2, lr=2e-3, seed=0, device="cuda", log=print,
-          sigma_range=(0.5, 2.5), validate_every=100):
+def train(params=None, matrices=2000, imitate=1000, updates_per_matrix=2, lr=2e-3, probe_lr=5e-4, seed=0,
+          device="cuda", log=print, sigma_range=(0.5, 2.5), validate_every=100):
     """Train the edge MLP on freshly generated matrices.
 
     Matrices 0 .. imitate-1: regression of the output onto classical FSAI
@@ -231,7 +231,7 @@ def train(params=None, matrices=2000, imitate=300, updates_per_matrix=2, lr=2e-3
         else:
             W = harvest_probes(A, n, gen, device)
             kk = k - imitate
-            lr_t = lr * 0.5 * (1 + math.cos(math.pi * kk / max(1, matrices - imitate)))
+            lr_t = probe_lr * 0.5 * (1 + math.cos(math.pi * kk / max(1, matrices - imitate)))
             for _ in range(updates_per_matrix):
                 loss = F.train_step(params, W, grad) / W.shape[0]
                 step += 1
@@ -258,9 +258,10 @@ def main(argv=None):
 
     ap = argparse.ArgumentParser(description="train the learned-FSAI edge MLP on generated diffusion matrices")
     ap.add_argument("--matrices", type=int, default=2000)
-    ap.add_argument("--imitate", type=int, default=300, help="matrices of regression onto classical FSAI first")
+    ap.add_argument("--imitate", type=int, default=1000, help="matrices of regression onto classical FSAI first")
     ap.add_argument("--updates", type=int, default=2, help="Adam steps per matrix")
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=2e-3, help="Adam step of the regression phase")
+    ap.add_argument("--probe-lr", type=float, default=5e-4, help="peak Adam step of the probe phase (cosine decay)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--init", default=None, help="start from these weights")
@@ -268,10 +269,11 @@ def main(argv=None):
     import torch
 
     P = load_params(a.init) if a.init else None
-    P, hist = train(P, matrices=a.matrices, imitate=a.imitate, updates_per_matrix=a.updates, lr=a.lr, seed=a.seed,
-                    log=lambda s: print(s, flush=True))
+    P, hist = train(P, matrices=a.matrices, imitate=a.imitate, updates_per_matrix=a.updates, lr=a.lr,
+                    probe_lr=a.probe_lr, seed=a.seed, log=lambda s: print(s, flush=True))
     torch.cuda.synchronize()
     save_params(P, a.out, {"matrices": a.matrices, "imitate": a.imitate, "updates": a.updates, "lr": a.lr,
+                           "probe_lr": a.probe_lr,
                            "seed": a.seed, "validation": hist["validation"],
                            "best_validation_its": hist["best_validation_its"],
                            "classical_its": hist["classical_its"], "jacobi_its": hist["jacobi_its"]})

@@ -75,6 +75,7 @@ def features(row_ptr, col_ind, vals):
     np.add.at(s_low, rows_all, np.where(cols_all < rows_all, q, 0.0))
     f[:, 5] = s_low[r]
     f[:, 6] = s_all[r]
+    f[:, 7] = 1.0 / np.sqrt(np.maximum(1.0 - s_low[r], 1.0 / 64.0))
     return f, 1.0 / np.sqrt(d[c])
 
 

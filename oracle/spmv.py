@@ -10,6 +10,11 @@
   blocks, duplicates summed), an explicit restatement checked against scipy.
 * :func:`spmv_error_bound` -- the standard forward bound |y^ - y| <= gamma_k |A||x|
   with k the row length, used to size FP32 tolerances.
+* :func:`round16_rowsum` -- the library's 16-bit value copy (BF16 / FP16
+  off-diagonals, diagonal absorbing their rounding defects as a 16-bit slot
+  value plus an FP32 remainder). Pinned by row sums equal to A's, off-diagonals
+  equal to plain rounding, and, for diffusion M-matrices, the energy bound
+  |x^T (A16 - A) x| <= u x^T A x with u the format's unit roundoff.
 """
 from __future__ import annotations
 
@@ -79,3 +84,43 @@ def bsr_spmv(brow_ptr, bcol, blocks, x, dtype=np.float64):
     brows = np.repeat(np.arange(nbr), np.diff(brow_ptr))
     np.add.at(y, brows, prod)
     return y.reshape(-1)
+
+
+def round_bf16(a):
+    """Round-to-nearest-even FP32 -> BF16, widened back to float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def round16_rowsum(row_ptr, col_ind, vals, fmt="bf16"):
+    """(vals16 float64[nnz], dlo float64[n]): off-diagonals rounded to `fmt`,
+    d_i = a_ii + sum_{j != i}(a_ij - round(a_ij)) (FP64) stored as round(d_i) in
+    the first diagonal slot (other diagonal slots 0) and dlo_i = d_i - round(d_i)
+    rounded to FP32. y = vals16-SpMV + dlo * x is the library's 16-bit product."""
+    rnd = round_bf16 if fmt == "bf16" else (lambda a: np.asarray(a, np.float32).astype(np.float16).astype(np.float32))
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    col = np.asarray(col_ind, dtype=np.int64)
+    v = np.asarray(vals, dtype=np.float32)
+    n = row_ptr.size - 1
+    rows = np.repeat(np.arange(n), np.diff(row_ptr))
+    diag = col == rows
+    out = rnd(v).astype(np.float64)
+    defect = np.where(diag, v.astype(np.float64), v.astype(np.float64) - out)
+    d = np.zeros(n)
+    np.add.at(d, rows, defect)
+    out[diag] = 0.0
+    first = np.full(n, -1, dtype=np.int64)
+    idx = np.nonzero(diag)[0]
+    # first diagonal slot per row (indices ascend, so the minimum per row)
+    first_rows = rows[idx]
+    order = np.lexsort((idx, first_rows))
+    idx, first_rows = idx[order], first_rows[order]
+    keep = np.r_[True, first_rows[1:] != first_rows[:-1]] if idx.size else np.zeros(0, bool)
+    first[first_rows[keep]] = idx[keep]
+    hi = np.zeros(n)
+    has = first >= 0
+    hi[has] = rnd(d[has].astype(np.float32)).astype(np.float64)
+    out[first[has]] = hi[has]
+    dlo = (d - hi).astype(np.float32).astype(np.float64)
+    return out, dlo

@@ -52,13 +52,6 @@ def csr_from_arrays(row_ptr, col_ind, vals, n, m):
     return build_csr(keys, vals, n, m, export=False)[0]
 
 
-def bf16_round(a):
-    """Round-to-nearest-even FP32 -> BF16, returned widened to float32."""
-    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
-    return (u.astype(np.uint32) << 16).view(np.float32)
-
-
 class Lib:
     """Direct ctypes view of the C ABI (same library the extension links)."""
 

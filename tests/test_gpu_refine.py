@@ -52,3 +52,22 @@ def test_refined_bicgstab_nonsymmetric_host_vectors():
     info = A.solve_refined(torch.from_numpy(b), x, tol=1e-10, solver="bicgstab", values="bf16")
     rel = K.true_relres(S, x.numpy(), b)
     assert info["status"] == "converged" and rel <= 1.05e-10, (info, rel)
+
+
+@pytest.mark.parametrize("values", ["bf16", "f16"])
+def test_refined_16bit_contracts_on_high_contrast(values):
+    """sigma = 2 log-normal diffusion: the 16-bit copy keeps row sums (the
+    diagonal carries the off-diagonal rounding defects), so every outer step
+    gains about the inner tolerance: 1e-10 in at most 6 corrections."""
+    import torch
+
+    M = diffusion_3d(20, sigma=2.0, seed=5)
+    S = M.to_scipy()
+    b = np.random.default_rng(1).standard_normal(M.num_rows)
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    x = torch.zeros(M.num_rows, dtype=torch.float64, device="cuda")
+    say("refined cg sigma 2", values)
+    info = A.solve_refined(torch.from_numpy(b).cuda(), x, tol=1e-10, values=values, precond="jacobi")
+    rel = K.true_relres(S, x.cpu().numpy(), b)
+    assert info["status"] == "converged" and rel <= 1.05e-10, (info, rel)
+    assert info["outer_iterations"] <= 6, info

@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from gpu_util import Lib, bf16_round, build_csr, csr_from_arrays, say, t_f32
+from gpu_util import Lib, build_csr, csr_from_arrays, say, t_f32
 from mpsparse_amd.problems import block_sparse, diffusion_3d, laplacian_2d, pack_keys
 from oracle import coo_csr as oc
 from oracle import spmv as S
@@ -18,7 +18,7 @@ from oracle import spmv as S
 pytestmark = pytest.mark.gpu
 
 
-def _check(A, rp, ci, v, m, rng, values="f32", vals_used=None):
+def _check(A, rp, ci, v, m, rng, values="f32", vals_used=None, dlo=None):
     import torch
 
     x = rng.standard_normal(m).astype(np.float32)
@@ -28,6 +28,9 @@ def _check(A, rp, ci, v, m, rng, values="f32", vals_used=None):
     vu = v if vals_used is None else vals_used
     y64 = S.csr_spmv(rp, ci, vu, x)
     bound = 2 * S.spmv_error_bound(rp, ci, vu, x) + 1e-30
+    if dlo is not None:  # 16-bit values: the FP32 diagonal remainder is one more term
+        y64 = y64 + dlo * x[:dlo.size]
+        bound = bound + 4 * 2.0 ** -24 * np.abs(dlo * x[:dlo.size])
     err = np.abs(y.astype(np.float64) - y64)
     assert np.all(np.isfinite(y))
     assert np.all(err <= bound), (err.max(), np.argmax(err - bound))
@@ -88,9 +91,9 @@ def test_reduced_precision_values(values):
     M = diffusion_3d(32, sigma=1.0, seed=3)
     say("spmv", values)
     A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
-    used = bf16_round(M.vals) if values == "bf16" else M.vals.astype(np.float16).astype(np.float32)
+    used, dlo = S.round16_rowsum(M.row_ptr, M.col_ind, M.vals, values)
     _check(A, M.row_ptr, M.col_ind, M.vals, M.num_cols, np.random.default_rng(4), values=values,
-           vals_used=used)
+           vals_used=used, dlo=dlo)
 
 
 def test_direct_c_abi_call():

@@ -108,3 +108,5 @@ def test_row_coupling_features_give_classical_fsai_on_stencils():
                         minlength=S.shape[0])
     assert np.allclose(f[:, 6], s_all[r], rtol=1e-12)
     assert np.all(f[:, 6] >= f[:, 5])
+    # t (f[7]) is classical FSAI's diagonal output; off the diagonal -f0 t
+    assert np.allclose(out, np.where(r == c, 1.0, -f[:, 0]) * f[:, 7], rtol=1e-12)

@@ -162,3 +162,40 @@ def test_richardson_closed_form():
     for iters in (0, 1, 5, 40):
         x = K.richardson(A, b, omega, iters)
         assert np.allclose(x, K.richardson_closed_form(Ad, b, omega, iters), rtol=1e-12, atol=1e-12)
+
+
+def test_round16_rowsum_pins():
+    """The 16-bit value copy keeps A's row sums, rounds off-diagonals plainly and,
+    on a diffusion M-matrix, changes the energy by at most the unit roundoff."""
+    from mpsparse_amd.problems import diffusion_3d
+    from oracle import spmv as SP
+
+    M = diffusion_3d(6, 5, 4, sigma=2.0, seed=1)
+    rp, ci, v = M.row_ptr.astype(np.int64), M.col_ind.astype(np.int64), M.vals
+    n = M.num_rows
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    A = sp.csr_matrix((v.astype(np.float64), ci, rp), shape=(n, n))
+    rng = np.random.default_rng(0)
+    for fmt, u in (("bf16", 2.0 ** -8), ("f16", 2.0 ** -11)):
+        v16, dlo = SP.round16_rowsum(rp, ci, v, fmt)
+        A16 = sp.csr_matrix((v16, ci, rp), shape=(n, n)) + sp.diags(dlo)
+        rs, rs16 = np.asarray(A.sum(1)).ravel(), np.asarray(A16.sum(1)).ravel()
+        assert np.all(np.abs(rs16 - rs) <= 1e-6 * np.asarray(abs(A).sum(1)).ravel())
+        off = ci != rows
+        plain = SP.round_bf16(v) if fmt == "bf16" else v.astype(np.float16).astype(np.float32)
+        assert np.array_equal(v16[off], plain[off].astype(np.float64))
+        for _ in range(5):
+            x = rng.standard_normal(n)
+            e, e16 = x @ (A @ x), x @ (A16 @ x)
+            assert abs(e16 - e) <= u * e, (fmt, e, e16)
+        # the SpMV the kernels compute
+        y = SP.csr_spmv(rp, ci, v16, x) + dlo * x
+        assert np.allclose(y, A16 @ x, rtol=0, atol=1e-12 * np.abs(A16) @ np.abs(x))
+    # duplicate diagonal entries and a row without a diagonal
+    rp2 = np.array([0, 3, 4])
+    ci2 = np.array([0, 0, 1, 0])
+    v2 = np.array([2.0, 1.0, -0.3333333, 5.0], dtype=np.float32)
+    v16, dlo = SP.round16_rowsum(rp2, ci2, v2, "bf16")
+    assert v16[1] == 0.0 and v16[0] == SP.round_bf16(np.float32(3.0 + (-0.3333333 - SP.round_bf16(-0.3333333))))
+    assert abs(v16[0] + dlo[0] + v16[2] - (3.0 - 0.3333333)) < 1e-6
+    assert v16[3] == SP.round_bf16(5.0) and dlo[1] == (5.0 - SP.round_bf16(5.0))

@@ -105,9 +105,35 @@ typedef struct msp_refine_info {
   double seconds;           /* wall time of the whole refinement */
 } msp_refine_info;
 
+/* Batched Jacobi-PCG (msp_batch_pcg). */
+typedef struct msp_batch_opts {
+  double tol;              /* relative residual per (system, rhs); default 1e-6 */
+  int32_t maxiter;         /* per (system, rhs); default 10000 */
+  int32_t nrhs;            /* right-hand sides per system, 1..4; default 4 */
+  int32_t steps_per_graph; /* iterations per hipGraph launch (even); default 16 */
+  int32_t tile_rows;       /* rows per tile, multiple of 64; 0 = autotune (default) */
+  int32_t rows_per_thread; /* SpMM rows per thread, 1 or 2; 0 = autotune (default) */
+  int32_t max_verify;      /* true-residual restarts per pair; default 3 */
+  int32_t reserved[4];
+} msp_batch_opts;
+
+typedef struct msp_batch_info {
+  int32_t converged_pairs;
+  int32_t total_pairs;     /* num_systems * nrhs */
+  int32_t max_iterations;  /* over all pairs */
+  int32_t graph_launches;
+  int32_t verifications;   /* batch-wide true-residual passes */
+  int32_t tile_rows;       /* the schedule used */
+  int32_t rows_per_thread;
+  int32_t reserved;
+  double seconds;          /* device time of the solve (events) */
+  double setup_seconds;    /* host time of staging and tuning */
+} msp_batch_info;
+
 typedef struct msp_csr_s *msp_csr;
 typedef struct msp_bcsr_s *msp_bcsr;
 typedef struct msp_fsai_s *msp_fsai;
+typedef struct msp_batch_s *msp_batch;
 
 /* Library ----------------------------------------------------------------- */
 int32_t msp_abi_version(void);
@@ -235,6 +261,32 @@ msp_status msp_fsai_apply_network(msp_fsai F, const float *params, int32_t *nonp
 /* forward + probe loss (msp_fsai_probe_loss_grad) + backward: grad_params overwritten. */
 msp_status msp_fsai_train_step(msp_fsai F, const float *params, const float *W, int32_t k, float *grad_params,
                                double *loss, void *stream);
+
+/* Batched Jacobi-PCG ------------------------------------------------------
+ * Many independent SPD systems stored as the diagonal blocks of one CSR matrix
+ * A: system s owns rows and columns [sys_off[s], sys_off[s+1]) (host or
+ * device array of num_systems + 1 entries, from 0 to num_rows). Entries
+ * coupling two systems and rows without a positive diagonal are rejected. The
+ * handle borrows A (A must outlive it). The SpMM tile schedule is autotuned on
+ * the first solve; iterations run as hipGraph launches of steps_per_graph
+ * PCG steps over all still-active systems. The reference has no batched
+ * solver (one csr_tensor per system, iterative/BiCGSTAB/spmv.cpp:892-905);
+ * this is the batching addition.
+ * b, x: num_rows x nrhs, row-major (rhs fastest), host or device. The
+ * iterate is carried as a compensated FP32 pair; msp_batch_pcg returns it
+ * rounded to FP32 (a pair whose FP32-rounded solution cannot meet tol stops
+ * as stagnated), msp_batch_pcg_x64 returns the pair's exact sum in FP64.
+ * pair_status / pair_iterations / pair_relres: optional host arrays of
+ * num_systems * nrhs (msp_solve_status, iterations, true relative residual of
+ * the returned x). Calls on one handle must not overlap. */
+void msp_default_batch_opts(msp_batch_opts *opts);
+msp_status msp_batch_create(msp_batch *out, msp_csr A, const uint32_t *sys_off, uint32_t num_systems, void *stream);
+msp_status msp_batch_pcg(msp_batch B, const float *b, float *x, const msp_batch_opts *opts, msp_batch_info *info,
+                         int32_t *pair_status, int32_t *pair_iterations, double *pair_relres, void *stream);
+msp_status msp_batch_pcg_x64(msp_batch B, const float *b, double *x, const msp_batch_opts *opts,
+                             msp_batch_info *info, int32_t *pair_status, int32_t *pair_iterations,
+                             double *pair_relres, void *stream);
+msp_status msp_batch_destroy(msp_batch B);
 
 /* Block CSR ---------------------------------------------------------------
  * Square B x B blocks (B = 4, 8 or 16), row-major inside a block (the layout of

@@ -10,8 +10,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/mpsparse_amd.h"
+#include "batched.hpp"
 #include "bcsr.hpp"
 #include "common.hpp"
 #include "fsai.hpp"
@@ -1762,4 +1764,430 @@ extern "C" msp_status msp_fsai_train_step(msp_fsai F, const float* params, const
                                                         go.as<float>());
   MSPCHK(launch_check("scale grad"));
   return msp_mlp_backward(params, F->feat.as<float>(), go.as<float>(), F->ne, grad_params, stream);
+}
+
+// ---------------------------------------------------------------------------
+// ABI: batched Jacobi-PCG over the diagonal blocks of one CSR matrix
+using batch::PairState;
+
+struct msp_batch_s {
+  int device = 0;
+  msp_csr A = nullptr;  // borrowed
+  uint32_t nsys = 0, n = 0;
+  std::vector<uint32_t> h_sys_off;
+  DevMem sys_off, dinv;
+  // tiling (chosen by the tuner)
+  uint32_t T = 0, ntiles = 0;
+  int rpt = 1;
+  DevMem tile_r0, tile_sys, sys_tile, active, n_active;
+  DevMem partials, state, sys_active;
+  DevMem vec;  // X, XL, R, P0, P1, Q, B: n * 4 floats each
+  int32_t* h_active = nullptr;  // pinned, 2 * nsys
+  hipStream_t cs = nullptr;     // capture stream
+  hipEvent_t ev[2] = {nullptr, nullptr}, join = nullptr, t0 = nullptr, t1 = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0, graph_rpt = 0, graph_maxiter = -1;
+  bool tuned = false;
+  int tuned_for[2] = {-1, -1};  // (tile_rows, rows_per_thread) requested when tuned
+  ~msp_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (join) (void)hipEventDestroy(join);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (h_active) (void)hipHostFree(h_active);
+  }
+  float* v(int i) const { return vec.as<float>() + (size_t)i * 4 * n; }
+  batch::View view() const {
+    batch::View V;
+    V.row_ptr = A->row_ptr.as<uint32_t>();
+    V.col = A->col.as<uint32_t>();
+    V.val = A->val.as<float>();
+    V.dinv = dinv.as<float>();
+    V.tile_r0 = tile_r0.as<uint32_t>();
+    V.tile_sys = tile_sys.as<uint32_t>();
+    V.active = active.as<uint32_t>();
+    V.n_active = n_active.as<uint32_t>();
+    V.sys_active = sys_active.as<int32_t>();
+    return V;
+  }
+};
+
+enum { kBX = 0, kBXL, kBR, kBP0, kBP1, kBQ, kBB, kBVecs };
+
+// Tiles of at most T rows inside each system.
+static msp_status batch_tile(msp_batch B, uint32_t T, hipStream_t s) {
+  std::vector<uint32_t> r0, sy, st(B->nsys + 1, 0);
+  for (uint32_t q = 0; q < B->nsys; ++q) {
+    st[q] = (uint32_t)r0.size();
+    for (uint32_t i = B->h_sys_off[q]; i < B->h_sys_off[q + 1]; i += T) {
+      r0.push_back(i);
+      sy.push_back(q);
+    }
+  }
+  st[B->nsys] = (uint32_t)r0.size();
+  const uint32_t nt = (uint32_t)r0.size();
+  r0.push_back(B->n);
+  sy.push_back(0);
+  MSPCHK(B->tile_r0.alloc(sizeof(uint32_t) * r0.size()));
+  MSPCHK(B->tile_sys.alloc(sizeof(uint32_t) * sy.size()));
+  MSPCHK(B->sys_tile.alloc(sizeof(uint32_t) * st.size()));
+  MSPCHK(B->active.alloc(sizeof(uint32_t) * (nt + 1)));
+  MSPCHK(B->partials.alloc(sizeof(CSum) * 12 * ((size_t)nt + 1)));
+  HIPCHK(hipMemcpyAsync(B->tile_r0.p, r0.data(), sizeof(uint32_t) * r0.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(B->tile_sys.p, sy.data(), sizeof(uint32_t) * sy.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(B->sys_tile.p, st.data(), sizeof(uint32_t) * st.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+  B->T = T;
+  B->ntiles = nt;
+  if (B->exec) {
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+  }
+  return MSP_OK;
+}
+
+static uint32_t batch_grid(const msp_batch B) {
+  return std::max<uint32_t>(1, std::min<uint32_t>(B->ntiles, (uint32_t)cu_count() * 8));
+}
+static uint32_t sys_grid(const msp_batch B) {
+  return std::max<uint32_t>(1, (B->nsys + kWavesPerBlock - 1) / kWavesPerBlock);
+}
+
+static void launch_spmm(msp_batch B, int rpt, int parity, hipStream_t s) {
+  const batch::View V = B->view();
+  const uint32_t g = batch_grid(B);
+  const float* pold = B->v(parity ? kBP1 : kBP0);
+  float* pnew = B->v(parity ? kBP0 : kBP1);
+  if (rpt == 2)
+    batch::k_bt_spmm<2><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), B->state.as<PairState>(),
+                                             B->partials.as<CSum>());
+  else
+    batch::k_bt_spmm<1><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), B->state.as<PairState>(),
+                                             B->partials.as<CSum>());
+}
+
+// One PCG iteration (4 kernels); `parity` selects the direction buffers.
+static void batch_iteration(msp_batch B, int rpt, int parity, int32_t maxiter, hipStream_t s) {
+  const batch::View V = B->view();
+  const uint32_t g = batch_grid(B), gs = sys_grid(B);
+  launch_spmm(B, rpt, parity, s);
+  batch::k_bt_fin_alpha<<<gs, kBlock, 0, s>>>(B->partials.as<CSum>(), B->sys_tile.as<uint32_t>(), B->nsys,
+                                               B->state.as<PairState>(), B->sys_active.as<int32_t>());
+  batch::k_bt_update<<<g, kBlock, 0, s>>>(V, B->v(kBX), B->v(kBXL), B->v(kBR), B->v(parity ? kBP0 : kBP1),
+                                          B->v(kBQ), B->state.as<PairState>(), B->partials.as<CSum>());
+  batch::k_bt_fin_beta<<<gs, kBlock, 0, s>>>(B->partials.as<CSum>(), B->sys_tile.as<uint32_t>(), B->nsys, maxiter,
+                                              B->state.as<PairState>(), B->sys_active.as<int32_t>());
+}
+
+static void batch_active_list(msp_batch B, hipStream_t s) {
+  batch::k_bt_active_list<<<1, kBlock, 0, s>>>(B->sys_tile.as<uint32_t>(), B->nsys, B->sys_active.as<int32_t>(),
+                                               B->active.as<uint32_t>(), B->n_active.as<uint32_t>());
+}
+
+// hipGraph of (active list + `steps` iterations), captured once per
+// configuration on the handle's own stream.
+static msp_status batch_graph(msp_batch B, int steps, int32_t maxiter) {
+  if (B->exec && B->graph_steps == steps && B->graph_rpt == B->rpt && B->graph_maxiter == maxiter) return MSP_OK;
+  if (B->exec) {
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(B->cs, hipStreamCaptureModeThreadLocal));
+  batch_active_list(B, B->cs);
+  for (int k = 0; k < steps; ++k) batch_iteration(B, B->rpt, k & 1, maxiter, B->cs);
+  hipError_t e = hipStreamEndCapture(B->cs, &graph);
+  if (e != hipSuccess) return fail(MSP_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    B->exec = nullptr;
+    return fail(MSP_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  B->graph_steps = steps;
+  B->graph_rpt = B->rpt;
+  B->graph_maxiter = maxiter;
+  return MSP_OK;
+}
+
+extern "C" void msp_default_batch_opts(msp_batch_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->tol = 1e-6;
+  o->maxiter = 10000;
+  o->nrhs = 4;
+  o->steps_per_graph = 16;
+  o->tile_rows = 0;
+  o->rows_per_thread = 0;
+  o->max_verify = 3;
+}
+
+extern "C" msp_status msp_batch_create(msp_batch* out, msp_csr A, const uint32_t* sys_off, uint32_t num_systems,
+                                       void* stream) {
+  if (!out) return fail(MSP_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  MSPCHK(check_handle(A));
+  if (A->n != A->m) return fail(MSP_ERR_INVALID, "batched systems need a square matrix");
+  if (!sys_off && num_systems) return fail(MSP_ERR_INVALID, "null sys_off");
+  hipStream_t s = (hipStream_t)stream;
+  msp_batch B = new msp_batch_s;
+  B->device = A->device;
+  B->A = A;
+  B->nsys = num_systems;
+  B->n = A->n;
+  auto body = [&]() -> msp_status {
+    B->h_sys_off.resize((size_t)num_systems + 1);
+    if (is_device_ptr(sys_off)) {
+      HIPCHK(hipMemcpy(B->h_sys_off.data(), sys_off, sizeof(uint32_t) * (num_systems + 1), hipMemcpyDeviceToHost));
+    } else {
+      std::memcpy(B->h_sys_off.data(), sys_off, sizeof(uint32_t) * (num_systems + 1));
+    }
+    if (B->h_sys_off[0] != 0 || B->h_sys_off[num_systems] != A->n)
+      return fail(MSP_ERR_INVALID, "sys_off must run from 0 to num_rows (%u)", A->n);
+    for (uint32_t q = 0; q < num_systems; ++q)
+      if (B->h_sys_off[q + 1] < B->h_sys_off[q]) return fail(MSP_ERR_INVALID, "sys_off must not decrease");
+    MSPCHK(B->sys_off.alloc(sizeof(uint32_t) * (num_systems + 1)));
+    HIPCHK(hipMemcpyAsync(B->sys_off.p, B->h_sys_off.data(), sizeof(uint32_t) * (num_systems + 1),
+                          hipMemcpyHostToDevice, s));
+    MSPCHK(batch_tile(B, 1024, s));
+    DevMem bad;
+    MSPCHK(bad.alloc(sizeof(uint32_t) * 2));
+    HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(uint32_t) * 2, s));
+    batch::k_bt_check<<<batch_grid(B), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
+                                                       B->tile_r0.as<uint32_t>(), B->tile_sys.as<uint32_t>(),
+                                                       B->sys_off.as<uint32_t>(), B->ntiles, bad.as<uint32_t>());
+    MSPCHK(B->dinv.alloc(sizeof(float) * ((size_t)A->n + 1)));
+    batch::k_bt_dinv<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
+                                                       A->val.as<float>(), A->n, B->dinv.as<float>(),
+                                                       bad.as<uint32_t>() + 1);
+    MSPCHK(launch_check("batch check"));
+    uint32_t hb[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(hb, bad.p, sizeof hb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hb[0]) return fail(MSP_ERR_INVALID, "%u entries couple different systems (not block diagonal)", hb[0]);
+    if (hb[1]) return fail(MSP_ERR_SINGULAR, "%u rows without a positive diagonal (Jacobi needs one)", hb[1]);
+    MSPCHK(B->state.alloc(sizeof(PairState) * batch::kRhs * ((size_t)num_systems + 1)));
+    MSPCHK(B->sys_active.alloc(sizeof(int32_t) * ((size_t)num_systems + 1)));
+    MSPCHK(B->n_active.alloc(sizeof(uint32_t)));
+    MSPCHK(B->vec.alloc(sizeof(float) * 4 * (size_t)A->n * kBVecs + 64));
+    HIPCHK(hipMemsetAsync(B->vec.p, 0, sizeof(float) * 4 * (size_t)A->n * kBVecs, s));
+    HIPCHK(hipMemsetAsync(B->state.p, 0, sizeof(PairState) * batch::kRhs * ((size_t)num_systems + 1), s));
+    HIPCHK(hipHostMalloc((void**)&B->h_active, sizeof(int32_t) * 2 * ((size_t)num_systems + 1),
+                         hipHostMallocDefault));
+    HIPCHK(hipStreamCreateWithFlags(&B->cs, hipStreamNonBlocking));
+    for (auto& e : B->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&B->join, hipEventDisableTiming));
+    HIPCHK(hipEventCreate(&B->t0));
+    HIPCHK(hipEventCreate(&B->t1));
+    HIPCHK(hipStreamSynchronize(s));
+    return MSP_OK;
+  };
+  msp_status st = body();
+  if (st != MSP_OK) {
+    delete B;
+    return st;
+  }
+  *out = B;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_batch_destroy(msp_batch B) {
+  if (!B) return MSP_OK;
+  (void)hipSetDevice(B->device);
+  (void)hipDeviceSynchronize();
+  delete B;
+  return MSP_OK;
+}
+
+// Tile size and rows per thread for k_bt_spmm: each candidate is timed (3
+// launches after one warm-up, all tiles active) and the fastest kept.
+static msp_status batch_tune(msp_batch B, const msp_batch_opts& o, hipStream_t s) {
+  std::vector<uint32_t> Ts = {256, 512, 1024, 2048};
+  std::vector<int> rpts = {1, 2};
+  if (o.tile_rows > 0) Ts = {(uint32_t)o.tile_rows};
+  if (o.rows_per_thread > 0) rpts = {o.rows_per_thread};
+  float best = 1e30f;
+  uint32_t bestT = Ts[0];
+  int bestR = rpts[0];
+  const bool single = Ts.size() == 1 && rpts.size() == 1;
+  for (uint32_t T : Ts) {
+    if (B->T != T) MSPCHK(batch_tile(B, T, s));
+    if (single) break;
+    // every system active, every beta 0 (the state is reset by the solve's init)
+    std::vector<int32_t> ones(B->nsys, 1);
+    HIPCHK(hipMemcpyAsync(B->sys_active.p, ones.data(), sizeof(int32_t) * B->nsys, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(B->state.p, 0, sizeof(PairState) * batch::kRhs * B->nsys, s));
+    batch_active_list(B, s);
+    for (int rpt : rpts) {
+      launch_spmm(B, rpt, 0, s);
+      HIPCHK(hipEventRecord(B->t0, s));
+      for (int k = 0; k < 3; ++k) launch_spmm(B, rpt, 0, s);
+      HIPCHK(hipEventRecord(B->t1, s));
+      HIPCHK(hipEventSynchronize(B->t1));
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, B->t0, B->t1));
+      if (ms < best) {
+        best = ms;
+        bestT = T;
+        bestR = rpt;
+      }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  if (B->T != bestT) MSPCHK(batch_tile(B, bestT, s));
+  B->rpt = bestR;
+  return MSP_OK;
+}
+
+static msp_status batch_pcg_impl(msp_batch B, const float* b, void* x, bool x64, const msp_batch_opts* opts,
+                                 msp_batch_info* info, int32_t* pair_status, int32_t* pair_iterations,
+                                 double* pair_relres, void* stream) {
+  if (!B) return fail(MSP_ERR_INVALID, "null batch handle");
+  HIPCHK(hipSetDevice(B->device));
+  msp_batch_opts o;
+  msp_default_batch_opts(&o);
+  if (opts) o = *opts;
+  if (o.nrhs < 1 || o.nrhs > batch::kRhs) return fail(MSP_ERR_INVALID, "nrhs must be 1..4");
+  if (o.tol < 0 || o.maxiter < 0) return fail(MSP_ERR_INVALID, "negative tol / maxiter");
+  if (o.steps_per_graph < 2 || (o.steps_per_graph & 1)) return fail(MSP_ERR_INVALID, "steps_per_graph: even, >= 2");
+  if (o.tile_rows < 0 || (o.tile_rows > 0 && o.tile_rows % 64)) return fail(MSP_ERR_INVALID, "tile_rows: multiple of 64");
+  if (o.rows_per_thread < 0 || o.rows_per_thread > 2) return fail(MSP_ERR_INVALID, "rows_per_thread: 0 (tune), 1, 2");
+  if ((!b || !x) && B->n) return fail(MSP_ERR_INVALID, "null b / x");
+  hipStream_t s = (hipStream_t)stream;
+  const auto wall0 = std::chrono::steady_clock::now();
+  const uint32_t n = B->n, gv = vec_grid(n);
+  // right-hand sides into the [n][4] layout
+  const size_t bytes = sizeof(float) * (size_t)n * o.nrhs;
+  InArr bin;
+  MSPCHK(bin.set(b, bytes, s));
+  if (o.nrhs == batch::kRhs)
+    HIPCHK(hipMemcpyAsync(B->v(kBB), bin.d, bytes, hipMemcpyDeviceToDevice, s));
+  else
+    batch::k_bt_pack<<<gv, kBlock, 0, s>>>(static_cast<const float*>(bin.d), n, o.nrhs, B->v(kBB));
+  if (!B->tuned || B->tuned_for[0] != o.tile_rows || B->tuned_for[1] != o.rows_per_thread) {
+    MSPCHK(batch_tune(B, o, s));
+    B->tuned = true;
+    B->tuned_for[0] = o.tile_rows;
+    B->tuned_for[1] = o.rows_per_thread;
+  }
+  const auto wall1 = std::chrono::steady_clock::now();
+  // the solve runs on the handle's capture stream, ordered after the caller's work
+  HIPCHK(hipEventRecord(B->join, s));
+  HIPCHK(hipStreamWaitEvent(B->cs, B->join, 0));
+  MSPCHK(batch_graph(B, o.steps_per_graph, o.maxiter));
+  hipStream_t cs = B->cs;
+  HIPCHK(hipEventRecord(B->t0, cs));
+  const batch::View V = B->view();
+  batch::k_bt_init<<<batch_grid(B), kBlock, 0, cs>>>(V, B->v(kBB), B->v(kBX), B->v(kBXL), B->v(kBR), B->v(kBP0),
+                                                     B->ntiles, B->partials.as<CSum>());
+  batch::k_bt_fin_init<<<sys_grid(B), kBlock, 0, cs>>>(B->partials.as<CSum>(), B->sys_tile.as<uint32_t>(), B->nsys,
+                                                       o.nrhs, (float)o.tol, o.maxiter, B->state.as<PairState>(),
+                                                       B->sys_active.as<int32_t>());
+  MSPCHK(launch_check("batch init"));
+  const size_t act_bytes = sizeof(int32_t) * B->nsys;
+  auto any_active = [&](int slot) {
+    const int32_t* a = B->h_active + (size_t)slot * (B->nsys + 1);
+    for (uint32_t q = 0; q < B->nsys; ++q)
+      if (a[q]) return true;
+    return false;
+  };
+  int launches = 0, verifications = 0;
+  const int max_launches = (o.maxiter / o.steps_per_graph + 3) * (o.max_verify + 2);
+  while (true) {
+    // graphs until the trailing poll sees no active system
+    int slot = 0;
+    bool have_prev = false;
+    while (launches < max_launches) {
+      HIPCHK(hipGraphLaunch(B->exec, cs));
+      ++launches;
+      HIPCHK(hipMemcpyAsync(B->h_active + (size_t)slot * (B->nsys + 1), B->sys_active.p, act_bytes,
+                            hipMemcpyDeviceToHost, cs));
+      HIPCHK(hipEventRecord(B->ev[slot], cs));
+      if (have_prev) {
+        HIPCHK(hipEventSynchronize(B->ev[slot ^ 1]));
+        if (!any_active(slot ^ 1)) break;
+      }
+      have_prev = true;
+      slot ^= 1;
+    }
+    // true residuals; converged pairs above tol restart
+    if (x64)
+      batch::k_bt_residual<true><<<batch_grid(B), kBlock, 0, cs>>>(V, B->v(kBB), B->v(kBX), B->v(kBXL), B->v(kBR),
+                                                                   B->ntiles, B->partials.as<CSum>());
+    else
+      batch::k_bt_residual<false><<<batch_grid(B), kBlock, 0, cs>>>(V, B->v(kBB), B->v(kBX), B->v(kBXL), B->v(kBR),
+                                                                    B->ntiles, B->partials.as<CSum>());
+    batch::k_bt_fin_verify<<<sys_grid(B), kBlock, 0, cs>>>(B->partials.as<CSum>(), B->sys_tile.as<uint32_t>(),
+                                                           B->nsys, o.maxiter, o.max_verify, x64 ? 1 : 0,
+                                                           B->state.as<PairState>(), B->sys_active.as<int32_t>());
+    MSPCHK(launch_check("batch verify"));
+    ++verifications;
+    HIPCHK(hipMemcpyAsync(B->h_active, B->sys_active.p, act_bytes, hipMemcpyDeviceToHost, cs));
+    HIPCHK(hipStreamSynchronize(cs));
+    if (!any_active(0) || launches >= max_launches) break;
+  }
+  HIPCHK(hipEventRecord(B->t1, cs));
+  // solution out (x + xl), ordered before the caller's later work
+  {
+    const size_t xbytes = (x64 ? sizeof(double) : sizeof(float)) * (size_t)n * o.nrhs;
+    DevMem tmp;
+    void* dst = x;
+    if (!is_device_ptr(x)) {
+      MSPCHK(tmp.alloc(xbytes + 16));
+      dst = tmp.p;
+    }
+    if (x64)
+      batch::k_bt_unpack64<<<gv, kBlock, 0, cs>>>(B->v(kBX), B->v(kBXL), n, o.nrhs, static_cast<double*>(dst));
+    else
+      batch::k_bt_unpack<<<gv, kBlock, 0, cs>>>(B->v(kBX), B->v(kBXL), n, o.nrhs, static_cast<float*>(dst));
+    MSPCHK(launch_check("batch unpack"));
+    if (dst != x) MSPCHK(copy_out(x, dst, xbytes, cs));
+  }
+  std::vector<PairState> ps((size_t)B->nsys * batch::kRhs);
+  HIPCHK(hipMemcpyAsync(ps.data(), B->state.p, sizeof(PairState) * ps.size(), hipMemcpyDeviceToHost, cs));
+  HIPCHK(hipStreamSynchronize(cs));
+  HIPCHK(hipEventRecord(B->join, cs));
+  HIPCHK(hipStreamWaitEvent(s, B->join, 0));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, B->t0, B->t1);
+  msp_batch_info inf{};
+  inf.total_pairs = (int32_t)(B->nsys * o.nrhs);
+  for (uint32_t q = 0; q < B->nsys; ++q)
+    for (int k = 0; k < o.nrhs; ++k) {
+      const PairState& p = ps[(size_t)q * batch::kRhs + k];
+      const size_t idx = (size_t)q * o.nrhs + k;
+      const int32_t st = p.status == batch::kPairRunning ? MSP_SOLVE_MAXITER
+                         : p.status == batch::kPairConverged ? MSP_SOLVE_CONVERGED
+                         : p.status == batch::kPairBreakdown ? MSP_SOLVE_BREAKDOWN
+                         : p.status == batch::kPairStagnated ? MSP_SOLVE_STAGNATED
+                                                             : MSP_SOLVE_MAXITER;
+      if (st == MSP_SOLVE_CONVERGED) inf.converged_pairs += 1;
+      inf.max_iterations = std::max(inf.max_iterations, p.iter);
+      if (pair_status) pair_status[idx] = st;
+      if (pair_iterations) pair_iterations[idx] = p.iter;
+      if (pair_relres) pair_relres[idx] = p.bb > 0 ? std::sqrt((double)p.rr_true / (double)p.bb) : 0.0;
+    }
+  inf.graph_launches = launches;
+  inf.verifications = verifications;
+  inf.tile_rows = (int32_t)B->T;
+  inf.rows_per_thread = B->rpt;
+  inf.seconds = ms * 1e-3;
+  inf.setup_seconds = std::chrono::duration<double>(wall1 - wall0).count();
+  if (info) *info = inf;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_batch_pcg(msp_batch B, const float* b, float* x, const msp_batch_opts* opts,
+                                    msp_batch_info* info, int32_t* pair_status, int32_t* pair_iterations,
+                                    double* pair_relres, void* stream) {
+  return batch_pcg_impl(B, b, x, false, opts, info, pair_status, pair_iterations, pair_relres, stream);
+}
+
+extern "C" msp_status msp_batch_pcg_x64(msp_batch B, const float* b, double* x, const msp_batch_opts* opts,
+                                        msp_batch_info* info, int32_t* pair_status, int32_t* pair_iterations,
+                                        double* pair_relres, void* stream) {
+  return batch_pcg_impl(B, b, x, true, opts, info, pair_status, pair_iterations, pair_relres, stream);
 }

@@ -21,7 +21,9 @@ constexpr int kNumXcd = 8;            // MI355X: 8 XCDs, workgroups dispatched r
 
 // ---------------------------------------------------------------------------
 // Compensated (TwoSum) FP32 accumulation: s + c carries ~2x the precision of a
-// plain float sum. No multiplications, so FMA contraction cannot disturb it.
+// plain float sum. The error-free transformations below must not be contracted
+// into FMAs with a product computed by the caller (hipcc contracts across
+// inlined statements by default), hence `fp contract(off)` in each body.
 struct CSum {
   float s, c;
 };
@@ -29,6 +31,7 @@ struct CSum {
 __device__ __forceinline__ CSum csum_zero() { return CSum{0.0f, 0.0f}; }
 
 __device__ __forceinline__ CSum csum_add(CSum x, float v) {
+#pragma clang fp contract(off)
   float s = x.s + v;
   float bp = s - x.s;
   float e = (x.s - (s - bp)) + (v - bp);
@@ -36,6 +39,7 @@ __device__ __forceinline__ CSum csum_add(CSum x, float v) {
 }
 
 __device__ __forceinline__ CSum csum_merge(CSum x, CSum y) {
+#pragma clang fp contract(off)
   float s = x.s + y.s;
   float bp = s - x.s;
   float e = (x.s - (s - bp)) + (y.s - bp);
@@ -43,6 +47,19 @@ __device__ __forceinline__ CSum csum_merge(CSum x, CSum y) {
 }
 
 __device__ __forceinline__ float csum_val(CSum x) { return x.s + x.c; }
+
+// (x, xl) += a * p with the rounding errors of the product (FMA) and of the sum
+// (TwoSum) carried in xl: x + xl holds the iterate to about twice FP32 precision.
+__device__ __forceinline__ void comp_axpy(float& x, float& xl, float a, float p) {
+#pragma clang fp contract(off)
+  const float t = a * p;
+  const float te = __builtin_fmaf(a, p, -t);
+  const float s = x + t;
+  const float bp = s - x;
+  const float e = (x - (s - bp)) + (t - bp);
+  x = s;
+  xl += e + te;
+}
 
 __device__ __forceinline__ CSum wave_reduce(CSum v) {
 #pragma unroll

@@ -136,17 +136,6 @@ __device__ __forceinline__ bool step_converged(const Scalars* S, int it, float r
 
 __device__ __forceinline__ uint32_t vec_chunks(uint32_t n) { return (n + 3u) / 4u; }
 
-// (x, xl) += a * p with the rounding errors of the product (FMA) and of the sum
-// (TwoSum) carried in xl: x + xl holds the iterate to about twice FP32 precision.
-__device__ __forceinline__ void comp_axpy(float& x, float& xl, float a, float p) {
-  const float t = a * p;
-  const float te = __builtin_fmaf(a, p, -t);
-  const float s = x + t;
-  const float bp = s - x;
-  const float e = (x - (s - bp)) + (t - bp);
-  x = s;
-  xl += e + te;
-}
 
 // Initialisation: x = 0 (or kept), r = b (or precomputed b - A x0), p = z,
 // <b,b>, <r,r>, <r,z>.

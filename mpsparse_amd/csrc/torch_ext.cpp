@@ -13,6 +13,7 @@
 #include <c10/hip/HIPGuard.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/mpsparse_amd.h"
 
@@ -418,6 +419,86 @@ py::dict pcg_fsai_refined(CsrTensor& A, FsaiTensor& F, torch::Tensor b, torch::T
   return refine_dict(info);
 }
 
+// Batched Jacobi-PCG over the diagonal blocks of a csr_tensor (kept alive by a
+// reference held on the Python side).
+class BatchTensor {
+ public:
+  BatchTensor(CsrTensor& A, torch::Tensor sys_off) : device_(A.device()) {
+    TORCH_CHECK(sys_off.dim() == 1 && sys_off.numel() >= 1, "sys_off must be 1-D with num_systems + 1 entries");
+    auto so = sys_off.to(torch::kCPU).to(torch::kInt64).contiguous();
+    std::vector<uint32_t> h((size_t)so.numel());
+    const int64_t* p = so.data_ptr<int64_t>();
+    for (size_t i = 0; i < h.size(); ++i) {
+      TORCH_CHECK(p[i] >= 0 && p[i] < (1ll << 32), "sys_off out of range");
+      h[i] = (uint32_t)p[i];
+    }
+    nsys_ = (uint32_t)h.size() - 1;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    py::gil_scoped_release nogil;
+    check(msp_batch_create(&h_, A.raw(), h.data(), nsys_, s), "batch_tensor");
+  }
+  ~BatchTensor() {
+    if (h_) msp_batch_destroy(h_);
+  }
+  int64_t num_systems() const { return nsys_; }
+  py::dict pcg(torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t steps_per_graph,
+               int64_t tile_rows, int64_t rows_per_thread, int64_t max_verify) {
+    TORCH_CHECK(b.dim() == 1 || b.dim() == 2, "b must be (n,) or (n, nrhs)");
+    const int64_t nrhs = b.dim() == 1 ? 1 : b.size(1);
+    TORCH_CHECK(nrhs >= 1 && nrhs <= 4, "1..4 right-hand sides");
+    expect(b, torch::kFloat32, -1, "b");
+    const bool x64 = x.scalar_type() == torch::kFloat64;
+    expect(x, x64 ? torch::kFloat64 : torch::kFloat32, b.numel(), "x");
+    msp_batch_opts o;
+    msp_default_batch_opts(&o);
+    o.tol = tol;
+    o.maxiter = (int32_t)maxiter;
+    o.nrhs = (int32_t)nrhs;
+    o.steps_per_graph = (int32_t)steps_per_graph;
+    o.tile_rows = (int32_t)tile_rows;
+    o.rows_per_thread = (int32_t)rows_per_thread;
+    o.max_verify = (int32_t)max_verify;
+    const int64_t np = (int64_t)nsys_ * nrhs;
+    auto st = torch::empty({np}, torch::kInt32);
+    auto it = torch::empty({np}, torch::kInt32);
+    auto rel = torch::empty({np}, torch::kFloat64);
+    msp_batch_info info{};
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_for(device_);
+    {
+      py::gil_scoped_release nogil;
+      if (x64)
+        check(msp_batch_pcg_x64(h_, b.data_ptr<float>(), x.data_ptr<double>(), &o, &info, st.data_ptr<int32_t>(),
+                                it.data_ptr<int32_t>(), rel.data_ptr<double>(), s),
+              "batch pcg");
+      else
+        check(msp_batch_pcg(h_, b.data_ptr<float>(), x.data_ptr<float>(), &o, &info, st.data_ptr<int32_t>(),
+                            it.data_ptr<int32_t>(), rel.data_ptr<double>(), s),
+              "batch pcg");
+    }
+    py::dict d;
+    d["converged_pairs"] = info.converged_pairs;
+    d["total_pairs"] = info.total_pairs;
+    d["max_iterations"] = info.max_iterations;
+    d["graph_launches"] = info.graph_launches;
+    d["verifications"] = info.verifications;
+    d["tile_rows"] = info.tile_rows;
+    d["rows_per_thread"] = info.rows_per_thread;
+    d["seconds"] = info.seconds;
+    d["setup_seconds"] = info.setup_seconds;
+    d["status"] = st.view({(int64_t)nsys_, nrhs});
+    d["iterations"] = it.view({(int64_t)nsys_, nrhs});
+    d["relres"] = rel.view({(int64_t)nsys_, nrhs});
+    return d;
+  }
+
+ private:
+  int device_ = 0;
+  uint32_t nsys_ = 0;
+  msp_batch h_ = nullptr;
+};
+
 // bcsr.bcsr_tensor(keys, values, row_ptr, col_ind, num_rows, num_cols, block_size)
 // (iterative/BiCGSTAB/bcsr.cpp:357-422, 953-965): with block arrays in
 // values/row_ptr/col_ind (keys unused, as in the reference) or, when row_ptr is
@@ -669,6 +750,15 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def_property_readonly("nnz", &CsrTensor::nnz)
       .def_property_readonly("device", &CsrTensor::device)
       .def_property_readonly("handle", &CsrTensor::handle);
+  py::class_<BatchTensor>(m, "batch_tensor")
+      .def(py::init<CsrTensor&, torch::Tensor>(), py::arg("A"), py::arg("sys_off"), py::keep_alive<1, 2>())
+      .def_property_readonly("num_systems", &BatchTensor::num_systems)
+      .def("pcg", &BatchTensor::pcg,
+           "Jacobi-PCG of every (system, rhs) pair (x float32, or float64 for the compensated iterate's exact "
+           "sum); status codes 1 converged, 2 breakdown, 3 maxiter, 4 stagnated",
+           py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("maxiter") = 10000,
+           py::arg("steps_per_graph") = 16, py::arg("tile_rows") = 0, py::arg("rows_per_thread") = 0,
+           py::arg("max_verify") = 3);
   py::class_<FsaiTensor>(m, "fsai_tensor")
       .def(py::init<CsrTensor&>(), py::arg("A"), py::keep_alive<1, 2>())
       .def_property_readonly("num_edges", &FsaiTensor::num_edges)

@@ -23,7 +23,7 @@ import time
 import numpy as np
 
 from . import load_ext
-from .problems import diffusion_stencil
+from .problems import diffusion_keys_from_k, diffusion_keys_torch  # noqa: F401 (re-exported)
 
 _C = load_ext()
 fsai_tensor = _C.fsai_tensor
@@ -86,32 +86,6 @@ def solve_refined(A, F, b, x, **kw):
 
 # ---------------------------------------------------------------------------
 # training data on the GPU
-def diffusion_keys_torch(shape, sigma, gen, device="cuda"):
-    """Packed keys + FP32 values of a log-normal diffusion matrix
-    (problems.diffusion_stencil), generated with torch on ``device``.
-    shape = (nz, ny, nx); nz = 1 gives the 5-point 2D operator."""
-    import torch
-
-    k = torch.exp(sigma * torch.randn(shape, generator=gen, device=device, dtype=torch.float64))
-    return diffusion_keys_from_k(k)
-
-
-def diffusion_keys_from_k(k):
-    """(keys int64[nnz], vals float32[nnz], n) of the diffusion operator of a
-    torch conductivity array k (nz, ny, nx), in row-major stencil order."""
-    import torch
-
-    dev = k.device
-    cols, vals = diffusion_stencil(k, lambda s, f: torch.full(s, f, dtype=torch.int64, device=dev),
-                                   lambda s: torch.zeros(s, dtype=torch.float64, device=dev),
-                                   lambda m: torch.arange(m, dtype=torch.int64, device=dev))
-    n = cols.shape[0]
-    rows = torch.arange(n, device=dev, dtype=torch.int64)[:, None].expand(n, 7)
-    m = cols >= 0
-    keys = (rows[m] << 32) | cols[m]
-    return keys.contiguous(), vals[m].float().contiguous(), n
-
-
 def sample_shape(rng: np.random.Generator):
     """2D side in [128, 1000] or 3D side in [26, 100] (n between ~16k and 1M)."""
     if rng.random() < 0.5:

@@ -318,6 +318,32 @@ def diffusion_3d(nx: int, ny: int | None = None, nz: int | None = None, sigma: f
     return _stencil_to_csr(cols, vals, n, meta)
 
 
+def diffusion_keys_torch(shape, sigma, gen, device="cuda"):
+    """Packed keys + FP32 values of a log-normal diffusion matrix
+    (:func:`diffusion_stencil`), generated with torch on ``device``.
+    shape = (nz, ny, nx); nz = 1 gives the 5-point 2D operator."""
+    import torch
+
+    k = torch.exp(sigma * torch.randn(shape, generator=gen, device=device, dtype=torch.float64))
+    return diffusion_keys_from_k(k)
+
+
+def diffusion_keys_from_k(k):
+    """(keys int64[nnz], vals float32[nnz], n) of the diffusion operator of a
+    torch conductivity array k (nz, ny, nx), in row-major stencil order."""
+    import torch
+
+    dev = k.device
+    cols, vals = diffusion_stencil(k, lambda s, f: torch.full(s, f, dtype=torch.int64, device=dev),
+                                   lambda s: torch.zeros(s, dtype=torch.float64, device=dev),
+                                   lambda m: torch.arange(m, dtype=torch.int64, device=dev))
+    n = cols.shape[0]
+    rows = torch.arange(n, device=dev, dtype=torch.int64)[:, None].expand(n, 7)
+    m = cols >= 0
+    keys = (rows[m] << 32) | cols[m]
+    return keys.contiguous(), vals[m].float().contiguous(), n
+
+
 def diffusion_2d(nx: int, ny: int | None = None, sigma: float = 1.5, seed: int = 0) -> CsrMatrix:
     """5-point analogue of :func:`diffusion_3d` on an (ny, nx) grid."""
     ny = nx if ny is None else ny

@@ -1,0 +1,143 @@
+"""Batched Jacobi-PCG against the FP64 oracle, system by system.
+
+Bars (FP32 device arithmetic, compensated iterate): with the FP64 result every
+(system, rhs) pair converges to a true relative residual ||b - A x|| / ||b||
+<= 1.05 * tol in FP64 (oracle), the reported per-pair relres agrees with it to
+1%, and the iteration count lies between 0.85x - 3 and 1.5x + 5 of FP64
+Jacobi-PCG (oracle.krylov.cg with M = D^-1) on the same system and right-hand
+side (the upper slack covers true-residual restarts). With the FP32 result a
+pair either converges the same way or stops as stagnated where the FP32
+rounding of the exact solution is within 3x of tol, at most 4x that floor.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from gpu_util import say
+from oracle import krylov as K
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(num_systems, lo, hi, sigma=1.5, seed=3):
+    import torch
+
+    from mpsparse_amd import batched, spmv
+
+    keys, vals, off, sides = batched.batched_diffusion(num_systems, lo, hi, sigma, seed)
+    n = int(off[-1])
+    rp = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    ci = torch.empty(keys.numel(), dtype=torch.int32, device="cuda")
+    cv = torch.empty(keys.numel(), dtype=torch.float32, device="cuda")
+    A = spmv.csr_tensor(keys, vals, rp, ci, cv, n, n)
+    S = sp.csr_matrix((cv.cpu().numpy().astype(np.float64), ci.cpu().numpy().astype(np.int64),
+                       rp.cpu().numpy().astype(np.int64)), shape=(n, n))
+    return A, S, off, sides
+
+
+def _check_pairs(S, off, b, x, info, tol, nrhs, its_check=True):
+    import scipy.sparse.linalg as spla
+
+    st, its, rel = (info[k].numpy() for k in ("status", "iterations", "relres"))
+    fp32 = x.dtype == np.float32
+    for s in range(len(off) - 1):
+        lo, hi = int(off[s]), int(off[s + 1])
+        Ss = S[lo:hi, lo:hi]
+        d = Ss.diagonal()
+        for k in range(nrhs):
+            bs = b[lo:hi, k].astype(np.float64)
+            if not np.any(bs):
+                assert st[s, k] == 1 and its[s, k] == 0 and not np.any(x[lo:hi, k])
+                continue
+            r = K.true_relres(Ss, x[lo:hi, k], bs)
+            assert abs(rel[s, k] - r) <= 0.01 * r + 1e-12, (x.dtype, s, k, st[s, k], its[s, k], rel[s, k], r,
+                                                            st.tolist(), its.tolist())
+            if st[s, k] != 1 and fp32:
+                floor = K.true_relres(Ss, spla.spsolve(Ss.tocsc(), bs).astype(np.float32), bs)
+                assert st[s, k] == 4 and floor > tol / 3 and r <= 4 * floor, (s, k, st[s, k], r, floor)
+                continue
+            assert st[s, k] == 1, (s, k, st[s, k], its[s, k], r)
+            assert r <= 1.05 * tol, (s, k, r)
+            if its_check:
+                _, it64, _, _ = K.cg(Ss, bs, tol=tol, M_apply=lambda v, d=d: v / d)
+                assert 0.85 * it64 - 3 <= its[s, k] <= 1.5 * it64 + 5, (s, k, its[s, k], it64)
+
+
+@pytest.mark.parametrize("nrhs", [4, 1, 3])
+def test_batched_matches_oracle(nrhs):
+    import torch
+
+    from mpsparse_amd import batched
+
+    A, S, off, sides = _build(12, 4, 40)
+    n = int(off[-1])
+    B = batched.batch_tensor(A, torch.from_numpy(off))
+    assert B.num_systems == 12
+    b = np.random.default_rng(nrhs).standard_normal((n, nrhs)).astype(np.float32)
+    b[off[2]:off[3], 0] = 0.0  # a right-hand side that is already solved
+    x = torch.full((n, nrhs), np.nan, dtype=torch.float64, device="cuda")
+    say("batched pcg", nrhs, "rhs, sides", sides.tolist())
+    info = B.pcg(torch.from_numpy(b).cuda(), x)
+    assert info["converged_pairs"] == info["total_pairs"] == 12 * nrhs, (info["status"], info["relres"])
+    _check_pairs(S, off, b, x.cpu().numpy(), info, 1e-6, nrhs)
+    # the FP32 result: converged, or stagnated at the FP32 rounding floor
+    x32 = torch.full((n, nrhs), np.nan, dtype=torch.float32, device="cuda")
+    info = B.pcg(torch.from_numpy(b).cuda(), x32)
+    _check_pairs(S, off, b, x32.cpu().numpy(), info, 1e-6, nrhs, its_check=False)
+
+
+def test_batched_fixed_schedules_agree():
+    """Every tile size / rows-per-thread choice converges (the tuner only picks speed)."""
+    import torch
+
+    from mpsparse_amd import batched
+
+    A, S, off, _ = _build(6, 16, 48, seed=5)
+    n = int(off[-1])
+    B = batched.batch_tensor(A, torch.from_numpy(off))
+    b = np.random.default_rng(0).standard_normal((n, 4)).astype(np.float32)
+    for T, rpt in ((256, 1), (512, 2), (2048, 1)):
+        x = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+        say("batched schedule", T, rpt)
+        info = B.pcg(torch.from_numpy(b).cuda(), x, tile_rows=T, rows_per_thread=rpt, steps_per_graph=8)
+        assert info["tile_rows"] == T and info["rows_per_thread"] == rpt
+        _check_pairs(S, off, b, x.cpu().numpy(), info, 1e-6, 4, its_check=False)
+
+
+def test_batched_maxiter_and_host_vectors():
+    import torch
+
+    from mpsparse_amd import batched
+
+    A, S, off, _ = _build(5, 20, 60, seed=7)
+    n = int(off[-1])
+    B = batched.batch_tensor(A, torch.from_numpy(off))
+    b = np.random.default_rng(2).standard_normal((n, 2)).astype(np.float32)
+    x = torch.zeros((n, 2), dtype=torch.float64)
+    say("batched maxiter 6, host vectors")
+    info = B.pcg(torch.from_numpy(b), x, maxiter=6)
+    st, its = info["status"].numpy(), info["iterations"].numpy()
+    assert np.all((st == 3) & (its == 6)), (st, its)
+    # the same handle solves fully afterwards (graph rebuilt for the new maxiter)
+    info = B.pcg(torch.from_numpy(b), x)
+    assert info["converged_pairs"] == 10
+    _check_pairs(S, off, b, x.numpy(), info, 1e-6, 2, its_check=False)
+
+
+def test_batched_rejects_coupled_systems():
+    import torch
+
+    from mpsparse_amd import batched
+    from mpsparse_amd.problems import laplacian_2d
+
+    from gpu_util import build_csr
+
+    p = laplacian_2d(8)  # 64 rows; cutting at 30 splits coupled rows
+    A = build_csr(p.keys, p.vals, p.num_rows, p.num_cols, export=False)[0]
+    say("batched rejects coupled systems")
+    with pytest.raises(RuntimeError, match="block diagonal"):
+        batched.batch_tensor(A, torch.tensor([0, 30, 64]))
+    with pytest.raises(RuntimeError, match="sys_off"):
+        batched.batch_tensor(A, torch.tensor([0, 30, 60]))
+    B = batched.batch_tensor(A, torch.tensor([0, 64]))
+    assert B.num_systems == 1

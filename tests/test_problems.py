@@ -70,7 +70,7 @@ def test_torch_diffusion_generator_matches_numpy():
     diffusion_3d from the same conductivities (2D and 3D)."""
     import torch
 
-    from mpsparse_amd.learned_fsai import diffusion_keys_from_k
+    from mpsparse_amd.problems import diffusion_keys_from_k
 
     for shape in ((4, 5, 6), (1, 7, 9)):
         nz, ny, nx = shape

@@ -40,6 +40,14 @@ typedef enum msp_value_type {
   MSP_VALUES_F16 = 2
 } msp_value_type;
 
+/* Storage used by the SpMV of a CSR handle (msp_csr_set_format). AUTO keeps
+ * the CSR arrays and adds diagonal storage when every entry lies on one of at
+ * most 16 diagonals that are mostly full (stencil matrices). */
+typedef enum msp_format {
+  MSP_FORMAT_AUTO = 0,
+  MSP_FORMAT_CSR = 1
+} msp_format;
+
 typedef enum msp_criterion {
   MSP_CRIT_RELRES = 0,   /* stop when ||r|| <= tol * ||b|| (checked every step) */
   MSP_CRIT_REFERENCE = 1 /* stop when sum(r^2) < tol, tested every 50 steps
@@ -171,8 +179,17 @@ msp_status msp_csr_export(msp_csr A, uint32_t *row_ptr, uint32_t *col_ind, float
 
 msp_status msp_csr_shape(msp_csr A, uint32_t *num_rows, uint32_t *num_cols, uint64_t *nnz);
 
-/* Build the reduced-precision value copy used when opts.values != F32. */
+/* Build the reduced-precision value copy used when opts.values != F32: BF16 or
+ * FP16 off-diagonal values, with the diagonal absorbing their rounding defects
+ * (row sums kept) and stored as a 16-bit value plus an FP32 remainder. */
 msp_status msp_csr_prepare_values(msp_csr A, int32_t value_type, void *stream);
+
+/* Storage choice for the handle's SpMV (msp_format; default AUTO). The
+ * reference always streams CSR (spmv.metal:11-49); AUTO adds diagonal storage
+ * for stencil-like patterns. format_in_use (optional): MSP_FORMAT_CSR, or 2
+ * when diagonal storage is in use (known after the first SpMV or solve). */
+msp_status msp_csr_set_format(msp_csr A, int32_t format);
+msp_status msp_csr_get_format(msp_csr A, int32_t *format_in_use);
 
 /* Release the handle (device buffers included). Replaces ~csr_tensor
  * (iterative/BiCGSTAB/spmv.cpp:354-366). */

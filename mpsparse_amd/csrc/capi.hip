@@ -16,6 +16,7 @@
 #include "batched.hpp"
 #include "bcsr.hpp"
 #include "common.hpp"
+#include "dia.hpp"
 #include "fsai.hpp"
 #include "krylov.hpp"
 #include "mlp.hpp"
@@ -78,6 +79,10 @@ static uint32_t vec_grid(uint64_t n) {
   return (uint32_t)std::max<uint64_t>(1, std::min(g, cap));
 }
 
+static uint32_t grid_for(uint64_t work) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + kBlock - 1) / kBlock, 65536));
+}
+
 static uint32_t bitlen(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v) : 0u; }
 
 // ---------------------------------------------------------------------------
@@ -102,6 +107,10 @@ struct DevMem {
       return fail(MSP_ERR_HIP, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
     }
     return MSP_OK;
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
@@ -177,7 +186,9 @@ struct Workspace {
   enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL, RH };
   float* v(int i) const { return vecs.as<float>() + (size_t)i * npad; }
   Scalars* S_dev() const { return scal.as<Scalars>(); }
+  // two halves: SpMV partials, and those of the kernel that consumes them
   GridRed gr() const { return GridRed{red.as<CSum>()}; }
+  GridRed gr2() const { return GridRed{red.as<CSum>() + (size_t)16 * cu_count()}; }
   ~Workspace() {
     if (hS) (void)hipHostFree(hS);
     if (hflag) (void)hipHostFree(hflag);
@@ -203,6 +214,12 @@ struct msp_csr_s : MatBase {
   DevMem row_ptr, col, val, val16, dlo, blk;  // val16 + dlo: 16-bit values, FP32 diagonal remainder
   int val16_type = -1;
   uint32_t nblk = 0, T = 0, max_row = 0;
+  // diagonal storage (stencil-like matrices): state 0 not probed, 1 in use, 2 not applicable
+  int format = MSP_FORMAT_AUTO;
+  int dia_state = 0, dia_dd = -1, dia16_type = -1;
+  DiaOffsets dia_o{};
+  uint64_t dia_ld = 0;
+  DevMem dia, dia16, dia_dlo;
 };
 
 struct msp_bcsr_s : MatBase {
@@ -241,6 +258,105 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
 }
 
 // Row blocks for the SpMV (and the max row length).
+// Diagonal storage when every entry lies on one of at most kMaxDiag
+// diagonals that are at least 60% full: candidate offsets come from a few
+// interior sample rows, then one pass over all entries confirms that none lies
+// elsewhere. Probed once per matrix (its pattern never changes).
+static msp_status ensure_dia(msp_csr A, hipStream_t s) {
+  if (A->dia_state != 0) return MSP_OK;
+  A->dia_state = 2;
+  if (A->format != MSP_FORMAT_AUTO || A->n == 0 || A->nnz == 0 || A->max_row > (uint32_t)kMaxDiag) return MSP_OK;
+  const uint32_t n = A->n;
+  std::vector<int64_t> offs;
+  // 64 sample rows from a fixed-seed LCG (rows on a grid face lack some offsets;
+  // a sample that misses one only sends the matrix back to CSR)
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  for (int k = 0; k < 64; ++k) {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    const uint32_t r = (uint32_t)((st >> 33) % n);
+    uint32_t rp[2] = {0, 0};
+    HIPCHK(hipMemcpy(rp, A->row_ptr.as<uint32_t>() + r, sizeof rp, hipMemcpyDeviceToHost));
+    const uint32_t len = rp[1] - rp[0];
+    if (len == 0) continue;
+    uint32_t cols[kMaxDiag];
+    HIPCHK(hipMemcpy(cols, A->col.as<uint32_t>() + rp[0], sizeof(uint32_t) * len, hipMemcpyDeviceToHost));
+    for (uint32_t e = 0; e < len; ++e) {
+      const int64_t o = (int64_t)cols[e] - (int64_t)r;
+      if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+    }
+    if (offs.size() > (size_t)kMaxDiag) return MSP_OK;
+  }
+  if (offs.empty()) return MSP_OK;
+  std::sort(offs.begin(), offs.end());
+  for (int64_t o : offs)
+    if (o < INT32_MIN || o > INT32_MAX) return MSP_OK;
+  if ((double)A->nnz < 0.6 * (double)n * (double)offs.size()) return MSP_OK;
+  DiaOffsets o{};
+  o.ndiag = (int32_t)offs.size();
+  int dd = -1;
+  for (size_t d = 0; d < offs.size(); ++d) {
+    o.off[d] = (int32_t)offs[d];
+    if (offs[d] == 0) dd = (int)d;
+  }
+  DevMem miss;
+  MSPCHK(miss.alloc(sizeof(uint32_t)));
+  HIPCHK(hipMemsetAsync(miss.p, 0, sizeof(uint32_t), s));
+  k_dia_check<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), n, o,
+                                             miss.as<uint32_t>());
+  MSPCHK(launch_check("k_dia_check"));
+  uint32_t hm = 0;
+  HIPCHK(hipMemcpyAsync(&hm, miss.p, sizeof hm, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hm) return MSP_OK;
+  A->dia_ld = ((uint64_t)n + 63) / 64 * 64;
+  const size_t bytes = sizeof(float) * A->dia_ld * o.ndiag;
+  MSPCHK(A->dia.alloc(bytes));
+  HIPCHK(hipMemsetAsync(A->dia.p, 0, bytes, s));
+  k_dia_fill<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n,
+                                            o, A->dia_ld, A->dia.as<float>());
+  MSPCHK(launch_check("k_dia_fill"));
+  A->dia_o = o;
+  A->dia_dd = dd;
+  A->dia_state = 1;
+  A->dia16_type = -1;
+  return MSP_OK;
+}
+
+// Refill the FP32 diagonal storage after A's values changed (pattern fixed).
+static msp_status refresh_dia(msp_csr A, hipStream_t s) {
+  if (A->dia_state != 1) return MSP_OK;
+  HIPCHK(hipMemsetAsync(A->dia.p, 0, sizeof(float) * A->dia_ld * A->dia_o.ndiag, s));
+  k_dia_fill<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
+                                               A->n, A->dia_o, A->dia_ld, A->dia.as<float>());
+  A->dia16_type = -1;
+  return launch_check("k_dia_fill");
+}
+
+static msp_status ensure_dia16(msp_csr A, int32_t vt, hipStream_t s) {
+  if (A->dia_state != 1 || vt == MSP_VALUES_F32 || A->dia16_type == vt) return MSP_OK;
+  const size_t cnt = A->dia_ld * A->dia_o.ndiag;
+  MSPCHK(A->dia16.alloc(sizeof(uint16_t) * cnt));
+  HIPCHK(hipMemsetAsync(A->dia16.p, 0, sizeof(uint16_t) * cnt, s));
+  MSPCHK(A->dia_dlo.alloc(sizeof(float) * ((size_t)A->n + 1)));
+  if (vt == MSP_VALUES_BF16)
+    k_dia_cvt16<true><<<grid_for(A->n), kBlock, 0, s>>>(A->dia.as<float>(), A->n, A->m, A->dia_o, A->dia_dd,
+                                                        A->dia_ld, A->dia16.as<uint16_t>(), A->dia_dlo.as<float>());
+  else
+    k_dia_cvt16<false><<<grid_for(A->n), kBlock, 0, s>>>(A->dia.as<float>(), A->n, A->m, A->dia_o, A->dia_dd,
+                                                         A->dia_ld, A->dia16.as<uint16_t>(), A->dia_dlo.as<float>());
+  MSPCHK(launch_check("k_dia_cvt16"));
+  A->dia16_type = vt;
+  return MSP_OK;
+}
+
+// Everything an SpMV with values of type vt needs: the diagonal-storage probe,
+// and the 16-bit copies (CSR and, when in use, diagonal).
+static msp_status ensure_format(msp_csr A, int32_t vt, hipStream_t s) {
+  MSPCHK(ensure_dia(A, s));
+  if (vt != MSP_VALUES_F32) MSPCHK(ensure_dia16(A, vt, s));
+  return MSP_OK;
+}
+
 static msp_status finalize_matrix(msp_csr A, hipStream_t s) {
   const uint32_t n = A->n;
   DevMem mx;
@@ -541,13 +657,35 @@ extern "C" msp_status msp_csr_prepare_values(msp_csr A, int32_t value_type, void
   const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(((uint64_t)A->n + kBlock - 1) / kBlock, 65536));
   if (value_type == MSP_VALUES_BF16)
     k_cvt16_rowsum<true><<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
-                                              A->n, A->nnz, A->val16.as<uint16_t>(), A->dlo.as<float>());
+                                              A->n, A->m, A->nnz, A->val16.as<uint16_t>(), A->dlo.as<float>());
   else
     k_cvt16_rowsum<false><<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
-                                               A->n, A->nnz, A->val16.as<uint16_t>(), A->dlo.as<float>());
+                                               A->n, A->m, A->nnz, A->val16.as<uint16_t>(), A->dlo.as<float>());
   MSPCHK(launch_check("k_cvt16_rowsum"));
   A->val16_type = value_type;
   HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_set_format(msp_csr A, int32_t format) {
+  MSPCHK(check_handle(A));
+  if (format != MSP_FORMAT_AUTO && format != MSP_FORMAT_CSR) return fail(MSP_ERR_INVALID, "unknown format %d", format);
+  if (format == A->format) return MSP_OK;
+  A->format = format;
+  A->dia_state = 0;  // re-probe on the next use
+  A->dia16_type = -1;
+  if (format == MSP_FORMAT_CSR) {
+    A->dia_state = 2;
+    A->dia.reset();
+    A->dia16.reset();
+    A->dia_dlo.reset();
+  }
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_csr_get_format(msp_csr A, int32_t* format_in_use) {
+  MSPCHK(check_handle(A));
+  if (format_in_use) *format_in_use = A->dia_state == 1 ? 2 : MSP_FORMAT_CSR;
   return MSP_OK;
 }
 
@@ -577,6 +715,7 @@ static CsrView<V> view_of(msp_csr A) {
   v.blk = A->blk.as<uint32_t>();
   v.nblk = A->nblk;
   v.n = A->n;
+  v.m = A->m;
   return v;
 }
 
@@ -590,22 +729,59 @@ static void finish(const Fin& fin, GridRed rb, uint32_t nb, hipStream_t s) {
 }
 
 template <typename V>
+static DiaView<V> dia_view_of(msp_csr A) {
+  DiaView<V> v;
+  if constexpr (std::is_same<V, ValF32>::value) {
+    v.val = A->dia.as<float>();
+    v.dlo = nullptr;
+  } else {
+    v.val = A->dia16.as<uint16_t>();
+    v.dlo = A->dia_dlo.as<float>();
+  }
+  v.ld = A->dia_ld;
+  v.n = A->n;
+  v.m = A->m;
+  v.o = A->dia_o;
+  return v;
+}
+
+static uint32_t dia_grid(uint32_t n) {
+  return std::max<uint32_t>(1, std::min<uint32_t>((n + kBlock - 1) / kBlock, (uint32_t)cu_count() * 8));
+}
+
+// y = A x with a fused epilogue: diagonal storage when the matrix has it (and
+// its values of type V are prepared), the CSR row-block kernel otherwise.
+template <typename V, int ND, typename XL, typename Epi, typename Fin>
+static uint32_t spmv_any(msp_csr A, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
+  const bool dia = A->dia_state == 1 && (V::kType == MSP_VALUES_F32 || A->dia16_type == V::kType);
+  const uint32_t g = dia ? dia_grid(A->n) : spmv_grid(A->nblk);
+  if (dia) k_dia<V, ND, XL, Epi><<<g, kBlock, 0, s>>>(dia_view_of<V>(A), xl, epi, rb, done);
+  else k_spmv<V, ND, XL, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), xl, epi, rb, done);
+  finish<ND>(fin, rb, g, s);
+  return g;
+}
+
+template <typename V>
 struct OpCsr {
   msp_csr A;
   MatBase* base() const { return A; }
   template <int ND, typename Epi, typename Fin>
   void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
-    const uint32_t g = spmv_grid(A->nblk);
-    k_spmv<V, ND, XPlain, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), XPlain{x}, epi, red_of(A), done);
-    finish<ND>(fin, red_of(A), g, s);
+    spmv_any<V, ND>(A, XPlain{x}, epi, fin, red_of(A), done, s);
   }
   // y = A (r - alpha v)
   template <int ND, typename Epi, typename Fin>
   void spmv_axpy(const float* r, const float* v, const float* alpha, Epi epi, Fin fin, const int* done,
                  hipStream_t s) const {
-    const uint32_t g = spmv_grid(A->nblk);
-    k_spmv<V, ND, XAxpy, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), XAxpy{r, v, alpha}, epi, red_of(A), done);
-    finish<ND>(fin, red_of(A), g, s);
+    spmv_any<V, ND>(A, XAxpy{r, v, alpha}, epi, fin, red_of(A), done, s);
+  }
+  // CG step head: p = z + beta p_old formed in the gather, q = A p, <p,q>
+  // partials (first reduction half); returns the number of partials.
+  template <int PC>
+  uint32_t cg_dir(const float* r, const float* pold, const float* dinv, const float* beta, float* q, float* pnew,
+                  const int* done, hipStream_t s) const {
+    const XPNew<PC> xl{r, pold, dinv, beta};
+    return spmv_any<V, 1>(A, xl, EpiCgQ<XPNew<PC>>{q, pnew, xl}, FinNone{}, red_of(A), done, s);
   }
   RowsCsr rows() const { return RowsCsr{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()}; }
   void inv_diag(float* dinv, int* bad, hipStream_t s) const {
@@ -634,12 +810,19 @@ struct OpBcsr {
   msp_bcsr A;
   MatBase* base() const { return A; }
   template <int ND, typename XL, typename Epi, typename Fin>
-  void launch(XL xl, Epi epi, Fin fin, const int* done, hipStream_t s) const {
+  uint32_t launch(XL xl, Epi epi, Fin fin, const int* done, hipStream_t s) const {
     const uint32_t g = bspmv_grid(A);
     if (A->B == 4) k_bspmv<4, ND, XL, Epi><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, red_of(A), done);
     else if (A->B == 8) k_bspmv<8, ND, XL, Epi><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, red_of(A), done);
     else k_bspmv<16, ND, XL, Epi><<<g, kBlock, 0, s>>>(bview_of(A), xl, epi, red_of(A), done);
     finish<ND>(fin, red_of(A), g, s);
+    return g;
+  }
+  template <int PC>
+  uint32_t cg_dir(const float* r, const float* pold, const float* dinv, const float* beta, float* q, float* pnew,
+                  const int* done, hipStream_t s) const {
+    const X4PNew<PC> xl{r, pold, dinv, beta};
+    return launch<1>(xl, EpiCgQ<X4PNew<PC>>{q, pnew, xl}, FinNone{}, done, s);
   }
   template <int ND, typename Epi, typename Fin>
   void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
@@ -666,8 +849,9 @@ static msp_status check_values(msp_csr A, int32_t vt) {
 
 // Run f(OpCsr<V>{A}) for the requested value type.
 template <typename F>
-static msp_status with_csr_op(msp_csr A, int32_t vt, F&& f) {
+static msp_status with_csr_op(msp_csr A, int32_t vt, hipStream_t s, F&& f) {
   MSPCHK(check_values(A, vt));
+  MSPCHK(ensure_format(A, vt, s));
   if (vt == MSP_VALUES_BF16) return f(OpCsr<ValBF16>{A});
   if (vt == MSP_VALUES_F16) return f(OpCsr<ValF16>{A});
   return f(OpCsr<ValF32>{A});
@@ -790,24 +974,31 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
     finish<3>(FinCgInit{S}, rb, gv, s);
     return launch_check("cg init");
   };
+  // ping-pong directions: step k reads P[k & 1], writes P[(k + 1) & 1] (P, Z)
+  float* Pb[2] = {P, w.v(Workspace::Z)};
+  int k = 0;
+  const GridRed rb2 = w.gr2();
   auto step = [&]() {
-    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->halt, s);
+    const float* pold = Pb[k & 1];
+    float* pnew = Pb[(k + 1) & 1];
     if (jac) {
-      k_cg_update_xr<kPcJacobi><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, D, n, S, rb);
-      finish<2>(FinCgStep<kPcJacobi>{S}, rb, gv, s);
-      k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
+      const uint32_t g = op.template cg_dir<kPcJacobi>(R, pold, D, &S->beta, Q, pnew, &S->halt, s);
+      k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      finish<2>(FinCgStep<kPcJacobi>{S}, rb2, gv, s);
     } else {
-      k_cg_update_xr<kPcNone><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, D, n, S, rb);
-      finish<2>(FinCgStep<kPcNone>{S}, rb, gv, s);
-      k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(P, R, D, nullptr, n, S);
+      const uint32_t g = op.template cg_dir<kPcNone>(R, pold, D, &S->beta, Q, pnew, &S->halt, s);
+      k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      finish<2>(FinCgStep<kPcNone>{S}, rb2, gv, s);
     }
+    ++k;
   };
+  // a restart (beta = 0) makes the next direction z whatever P[k & 1] holds
   auto verify = [&]() {
     if (jac) {
-      k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
+      k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb);
       finish<2>(FinVerify<1>{S}, rb, gv, s);
     } else {
-      k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, D, n, S, rb);
+      k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb);
       finish<2>(FinVerify<0>{S}, rb, gv, s);
     }
   };
@@ -914,7 +1105,7 @@ extern "C" msp_status msp_csr_spmv(msp_csr A, const float* x, float* y, int32_t 
   MSPCHK(check_handle(A));
   hipStream_t s = (hipStream_t)stream;
   // the kernel may read x in aligned 16-byte quads: stage a misaligned x
-  return with_csr_op(A, value_type, [&](auto op) { return spmv_impl(op, x, y, !aligned16(x), s); });
+  return with_csr_op(A, value_type, s, [&](auto op) { return spmv_impl(op, x, y, !aligned16(x), s); });
 }
 
 extern "C" msp_status msp_csr_cg(msp_csr A, const float* b, float* x, const msp_solve_opts* opts,
@@ -923,7 +1114,7 @@ extern "C" msp_status msp_csr_cg(msp_csr A, const float* b, float* x, const msp_
   msp_solve_opts o;
   fill_opts(o, opts);
   hipStream_t s = (hipStream_t)stream;
-  return with_csr_op(A, o.values, [&](auto op) { return cg_impl(op, b, x, o, info, s); });
+  return with_csr_op(A, o.values, s, [&](auto op) { return cg_impl(op, b, x, o, info, s); });
 }
 
 extern "C" msp_status msp_csr_bicgstab(msp_csr A, const float* b, float* x, const msp_solve_opts* opts,
@@ -932,14 +1123,14 @@ extern "C" msp_status msp_csr_bicgstab(msp_csr A, const float* b, float* x, cons
   msp_solve_opts o;
   fill_opts(o, opts);
   hipStream_t s = (hipStream_t)stream;
-  return with_csr_op(A, o.values, [&](auto op) { return bicgstab_impl(op, b, x, o, info, s); });
+  return with_csr_op(A, o.values, s, [&](auto op) { return bicgstab_impl(op, b, x, o, info, s); });
 }
 
 extern "C" msp_status msp_csr_richardson(msp_csr A, const float* b, float* x, float omega, int32_t iters,
                                          int32_t use_x0, int32_t value_type, void* stream) {
   MSPCHK(check_handle(A));
   hipStream_t s = (hipStream_t)stream;
-  return with_csr_op(A, value_type,
+  return with_csr_op(A, value_type, s,
                      [&](auto op) { return richardson_impl(op, b, x, omega, iters, use_x0, s); });
 }
 
@@ -1266,7 +1457,7 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
     io.tol = itol;
     auto run = [&](int32_t vt) -> msp_status {
       io.values = vt;
-      return with_csr_op(A, vt, [&](auto op) {
+      return with_csr_op(A, vt, s, [&](auto op) {
         return ro.inner_solver == 0 ? cg_impl(op, rhs, d, io, &ii, s) : bicgstab_impl(op, rhs, d, io, &ii, s);
       });
     };
@@ -1307,9 +1498,6 @@ static msp_status exclusive_scan_u32(uint32_t* data, uint64_t N, hipStream_t s) 
   return MSP_OK;
 }
 
-static uint32_t grid_for(uint64_t work) {
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + kBlock - 1) / kBlock, 65536));
-}
 
 extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
   if (!out) return fail(MSP_ERR_INVALID, "null output handle");
@@ -1359,6 +1547,8 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     HIPCHK(hipMemsetAsync(F->GT.val.p, 0, sizeof(float) * (ne + 64), s));
     MSPCHK(zero_pad(&F->GT, s));
     MSPCHK(finalize_matrix(&F->GT, s));
+    MSPCHK(ensure_dia(&F->G, s));
+    MSPCHK(ensure_dia(&F->GT, s));
     // features and column scaling
     MSPCHK(F->diag.alloc(sizeof(float) * ((size_t)n + 1)));
     MSPCHK(F->feat.alloc(sizeof(float) * 8 * (ne + 64)));
@@ -1427,6 +1617,8 @@ static msp_status fsai_commit(msp_fsai F, hipStream_t s) {
   k_gather_by_payload<<<grid_for(F->ne), kBlock, 0, s>>>(F->GT.val.as<float>(), F->G.val.as<float>(),
                                                          F->GTmap.as<float>(), F->ne);
   MSPCHK(launch_check("gather G^T"));
+  MSPCHK(refresh_dia(&F->G, s));
+  MSPCHK(refresh_dia(&F->GT, s));
   F->ready = 1;
   return MSP_OK;
 }
@@ -1508,9 +1700,7 @@ extern "C" msp_status msp_fsai_export_values(msp_fsai F, float* g, void* stream)
 
 template <int ND, typename XL, typename Epi, typename Fin>
 static void spmv_with(msp_csr M, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
-  const uint32_t g = spmv_grid(M->nblk);
-  k_spmv<ValF32, ND, XL, Epi><<<g, kBlock, 0, s>>>(view_of<ValF32>(M), xl, epi, rb, done);
-  finish<ND>(fin, rb, g, s);
+  spmv_any<ValF32, ND>(M, xl, epi, fin, rb, done, s);
 }
 
 extern "C" msp_status msp_fsai_apply(msp_fsai F, const float* r, float* z, void* stream) {
@@ -1600,7 +1790,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinPcgBeta{S}, rb, &S->halt, s);
     k_cg_update_p<kPcExplicit><<<gv, kBlock, 0, s>>>(P, R, nullptr, Z, n, S);
     op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->halt, s);
-    k_cg_update_xr<kPcExplicit><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb);
+    k_cg_update_xr<kPcExplicit, false><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb, rb, 0);
     finish<2>(FinCgStep<kPcExplicit>{S}, rb, gv, s);
   };
   auto verify = [&]() {
@@ -1625,11 +1815,11 @@ extern "C" msp_status msp_csr_pcg_fsai(msp_csr A, msp_fsai F, const float* b, fl
   if (F->nonpos > 0) {
     // G has a non-positive diagonal: M = G^T G may be singular -> plain CG
     o.precond = MSP_PRECOND_NONE;
-    msp_status st = with_csr_op(A, o.values, [&](auto op) { return cg_impl(op, b, x, o, info, s); });
+    msp_status st = with_csr_op(A, o.values, s, [&](auto op) { return cg_impl(op, b, x, o, info, s); });
     if (info) info->reserved = 1;
     return st;
   }
-  msp_status st = with_csr_op(A, o.values, [&](auto op) { return pcg_fsai_impl(op, F, b, x, o, info, s); });
+  msp_status st = with_csr_op(A, o.values, s, [&](auto op) { return pcg_fsai_impl(op, F, b, x, o, info, s); });
   if (info) info->reserved = 0;
   return st;
 }
@@ -1656,12 +1846,12 @@ extern "C" msp_status msp_csr_solve_refined_fsai(msp_csr A, msp_fsai F, const do
     io.tol = itol;
     const bool use_fsai = F->nonpos == 0;
     if (use_fsai)
-      MSPCHK(with_csr_op(A, io.values, [&](auto op) { return pcg_fsai_impl(op, F, rhs, d, io, &ii, s); }));
+      MSPCHK(with_csr_op(A, io.values, s, [&](auto op) { return pcg_fsai_impl(op, F, rhs, d, io, &ii, s); }));
     inf->inner_iterations += ii.iterations;
     const bool reached = ii.status == MSP_SOLVE_CONVERGED;
     if (!use_fsai || !reached) {
       if (use_fsai) inf->fallbacks += 1;
-      MSPCHK(with_csr_op(A, io.values, [&](auto op) { return cg_impl(op, rhs, d, io, &ii, s); }));
+      MSPCHK(with_csr_op(A, io.values, s, [&](auto op) { return cg_impl(op, rhs, d, io, &ii, s); }));
       inf->inner_iterations += ii.iterations;
     }
     *stop = ii.status == MSP_SOLVE_BREAKDOWN && ii.iterations == 0;

@@ -168,26 +168,31 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t wg, uint32_t nb) {
 struct ValF32 {
   typedef float storage;
   static constexpr bool kLowDiag = false;
+  static constexpr int kType = 0;  // msp_value_type
   static __device__ __forceinline__ f4 load4(const float* p) {
     return __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
   }
   static __device__ __forceinline__ float load1(const float* p) { return __builtin_nontemporal_load(p); }
+  static __device__ __forceinline__ float cvt1(float v) { return v; }
 };
 
 struct ValBF16 {
   typedef uint16_t storage;
   static constexpr bool kLowDiag = true;
+  static constexpr int kType = 1;
   static __device__ __forceinline__ float cvt(uint32_t h) { return __uint_as_float(h << 16); }
   static __device__ __forceinline__ f4 load4(const uint16_t* p) {
     u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
     return f4{cvt(w.x & 0xFFFFu), cvt(w.x >> 16), cvt(w.y & 0xFFFFu), cvt(w.y >> 16)};
   }
   static __device__ __forceinline__ float load1(const uint16_t* p) { return cvt(*p); }
+  static __device__ __forceinline__ float cvt1(uint16_t v) { return cvt(v); }
 };
 
 struct ValF16 {
   typedef uint16_t storage;
   static constexpr bool kLowDiag = true;
+  static constexpr int kType = 2;
   static __device__ __forceinline__ float cvt(uint32_t h) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
   }
@@ -196,6 +201,7 @@ struct ValF16 {
     return f4{cvt(w.x & 0xFFFFu), cvt(w.x >> 16), cvt(w.y & 0xFFFFu), cvt(w.y >> 16)};
   }
   static __device__ __forceinline__ float load1(const uint16_t* p) { return cvt(*p); }
+  static __device__ __forceinline__ float cvt1(uint16_t v) { return cvt(v); }
 };
 
 // Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).

@@ -1,0 +1,128 @@
+// Diagonal (offset) storage for stencil-like matrices, used in place of the
+// CSR stream when every nonzero lies on one of at most kMaxDiag diagonals
+// (column - row = off[d]) and the diagonals are mostly full. Values are stored
+// diagonal-major, val[d * ld + i] = A(i, i + off[d]) (zero where the diagonal
+// leaves the matrix or has no entry), so a row costs only its values (no
+// column indices, no row pointers) and every load is coalesced: the 7-point
+// operator moves 28 B of values per row instead of 56 B of values + columns.
+// The row results feed the same epilogues as the CSR kernel.
+#pragma once
+
+#include "common.hpp"
+#include "spmv.hpp"
+
+namespace msp {
+
+constexpr int kMaxDiag = 16;
+
+struct DiaOffsets {
+  int32_t off[kMaxDiag];
+  int32_t ndiag;
+};
+
+template <typename V>
+struct DiaView {
+  const typename V::storage* __restrict__ val;  // [ndiag][ld]
+  const float* __restrict__ dlo;                // 16-bit formats: FP32 remainder of the diagonal
+  uint64_t ld;
+  uint32_t n, m;
+  DiaOffsets o;
+};
+
+// One row per thread, grid-stride (measured faster than contiguous per-XCD
+// chunks at 256^3 and within 10% at 128^3), plain coalesced loads (a
+// non-temporal hint on these 4-byte lane loads more than doubled the time);
+// partial sums of the epilogue go to rb.
+template <typename V, int ND, typename XL, typename Epi>
+__global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, GridRed rb,
+                                                const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int NA = ND > 0 ? ND : 1;
+  CSum acc[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
+  const int nd = A.o.ndiag;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += gridDim.x * kBlock) {
+    float s = 0.0f;
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d) {
+      if (d < nd) {
+        const int64_t c = (int64_t)i + A.o.off[d];
+        if (c >= 0 && c < (int64_t)A.m) s += V::cvt1(A.val[d * A.ld + i]) * xl((uint32_t)c);
+      }
+    }
+    if constexpr (V::kLowDiag) {
+      if (i < A.m) s += A.dlo[i] * xl(i);
+    }
+    epi.row(i, s, acc);
+  }
+  if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
+}
+
+// Entries whose offset is not in the candidate list (counted), per row.
+__global__ __launch_bounds__(kBlock) void k_dia_check(const uint32_t* __restrict__ row_ptr,
+                                                      const uint32_t* __restrict__ col, uint32_t n, DiaOffsets o,
+                                                      uint32_t* __restrict__ misses) {
+  uint32_t miss = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+      const int64_t off = (int64_t)col[e] - (int64_t)i;
+      bool hit = false;
+      for (int d = 0; d < o.ndiag; ++d) hit |= (off == o.off[d]);
+      miss += hit ? 0u : 1u;
+    }
+  }
+#pragma unroll
+  for (int sh = kWave / 2; sh > 0; sh >>= 1) miss += __shfl_xor(miss, sh, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && miss) atomicAdd(misses, miss);
+}
+
+// Scatter the FP32 CSR values into diagonal storage (zero-filled by the
+// caller); duplicates of an entry are added in CSR order.
+__global__ __launch_bounds__(kBlock) void k_dia_fill(const uint32_t* __restrict__ row_ptr,
+                                                     const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                                     uint32_t n, DiaOffsets o, uint64_t ld, float* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+      const int64_t off = (int64_t)col[e] - (int64_t)i;
+      for (int d = 0; d < o.ndiag; ++d) {
+        if (off == o.off[d]) {
+          out[d * ld + i] += val[e];
+          break;
+        }
+      }
+    }
+  }
+}
+
+// 16-bit diagonal storage with exact row sums, as k_cvt16_rowsum does for CSR:
+// off-diagonals rounded, the diagonal (offset 0, index dd; -1 if absent)
+// absorbs their rounding defects and is stored as a 16-bit value plus the FP32
+// remainder dlo.
+template <bool BF16>
+__global__ __launch_bounds__(kBlock) void k_dia_cvt16(const float* __restrict__ in, uint32_t n, uint32_t m,
+                                                      DiaOffsets o, int dd, uint64_t ld, uint16_t* __restrict__ out,
+                                                      float* __restrict__ dlo) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double dsum = 0.0;
+    for (int d = 0; d < o.ndiag; ++d) {
+      const float a = in[d * ld + i];
+      if (d == dd) {
+        dsum += (double)a;
+        continue;
+      }
+      const uint16_t h = BF16 ? f32_to_bf16(a) : f32_to_f16(a);
+      out[d * ld + i] = h;
+      dsum += (double)a - (double)(BF16 ? ValBF16::cvt(h) : ValF16::cvt(h));
+    }
+    float hi = 0.0f;
+    if (dd >= 0 && i < m) {
+      const uint16_t h = BF16 ? f32_to_bf16((float)dsum) : f32_to_f16((float)dsum);
+      out[dd * ld + i] = h;
+      hi = BF16 ? ValBF16::cvt(h) : ValF16::cvt(h);
+    }
+    dlo[i] = i < m ? (float)(dsum - (double)hi) : 0.0f;
+  }
+}
+
+}  // namespace msp

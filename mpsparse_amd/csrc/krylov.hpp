@@ -3,10 +3,15 @@
 // and scalar bookkeeping in the k_finish launched after it), all scalars kept
 // on the device.
 //
-// CG (reference: iterative/BiCGSTAB/cgd.metal:11-90, spmv.cpp:528-659) per step:
-//   q = A p, <p,q> -> alpha              (k_spmv + FinCgAlpha)
-//   x += alpha p, r -= alpha q, <r,r>, <r,z> -> convergence, beta  (k_cg_update_xr)
-//   p = z + beta p                       (k_cg_update_p)
+// CG (reference: iterative/BiCGSTAB/cgd.metal:11-90, spmv.cpp:528-659) per step,
+// three launches:
+//   p = z + beta p_old for every gathered column (XPNew, ping-pong p buffers),
+//   q = A p, block sums of <p,q>          (k_spmv / k_dia + EpiCgQ)
+//   alpha = <r,z> / <p,q> from those sums in every workgroup, x += alpha p,
+//   r -= alpha q, sums of <r,r>, <r,z>   (k_cg_update_xr<PC, true>)
+//   convergence, beta                    (k_finish + FinCgStep)
+// FSAI-preconditioned CG keeps z explicit (two more SpMVs) and the separate
+// alpha finisher and p update.
 // BiCGSTAB (reference: iterative/BiCGSTAB/bicgstab.metal:12-326, spmv.cpp:699-888):
 //   v = A p, <r0,v> -> alpha             (k_spmv + FinBicgAlpha)
 //   s = r - alpha v formed inside the second SpMV's gather, t = A s,
@@ -18,6 +23,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "spmv.hpp"
 
 namespace msp {
 
@@ -126,6 +132,62 @@ struct FinPcgBeta {
 
 __global__ void k_set_restart(Scalars* S) { S->restart = 1; }
 
+// CG direction formed on the fly: p = z + beta p_old, z = r (PC none) or
+// D^-1 r (Jacobi); beta = 0 on the first step and after a restart.
+template <int PC>
+struct XPNew {
+  const float* __restrict__ r;
+  const float* __restrict__ pold;
+  const float* __restrict__ dinv;
+  const float* __restrict__ beta;
+  __device__ __forceinline__ float operator()(uint32_t c) const {
+    float z = r[c];
+    if (PC == kPcJacobi) z *= dinv[c];
+    return z + (*beta) * pold[c];
+  }
+  __device__ __forceinline__ f4 load4(u4 c) const {
+    if (quad_run(c)) {
+      f4 z = *reinterpret_cast<const f4*>(r + c.x);
+      if (PC == kPcJacobi) z *= *reinterpret_cast<const f4*>(dinv + c.x);
+      return z + (*beta) * *reinterpret_cast<const f4*>(pold + c.x);
+    }
+    return f4{(*this)(c.x), (*this)(c.y), (*this)(c.z), (*this)(c.w)};
+  }
+};
+
+// The same direction for the BCSR kernel's 4-wide loads.
+template <int PC>
+struct X4PNew {
+  const float* __restrict__ r;
+  const float* __restrict__ pold;
+  const float* __restrict__ dinv;
+  const float* __restrict__ beta;
+  __device__ __forceinline__ float operator()(uint32_t c) const {
+    float z = r[c];
+    if (PC == kPcJacobi) z *= dinv[c];
+    return z + (*beta) * pold[c];
+  }
+  __device__ __forceinline__ f4 load4(uint32_t c) const {
+    f4 z = *reinterpret_cast<const f4*>(r + c);
+    if (PC == kPcJacobi) z *= *reinterpret_cast<const f4*>(dinv + c);
+    return z + (*beta) * *reinterpret_cast<const f4*>(pold + c);
+  }
+};
+
+// q = A p with p formed by XPNew / X4PNew: the owned rows also store p, and <p, q>.
+template <class XL>
+struct EpiCgQ {
+  float* __restrict__ q;
+  float* __restrict__ pnew;
+  XL xl;
+  __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc) const {
+    const float pi = xl(i);
+    pnew[i] = pi;
+    q[i] = s;
+    acc[0] = csum_add(acc[0], pi * s);
+  }
+};
+
 // Convergence test shared by the solvers; `rr_old` is ||r||^2 at the start of
 // the step (the reference criterion tests it every 50 steps,
 // iterative/BiCGSTAB/spmv.cpp:603-629).
@@ -191,14 +253,43 @@ struct FinCgInit {
   }
 };
 
+// Total of nb block partials (value d of ND) computed by the whole workgroup
+// in a fixed order (every workgroup gets the same bits); all threads get it.
+template <int ND>
+__device__ __forceinline__ float block_total(const CSum* __restrict__ partials, uint32_t nb, int d) {
+  __shared__ float s_tot;
+  CSum a[1] = {csum_zero()};
+  for (uint32_t i = threadIdx.x; i < nb; i += kBlock) a[0] = csum_merge(a[0], partials[i * ND + d]);
+  block_reduce<1>(a);
+  if (threadIdx.x == 0) s_tot = csum_val(a[0]);
+  __syncthreads();
+  const float t = s_tot;
+  __syncthreads();
+  return t;
+}
+
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
-template <int PC>
+// ALPHA_LOCAL: alpha = <r,z> / <p,q> from the SpMV's block partials (pq, npq
+// blocks), computed identically by every workgroup; block 0 records pq and
+// alpha, and a non-positive <p,q> leaves x and r alone (FinCgStep then stops
+// with a breakdown). Otherwise alpha comes from the state.
+template <int PC, bool ALPHA_LOCAL>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ xl,
                                                          float* __restrict__ r, const float* __restrict__ p,
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
-                                                         uint32_t n, Scalars* S, GridRed rb) {
+                                                         uint32_t n, Scalars* S, GridRed rb, GridRed pq,
+                                                         uint32_t npq) {
   if (S->halt) return;
-  const float alpha = S->alpha;
+  float alpha = S->alpha;
+  if constexpr (ALPHA_LOCAL) {
+    const float pqv = block_total<1>(pq.partials, npq, 0);
+    const bool ok = finite_pos(pqv);
+    alpha = ok ? S->rz / pqv : 0.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S->pq = pqv;
+      S->alpha = alpha;
+    }
+  }
   const bool fold = S->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
@@ -242,6 +333,10 @@ struct FinCgStep {
   __device__ __forceinline__ void after() const { refresh_halt(S); }
   __device__ __forceinline__ bool active() const { return S->halt == 0; }
   __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
+    if (!finite_pos(S->pq)) {  // alpha was not applied
+      S->done = kBreakdown;
+      return;
+    }
     const int it = S->iter;
     const float rr_old = S->rr;
     S->iter = it + 1;

@@ -32,7 +32,7 @@ struct CsrView {
   const uint32_t* __restrict__ blk;  // [nblk + 1] first row of each block
   const float* __restrict__ dlo;     // 16-bit formats: FP32 remainder of the diagonal
   uint32_t nblk;
-  uint32_t n;
+  uint32_t n, m;
 };
 
 // x loaders --------------------------------------------------------------
@@ -107,7 +107,9 @@ __device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const
       }
       s = group_reduce_sum(s, L);
       if (lane == 0 && ri < nr) {
-        if constexpr (V::kLowDiag) s += A.dlo[r0 + ri] * xl(r0 + ri);
+        if constexpr (V::kLowDiag) {
+          if (r0 + ri < A.m) s += A.dlo[r0 + ri] * xl(r0 + ri);
+        }
         epi.row(r0 + ri, s, acc);
       }
     }
@@ -126,7 +128,9 @@ __device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const
       if (tid == 0) {
         float t = 0.0f;
         for (int w = 0; w < kWavesPerBlock; ++w) t += wpart[w];
-        if constexpr (V::kLowDiag) t += A.dlo[r] * xl(r);
+        if constexpr (V::kLowDiag) {
+          if (r < A.m) t += A.dlo[r] * xl(r);
+        }
         epi.row(r, t, acc);
       }
       __syncthreads();
@@ -288,13 +292,15 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag(const uint32_t* __restrict_
 // dlo_i = d_i - round(d_i). For an M-matrix (diffusion) the 16-bit operator is
 // then the same operator with every coupling perturbed by at most half an ulp
 // of the format, so its energy norm is within that factor of A's. Duplicate
-// diagonal entries: the first slot carries round(d_i), the others 0. One
-// thread per row; padding past nnz is written as 0.
+// diagonal entries: the first slot carries round(d_i), the others 0. Rows
+// i >= m (no diagonal column) keep plain rounding. One thread per row;
+// padding past nnz is written as 0.
 template <bool BF16>
 __global__ __launch_bounds__(kBlock) void k_cvt16_rowsum(const uint32_t* __restrict__ row_ptr,
                                                           const uint32_t* __restrict__ col,
-                                                          const float* __restrict__ in, uint32_t n, uint64_t nnz,
-                                                          uint16_t* __restrict__ out, float* __restrict__ dlo) {
+                                                          const float* __restrict__ in, uint32_t n, uint32_t m,
+                                                          uint64_t nnz, uint16_t* __restrict__ out,
+                                                          float* __restrict__ dlo) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t e = nnz + tid; e < nnz + 64; e += stride) out[e] = 0;
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_cvt16_rowsum(const uint32_t* __restr
       out[dslot] = h;
       hi = BF16 ? ValBF16::cvt(h) : ValF16::cvt(h);
     }
-    dlo[i] = (float)(d - (double)hi);
+    dlo[i] = i < m ? (float)(d - (double)hi) : 0.0f;  // rows past the last column have no diagonal
   }
 }
 

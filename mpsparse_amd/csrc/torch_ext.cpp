@@ -237,6 +237,15 @@ class CsrTensor {
     return refine_dict(info);
   }
 
+  void set_format(const std::string& f) {
+    TORCH_CHECK(f == "auto" || f == "csr", "format must be 'auto' or 'csr'");
+    check(msp_csr_set_format(h_, f == "auto" ? MSP_FORMAT_AUTO : MSP_FORMAT_CSR), "set_format");
+  }
+  std::string format() const {
+    int32_t f = 0;
+    check(msp_csr_get_format(h_, &f), "format");
+    return f == 2 ? "dia" : "csr";
+  }
   int64_t num_rows() const {
     uint32_t n = 0;
     msp_csr_shape(h_, &n, nullptr, nullptr);
@@ -745,6 +754,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
            py::arg("values") = "f32", py::arg("precond") = "none", py::arg("use_x0") = false)
       .def("export", &CsrTensor::export_into, "write row_ptr / col_ind / values", py::arg("row_ptr"),
            py::arg("col_ind"), py::arg("out_vals"))
+      .def("set_format", &CsrTensor::set_format,
+           "'auto' (diagonal storage for stencil-like patterns, the default) or 'csr'", py::arg("format"))
+      .def_property_readonly("format", &CsrTensor::format, "storage the SpMV uses: 'dia' or 'csr'")
       .def_property_readonly("num_rows", &CsrTensor::num_rows)
       .def_property_readonly("num_cols", &CsrTensor::num_cols)
       .def_property_readonly("nnz", &CsrTensor::nnz)

@@ -93,11 +93,12 @@ def round_bf16(a):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-def round16_rowsum(row_ptr, col_ind, vals, fmt="bf16"):
+def round16_rowsum(row_ptr, col_ind, vals, fmt="bf16", num_cols=None):
     """(vals16 float64[nnz], dlo float64[n]): off-diagonals rounded to `fmt`,
     d_i = a_ii + sum_{j != i}(a_ij - round(a_ij)) (FP64) stored as round(d_i) in
     the first diagonal slot (other diagonal slots 0) and dlo_i = d_i - round(d_i)
-    rounded to FP32. y = vals16-SpMV + dlo * x is the library's 16-bit product."""
+    rounded to FP32 (0 for rows i >= num_cols, which have no diagonal column).
+    y = vals16-SpMV + dlo * x is the library's 16-bit product."""
     rnd = round_bf16 if fmt == "bf16" else (lambda a: np.asarray(a, np.float32).astype(np.float16).astype(np.float32))
     row_ptr = np.asarray(row_ptr, dtype=np.int64)
     col = np.asarray(col_ind, dtype=np.int64)
@@ -123,4 +124,6 @@ def round16_rowsum(row_ptr, col_ind, vals, fmt="bf16"):
     hi[has] = rnd(d[has].astype(np.float32)).astype(np.float64)
     out[first[has]] = hi[has]
     dlo = (d - hi).astype(np.float32).astype(np.float64)
+    if num_cols is not None:
+        dlo[np.arange(n) >= num_cols] = 0.0
     return out, dlo

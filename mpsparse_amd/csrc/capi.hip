@@ -203,6 +203,9 @@ struct Workspace {
 struct MatBase {
   int device = 0;
   uint32_t n = 0, m = 0;
+  // CG step form per preconditioner (none, Jacobi): -1 not measured yet,
+  // 1 direction formed in the gather, 0 separate update
+  int cg_fused[2] = {-1, -1};
   Workspace ws;
   DevMem dinv;
   int dinv_state = 0;  // 0 not built, 1 ok, 2 singular
@@ -265,7 +268,8 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
 static msp_status ensure_dia(msp_csr A, hipStream_t s) {
   if (A->dia_state != 0) return MSP_OK;
   A->dia_state = 2;
-  if (A->format != MSP_FORMAT_AUTO || A->n == 0 || A->nnz == 0 || A->max_row > (uint32_t)kMaxDiag) return MSP_OK;
+  if (A->format != MSP_FORMAT_AUTO || A->n == 0 || A->m == 0 || A->nnz == 0 || A->max_row > (uint32_t)kMaxDiag)
+    return MSP_OK;
   const uint32_t n = A->n;
   std::vector<int64_t> offs;
   // 64 sample rows from a fixed-seed LCG (rows on a grid face lack some offsets;
@@ -673,6 +677,7 @@ extern "C" msp_status msp_csr_set_format(msp_csr A, int32_t format) {
   if (format == A->format) return MSP_OK;
   A->format = format;
   A->dia_state = 0;  // re-probe on the next use
+  A->cg_fused[0] = A->cg_fused[1] = -1;
   A->dia16_type = -1;
   if (format == MSP_FORMAT_CSR) {
     A->dia_state = 2;
@@ -766,8 +771,8 @@ struct OpCsr {
   msp_csr A;
   MatBase* base() const { return A; }
   template <int ND, typename Epi, typename Fin>
-  void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
-    spmv_any<V, ND>(A, XPlain{x}, epi, fin, red_of(A), done, s);
+  uint32_t spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
+    return spmv_any<V, ND>(A, XPlain{x}, epi, fin, red_of(A), done, s);
   }
   // y = A (r - alpha v)
   template <int ND, typename Epi, typename Fin>
@@ -825,8 +830,8 @@ struct OpBcsr {
     return launch<1>(xl, EpiCgQ<X4PNew<PC>>{q, pnew, xl}, FinNone{}, done, s);
   }
   template <int ND, typename Epi, typename Fin>
-  void spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
-    launch<ND>(X4Plain{x}, epi, fin, done, s);
+  uint32_t spmv(const float* x, Epi epi, Fin fin, const int* done, hipStream_t s) const {
+    return launch<ND>(X4Plain{x}, epi, fin, done, s);
   }
   template <int ND, typename Epi, typename Fin>
   void spmv_axpy(const float* r, const float* v, const float* alpha, Epi epi, Fin fin, const int* done,
@@ -974,19 +979,48 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
     finish<3>(FinCgInit{S}, rb, gv, s);
     return launch_check("cg init");
   };
-  // ping-pong directions: step k reads P[k & 1], writes P[(k + 1) & 1] (P, Z)
+  // ping-pong directions: step k reads P[k & 1], writes P[(k + 1) & 1] (P, Z).
+  // Two forms of the step compute the same recurrence: the direction formed in
+  // the SpMV's gather (3 launches; best while the gathered neighbourhood of r
+  // and p_old stays in cache), or a separate p update feeding a plain SpMV
+  // (4 launches; best for large stencils). The first solve on a matrix times
+  // steps 0-7 in one form and 8-15 in the other and keeps the faster.
   float* Pb[2] = {P, w.v(Workspace::Z)};
   int k = 0;
   const GridRed rb2 = w.gr2();
+  hipEvent_t tev[3] = {nullptr, nullptr, nullptr};
+  int& form = A->cg_fused[jac ? 1 : 0];
+  if (o.cg_form < 0 || o.cg_form > 2) return fail(MSP_ERR_INVALID, "cg_form must be 0, 1 or 2");
+  const bool tune = o.cg_form == 0 && form < 0;
+  if (tune)
+    for (auto& e : tev) HIPCHK(hipEventCreate(&e));
   auto step = [&]() {
     const float* pold = Pb[k & 1];
     float* pnew = Pb[(k + 1) & 1];
+    const bool fused = o.cg_form == 1 || (o.cg_form == 0 && (tune ? k < 8 : form == 1));
+    if (tune && (k == 0 || k == 8 || k == 16)) (void)hipEventRecord(tev[k / 8], s);
+    if (tune && k == 16) {
+      // decide only if all 16 steps did work (none halted by convergence)
+      float ms_f = 0.f, ms_u = 0.f;
+      int32_t it = 0;
+      if (hipEventSynchronize(tev[2]) == hipSuccess && hipEventElapsedTime(&ms_f, tev[0], tev[1]) == hipSuccess &&
+          hipEventElapsedTime(&ms_u, tev[1], tev[2]) == hipSuccess &&
+          hipMemcpy(&it, &S->iter, sizeof it, hipMemcpyDeviceToHost) == hipSuccess && it >= 16)
+        form = ms_f <= ms_u ? 1 : 0;
+    }
+    uint32_t g;
+    if (fused) {
+      g = jac ? op.template cg_dir<kPcJacobi>(R, pold, D, &S->beta, Q, pnew, &S->halt, s)
+              : op.template cg_dir<kPcNone>(R, pold, D, &S->beta, Q, pnew, &S->halt, s);
+    } else {
+      if (jac) k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, nullptr, n, S);
+      else k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, nullptr, n, S);
+      g = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S->halt, s);
+    }
     if (jac) {
-      const uint32_t g = op.template cg_dir<kPcJacobi>(R, pold, D, &S->beta, Q, pnew, &S->halt, s);
       k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcJacobi>{S}, rb2, gv, s);
     } else {
-      const uint32_t g = op.template cg_dir<kPcNone>(R, pold, D, &S->beta, Q, pnew, &S->halt, s);
       k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcNone>{S}, rb2, gv, s);
     }
@@ -1002,7 +1036,10 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       finish<2>(FinVerify<0>{S}, rb, gv, s);
     }
   };
-  MSPCHK(run_solver(A, o, info, s, init, step, verify));
+  const msp_status st = run_solver(A, o, info, s, init, step, verify);
+  for (auto& e : tev)
+    if (e) (void)hipEventDestroy(e);
+  MSPCHK(st);
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1788,7 +1825,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   auto step = [&]() {
     spmv_with<0>(&F->G, XPlain{R}, EpiStore{Wv}, FinNone{}, rb, &S->halt, s);
     spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinPcgBeta{S}, rb, &S->halt, s);
-    k_cg_update_p<kPcExplicit><<<gv, kBlock, 0, s>>>(P, R, nullptr, Z, n, S);
+    k_cg_update_p<kPcExplicit><<<gv, kBlock, 0, s>>>(P, P, R, nullptr, Z, n, S);
     op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->halt, s);
     k_cg_update_xr<kPcExplicit, false><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb, rb, 0);
     finish<2>(FinCgStep<kPcExplicit>{S}, rb, gv, s);

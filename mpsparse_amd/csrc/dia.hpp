@@ -32,7 +32,12 @@ struct DiaView {
 // One row per thread, grid-stride (measured faster than contiguous per-XCD
 // chunks at 256^3 and within 10% at 128^3), plain coalesced loads (a
 // non-temporal hint on these 4-byte lane loads more than doubled the time);
-// partial sums of the epilogue go to rb.
+// partial sums of the epilogue go to rb. Every load of a row is issued before
+// the first product: with load and multiply-add together under the per-diagonal
+// branch, each diagonal waited for its own loads and a row had one value/x pair
+// in flight (2.5 TB/s on the 7-point operator at 256^3). Columns outside the
+// matrix read a clamped in-range x and are then left out of the sum, so the
+// result equals the guarded form bit for bit. Requires m > 0.
 template <typename V, int ND, typename XL, typename Epi>
 __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, GridRed rb,
                                                 const int* __restrict__ done) {
@@ -42,13 +47,25 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
 #pragma unroll
   for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
   const int nd = A.o.ndiag;
+  const int64_t m = A.m;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += gridDim.x * kBlock) {
+    float a[kMaxDiag], xv[kMaxDiag];
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d) {
+      a[d] = 0.0f;
+      xv[d] = 0.0f;
+      if (d < nd) {
+        const int64_t c = (int64_t)i + A.o.off[d];
+        a[d] = V::cvt1(A.val[d * A.ld + i]);
+        xv[d] = xl((uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c)));
+      }
+    }
     float s = 0.0f;
 #pragma unroll
     for (int d = 0; d < kMaxDiag; ++d) {
       if (d < nd) {
         const int64_t c = (int64_t)i + A.o.off[d];
-        if (c >= 0 && c < (int64_t)A.m) s += V::cvt1(A.val[d * A.ld + i]) * xl((uint32_t)c);
+        s = (c >= 0 && c < m) ? s + a[d] * xv[d] : s;
       }
     }
     if constexpr (V::kLowDiag) {

@@ -362,9 +362,10 @@ struct FinCgStep {
   }
 };
 
-// p = z + beta p with z = r, dinv*r, or an explicit vector.
+// pnew = z + beta pold with z = r, dinv*r, or an explicit vector (pnew may
+// alias pold).
 template <int PC>
-__global__ __launch_bounds__(kBlock) void k_cg_update_p(float* __restrict__ p, const float* __restrict__ r,
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(float* pnew, const float* pold, const float* __restrict__ r,
                                                         const float* __restrict__ dinv, const float* __restrict__ z,
                                                         uint32_t n, const Scalars* S) {
   if (S->halt) return;
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(float* __restrict__ p, c
     if (PC == kPcExplicit) zv = reinterpret_cast<const f4*>(z)[c];
     else zv = reinterpret_cast<const f4*>(r)[c];
     if (PC == kPcJacobi) zv *= reinterpret_cast<const f4*>(dinv)[c];
-    reinterpret_cast<f4*>(p)[c] = zv + beta * reinterpret_cast<const f4*>(p)[c];
+    reinterpret_cast<f4*>(pnew)[c] = zv + beta * reinterpret_cast<const f4*>(pold)[c];
   }
 }
 

@@ -153,7 +153,8 @@ class CsrTensor {
   }
 
   msp_solve_opts make_opts(double tol, int64_t maxiter, int64_t check_every, const std::string& criterion,
-                           bool use_x0, const std::string& precond, const std::string& values, bool verify) {
+                           bool use_x0, const std::string& precond, const std::string& values, bool verify,
+                           const std::string& cg_form = "auto") {
     msp_solve_opts o;
     msp_default_opts(&o);
     o.tol = tol;
@@ -168,17 +169,21 @@ class CsrTensor {
     else TORCH_CHECK(false, "precond must be 'none' or 'jacobi'");
     o.values = value_type_of(values);
     o.verify = verify ? 1 : 0;
+    if (cg_form == "auto") o.cg_form = 0;
+    else if (cg_form == "fused") o.cg_form = 1;
+    else if (cg_form == "separate") o.cg_form = 2;
+    else TORCH_CHECK(false, "cg_form must be 'auto', 'fused' or 'separate'");
     return o;
   }
 
   py::dict solve(bool bicg, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
                  const std::string& criterion, bool use_x0, const std::string& precond, const std::string& values,
-                 bool verify) {
+                 bool verify, const std::string& cg_form = "auto") {
     uint32_t n = 0, m = 0;
     check(msp_csr_shape(h_, &n, &m, nullptr), "shape");
     expect(b, torch::kFloat32, n, "b");
     expect(x, torch::kFloat32, m, "x");
-    msp_solve_opts o = make_opts(tol, maxiter, check_every, criterion, use_x0, precond, values, verify);
+    msp_solve_opts o = make_opts(tol, maxiter, check_every, criterion, use_x0, precond, values, verify, cg_form);
     msp_solve_info info{};
     c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
     void* s = stream_for(device_);
@@ -730,12 +735,14 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           "cgd",
           [](CsrTensor& self, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,
              const std::string& criterion, bool use_x0, const std::string& precond, const std::string& values,
-             bool verify) {
-            return self.solve(false, b, x, tol, maxiter, check_every, criterion, use_x0, precond, values, verify);
+             bool verify, const std::string& cg_form) {
+            return self.solve(false, b, x, tol, maxiter, check_every, criterion, use_x0, precond, values, verify,
+                              cg_form);
           },
           "conjugate gradient", py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("maxiter") = 10000,
           py::arg("check_every") = 16, py::arg("criterion") = "relres", py::arg("use_x0") = false,
-          py::arg("precond") = "none", py::arg("values") = "f32", py::arg("verify") = true)
+          py::arg("precond") = "none", py::arg("values") = "f32", py::arg("verify") = true,
+          py::arg("cg_form") = "auto")
       .def(
           "bicgstab",
           [](CsrTensor& self, torch::Tensor b, torch::Tensor x, double tol, int64_t maxiter, int64_t check_every,

@@ -102,6 +102,30 @@ def test_jacobi_pcg_lognormal():
     check_fp32_solution(info, S, x, b, 1e-6)
 
 
+@pytest.mark.parametrize("fmt", ["auto", "csr"])
+@pytest.mark.parametrize("precond", ["none", "jacobi"])
+def test_cg_step_forms(fmt, precond):
+    """Both CG step forms (direction formed in the SpMV gather, or by a separate
+    update kernel) and the timed choice between them reach the FP64 true
+    residual bar with the FP64 oracle's iteration count (+15% + 2)."""
+    M = diffusion_2d(80, sigma=1.0, seed=6)
+    b = np.random.default_rng(6).standard_normal(M.num_rows).astype(np.float32)
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    A.set_format(fmt)
+    S = M.to_scipy()
+    dinv = 1.0 / S.diagonal()
+    _, it64, _, _ = K.cg(S, b.astype(np.float64), tol=1e-5,
+                         M_apply=(lambda r: dinv * r) if precond == "jacobi" else None)
+    for form in ("fused", "separate", "auto", "auto"):
+        say("cg step form", fmt, precond, form)
+        x, info = _solve(A, b, precond=precond, tol=1e-5, cg_form=form)
+        assert info["status"] == "converged", info
+        assert K.true_relres(S, x, b) <= 1.05e-5
+        assert abs(info["iterations"] - it64) <= 0.15 * it64 + 2, (form, info, it64)
+    with pytest.raises(RuntimeError):
+        _solve(A, b, cg_form="both")
+
+
 def test_bicgstab_block_sparse():
     q = block_sparse(1024, 16, 8, seed=2)
     say("bicgstab block sparse 1024")

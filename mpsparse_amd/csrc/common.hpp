@@ -174,6 +174,13 @@ struct ValF32 {
   }
   static __device__ __forceinline__ float load1(const float* p) { return __builtin_nontemporal_load(p); }
   static __device__ __forceinline__ float cvt1(float v) { return v; }
+  // four values as loaded (raw4) and converted (cvt4), so a load can be issued
+  // well before its use
+  typedef f4 raw4;
+  static __device__ __forceinline__ raw4 ld4(const float* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+  }
+  static __device__ __forceinline__ f4 cvt4(raw4 w) { return w; }
 };
 
 struct ValBF16 {
@@ -187,6 +194,13 @@ struct ValBF16 {
   }
   static __device__ __forceinline__ float load1(const uint16_t* p) { return cvt(*p); }
   static __device__ __forceinline__ float cvt1(uint16_t v) { return cvt(v); }
+  typedef u2 raw4;
+  static __device__ __forceinline__ raw4 ld4(const uint16_t* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
+  }
+  static __device__ __forceinline__ f4 cvt4(raw4 w) {
+    return f4{cvt(w.x & 0xFFFFu), cvt(w.x >> 16), cvt(w.y & 0xFFFFu), cvt(w.y >> 16)};
+  }
 };
 
 struct ValF16 {
@@ -202,6 +216,13 @@ struct ValF16 {
   }
   static __device__ __forceinline__ float load1(const uint16_t* p) { return cvt(*p); }
   static __device__ __forceinline__ float cvt1(uint16_t v) { return cvt(v); }
+  typedef u2 raw4;
+  static __device__ __forceinline__ raw4 ld4(const uint16_t* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const u2*>(p));
+  }
+  static __device__ __forceinline__ f4 cvt4(raw4 w) {
+    return f4{cvt(w.x & 0xFFFFu), cvt(w.x >> 16), cvt(w.y & 0xFFFFu), cvt(w.y >> 16)};
+  }
 };
 
 // Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).

@@ -12,7 +12,7 @@
 //   3. hand every row result to an epilogue (store, store + dot, Richardson
 //      update, ...), whose dot contributions go through one deterministic grid
 //      reduction.
-// Blocks with more than kCap nonzeros (a row longer than the window) take a
+// Blocks with more than kWin nonzeros (a row longer than the window) take a
 // slower whole-workgroup-per-row path inside the same kernel.
 // Workgroups are persistent (grid capped near residency) and walk blocks in
 // XCD-contiguous order.
@@ -67,73 +67,145 @@ struct XAxpy {
   }
 };
 
-// One row block. Products of the block are staged in `sprod`. Column index
-// padding past nnz must be valid columns (the library pads with zeros).
-template <typename V, typename XL, typename Epi>
-__device__ __forceinline__ void csr_block(const CsrView<V>& A, uint32_t b, const XL& xl, const Epi& epi,
-                                          float* sprod, CSum* acc) {
-  const uint32_t tid = threadIdx.x;
-  const uint32_t r0 = A.blk[b], r1 = A.blk[b + 1];
-  if (r0 >= r1) return;
-  const uint32_t nz0 = A.row_ptr[r0], nz1 = A.row_ptr[r1];
-  const uint32_t cnt = nz1 - nz0;
-  if (cnt <= kCap) {
-    const uint32_t a0 = nz0 & ~3u;
-    for (uint32_t i = a0 + 4u * tid; i < nz1; i += 4u * kBlock) {
-      u4 c = __builtin_nontemporal_load(reinterpret_cast<const u4*>(A.col + i));
-      f4 v = V::load4(A.val + i);
-      f4 p = v * xl.load4(c);  // quads at the block edges read valid columns of neighbours / padding
+// Quads of one short block held in registers: kSlots quads per thread from a
+// 4-aligned start cover any window of at most kWin nonzeros. The host's block
+// size keeps every block below kCap - 4 nonzeros unless a single row is longer.
+constexpr int kSlots = (int)(kCap / (4 * kBlock));
+constexpr uint32_t kWin = 4u * kBlock * kSlots - 3u;
+static_assert(kWin <= kCap && kWin + 4 > kCap - 4, "window and LDS size disagree");
+
+template <typename V>
+struct BlockLoad {
+  uint32_t r0, r1, nz0, nz1;
+  u4 c[kSlots];
+  typename V::raw4 v[kSlots];
+};
+
+__device__ __forceinline__ bool short_block(uint32_t r0, uint32_t r1, uint32_t nz0, uint32_t nz1) {
+  return r0 < r1 && nz1 - nz0 <= kWin;
+}
+
+// Issue the loads of block b (bounds, then column indices and values of a
+// short block) without waiting for them. Slots past the block re-read its
+// first quad instead of branching round the load (a lane-divergent branch per
+// slot made the compiler wait for each slot's loads before the next). Column
+// index and value padding past nnz must be valid (the library pads with zeros).
+template <typename V>
+__device__ __forceinline__ void block_fetch(const CsrView<V>& A, uint32_t b, BlockLoad<V>& L) {
+  L.r0 = A.blk[b];
+  L.r1 = A.blk[b + 1];
+  L.nz0 = A.row_ptr[L.r0];
+  L.nz1 = A.row_ptr[L.r1];
+  if (!short_block(L.r0, L.r1, L.nz0, L.nz1)) return;
+  const uint32_t q0 = L.nz0 & ~3u, a0 = q0 + 4u * threadIdx.x;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        uint32_t j = i + k;
-        if (j >= nz0 && j < nz1) sprod[j - nz0] = p[k];
+  for (int k = 0; k < kSlots; ++k) {
+    const uint32_t i = a0 + (uint32_t)k * 4u * kBlock;
+    const uint32_t ic = i < L.nz1 ? i : q0;
+    L.c[k] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(A.col + ic));
+    L.v[k] = V::ld4(A.val + ic);
+  }
+}
+
+// Products of a fetched short block into sprod (quads at the block edges and
+// the re-read slots gather valid columns of neighbours / padding; only the
+// block's own entries are kept).
+template <typename V, typename XL>
+__device__ __forceinline__ void block_products(const BlockLoad<V>& L, const XL& xl, float* sprod) {
+  const uint32_t a0 = (L.nz0 & ~3u) + 4u * threadIdx.x;
+  f4 xv[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) xv[k] = xl.load4(L.c[k]);  // every gather in flight at once
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const uint32_t i = a0 + (uint32_t)k * 4u * kBlock;
+    if (i < L.nz1) {
+      const f4 p = V::cvt4(L.v[k]) * xv[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t j = i + q;
+        if (j >= L.nz0 && j < L.nz1) sprod[j - L.nz0] = p[q];
       }
+    }
+  }
+}
+
+// Row sums of a short block from sprod, L lanes per row, L the largest power
+// of two with L * rows <= kBlock (at most 64).
+template <typename V, typename XL, typename Epi>
+__device__ __forceinline__ void block_rows(const CsrView<V>& A, uint32_t r0, uint32_t r1, uint32_t nz0, const XL& xl,
+                                           const Epi& epi, const float* sprod, CSum* acc) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nr = r1 - r0;
+  uint32_t L = 1;
+  if (nr < kBlock) {
+    uint32_t q = kBlock / nr;  // >= 1
+    L = 1u << (31 - __builtin_clz(q));
+    if (L > 64) L = 64;
+  }
+  const uint32_t G = kBlock / L, g = tid / L, lane = tid % L;
+  for (uint32_t rb = 0; rb < nr; rb += G) {
+    const uint32_t ri = rb + g;
+    float s = 0.0f;
+    if (ri < nr) {
+      const uint32_t s0 = A.row_ptr[r0 + ri] - nz0, s1 = A.row_ptr[r0 + ri + 1] - nz0;
+      for (uint32_t j = s0 + lane; j < s1; j += L) s += sprod[j];
+    }
+    s = group_reduce_sum(s, L);
+    if (lane == 0 && ri < nr) {
+      if constexpr (V::kLowDiag) {
+        if (r0 + ri < A.m) s += A.dlo[r0 + ri] * xl(r0 + ri);
+      }
+      epi.row(r0 + ri, s, acc);
+    }
+  }
+}
+
+// A block longer than the LDS window: every row reduced by the whole
+// workgroup, straight from HBM.
+template <typename V, typename XL, typename Epi>
+__device__ __forceinline__ void long_rows(const CsrView<V>& A, uint32_t r0, uint32_t r1, const XL& xl,
+                                          const Epi& epi, CSum* acc) {
+  __shared__ float wpart[kWavesPerBlock];
+  const uint32_t tid = threadIdx.x;
+  const int wl = tid & (kWave - 1), wid = tid / kWave;
+  for (uint32_t r = r0; r < r1; ++r) {
+    const uint32_t s0 = A.row_ptr[r], s1 = A.row_ptr[r + 1];
+    float s = 0.0f;
+    for (uint32_t j = s0 + tid; j < s1; j += kBlock) s += V::load1(A.val + j) * xl(A.col[j]);
+    s = wave_reduce_sum(s);
+    if (wl == 0) wpart[wid] = s;
+    __syncthreads();
+    if (tid == 0) {
+      float t = 0.0f;
+      for (int w = 0; w < kWavesPerBlock; ++w) t += wpart[w];
+      if constexpr (V::kLowDiag) {
+        if (r < A.m) t += A.dlo[r] * xl(r);
+      }
+      epi.row(r, t, acc);
     }
     __syncthreads();
-    const uint32_t nr = r1 - r0;
-    uint32_t L = 1;
-    if (nr < kBlock) {
-      uint32_t q = kBlock / nr;  // >= 1
-      L = 1u << (31 - __builtin_clz(q));
-      if (L > 64) L = 64;
-    }
-    const uint32_t G = kBlock / L, g = tid / L, lane = tid % L;
-    for (uint32_t rb = 0; rb < nr; rb += G) {
-      const uint32_t ri = rb + g;
-      float s = 0.0f;
-      if (ri < nr) {
-        const uint32_t s0 = A.row_ptr[r0 + ri] - nz0, s1 = A.row_ptr[r0 + ri + 1] - nz0;
-        for (uint32_t j = s0 + lane; j < s1; j += L) s += sprod[j];
-      }
-      s = group_reduce_sum(s, L);
-      if (lane == 0 && ri < nr) {
-        if constexpr (V::kLowDiag) {
-          if (r0 + ri < A.m) s += A.dlo[r0 + ri] * xl(r0 + ri);
-        }
-        epi.row(r0 + ri, s, acc);
-      }
-    }
-    __syncthreads();
-  } else {
-    // long block: every row reduced by the whole workgroup, straight from HBM
-    __shared__ float wpart[kWavesPerBlock];
-    const int wl = tid & (kWave - 1), wid = tid / kWave;
-    for (uint32_t r = r0; r < r1; ++r) {
-      const uint32_t s0 = A.row_ptr[r], s1 = A.row_ptr[r + 1];
-      float s = 0.0f;
-      for (uint32_t j = s0 + tid; j < s1; j += kBlock) s += V::load1(A.val + j) * xl(A.col[j]);
-      s = wave_reduce_sum(s);
-      if (wl == 0) wpart[wid] = s;
+  }
+}
+
+// Blocks [lo, hi) of one workgroup. Every quad of a block is requested
+// before the first product. (Fetching block b + 1 during block b's row sums
+// was measured 15% slower: the row sums' own vector loads (row offsets,
+// epilogue operands) wait on the in-order load counter, so they queued behind
+// the prefetch.)
+template <typename V, typename XL, typename Epi>
+__device__ __forceinline__ void csr_blocks(const CsrView<V>& A, uint32_t lo, uint32_t hi, uint32_t step,
+                                           const XL& xl, const Epi& epi, float* sprod, CSum* acc) {
+  BlockLoad<V> L;
+  for (uint32_t b = lo; b < hi; b += step) {
+    block_fetch<V>(A, b, L);
+    if (short_block(L.r0, L.r1, L.nz0, L.nz1)) {
+      block_products<V>(L, xl, sprod);
       __syncthreads();
-      if (tid == 0) {
-        float t = 0.0f;
-        for (int w = 0; w < kWavesPerBlock; ++w) t += wpart[w];
-        if constexpr (V::kLowDiag) {
-          if (r < A.m) t += A.dlo[r] * xl(r);
-        }
-        epi.row(r, t, acc);
-      }
+      block_rows<V>(A, L.r0, L.r1, L.nz0, xl, epi, sprod, acc);
       __syncthreads();
+    } else if (L.r0 < L.r1) {
+      long_rows<V>(A, L.r0, L.r1, xl, epi, acc);
     }
   }
 }
@@ -223,12 +295,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, G
 #pragma unroll
   for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
   const uint32_t G = gridDim.x;
-  // XCD-contiguous walk: workgroup wg owns blocks [lo, hi) of its XCD's share
+  // grid-stride over blocks in XCD-swizzled order: at every step each XCD
+  // takes a contiguous run of blocks, and the grid as a whole sweeps memory
+  // front to back
   const uint32_t wg = xcd_swizzle(blockIdx.x, G);
-  const uint32_t per = A.nblk / G, rem = A.nblk % G;
-  const uint32_t lo = wg * per + (wg < rem ? wg : rem);
-  const uint32_t hi = lo + per + (wg < rem ? 1u : 0u);
-  for (uint32_t b = lo; b < hi; ++b) csr_block<V>(A, b, xl, epi, sprod, acc);
+  csr_blocks<V>(A, wg, A.nblk, G, xl, epi, sprod, acc);
   if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
 }
 

@@ -55,11 +55,9 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(BcsrView A, XL xl, Epi epi, Gr
   for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
   const uint32_t units = (A.nbr + RPB - 1) / RPB;
   const uint32_t Gd = gridDim.x;
+  // grid-stride in XCD-swizzled order (as the CSR kernel)
   const uint32_t wg = xcd_swizzle(blockIdx.x, Gd);
-  const uint32_t per = units / Gd, rem = units % Gd;
-  const uint32_t lo = wg * per + (wg < rem ? wg : rem);
-  const uint32_t hi = lo + per + (wg < rem ? 1u : 0u);
-  for (uint32_t u = lo; u < hi; ++u) {
+  for (uint32_t u = wg; u < units; u += Gd) {
     const uint32_t br = u * RPB + wid * W + g;
     float sum = 0.0f;
     if (br < A.nbr) {

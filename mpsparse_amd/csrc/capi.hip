@@ -750,10 +750,6 @@ static DiaView<V> dia_view_of(msp_csr A) {
   return v;
 }
 
-static uint32_t dia_grid(uint32_t n) {
-  return std::max<uint32_t>(1, std::min<uint32_t>((n + kBlock - 1) / kBlock, (uint32_t)cu_count() * 8));
-}
-
 // Workgroups of a kernel resident on one CU at once (1..8): the CSR kernel's
 // persistent grid must not exceed residency, or the surplus workgroups run as
 // a second wave after the first has finished.
@@ -770,12 +766,20 @@ static uint32_t csr_grid(uint32_t nblk) {
   return std::max(1u, std::min(nblk, per_cu * (uint32_t)cu_count()));
 }
 
+// The diagonal kernel is grid-stride over rows: a grid beyond residency would
+// leave its last workgroups running alone at the end.
+template <typename V, int ND, typename XL, typename Epi>
+static uint32_t dia_grid(uint32_t n) {
+  static const uint32_t per_cu = resident_per_cu(k_dia<V, ND, XL, Epi>);
+  return std::max<uint32_t>(1, std::min<uint32_t>((n + kBlock - 1) / kBlock, per_cu * (uint32_t)cu_count()));
+}
+
 // y = A x with a fused epilogue: diagonal storage when the matrix has it (and
 // its values of type V are prepared), the CSR row-block kernel otherwise.
 template <typename V, int ND, typename XL, typename Epi, typename Fin>
 static uint32_t spmv_any(msp_csr A, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
   const bool dia = A->dia_state == 1 && (V::kType == MSP_VALUES_F32 || A->dia16_type == V::kType);
-  const uint32_t g = dia ? dia_grid(A->n) : csr_grid<V, ND, XL, Epi>(A->nblk);
+  const uint32_t g = dia ? dia_grid<V, ND, XL, Epi>(A->n) : csr_grid<V, ND, XL, Epi>(A->nblk);
   if (dia) k_dia<V, ND, XL, Epi><<<g, kBlock, 0, s>>>(dia_view_of<V>(A), xl, epi, rb, done);
   else k_spmv<V, ND, XL, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), xl, epi, rb, done);
   finish<ND>(fin, rb, g, s);

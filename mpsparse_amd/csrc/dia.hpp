@@ -8,6 +8,8 @@
 // The row results feed the same epilogues as the CSR kernel.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "spmv.hpp"
 
@@ -33,9 +35,10 @@ struct DiaView {
 // chunks at 256^3 and within 10% at 128^3), plain coalesced loads (a
 // non-temporal hint on these 4-byte lane loads more than doubled the time);
 // partial sums of the epilogue go to rb. Every load of a row is issued before
-// the first product: with load and multiply-add together under the per-diagonal
-// branch, each diagonal waited for its own loads and a row had one value/x pair
-// in flight (2.5 TB/s on the 7-point operator at 256^3). Columns outside the
+// any arithmetic on loaded data (value widening, the gather's own arithmetic):
+// arithmetic under the per-diagonal branch makes each diagonal wait for its own
+// loads, leaving a row one value/x pair in flight (2.5 TB/s on the 7-point
+// operator at 256^3). Columns outside the
 // matrix read a clamped in-range x and are then left out of the sum, so the
 // result equals the guarded form bit for bit. Requires m > 0.
 template <typename V, int ND, typename XL, typename Epi>
@@ -49,15 +52,19 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
   const int nd = A.o.ndiag;
   const int64_t m = A.m;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += gridDim.x * kBlock) {
-    float a[kMaxDiag], xv[kMaxDiag];
+    // 16-bit values held one per register (packing two per register would
+    // wait for the loads)
+    using Held = typename std::conditional<sizeof(typename V::storage) == 2, uint32_t, float>::type;
+    Held a[kMaxDiag];
+    typename XL::Raw xr[kMaxDiag];
 #pragma unroll
     for (int d = 0; d < kMaxDiag; ++d) {
-      a[d] = 0.0f;
-      xv[d] = 0.0f;
-      if (d < nd) {
+      a[d] = {};
+      xr[d] = {};
+      if (d < nd) {  // loads only: no arithmetic on loaded data before the last load
         const int64_t c = (int64_t)i + A.o.off[d];
-        a[d] = V::cvt1(A.val[d * A.ld + i]);
-        xv[d] = xl((uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c)));
+        a[d] = A.val[d * A.ld + i];
+        xr[d] = xl.raw((uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c)));
       }
     }
     float s = 0.0f;
@@ -65,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
     for (int d = 0; d < kMaxDiag; ++d) {
       if (d < nd) {
         const int64_t c = (int64_t)i + A.o.off[d];
-        s = (c >= 0 && c < m) ? s + a[d] * xv[d] : s;
+        s = (c >= 0 && c < m) ? s + V::cvt1((typename V::storage)a[d]) * xl.make(xr[d]) : s;
       }
     }
     if constexpr (V::kLowDiag) {

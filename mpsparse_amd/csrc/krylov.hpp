@@ -140,6 +140,17 @@ struct XPNew {
   const float* __restrict__ pold;
   const float* __restrict__ dinv;
   const float* __restrict__ beta;
+  struct Raw {
+    float r, p, d;
+  };
+  __device__ __forceinline__ Raw raw(uint32_t c) const {
+    return Raw{r[c], pold[c], PC == kPcJacobi ? dinv[c] : 1.0f};
+  }
+  __device__ __forceinline__ float make(Raw w) const {
+    float z = w.r;
+    if (PC == kPcJacobi) z *= w.d;
+    return z + (*beta) * w.p;
+  }
   __device__ __forceinline__ float operator()(uint32_t c) const {
     float z = r[c];
     if (PC == kPcJacobi) z *= dinv[c];

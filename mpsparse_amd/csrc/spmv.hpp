@@ -43,8 +43,13 @@ __device__ __forceinline__ bool quad_run(u4 c) {
   return c.y == c.x + 1 && c.z == c.x + 2 && c.w == c.x + 3 && (c.x & 3u) == 0;
 }
 
+// Every loader also splits a gather into raw(c) (loads only) and make(raw)
+// (the arithmetic), so a kernel can issue many gathers before the first use.
 struct XPlain {
   const float* __restrict__ x;
+  typedef float Raw;
+  __device__ __forceinline__ Raw raw(uint32_t c) const { return x[c]; }
+  __device__ __forceinline__ float make(Raw w) const { return w; }
   __device__ __forceinline__ float operator()(uint32_t c) const { return x[c]; }
   __device__ __forceinline__ f4 load4(u4 c) const {
     if (quad_run(c)) return *reinterpret_cast<const f4*>(x + c.x);
@@ -58,6 +63,11 @@ struct XAxpy {
   const float* __restrict__ r;
   const float* __restrict__ v;
   const float* __restrict__ alpha;
+  struct Raw {
+    float r, v;
+  };
+  __device__ __forceinline__ Raw raw(uint32_t c) const { return Raw{r[c], v[c]}; }
+  __device__ __forceinline__ float make(Raw w) const { return w.r - (*alpha) * w.v; }
   __device__ __forceinline__ float operator()(uint32_t c) const { return r[c] - (*alpha) * v[c]; }
   __device__ __forceinline__ f4 load4(u4 c) const {
     const float a = *alpha;

@@ -768,10 +768,13 @@ static uint32_t csr_grid(uint32_t nblk) {
 
 // The diagonal kernel is grid-stride over rows: a grid beyond residency would
 // leave its last workgroups running alone at the end.
-template <typename V, int ND, typename XL, typename Epi>
-static uint32_t dia_grid(uint32_t n) {
-  static const uint32_t per_cu = resident_per_cu(k_dia<V, ND, XL, Epi>);
-  return std::max<uint32_t>(1, std::min<uint32_t>((n + kBlock - 1) / kBlock, per_cu * (uint32_t)cu_count()));
+template <typename V, int ND, typename XL, typename Epi, int NDIAG>
+static uint32_t dia_launch(msp_csr A, XL xl, Epi epi, GridRed rb, const int* done, hipStream_t s) {
+  static const uint32_t per_cu = resident_per_cu(k_dia<V, ND, XL, Epi, NDIAG>);
+  const uint32_t g =
+      std::max<uint32_t>(1, std::min<uint32_t>((A->n + kBlock - 1) / kBlock, per_cu * (uint32_t)cu_count()));
+  k_dia<V, ND, XL, Epi, NDIAG><<<g, kBlock, 0, s>>>(dia_view_of<V>(A), xl, epi, rb, done);
+  return g;
 }
 
 // y = A x with a fused epilogue: diagonal storage when the matrix has it (and
@@ -779,9 +782,18 @@ static uint32_t dia_grid(uint32_t n) {
 template <typename V, int ND, typename XL, typename Epi, typename Fin>
 static uint32_t spmv_any(msp_csr A, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
   const bool dia = A->dia_state == 1 && (V::kType == MSP_VALUES_F32 || A->dia16_type == V::kType);
-  const uint32_t g = dia ? dia_grid<V, ND, XL, Epi>(A->n) : csr_grid<V, ND, XL, Epi>(A->nblk);
-  if (dia) k_dia<V, ND, XL, Epi><<<g, kBlock, 0, s>>>(dia_view_of<V>(A), xl, epi, rb, done);
-  else k_spmv<V, ND, XL, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), xl, epi, rb, done);
+  uint32_t g;
+  if (dia) {
+    // the diagonal kernel is grid-stride over rows: a grid beyond residency
+    // would leave its last workgroups running alone at the end
+    const int nd = A->dia_o.ndiag;
+    if (nd == 7) g = dia_launch<V, ND, XL, Epi, 7>(A, xl, epi, rb, done, s);
+    else if (nd == 5) g = dia_launch<V, ND, XL, Epi, 5>(A, xl, epi, rb, done, s);
+    else g = dia_launch<V, ND, XL, Epi, 0>(A, xl, epi, rb, done, s);
+  } else {
+    g = csr_grid<V, ND, XL, Epi>(A->nblk);
+    k_spmv<V, ND, XL, Epi><<<g, kBlock, 0, s>>>(view_of<V>(A), xl, epi, rb, done);
+  }
   finish<ND>(fin, rb, g, s);
   return g;
 }

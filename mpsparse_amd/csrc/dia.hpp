@@ -41,24 +41,27 @@ struct DiaView {
 // operator at 256^3). Columns outside the
 // matrix read a clamped in-range x and are then left out of the sum, so the
 // result equals the guarded form bit for bit. Requires m > 0.
-template <typename V, int ND, typename XL, typename Epi>
+template <typename V, int ND, typename XL, typename Epi, int NDIAG>
 __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, GridRed rb,
                                                 const int* __restrict__ done) {
+  // NDIAG > 0: exactly NDIAG diagonals (no per-diagonal branch; the 5- and
+  // 7-point stencils); 0: A.o.ndiag <= kMaxDiag
+  constexpr int DMAX = NDIAG > 0 ? NDIAG : kMaxDiag;
   if (done && *done) return;
   constexpr int NA = ND > 0 ? ND : 1;
   CSum acc[NA];
 #pragma unroll
   for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
-  const int nd = A.o.ndiag;
+  const int nd = NDIAG > 0 ? NDIAG : A.o.ndiag;
   const int64_t m = A.m;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += gridDim.x * kBlock) {
     // 16-bit values held one per register (packing two per register would
     // wait for the loads)
     using Held = typename std::conditional<sizeof(typename V::storage) == 2, uint32_t, float>::type;
-    Held a[kMaxDiag];
-    typename XL::Raw xr[kMaxDiag];
+    Held a[DMAX];
+    typename XL::Raw xr[DMAX];
 #pragma unroll
-    for (int d = 0; d < kMaxDiag; ++d) {
+    for (int d = 0; d < DMAX; ++d) {
       a[d] = {};
       xr[d] = {};
       if (d < nd) {  // loads only: no arithmetic on loaded data before the last load
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
     }
     float s = 0.0f;
 #pragma unroll
-    for (int d = 0; d < kMaxDiag; ++d) {
+    for (int d = 0; d < DMAX; ++d) {
       if (d < nd) {
         const int64_t c = (int64_t)i + A.o.off[d];
         s = (c >= 0 && c < m) ? s + V::cvt1((typename V::storage)a[d]) * xl.make(xr[d]) : s;

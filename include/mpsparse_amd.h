@@ -152,6 +152,12 @@ const char *msp_last_error(void);
 void msp_default_opts(msp_solve_opts *opts);
 /* Number of visible GPUs (0 when none); never fails. */
 int32_t msp_device_count(void);
+/* Temporaries of matrix builds (sort buffers, probe counters) come from a
+ * library-owned memory pool per device that keeps freed memory for the next
+ * build. This waits for every device the pool serves and returns the cached
+ * memory to the system. No reference counterpart (the reference allocates
+ * and frees Metal buffers per call, iterative/BiCGSTAB/spmv.cpp:62-228). */
+msp_status msp_release_workspace(void);
 
 /* Standalone COO -> CSR.
  * Replaces coo_csr.coo_csr(keys, values, row_ptr, col_ind, out_vals, num_rows, num_cols)

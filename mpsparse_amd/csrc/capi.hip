@@ -9,6 +9,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -110,6 +112,67 @@ struct DevMem {
   }
   void reset() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Stream-ordered temporaries (sort buffers, probe counters, ...) come from one
+// library-owned memory pool per device that keeps freed memory for reuse:
+// repeated builds pay neither hipMalloc nor hipFree, whose implicit device
+// synchronisation stalled the stream. msp_release_workspace() returns the
+// cached memory to the system.
+static std::mutex g_pool_mu;
+static std::map<int, hipMemPool_t> g_pools;
+
+static msp_status tmp_pool(hipMemPool_t* out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  auto it = g_pools.find(dev);
+  if (it != g_pools.end()) {
+    *out = it->second;
+    return MSP_OK;
+  }
+  hipMemPoolProps props;
+  std::memset(&props, 0, sizeof props);
+  props.allocType = hipMemAllocationTypePinned;
+  props.handleTypes = hipMemHandleTypeNone;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = dev;
+  hipMemPool_t pool = nullptr;
+  HIPCHK(hipMemPoolCreate(&pool, &props));
+  uint64_t keep = UINT64_MAX;
+  HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  g_pools[dev] = pool;
+  *out = pool;
+  return MSP_OK;
+}
+
+struct TmpMem {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  TmpMem() = default;
+  TmpMem(const TmpMem&) = delete;
+  TmpMem& operator=(const TmpMem&) = delete;
+  ~TmpMem() { reset(); }
+  // freed on stream st after the work enqueued there so far
+  msp_status alloc(size_t bytes, hipStream_t st) {
+    reset();
+    if (bytes == 0) bytes = 256;
+    hipMemPool_t pool;
+    MSPCHK(tmp_pool(&pool));
+    hipError_t e = hipMallocFromPoolAsync(&p, bytes, pool, st);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(MSP_ERR_HIP, "hipMallocFromPoolAsync(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    s = st;
+    return MSP_OK;
+  }
+  void reset() {
+    if (p) (void)hipFreeAsync(p, s);
     p = nullptr;
   }
   template <typename T>
@@ -302,8 +365,8 @@ static msp_status ensure_dia(msp_csr A, hipStream_t s) {
     o.off[d] = (int32_t)offs[d];
     if (offs[d] == 0) dd = (int)d;
   }
-  DevMem miss;
-  MSPCHK(miss.alloc(sizeof(uint32_t)));
+  TmpMem miss;
+  MSPCHK(miss.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(miss.p, 0, sizeof(uint32_t), s));
   k_dia_check<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), n, o,
                                              miss.as<uint32_t>());
@@ -363,8 +426,8 @@ static msp_status ensure_format(msp_csr A, int32_t vt, hipStream_t s) {
 
 static msp_status finalize_matrix(msp_csr A, hipStream_t s) {
   const uint32_t n = A->n;
-  DevMem mx;
-  MSPCHK(mx.alloc(sizeof(uint32_t)));
+  TmpMem mx;
+  MSPCHK(mx.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), s));
   if (n > 0) {
     uint32_t g = std::min<uint32_t>((n + kBlock - 1) / kBlock, 4096);
@@ -409,16 +472,16 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
   const uint32_t num_tiles = (uint32_t)((nnz + kTile - 1) / kTile);
   const uint64_t N = (uint64_t)kRadix * num_tiles;
   const uint32_t nscan = (uint32_t)((N + kScanTile - 1) / kScanTile);
-  DevMem counts, sums, ka, kb, va, vb;
-  MSPCHK(counts.alloc(sizeof(uint32_t) * N));
-  MSPCHK(sums.alloc(sizeof(uint32_t) * nscan));
+  TmpMem counts, sums, ka, kb, va, vb;
+  MSPCHK(counts.alloc(sizeof(uint32_t) * N, s));
+  MSPCHK(sums.alloc(sizeof(uint32_t) * nscan, s));
   if (passes > 1) {
-    MSPCHK(ka.alloc(sizeof(K) * nnz));
-    MSPCHK(va.alloc(sizeof(float) * nnz));
+    MSPCHK(ka.alloc(sizeof(K) * nnz, s));
+    MSPCHK(va.alloc(sizeof(float) * nnz, s));
   }
   if (passes > 2) {
-    MSPCHK(kb.alloc(sizeof(K) * nnz));
-    MSPCHK(vb.alloc(sizeof(float) * nnz));
+    MSPCHK(kb.alloc(sizeof(K) * nnz, s));
+    MSPCHK(vb.alloc(sizeof(float) * nnz, s));
   }
   K* kbuf[2] = {ka.as<K>(), kb.as<K>()};
   float* vbuf[2] = {va.as<float>(), vb.as<float>()};
@@ -449,7 +512,7 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
     kin = kout;
     vin = vout;
   }
-  // temporaries are freed on return; hipFree waits for the device
+  // temporaries go back to the pool on return, ordered after the passes on s
   return MSP_OK;
 }
 
@@ -458,8 +521,8 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
 static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n, uint32_t m,
                                     uint32_t* row_ptr, uint32_t* col_out, float* val_out, hipStream_t s,
                                     uint32_t* rows_out = nullptr) {
-  DevMem err, rows_own;
-  MSPCHK(err.alloc(sizeof(int)));
+  TmpMem err, rows_own;
+  MSPCHK(err.alloc(sizeof(int), s));
   HIPCHK(hipMemsetAsync(err.p, 0, sizeof(int), s));
   if (nnz > 0) {
     sortk::KeyMap km;
@@ -471,7 +534,7 @@ static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uin
     const uint32_t tb = km.cbits + rbits;
     const int passes = std::max(1, (int)((tb + sortk::kRadixBits - 1) / sortk::kRadixBits));
     if (!rows_out) {
-      MSPCHK(rows_own.alloc(sizeof(uint32_t) * nnz));
+      MSPCHK(rows_own.alloc(sizeof(uint32_t) * nnz, s));
       rows_out = rows_own.as<uint32_t>();
     }
     if (tb <= 32)
@@ -495,6 +558,20 @@ static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uin
 extern "C" int32_t msp_abi_version(void) { return MSP_ABI_VERSION; }
 
 extern "C" const char* msp_last_error(void) { return g_err.c_str(); }
+
+extern "C" msp_status msp_release_workspace(void) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  if (g_pools.empty()) return MSP_OK;
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  for (auto& kv : g_pools) {
+    HIPCHK(hipSetDevice(kv.first));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemPoolTrimTo(kv.second, 0));
+  }
+  HIPCHK(hipSetDevice(cur));
+  return MSP_OK;
+}
 
 extern "C" void msp_default_opts(msp_solve_opts* o) {
   if (!o) return;

@@ -9,10 +9,14 @@ residual, default 1e-6), ``maxiter`` (10000), ``check_every``, ``criterion``
 (``"relres"`` or ``"reference"`` = sum(r^2) < tol tested every 50 steps),
 ``use_x0``, ``precond`` (``"none"``/``"jacobi"``, CG), ``values``
 (``"f32"``/``"bf16"``/``"f16"``). Solvers return an info dict.
+
+``release_workspace()`` returns the cached temporaries of matrix builds (a
+per-device memory pool the library keeps for the next build) to the system.
 """
 from . import load_ext
 
 _C = load_ext()
 csr_tensor = _C.csr_tensor
+release_workspace = _C.release_workspace
 
-__all__ = ["csr_tensor"]
+__all__ = ["csr_tensor", "release_workspace"]

@@ -104,3 +104,20 @@ def test_repeat_is_deterministic():
     a = _run_coo_csr(keys, vals, 50, 50, "cuda")
     b = _run_coo_csr(keys, vals, 50, 50, "cuda")
     _same(a, b)
+
+
+def test_pooled_temporaries_reused_and_released():
+    """Builds of different sizes reuse the pooled sort buffers (a larger build
+    after a smaller one, then a smaller one again), each bit-exact against the
+    oracle; releasing the pool between builds changes nothing."""
+    from mpsparse_amd import spmv
+
+    rng = np.random.default_rng(11)
+    for nnz, n in ((200000, 3000), (50000, 700), (400000, 5000), (30000, 400)):
+        rows, cols = rng.integers(0, n, nnz), rng.integers(0, n, nnz)
+        keys = pack_keys(rows, cols)
+        vals = rng.standard_normal(nnz).astype(np.float32)
+        say("pooled build", nnz, n)
+        _same(_run_coo_csr(keys, vals, n, n, "cuda"), oc.coo_to_csr(keys, vals, n, n))
+        if nnz == 400000:
+            spmv.release_workspace()

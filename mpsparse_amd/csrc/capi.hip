@@ -1065,6 +1065,12 @@ static msp_status solver_prologue(MatBase* A, const float* b, const float* x, co
   return MSP_OK;
 }
 
+// The CG iterate carries its rounding error in a second vector (x + xl) only
+// when the target is near FP32's attainable accuracy; looser solves (the
+// corrections of iterative refinement, 1e-3 by default) update x alone and
+// move two vectors less per step.
+static bool compensate(const msp_solve_opts& o) { return o.tol < 1e-4; }
+
 template <class Op>
 static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solve_opts& o, msp_solve_info* info,
                           hipStream_t s) {
@@ -1079,6 +1085,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   const float* B = w.v(Workspace::B);
   const float* D = A->dinv.as<float>();
   Scalars* S = w.S_dev();
+  const bool comp = compensate(o);
   GridRed rb = w.gr();
   const uint32_t n = A->n, gv = vec_grid(n);
   const bool jac = o.precond == MSP_PRECOND_JACOBI;
@@ -1127,10 +1134,12 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       g = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S->halt, s);
     }
     if (jac) {
-      k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      if (comp) k_cg_update_xr<kPcJacobi, true, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      else k_cg_update_xr<kPcJacobi, true, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcJacobi>{S}, rb2, gv, s);
     } else {
-      k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      if (comp) k_cg_update_xr<kPcNone, true, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      else k_cg_update_xr<kPcNone, true, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcNone>{S}, rb2, gv, s);
     }
     ++k;
@@ -1924,6 +1933,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   Scalars* S = w.S_dev();
   GridRed rb = w.gr();
   const uint32_t n = A->n, gv = vec_grid(n);
+  const bool comp = compensate(o);
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
     k_cg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, P, B, nullptr, n, o.use_x0, kPcExplicit, S, rb);
@@ -1936,7 +1946,8 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinPcgBeta{S}, rb, &S->halt, s);
     k_cg_update_p<kPcExplicit><<<gv, kBlock, 0, s>>>(P, P, R, nullptr, Z, n, S);
     op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->halt, s);
-    k_cg_update_xr<kPcExplicit, false><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb, rb, 0);
+    if (comp) k_cg_update_xr<kPcExplicit, false, true><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb, rb, 0);
+    else k_cg_update_xr<kPcExplicit, false, false><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb, rb, 0);
     finish<2>(FinCgStep<kPcExplicit>{S}, rb, gv, s);
   };
   auto verify = [&]() {

@@ -283,8 +283,10 @@ __device__ __forceinline__ float block_total(const CSum* __restrict__ partials, 
 // ALPHA_LOCAL: alpha = <r,z> / <p,q> from the SpMV's block partials (pq, npq
 // blocks), computed identically by every workgroup; block 0 records pq and
 // alpha, and a non-positive <p,q> leaves x and r alone (FinCgStep then stops
-// with a breakdown). Otherwise alpha comes from the state.
-template <int PC, bool ALPHA_LOCAL>
+// with a breakdown). Otherwise alpha comes from the state. COMP: the iterate
+// is x + xl with xl carrying the rounding error of x += alpha p (for targets
+// near FP32's attainable accuracy); otherwise x alone, xl untouched.
+template <int PC, bool ALPHA_LOCAL, bool COMP>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ xl,
                                                          float* __restrict__ r, const float* __restrict__ p,
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
@@ -301,29 +303,33 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
       S->alpha = alpha;
     }
   }
-  const bool fold = S->collapse != 0;
+  [[maybe_unused]] const bool fold = S->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
     f4 xv = reinterpret_cast<f4*>(x)[c];
-    f4 xlv = reinterpret_cast<f4*>(xl)[c];
-    if (fold) {
-      xv += xlv;
-      xlv = f4{0.f, 0.f, 0.f, 0.f};
-    }
     f4 rv = reinterpret_cast<f4*>(r)[c];
     f4 pv = reinterpret_cast<const f4*>(p)[c];
     f4 qv = reinterpret_cast<const f4*>(q)[c];
+    if constexpr (COMP) {
+      f4 xlv = reinterpret_cast<f4*>(xl)[c];
+      if (fold) {
+        xv += xlv;
+        xlv = f4{0.f, 0.f, 0.f, 0.f};
+      }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      float xk = xv[k], lk = xlv[k];
-      comp_axpy(xk, lk, alpha, pv[k]);
-      xv[k] = xk;
-      xlv[k] = lk;
+      for (int k = 0; k < 4; ++k) {
+        float xk = xv[k], lk = xlv[k];
+        comp_axpy(xk, lk, alpha, pv[k]);
+        xv[k] = xk;
+        xlv[k] = lk;
+      }
+      reinterpret_cast<f4*>(xl)[c] = xlv;
+    } else {
+      xv += alpha * pv;
     }
     rv -= alpha * qv;
     reinterpret_cast<f4*>(x)[c] = xv;
-    reinterpret_cast<f4*>(xl)[c] = xlv;
     reinterpret_cast<f4*>(r)[c] = rv;
     f4 zv = rv;
     if (PC == kPcJacobi) zv = rv * reinterpret_cast<const f4*>(dinv)[c];

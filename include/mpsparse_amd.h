@@ -153,10 +153,12 @@ void msp_default_opts(msp_solve_opts *opts);
 /* Number of visible GPUs (0 when none); never fails. */
 int32_t msp_device_count(void);
 /* Temporaries of matrix builds (sort buffers, probe counters) come from a
- * library-owned memory pool per device that keeps freed memory for the next
- * build. This waits for every device the pool serves and returns the cached
- * memory to the system. No reference counterpart (the reference allocates
- * and frees Metal buffers per call, iterative/BiCGSTAB/spmv.cpp:62-228). */
+ * library-owned scratch arena per (device, stream) that is kept for the next
+ * call on that stream. This waits for each device with an idle arena and
+ * returns its memory to the system (arenas in use by a call in progress are
+ * kept). A stream handle must not be destroyed and recreated while work that
+ * used its arena is still running. No reference counterpart (the reference
+ * allocates and frees Metal buffers per call, iterative/BiCGSTAB/spmv.cpp:62-228). */
 msp_status msp_release_workspace(void);
 
 /* Standalone COO -> CSR.

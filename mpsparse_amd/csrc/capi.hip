@@ -118,61 +118,90 @@ struct DevMem {
   T* as() const { return static_cast<T*>(p); }
 };
 
-// Stream-ordered temporaries (sort buffers, probe counters, ...) come from one
-// library-owned memory pool per device that keeps freed memory for reuse:
-// repeated builds pay neither hipMalloc nor hipFree, whose implicit device
-// synchronisation stalled the stream. msp_release_workspace() returns the
-// cached memory to the system.
-static std::mutex g_pool_mu;
-static std::map<int, hipMemPool_t> g_pools;
+// Temporaries (sort buffers, probe counters, ...) come from a scratch arena
+// per (device, stream) that the library keeps between calls: a stack of
+// blocks carved from chunks that are never returned on their own, so repeated
+// builds pay neither hipMalloc nor hipFree (whose implicit synchronisation
+// stalled the host for ~1 ms per large buffer). Reuse is safe because every
+// user of a block is enqueued on the arena's stream: work enqueued after a
+// block is released runs after the work that used it. Blocks are released in
+// reverse order of allocation (a block released early is popped once those
+// above it are). msp_release_workspace() frees the chunks of idle arenas.
+struct ArenaChunk {
+  void* p;
+  size_t size;
+};
+struct ArenaRec {
+  size_t chunk, offset, end;
+  bool freed;
+};
+struct Arena {
+  std::mutex mu;
+  std::vector<ArenaChunk> chunks;
+  std::vector<ArenaRec> stack;
+};
+static std::mutex g_arena_mu;
+static std::map<std::pair<int, hipStream_t>, Arena*> g_arenas;
 
-static msp_status tmp_pool(hipMemPool_t* out) {
+static msp_status arena_for(hipStream_t s, Arena** out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_pool_mu);
-  auto it = g_pools.find(dev);
-  if (it != g_pools.end()) {
-    *out = it->second;
-    return MSP_OK;
-  }
-  hipMemPoolProps props;
-  std::memset(&props, 0, sizeof props);
-  props.allocType = hipMemAllocationTypePinned;
-  props.handleTypes = hipMemHandleTypeNone;
-  props.location.type = hipMemLocationTypeDevice;
-  props.location.id = dev;
-  hipMemPool_t pool = nullptr;
-  HIPCHK(hipMemPoolCreate(&pool, &props));
-  uint64_t keep = UINT64_MAX;
-  HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
-  g_pools[dev] = pool;
-  *out = pool;
+  std::lock_guard<std::mutex> lk(g_arena_mu);
+  Arena*& a = g_arenas[std::make_pair(dev, s)];
+  if (!a) a = new Arena;
+  *out = a;
   return MSP_OK;
+}
+
+static msp_status arena_alloc(Arena* a, size_t bytes, void** p, size_t* rec) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> lk(a->mu);
+  size_t c = 0, off = 0;
+  if (!a->stack.empty()) {
+    c = a->stack.back().chunk;
+    off = a->stack.back().end;
+  }
+  while (c < a->chunks.size() && off + bytes > a->chunks[c].size) {
+    ++c;
+    off = 0;
+  }
+  if (c == a->chunks.size()) {
+    size_t sz = std::max<size_t>(bytes, (size_t)64 << 20);
+    if (!a->chunks.empty()) sz = std::max(sz, a->chunks.back().size);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, sz);
+    if (e != hipSuccess) return fail(MSP_ERR_HIP, "hipMalloc(%zu bytes) for scratch: %s", sz, hipGetErrorString(e));
+    a->chunks.push_back(ArenaChunk{q, sz});
+  }
+  *p = static_cast<char*>(a->chunks[c].p) + off;
+  a->stack.push_back(ArenaRec{c, off, off + bytes, false});
+  *rec = a->stack.size() - 1;
+  return MSP_OK;
+}
+
+static void arena_free(Arena* a, size_t rec) {
+  std::lock_guard<std::mutex> lk(a->mu);
+  if (rec < a->stack.size()) a->stack[rec].freed = true;
+  while (!a->stack.empty() && a->stack.back().freed) a->stack.pop_back();
 }
 
 struct TmpMem {
   void* p = nullptr;
-  hipStream_t s = nullptr;
+  Arena* a = nullptr;
+  size_t rec = 0;
   TmpMem() = default;
   TmpMem(const TmpMem&) = delete;
   TmpMem& operator=(const TmpMem&) = delete;
   ~TmpMem() { reset(); }
-  // freed on stream st after the work enqueued there so far
+  // a block of the arena of stream st; every user must be enqueued on st
   msp_status alloc(size_t bytes, hipStream_t st) {
     reset();
     if (bytes == 0) bytes = 256;
-    hipMemPool_t pool;
-    MSPCHK(tmp_pool(&pool));
-    hipError_t e = hipMallocFromPoolAsync(&p, bytes, pool, st);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return fail(MSP_ERR_HIP, "hipMallocFromPoolAsync(%zu bytes): %s", bytes, hipGetErrorString(e));
-    }
-    s = st;
-    return MSP_OK;
+    MSPCHK(arena_for(st, &a));
+    return arena_alloc(a, bytes, &p, &rec);
   }
   void reset() {
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) arena_free(a, rec);
     p = nullptr;
   }
   template <typename T>
@@ -252,6 +281,7 @@ struct Workspace {
   // two halves: SpMV partials, and those of the kernel that consumes them
   GridRed gr() const { return GridRed{red.as<CSum>()}; }
   GridRed gr2() const { return GridRed{red.as<CSum>() + (size_t)16 * cu_count()}; }
+  GridRed gr3() const { return GridRed{red.as<CSum>() + (size_t)32 * cu_count()}; }
   ~Workspace() {
     if (hS) (void)hipHostFree(hS);
     if (hflag) (void)hipHostFree(hflag);
@@ -311,7 +341,7 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
     MSPCHK(w.scal.alloc(sizeof(Scalars)));
     HIPCHK(hipMemsetAsync(w.scal.p, 0, sizeof(Scalars), s));
     // partials of up to 4 values for grids of up to 8 workgroups per CU
-    size_t red_bytes = sizeof(CSum) * 4 * (size_t)cu_count() * 8;
+    size_t red_bytes = sizeof(CSum) * 6 * (size_t)cu_count() * 8;  // three regions of 16 CSums per CU
     MSPCHK(w.red.alloc(red_bytes));
     HIPCHK(hipMemsetAsync(w.red.p, 0, red_bytes, s));
     HIPCHK(hipHostMalloc((void**)&w.hS, sizeof(Scalars), hipHostMallocDefault));
@@ -335,20 +365,29 @@ static msp_status ensure_dia(msp_csr A, hipStream_t s) {
     return MSP_OK;
   const uint32_t n = A->n;
   std::vector<int64_t> offs;
-  // 64 sample rows from a fixed-seed LCG (rows on a grid face lack some offsets;
-  // a sample that misses one only sends the matrix back to CSR)
+  // kDiaSamples sample rows from a fixed-seed LCG (rows on a grid face lack
+  // some offsets; a sample that misses one only sends the matrix back to CSR),
+  // gathered by one kernel and read back with one copy
+  DiaSample smp;
   uint64_t st = 0x9E3779B97F4A7C15ull;
-  for (int k = 0; k < 64; ++k) {
+  for (int k = 0; k < kDiaSamples; ++k) {
     st = st * 6364136223846793005ull + 1442695040888963407ull;
-    const uint32_t r = (uint32_t)((st >> 33) % n);
-    uint32_t rp[2] = {0, 0};
-    HIPCHK(hipMemcpy(rp, A->row_ptr.as<uint32_t>() + r, sizeof rp, hipMemcpyDeviceToHost));
-    const uint32_t len = rp[1] - rp[0];
-    if (len == 0) continue;
-    uint32_t cols[kMaxDiag];
-    HIPCHK(hipMemcpy(cols, A->col.as<uint32_t>() + rp[0], sizeof(uint32_t) * len, hipMemcpyDeviceToHost));
+    smp.row[k] = (uint32_t)((st >> 33) % n);
+  }
+  constexpr size_t kSampleWords = (size_t)kDiaSamples * (kMaxDiag + 1);
+  TmpMem sbuf;
+  MSPCHK(sbuf.alloc(sizeof(uint32_t) * kSampleWords, s));
+  k_dia_sample<<<1, kDiaSamples, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), smp,
+                                         sbuf.as<uint32_t>());
+  MSPCHK(launch_check("k_dia_sample"));
+  std::vector<uint32_t> hs(kSampleWords);
+  HIPCHK(hipMemcpyAsync(hs.data(), sbuf.p, sizeof(uint32_t) * kSampleWords, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < kDiaSamples; ++k) {
+    const uint32_t* rec = hs.data() + (size_t)k * (kMaxDiag + 1);
+    const uint32_t len = std::min<uint32_t>(rec[0], (uint32_t)kMaxDiag);
     for (uint32_t e = 0; e < len; ++e) {
-      const int64_t o = (int64_t)cols[e] - (int64_t)r;
+      const int64_t o = (int64_t)rec[1 + e] - (int64_t)smp.row[k];
       if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
     }
     if (offs.size() > (size_t)kMaxDiag) return MSP_OK;
@@ -430,7 +469,7 @@ static msp_status finalize_matrix(msp_csr A, hipStream_t s) {
   MSPCHK(mx.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), s));
   if (n > 0) {
-    uint32_t g = std::min<uint32_t>((n + kBlock - 1) / kBlock, 4096);
+    uint32_t g = std::min<uint32_t>((n + kBlock - 1) / kBlock, 1024);
     k_max_row<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), n, mx.as<uint32_t>());
     MSPCHK(launch_check("k_max_row"));
   }
@@ -560,14 +599,18 @@ extern "C" int32_t msp_abi_version(void) { return MSP_ABI_VERSION; }
 extern "C" const char* msp_last_error(void) { return g_err.c_str(); }
 
 extern "C" msp_status msp_release_workspace(void) {
-  std::lock_guard<std::mutex> lk(g_pool_mu);
-  if (g_pools.empty()) return MSP_OK;
+  std::lock_guard<std::mutex> lk(g_arena_mu);
+  if (g_arenas.empty()) return MSP_OK;
   int cur = 0;
   HIPCHK(hipGetDevice(&cur));
-  for (auto& kv : g_pools) {
-    HIPCHK(hipSetDevice(kv.first));
+  for (auto& kv : g_arenas) {
+    Arena* a = kv.second;
+    std::lock_guard<std::mutex> la(a->mu);
+    if (!a->stack.empty()) continue;  // in use by a call in progress
+    HIPCHK(hipSetDevice(kv.first.first));
     HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemPoolTrimTo(kv.second, 0));
+    for (auto& c : a->chunks) HIPCHK(hipFree(c.p));
+    a->chunks.clear();
   }
   HIPCHK(hipSetDevice(cur));
   return MSP_OK;
@@ -863,9 +906,12 @@ static uint32_t spmv_any(msp_csr A, XL xl, Epi epi, Fin fin, GridRed rb, const i
   if (dia) {
     // the diagonal kernel is grid-stride over rows: a grid beyond residency
     // would leave its last workgroups running alone at the end
+    // (3, 5, 7: 1D/2D/3D stencils; 3 and 4 also their FSAI factors' lower triangles)
     const int nd = A->dia_o.ndiag;
     if (nd == 7) g = dia_launch<V, ND, XL, Epi, 7>(A, xl, epi, rb, done, s);
     else if (nd == 5) g = dia_launch<V, ND, XL, Epi, 5>(A, xl, epi, rb, done, s);
+    else if (nd == 4) g = dia_launch<V, ND, XL, Epi, 4>(A, xl, epi, rb, done, s);
+    else if (nd == 3) g = dia_launch<V, ND, XL, Epi, 3>(A, xl, epi, rb, done, s);
     else g = dia_launch<V, ND, XL, Epi, 0>(A, xl, epi, rb, done, s);
   } else {
     g = csr_grid<V, ND, XL, Epi>(A->nblk);
@@ -1134,12 +1180,12 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       g = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S->halt, s);
     }
     if (jac) {
-      if (comp) k_cg_update_xr<kPcJacobi, true, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      else k_cg_update_xr<kPcJacobi, true, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      if (comp) k_cg_update_xr<kPcJacobi, kAlphaPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      else k_cg_update_xr<kPcJacobi, kAlphaPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcJacobi>{S}, rb2, gv, s);
     } else {
-      if (comp) k_cg_update_xr<kPcNone, true, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      else k_cg_update_xr<kPcNone, true, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      if (comp) k_cg_update_xr<kPcNone, kAlphaPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      else k_cg_update_xr<kPcNone, kAlphaPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcNone>{S}, rb2, gv, s);
     }
     ++k;
@@ -1345,16 +1391,16 @@ static msp_status csr_to_bcsr_device(msp_bcsr A, const uint32_t* row_ptr, const 
   const uint32_t B = A->B, n = A->n, nbr = A->nbr, nbc = A->nbc;
   const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n + kBlock - 1) / kBlock, 65536));
   const uint32_t gz = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nnz + kBlock - 1) / kBlock, 65536));
-  DevMem bkeys, payload, brp_nnz, sbcol, sbrow, spayload, rows, flag, excl, sums;
-  MSPCHK(bkeys.alloc(sizeof(uint64_t) * nnz));
-  MSPCHK(payload.alloc(sizeof(float) * nnz));
-  MSPCHK(brp_nnz.alloc(sizeof(uint32_t) * ((size_t)nbr + 1)));
-  MSPCHK(sbcol.alloc(sizeof(uint32_t) * nnz));
-  MSPCHK(sbrow.alloc(sizeof(uint32_t) * nnz));
-  MSPCHK(spayload.alloc(sizeof(float) * nnz));
-  MSPCHK(rows.alloc(sizeof(uint32_t) * nnz));
-  MSPCHK(flag.alloc(sizeof(uint32_t) * nnz));
-  MSPCHK(excl.alloc(sizeof(uint32_t) * nnz));
+  TmpMem bkeys, payload, brp_nnz, sbcol, sbrow, spayload, rows, flag, excl, sums;
+  MSPCHK(bkeys.alloc(sizeof(uint64_t) * nnz, s));
+  MSPCHK(payload.alloc(sizeof(float) * nnz, s));
+  MSPCHK(brp_nnz.alloc(sizeof(uint32_t) * ((size_t)nbr + 1), s));
+  MSPCHK(sbcol.alloc(sizeof(uint32_t) * nnz, s));
+  MSPCHK(sbrow.alloc(sizeof(uint32_t) * nnz, s));
+  MSPCHK(spayload.alloc(sizeof(float) * nnz, s));
+  MSPCHK(rows.alloc(sizeof(uint32_t) * nnz, s));
+  MSPCHK(flag.alloc(sizeof(uint32_t) * nnz, s));
+  MSPCHK(excl.alloc(sizeof(uint32_t) * nnz, s));
   uint32_t L = 1;
   while (L < 64 && L * n < nnz) L <<= 1;  // lanes per row ~ mean row length
   k_block_keys<<<g, kBlock, 0, s>>>(row_ptr, col, n, B, L, bkeys.as<uint64_t>(), payload.as<float>(),
@@ -1365,7 +1411,7 @@ static msp_status csr_to_bcsr_device(msp_bcsr A, const uint32_t* row_ptr, const 
   k_block_heads<<<gz, kBlock, 0, s>>>(sbrow.as<uint32_t>(), sbcol.as<uint32_t>(), nnz, flag.as<uint32_t>());
   // exclusive scan of the head flags
   const uint32_t nscan = (uint32_t)((nnz + sortk::kScanTile - 1) / sortk::kScanTile);
-  MSPCHK(sums.alloc(sizeof(uint32_t) * nscan));
+  MSPCHK(sums.alloc(sizeof(uint32_t) * nscan, s));
   HIPCHK(hipMemcpyAsync(excl.p, flag.p, sizeof(uint32_t) * nnz, hipMemcpyDeviceToDevice, s));
   sortk::k_scan_reduce<<<nscan, kBlock, 0, s>>>(excl.as<uint32_t>(), nnz, sums.as<uint32_t>());
   sortk::k_scan_top<<<1, kBlock, 0, s>>>(sums.as<uint32_t>(), nscan);
@@ -1689,9 +1735,9 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     MSPCHK(zero_pad(&F->G, s));
     MSPCHK(finalize_matrix(&F->G, s));
     // transpose pattern by sorting (col << 32 | row) with the edge index as payload
-    DevMem keys, payload;
-    MSPCHK(keys.alloc(sizeof(uint64_t) * (ne + 1)));
-    MSPCHK(payload.alloc(sizeof(float) * (ne + 1)));
+    TmpMem keys, payload;
+    MSPCHK(keys.alloc(sizeof(uint64_t) * (ne + 1), s));
+    MSPCHK(payload.alloc(sizeof(float) * (ne + 1), s));
     k_transpose_keys<<<grid_for(ne), kBlock, 0, s>>>(F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), ne,
                                                      keys.as<uint64_t>(), payload.as<float>());
     MSPCHK(launch_check("transpose keys"));
@@ -1785,10 +1831,10 @@ extern "C" msp_status msp_fsai_set_values(msp_fsai F, const float* g, int32_t* n
   MSPCHK(copy_in(F->G.val.p, g, sizeof(float) * F->ne, s));
   // count non-positive diagonal entries with the scaling kernel (its scaled
   // output goes to a throw-away buffer; colscale > 0 keeps the signs)
-  DevMem cnt, scaled;
-  MSPCHK(cnt.alloc(sizeof(uint32_t)));
+  TmpMem cnt, scaled;
+  MSPCHK(cnt.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
-  MSPCHK(scaled.alloc(sizeof(float) * (F->ne + 1)));
+  MSPCHK(scaled.alloc(sizeof(float) * (F->ne + 1), s));
   k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(F->G.val.as<float>(), F->colscale.as<float>(), F->feat.as<float>(),
                                                   F->ne, scaled.as<float>(), cnt.as<uint32_t>());
   MSPCHK(fsai_commit(F, s));
@@ -1807,8 +1853,8 @@ extern "C" msp_status msp_fsai_set_network_output(msp_fsai F, const float* out, 
   hipStream_t s = (hipStream_t)stream;
   InArr o;
   MSPCHK(o.set(out, sizeof(float) * F->ne, s));
-  DevMem cnt;
-  MSPCHK(cnt.alloc(sizeof(uint32_t)));
+  TmpMem cnt;
+  MSPCHK(cnt.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
   k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(static_cast<const float*>(o.d), F->colscale.as<float>(),
                                                   F->feat.as<float>(), F->ne, F->G.val.as<float>(),
@@ -1828,8 +1874,8 @@ extern "C" msp_status msp_fsai_classical(msp_fsai F, int32_t* fallback_rows, voi
   HIPCHK(hipSetDevice(F->device));
   hipStream_t s = (hipStream_t)stream;
   msp_csr A = F->A;
-  DevMem cnt;
-  MSPCHK(cnt.alloc(sizeof(uint32_t)));
+  TmpMem cnt;
+  MSPCHK(cnt.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
   k_fsai_classical<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
                                                      A->val.as<float>(), A->n, F->G.row_ptr.as<uint32_t>(),
@@ -1854,8 +1900,8 @@ extern "C" msp_status msp_fsai_export_values(msp_fsai F, float* g, void* stream)
 }
 
 template <int ND, typename XL, typename Epi, typename Fin>
-static void spmv_with(msp_csr M, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
-  spmv_any<ValF32, ND>(M, xl, epi, fin, rb, done, s);
+static uint32_t spmv_with(msp_csr M, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
+  return spmv_any<ValF32, ND>(M, xl, epi, fin, rb, done, s);
 }
 
 extern "C" msp_status msp_fsai_apply(msp_fsai F, const float* r, float* z, void* stream) {
@@ -1941,14 +1987,22 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     k_set_restart<<<1, 1, 0, s>>>(S);
     return launch_check("pcg init");
   };
+  // one step, six launches: w = G r; z = G^T w with <r,z> partials (zr);
+  // p = z + beta p with beta from zr; q = A p with <p,q> partials (rb);
+  // alpha from zr and rb inside the x/r update (partials rb2); FinCgStep
+  GridRed rb2 = w.gr2(), zr = w.gr3();
   auto step = [&]() {
     spmv_with<0>(&F->G, XPlain{R}, EpiStore{Wv}, FinNone{}, rb, &S->halt, s);
-    spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinPcgBeta{S}, rb, &S->halt, s);
-    k_cg_update_p<kPcExplicit><<<gv, kBlock, 0, s>>>(P, P, R, nullptr, Z, n, S);
-    op.template spmv<1>(P, EpiStoreDot{Q, P}, FinCgAlpha{S}, &S->halt, s);
-    if (comp) k_cg_update_xr<kPcExplicit, false, true><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb, rb, 0);
-    else k_cg_update_xr<kPcExplicit, false, false><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb, rb, 0);
-    finish<2>(FinCgStep<kPcExplicit>{S}, rb, gv, s);
+    const uint32_t gz = spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinNone{}, zr, &S->halt, s);
+    k_cg_update_p<kPcExplicit, true><<<gv, kBlock, 0, s>>>(P, P, R, nullptr, Z, n, S, zr, gz);
+    const uint32_t gq = op.template spmv<1>(P, EpiStoreDot{Q, P}, FinNone{}, &S->halt, s);
+    if (comp)
+      k_cg_update_xr<kPcExplicit, kAlphaRzPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb2, rb,
+                                                                          gq, zr, gz);
+    else
+      k_cg_update_xr<kPcExplicit, kAlphaRzPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb2, rb,
+                                                                           gq, zr, gz);
+    finish<2>(FinCgStep<kPcExplicit>{S}, rb2, gv, s);
   };
   auto verify = [&]() {
     k_verify<3><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, nullptr, n, S, rb);
@@ -2087,8 +2141,8 @@ extern "C" msp_status msp_fsai_apply_network(msp_fsai F, const float* params, in
                                              void* stream) {
   if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
   HIPCHK(hipSetDevice(F->device));
-  DevMem out;
-  MSPCHK(out.alloc(sizeof(float) * (F->ne + 1)));
+  TmpMem out;
+  MSPCHK(out.alloc(sizeof(float) * (F->ne + 1), (hipStream_t)stream));
   MSPCHK(msp_mlp_forward(params, F->feat.as<float>(), F->ne, out.as<float>(), stream));
   return msp_fsai_set_network_output(F, out.as<float>(), nonpositive_diag, stream);
 }

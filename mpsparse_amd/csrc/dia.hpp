@@ -44,8 +44,8 @@ struct DiaView {
 template <typename V, int ND, typename XL, typename Epi, int NDIAG>
 __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, GridRed rb,
                                                 const int* __restrict__ done) {
-  // NDIAG > 0: exactly NDIAG diagonals (no per-diagonal branch; the 5- and
-  // 7-point stencils); 0: A.o.ndiag <= kMaxDiag
+  // NDIAG > 0: exactly NDIAG diagonals (no per-diagonal branch; stencils and
+  // their triangular factors); 0: A.o.ndiag <= kMaxDiag
   constexpr int DMAX = NDIAG > 0 ? NDIAG : kMaxDiag;
   if (done && *done) return;
   constexpr int NA = ND > 0 ? ND : 1;
@@ -84,6 +84,23 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
     epi.row(i, s, acc);
   }
   if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
+}
+
+// Rows sampled by the diagonal probe: for sample k (one thread each),
+// out[k * (kMaxDiag + 1)] = row length, then up to kMaxDiag column indices.
+constexpr int kDiaSamples = 64;
+struct DiaSample {
+  uint32_t row[kDiaSamples];
+};
+
+__global__ __launch_bounds__(kDiaSamples) void k_dia_sample(const uint32_t* __restrict__ row_ptr,
+                                                            const uint32_t* __restrict__ col, DiaSample smp,
+                                                            uint32_t* __restrict__ out) {
+  const uint32_t k = threadIdx.x, r = smp.row[k];
+  const uint32_t b = row_ptr[r], len = row_ptr[r + 1] - b;
+  uint32_t* o = out + k * (kMaxDiag + 1);
+  o[0] = len;
+  for (uint32_t j = 0; j < len && j < (uint32_t)kMaxDiag; ++j) o[1 + j] = col[b + j];
 }
 
 // Entries whose offset is not in the candidate list (counted), per row.

@@ -280,27 +280,42 @@ __device__ __forceinline__ float block_total(const CSum* __restrict__ partials, 
 }
 
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
-// ALPHA_LOCAL: alpha = <r,z> / <p,q> from the SpMV's block partials (pq, npq
-// blocks), computed identically by every workgroup; block 0 records pq and
-// alpha, and a non-positive <p,q> leaves x and r alone (FinCgStep then stops
-// with a breakdown). Otherwise alpha comes from the state. COMP: the iterate
-// is x + xl with xl carrying the rounding error of x += alpha p (for targets
-// near FP32's attainable accuracy); otherwise x alone, xl untouched.
-template <int PC, bool ALPHA_LOCAL, bool COMP>
+// ALPHA: kAlphaState takes alpha from the state. kAlphaPq: alpha = <r,z> /
+// <p,q> with <p,q> from the SpMV's block partials (pq, npq blocks), computed
+// identically by every workgroup; block 0 records pq and alpha, and a
+// non-positive <p,q> leaves x and r alone (FinCgStep then stops with a
+// breakdown). kAlphaRzPq (explicit preconditioner): <r,z> also from block
+// partials (zr, nz blocks, those of the z = M r product), and block 0 records
+// it as the new <r,z> and clears the restart flag; a non-positive <r,z> is
+// recorded as a breakdown the same way. COMP: the iterate is x + xl with xl
+// carrying the rounding error of x += alpha p (for targets near FP32's
+// attainable accuracy); otherwise x alone, xl untouched.
+enum : int { kAlphaState = 0, kAlphaPq = 1, kAlphaRzPq = 2 };
+
+template <int PC, int ALPHA, bool COMP>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ xl,
                                                          float* __restrict__ r, const float* __restrict__ p,
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
                                                          uint32_t n, Scalars* S, GridRed rb, GridRed pq,
-                                                         uint32_t npq) {
+                                                         uint32_t npq, GridRed zr = GridRed{}, uint32_t nz = 0) {
   if (S->halt) return;
   float alpha = S->alpha;
-  if constexpr (ALPHA_LOCAL) {
+  if constexpr (ALPHA != kAlphaState) {
     const float pqv = block_total<1>(pq.partials, npq, 0);
-    const bool ok = finite_pos(pqv);
-    alpha = ok ? S->rz / pqv : 0.0f;
+    float rz = S->rz;
+    bool ok = finite_pos(pqv);
+    if constexpr (ALPHA == kAlphaRzPq) {
+      rz = block_total<1>(zr.partials, nz, 0);
+      ok = ok && finite_pos(rz);
+    }
+    alpha = ok ? rz / pqv : 0.0f;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-      S->pq = pqv;
+      S->pq = ok ? pqv : -1.0f;
       S->alpha = alpha;
+      if constexpr (ALPHA == kAlphaRzPq) {
+        S->rz = rz;
+        S->restart = 0;
+      }
     }
   }
   [[maybe_unused]] const bool fold = S->collapse != 0;
@@ -380,13 +395,22 @@ struct FinCgStep {
 };
 
 // pnew = z + beta pold with z = r, dinv*r, or an explicit vector (pnew may
-// alias pold).
-template <int PC>
+// alias pold). BETA_LOCAL (explicit preconditioner): beta = <r,z>_new /
+// <r,z>_old from the block partials of the z product (zr, nz blocks) and the
+// state's <r,z>, which k_cg_update_xr<.., kAlphaRzPq, ..> replaces later in
+// the step; 0 on the first step, after a restart, or for a non-positive
+// <r,z>_new (the update then records the breakdown).
+template <int PC, bool BETA_LOCAL = false>
 __global__ __launch_bounds__(kBlock) void k_cg_update_p(float* pnew, const float* pold, const float* __restrict__ r,
                                                         const float* __restrict__ dinv, const float* __restrict__ z,
-                                                        uint32_t n, const Scalars* S) {
+                                                        uint32_t n, const Scalars* S, GridRed zr = GridRed{},
+                                                        uint32_t nz = 0) {
   if (S->halt) return;
-  const float beta = S->beta;
+  float beta = S->beta;
+  if constexpr (BETA_LOCAL) {
+    const float rz_new = block_total<1>(zr.partials, nz, 0);
+    beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
+  }
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
     f4 zv;

@@ -333,9 +333,11 @@ __global__ __launch_bounds__(kBlock) void k_row_blocks(const uint32_t* __restric
   blk[b] = lo;
 }
 
-// Maximum row length (for block sizing) via atomicMax.
+// Maximum row length (for block sizing): one atomicMax per workgroup (one per
+// wave on a single address serialised into ~0.2 ms at 2M rows).
 __global__ __launch_bounds__(kBlock) void k_max_row(const uint32_t* __restrict__ row_ptr, uint32_t n,
                                                     uint32_t* __restrict__ out) {
+  __shared__ uint32_t wmax[kWavesPerBlock];
   uint32_t m = 0;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
     uint32_t l = row_ptr[r + 1] - row_ptr[r];
@@ -345,7 +347,12 @@ __global__ __launch_bounds__(kBlock) void k_max_row(const uint32_t* __restrict__
     uint32_t o = __shfl_xor(m, off, kWave);
     m = o > m ? o : m;
   }
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, m);
+  if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWavesPerBlock; ++w) m = wmax[w] > m ? wmax[w] : m;
+    atomicMax(out, m);
+  }
 }
 
 // Diagonal (duplicates summed) -> inverse for Jacobi; zero diagonal -> flag.

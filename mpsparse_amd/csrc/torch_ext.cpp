@@ -726,7 +726,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("device_count", []() { return msp_device_count(); });
   m.def(
       "release_workspace", []() { check(msp_release_workspace(), "release_workspace"); },
-      "return the cached temporaries of matrix builds (a per-device memory pool) to the system");
+      "return the cached temporaries of matrix builds (a scratch arena per device and stream) to the system");
   m.def("coo_csr", &coo_csr, "COO -> CSR (stable packed-key sort)", py::arg("keys"), py::arg("values"),
         py::arg("row_ptr"), py::arg("col_ind"), py::arg("out_vals"), py::arg("num_rows"), py::arg("num_cols"));
   py::class_<CsrTensor>(m, "csr_tensor")

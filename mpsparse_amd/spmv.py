@@ -11,7 +11,8 @@ residual, default 1e-6), ``maxiter`` (10000), ``check_every``, ``criterion``
 (``"f32"``/``"bf16"``/``"f16"``). Solvers return an info dict.
 
 ``release_workspace()`` returns the cached temporaries of matrix builds (a
-per-device memory pool the library keeps for the next build) to the system.
+scratch arena per device and stream that the library keeps for the next
+build) to the system.
 """
 from . import load_ext
 

@@ -83,8 +83,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--sections", default="fsai,spmv,solve,bcsr,cpu,precision,batched",
-                    help="comma list of fsai, spmv, solve, bcsr, cpu, precision, batched, copy "
+    ap.add_argument("--sections", default="fsai,spmv,solve,bcsr,cpu,precision,batched,direct",
+                    help="comma list of fsai, spmv, solve, bcsr, cpu, precision, batched, direct, copy "
                          "(copy: 1 GiB device copy for counter calibration)")
     ap.add_argument("--block-rows", type=int, default=65536, help="block-rows of the block-sparse system")
     ap.add_argument("--fsai-weights", default=None, help="edge-MLP weights (.npz); default: the packaged ones")
@@ -270,6 +270,9 @@ def main():
 
     if "batched" in sections:
         out["batched_pcg"] = batched_section(dev)
+
+    if "direct" in sections:
+        out["direct_qr"] = direct_section(dev)
 
     if fsai is not None:
         lp = fsai["learned_fsai_refined_bf16"]
@@ -563,6 +566,67 @@ def fsai_section(dev, steps, weights=None):
                                                   / res["learned_fsai_refined_bf16"]["wall_s"])
     res["_csr"] = (rp, ci, cv, b32)  # for the CPU baseline; removed before printing
     return res
+
+
+def direct_section(dev, sizes=(1000, 4000, 10000, 20000, 100000), nrhs=128):
+    """The reference's direct-solver benchmark (sparse_bridge/benchmark.py:38-86):
+    1D Laplacian + 0.1 I of each size, factorisation, then one solve with 128
+    right-hand sides; scipy's sparse LU on the host next to it (the reference's
+    own comparison) and ||X - X_splu|| (the reference passes below 1e-8). Plus
+    a 2D 64x64 grid (band 64 after reordering) as a wider-band point."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+
+    from mpsparse_amd import problems
+    from mpsparse_amd.sparse_qr import SparseQRSolver
+
+    def case(name, cs, ri, v, n):
+        A = sp.csc_matrix((v, ri, cs), shape=(n, n))
+        tcs, tri, tv = (torch.from_numpy(a).to(dev) for a in (cs, ri, v))
+        SparseQRSolver(tcs, tri, tv, n, n)  # warm-up (code paths, scratch)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        S = SparseQRSolver(tcs, tri, tv, n, n)
+        torch.cuda.synchronize()
+        t_fact = time.perf_counter() - t
+        rng = np.random.default_rng(n)
+        Bh = rng.random((n, nrhs))
+        B = torch.from_numpy(Bh.T.copy()).to(dev).t()  # column-major, as the reference passes it
+        S.solve(B)
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            X = S.solve(B)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        lu = spla.splu(A)
+        c_fact = time.perf_counter() - t
+        t = time.perf_counter()
+        Xc = lu.solve(Bh)
+        c_solve = time.perf_counter() - t
+        diff = float(np.linalg.norm(X.cpu().numpy() - Xc))
+        r = {"n": n, "bandwidth": list(S.bandwidth), "nrhs": nrhs, "factor_ms": t_fact * 1e3,
+             "solve_ms": float(np.median(ts)) * 1e3, "scipy_splu_factor_ms": c_fact * 1e3,
+             "scipy_splu_solve_ms": c_solve * 1e3, "diff_vs_splu": diff, "pass": diff < 1e-8}
+        log("direct", name, r)
+        return r
+
+    out = {"workload": "1D Laplacian + 0.1 I (sparse_bridge/benchmark.py generate_data), 128 right-hand sides; "
+                       "FP64 banded Householder QR after reverse Cuthill-McKee", "cases": {}}
+    for n in sizes:
+        out["cases"][f"laplacian_1d_{n}"] = case(f"1d {n}", *problems.laplacian_1d_csc(n), n)
+    side = 64
+    T = sp.diags([-np.ones(side - 1), 4.1 * np.ones(side), -np.ones(side - 1)], [-1, 0, 1])
+    A2 = (sp.kron(sp.eye(side), T) + sp.kron(sp.diags([-np.ones(side - 1), -np.ones(side - 1)], [-1, 1]),
+                                              sp.eye(side))).tocsc()
+    A2.sort_indices()
+    out["cases"]["laplacian_2d_64x64"] = case("2d 64x64", A2.indptr.astype(np.int64), A2.indices.astype(np.int64),
+                                              A2.data.astype(np.float64), side * side)
+    return out
 
 
 def precision_section(dev, empty):

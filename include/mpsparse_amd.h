@@ -354,6 +354,31 @@ msp_status msp_bcsr_bicgstab(msp_bcsr A, const float *b, float *x, const msp_sol
 msp_status msp_bcsr_richardson(msp_bcsr A, const float *b, float *x, float omega, int32_t iters, int32_t use_x0,
                                void *stream);
 
+/* Direct sparse solve -----------------------------------------------------
+ * Replaces SparseQRSolver(col_starts, row_indices, values, n_rows, n_cols) and
+ * its solve(B) (sparse_bridge/bridge.cpp:14-191, Accelerate sparse QR on the
+ * CPU). The matrix comes as CSC with int64 indices and float64 values
+ * (bridge.cpp:24-57); duplicates are added. The host orders it into a band
+ * (reverse Cuthill-McKee on A + A^T for square matrices, or the identity when
+ * that gives a narrower band; rows sorted by leftmost column for tall ones) and
+ * the GPU factors the band by Householder QR in FP64. num_rows >= num_cols;
+ * the solve returns the least-squares solution (the solution for square
+ * nonsingular A). Orderings with kl + ku + 1 > 256, kl > 127, or a band window
+ * beyond the device's LDS are refused with MSP_ERR_UNSUPPORTED (tall matrices
+ * need kl >= num_rows - num_cols), and a zero or non-finite diagonal of R with
+ * MSP_ERR_SINGULAR. Arrays may be host or device memory. */
+typedef struct msp_qr_s *msp_qr;
+msp_status msp_qr_create_csc(msp_qr *out, const int64_t *col_starts, const int64_t *row_indices,
+                             const double *values, int64_t num_rows, int64_t num_cols, void *stream);
+/* X[:, k] = argmin ||A x - B[:, k]|| for k < nrhs; B column-major num_rows x nrhs
+ * with column stride ldb >= num_rows, X num_cols x nrhs with stride ldx >= num_cols
+ * (bridge.cpp:92-158; one right-hand side is nrhs = 1). */
+msp_status msp_qr_solve(msp_qr F, const double *B, int64_t ldb, int64_t nrhs, double *X, int64_t ldx,
+                        void *stream);
+/* Lower / upper bandwidth of the ordered matrix. */
+msp_status msp_qr_bandwidth(msp_qr F, int32_t *lower, int32_t *upper);
+msp_status msp_qr_destroy(msp_qr F);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,11 +10,13 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/mpsparse_amd.h"
+#include "bandqr.hpp"
 #include "batched.hpp"
 #include "bcsr.hpp"
 #include "common.hpp"
@@ -2591,4 +2593,320 @@ extern "C" msp_status msp_batch_pcg_x64(msp_batch B, const float* b, double* x, 
                                         msp_batch_info* info, int32_t* pair_status, int32_t* pair_iterations,
                                         double* pair_relres, void* stream) {
   return batch_pcg_impl(B, b, x, true, opts, info, pair_status, pair_iterations, pair_relres, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Direct sparse solve: banded Householder QR (bandqr.hpp). Replaces
+// sparse_bridge/bridge.cpp:14-191 (SparseQRSolver: symbolic + numeric QR at
+// construction, solve(B) for one or many right-hand sides).
+struct msp_qr_s {
+  int device = 0;
+  int64_t m = 0, n = 0;
+  int kl = 0, ku = 0, wc = 1;
+  DevMem Rb, Vb, tau;   // R rows [n][wc], reflectors [n][kl], tau [n]
+  DevMem src_row;       // ordered row -> caller's row (int32 [m])
+  DevMem pcol;          // caller's column -> ordered column (int32 [n])
+};
+
+// Reverse Cuthill-McKee order of the symmetric pattern adj (CSR, no self
+// loops): each component from a pseudo-peripheral start (the lowest-degree
+// node of the last BFS level, repeated while the eccentricity grows),
+// neighbours visited by increasing degree, the whole order reversed.
+static std::vector<int32_t> rcm_order(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>& adj) {
+  std::vector<int32_t> order;
+  order.reserve((size_t)n);
+  std::vector<int32_t> level((size_t)n, -1);
+  std::vector<char> done((size_t)n, 0);
+  auto deg = [&](int32_t v) { return ptr[v + 1] - ptr[v]; };
+  // BFS from s over unfinished nodes; returns the last level's nodes and the depth
+  auto bfs = [&](int32_t s, std::vector<int32_t>& last) {
+    std::vector<int32_t> cur{s}, seen{s};
+    level[s] = 0;
+    int depth = 0;
+    while (true) {
+      std::vector<int32_t> nxt;
+      for (int32_t v : cur)
+        for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+          const int32_t w = adj[e];
+          if (done[w] || level[w] >= 0) continue;
+          level[w] = depth + 1;
+          nxt.push_back(w);
+          seen.push_back(w);
+        }
+      if (nxt.empty()) break;
+      cur.swap(nxt);
+      ++depth;
+    }
+    last = cur;
+    for (int32_t v : seen) level[v] = -1;
+    return depth;
+  };
+  std::vector<int32_t> byDeg((size_t)n);
+  for (int64_t v = 0; v < n; ++v) byDeg[v] = (int32_t)v;
+  std::stable_sort(byDeg.begin(), byDeg.end(), [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+  for (int32_t s0 : byDeg) {
+    if (done[s0]) continue;
+    int32_t s = s0;
+    std::vector<int32_t> last;
+    int ecc = bfs(s, last);
+    for (int it = 0; it < 8; ++it) {
+      int32_t cand = last[0];
+      for (int32_t v : last)
+        if (deg(v) < deg(cand)) cand = v;
+      std::vector<int32_t> last2;
+      const int e2 = bfs(cand, last2);
+      if (e2 <= ecc) break;
+      s = cand;
+      ecc = e2;
+      last.swap(last2);
+    }
+    // Cuthill-McKee from s
+    size_t head = order.size();
+    order.push_back(s);
+    done[s] = 1;
+    std::vector<int32_t> nb;
+    while (head < order.size()) {
+      const int32_t v = order[head++];
+      nb.clear();
+      for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e)
+        if (!done[adj[e]]) {
+          done[adj[e]] = 1;
+          nb.push_back(adj[e]);
+        }
+      std::stable_sort(nb.begin(), nb.end(), [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+// Band of the pattern under row map prow (caller row -> ordered row) and
+// column map pcol.
+static void band_of(int64_t n, const std::vector<int64_t>& cs, const std::vector<int64_t>& ri,
+                    const std::vector<int32_t>& prow, const std::vector<int32_t>& pcol, int64_t* kl, int64_t* ku) {
+  int64_t l = 0, u = 0;
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t e = cs[c]; e < cs[c + 1]; ++e) {
+      const int64_t d = (int64_t)pcol[c] - (int64_t)prow[ri[e]];
+      if (d > u) u = d;
+      if (-d > l) l = -d;
+    }
+  *kl = l;
+  *ku = u;
+}
+
+static size_t bqr_factor_lds(int kl, int wc) {
+  return sizeof(double) * ((size_t)(kl + 1) * wc + (size_t)bqr::kStage * wc + (size_t)(kl + 1));
+}
+static size_t bqr_qt_lds(int kl) {
+  return sizeof(double) * ((size_t)(kl + 1) * bqr::kRhs + (size_t)bqr::kChunk * kl + bqr::kChunk);
+}
+static size_t bqr_rs_lds(int wc) { return sizeof(double) * ((size_t)wc * bqr::kRhs + (size_t)bqr::kChunk * wc); }
+
+static msp_status lds_allow(const void* kernel, size_t bytes) {
+  int dev = 0, maxb = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (bytes > (size_t)maxb)
+    return fail(MSP_ERR_UNSUPPORTED, "banded QR: the band window needs %zu bytes of LDS per workgroup (device allows %d)",
+                bytes, maxb);
+  if (bytes > 65536) HIPCHK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_qr_create_csc(msp_qr* out, const int64_t* col_starts, const int64_t* row_indices,
+                                        const double* values, int64_t num_rows, int64_t num_cols, void* stream) {
+  if (!out) return fail(MSP_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  MSPCHK(need_device());
+  if (!col_starts || num_rows <= 0 || num_cols <= 0) return fail(MSP_ERR_INVALID, "msp_qr_create_csc: empty matrix");
+  if (num_rows < num_cols)
+    return fail(MSP_ERR_UNSUPPORTED, "banded QR needs num_rows >= num_cols (got %lld x %lld)", (long long)num_rows,
+                (long long)num_cols);
+  if (num_rows > INT32_MAX) return fail(MSP_ERR_INVALID, "num_rows exceeds the int32 index range");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t m = num_rows, n = num_cols;
+  // symbolic phase on the host: the pattern, an ordering, the band
+  std::vector<int64_t> cs((size_t)n + 1);
+  HIPCHK(hipMemcpy(cs.data(), col_starts, sizeof(int64_t) * (n + 1), hipMemcpyDefault));
+  if (cs[0] != 0) return fail(MSP_ERR_INVALID, "col_starts[0] must be 0");
+  for (int64_t c = 0; c < n; ++c)
+    if (cs[c + 1] < cs[c]) return fail(MSP_ERR_INVALID, "col_starts must be non-decreasing");
+  const int64_t nnz = cs[n];
+  if (nnz > 0 && (!row_indices || !values)) return fail(MSP_ERR_INVALID, "null row_indices / values");
+  std::vector<int64_t> ri((size_t)nnz);
+  if (nnz) HIPCHK(hipMemcpy(ri.data(), row_indices, sizeof(int64_t) * nnz, hipMemcpyDefault));
+  for (int64_t e = 0; e < nnz; ++e)
+    if (ri[e] < 0 || ri[e] >= m) return fail(MSP_ERR_INVALID, "row index %lld out of range", (long long)ri[e]);
+  std::vector<int32_t> prow((size_t)m), pcol((size_t)n);
+  int64_t kl = 0, ku = 0;
+  if (m == n) {
+    // identity or reverse Cuthill-McKee on A + A^T, whichever gives the smaller window
+    std::vector<int64_t> ptr((size_t)n + 1, 0);
+    for (int64_t c = 0; c < n; ++c)
+      for (int64_t e = cs[c]; e < cs[c + 1]; ++e)
+        if (ri[e] != c) {
+          ++ptr[ri[e] + 1];
+          ++ptr[c + 1];
+        }
+    for (int64_t v = 0; v < n; ++v) ptr[v + 1] += ptr[v];
+    std::vector<int32_t> adj((size_t)ptr[n]);
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t c = 0; c < n; ++c)
+      for (int64_t e = cs[c]; e < cs[c + 1]; ++e)
+        if (ri[e] != c) {
+          adj[fill[ri[e]]++] = (int32_t)c;
+          adj[fill[c]++] = (int32_t)ri[e];
+        }
+    std::vector<int64_t> uptr((size_t)n + 1, 0);  // sorted, unique neighbour lists
+    std::vector<int32_t> uadj;
+    uadj.reserve(adj.size());
+    for (int64_t v = 0; v < n; ++v) {
+      std::sort(adj.begin() + ptr[v], adj.begin() + ptr[v + 1]);
+      for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e)
+        if (e == ptr[v] || adj[e] != adj[e - 1]) uadj.push_back(adj[e]);
+      uptr[v + 1] = (int64_t)uadj.size();
+    }
+    for (int64_t v = 0; v < n; ++v) prow[v] = pcol[v] = (int32_t)v;
+    band_of(n, cs, ri, prow, pcol, &kl, &ku);
+    const std::vector<int32_t> order = rcm_order(n, uptr, uadj);
+    std::vector<int32_t> inv((size_t)n);
+    for (int64_t k = 0; k < n; ++k) inv[order[k]] = (int32_t)k;
+    int64_t kl2 = 0, ku2 = 0;
+    band_of(n, cs, ri, inv, inv, &kl2, &ku2);
+    if ((kl2 + 1) * (kl2 + ku2 + 1) < (kl + 1) * (kl + ku + 1)) {
+      prow = inv;
+      pcol = inv;
+      kl = kl2;
+      ku = ku2;
+    }
+  } else {
+    // tall: columns as given, rows stably sorted by their leftmost column
+    std::vector<int64_t> lead((size_t)m, n);
+    for (int64_t c = 0; c < n; ++c)
+      for (int64_t e = cs[c]; e < cs[c + 1]; ++e) lead[ri[e]] = std::min(lead[ri[e]], c);
+    std::vector<int32_t> rows((size_t)m);
+    for (int64_t r = 0; r < m; ++r) rows[r] = (int32_t)r;
+    std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return lead[a] < lead[b]; });
+    for (int64_t k = 0; k < m; ++k) prow[rows[k]] = (int32_t)k;
+    for (int64_t c = 0; c < n; ++c) pcol[c] = (int32_t)c;
+    band_of(n, cs, ri, prow, pcol, &kl, &ku);
+  }
+  if (kl + ku + 1 > bqr::kMaxWC || kl > bqr::kMaxKl)
+    return fail(MSP_ERR_UNSUPPORTED,
+                "banded QR: the ordered matrix has bandwidths kl=%lld ku=%lld; kl+ku+1 must be at most %d and "
+                "kl at most %d",
+                (long long)kl, (long long)ku, bqr::kMaxWC, bqr::kMaxKl);
+  const int ikl = (int)kl, iku = (int)ku, wc = ikl + iku + 1;
+  const bool narrow = wc <= kWave;
+  const void* fk = narrow ? (const void*)bqr::k_bqr_factor<kWave> : (const void*)bqr::k_bqr_factor<kBlock>;
+  MSPCHK(lds_allow(fk, bqr_factor_lds(ikl, wc)));
+  MSPCHK(lds_allow((const void*)bqr::k_bqr_apply_qt, bqr_qt_lds(ikl)));
+  MSPCHK(lds_allow((const void*)bqr::k_bqr_rsolve, bqr_rs_lds(wc)));
+  msp_qr F = new msp_qr_s;
+  std::unique_ptr<msp_qr_s> guard(F);
+  HIPCHK(hipGetDevice(&F->device));
+  F->m = m;
+  F->n = n;
+  F->kl = ikl;
+  F->ku = iku;
+  F->wc = wc;
+  std::vector<int32_t> src((size_t)m);
+  for (int64_t r = 0; r < m; ++r) src[prow[r]] = (int32_t)r;
+  MSPCHK(F->src_row.alloc(sizeof(int32_t) * m));
+  MSPCHK(F->pcol.alloc(sizeof(int32_t) * n));
+  HIPCHK(hipMemcpyAsync(F->src_row.p, src.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(F->pcol.p, pcol.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  TmpMem d_prow, d_cs, d_ri, RB, bad;
+  MSPCHK(d_prow.alloc(sizeof(int32_t) * m, s));
+  HIPCHK(hipMemcpyAsync(d_prow.p, prow.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+  MSPCHK(d_cs.alloc(sizeof(int64_t) * (n + 1), s));
+  HIPCHK(hipMemcpyAsync(d_cs.p, cs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+  MSPCHK(d_ri.alloc(sizeof(int64_t) * std::max<int64_t>(nnz, 1), s));
+  if (nnz) HIPCHK(hipMemcpyAsync(d_ri.p, ri.data(), sizeof(int64_t) * nnz, hipMemcpyHostToDevice, s));
+  InArr v;
+  MSPCHK(v.set(values, sizeof(double) * nnz, s));
+  MSPCHK(RB.alloc(sizeof(double) * m * wc, s));
+  HIPCHK(hipMemsetAsync(RB.p, 0, sizeof(double) * m * wc, s));
+  MSPCHK(bad.alloc(sizeof(int), s));
+  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  bqr::k_bqr_fill<<<grid_for((uint64_t)n), kBlock, 0, s>>>(d_cs.as<int64_t>(), d_ri.as<int64_t>(),
+                                                          static_cast<const double*>(v.d), n, d_prow.as<int32_t>(),
+                                                          F->pcol.as<int32_t>(), ikl, wc, RB.as<double>(),
+                                                          bad.as<int>());
+  MSPCHK(launch_check("k_bqr_fill"));
+  MSPCHK(F->Rb.alloc(sizeof(double) * n * wc));
+  MSPCHK(F->Vb.alloc(sizeof(double) * std::max<int64_t>(n * ikl, 1)));
+  MSPCHK(F->tau.alloc(sizeof(double) * n));
+  const size_t lds = bqr_factor_lds(ikl, wc);
+  if (narrow)
+    bqr::k_bqr_factor<kWave><<<1, kWave, lds, s>>>(RB.as<double>(), m, n, ikl, iku, F->Rb.as<double>(),
+                                                   F->Vb.as<double>(), F->tau.as<double>());
+  else
+    bqr::k_bqr_factor<kBlock><<<1, kBlock, lds, s>>>(RB.as<double>(), m, n, ikl, iku, F->Rb.as<double>(),
+                                                     F->Vb.as<double>(), F->tau.as<double>());
+  MSPCHK(launch_check("k_bqr_factor"));
+  bqr::k_bqr_check<<<grid_for((uint64_t)n), kBlock, 0, s>>>(F->Rb.as<double>(), n, wc, bad.as<int>() + 0);
+  MSPCHK(launch_check("k_bqr_check"));
+  int hb = 0;
+  HIPCHK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hb) return fail(MSP_ERR_SINGULAR, "banded QR: R has a zero or non-finite diagonal entry (rank-deficient matrix)");
+  *out = guard.release();
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_qr_bandwidth(msp_qr F, int32_t* lower, int32_t* upper) {
+  if (!F) return fail(MSP_ERR_INVALID, "null QR handle");
+  if (lower) *lower = F->kl;
+  if (upper) *upper = F->ku;
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_qr_solve(msp_qr F, const double* B, int64_t ldb, int64_t nrhs, double* X, int64_t ldx,
+                                   void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null QR handle");
+  if (nrhs <= 0) return MSP_OK;
+  if (!B || !X) return fail(MSP_ERR_INVALID, "null B / X");
+  if (ldb < F->m || ldx < F->n) return fail(MSP_ERR_INVALID, "ldb must be >= num_rows and ldx >= num_cols");
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t m = F->m, n = F->n;
+  InArr bin;
+  MSPCHK(bin.set(B, sizeof(double) * (size_t)(ldb * (nrhs - 1) + m), s));
+  TmpMem Y, Xp, xo;
+  MSPCHK(Y.alloc(sizeof(double) * m * nrhs, s));
+  MSPCHK(Xp.alloc(sizeof(double) * n * nrhs, s));
+  const uint64_t gm = (uint64_t)(m * nrhs);
+  bqr::k_bqr_gather_rhs<<<grid_for(gm), kBlock, 0, s>>>(static_cast<const double*>(bin.d), ldb, m, nrhs,
+                                                        F->src_row.as<int32_t>(), Y.as<double>());
+  const uint32_t g = (uint32_t)((nrhs + bqr::kRhs - 1) / bqr::kRhs);
+  bqr::k_bqr_apply_qt<<<g, bqr::kRhs, bqr_qt_lds(F->kl), s>>>(F->Vb.as<double>(), F->tau.as<double>(), m, n, F->kl,
+                                                              Y.as<double>(), nrhs);
+  bqr::k_bqr_rsolve<<<g, bqr::kRhs, bqr_rs_lds(F->wc), s>>>(F->Rb.as<double>(), n, F->wc, Y.as<double>(),
+                                                            Xp.as<double>(), nrhs);
+  MSPCHK(launch_check("banded QR solve"));
+  const bool xdev = is_device_ptr(X);
+  double* xd = X;
+  if (!xdev) {
+    MSPCHK(xo.alloc(sizeof(double) * (size_t)(ldx * (nrhs - 1) + n), s));
+    xd = xo.as<double>();
+  }
+  bqr::k_bqr_scatter_x<<<grid_for((uint64_t)(n * nrhs)), kBlock, 0, s>>>(Xp.as<double>(), n, nrhs,
+                                                                        F->pcol.as<int32_t>(), xd, ldx);
+  MSPCHK(launch_check("k_bqr_scatter_x"));
+  if (!xdev)  // column by column: the caller's rows n .. ldx-1 stay untouched
+    HIPCHK(hipMemcpy2DAsync(X, sizeof(double) * ldx, xd, sizeof(double) * ldx, sizeof(double) * n, nrhs,
+                            hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_qr_destroy(msp_qr F) {
+  if (!F) return MSP_OK;
+  (void)hipSetDevice(F->device);
+  (void)hipDeviceSynchronize();
+  delete F;
+  return MSP_OK;
 }

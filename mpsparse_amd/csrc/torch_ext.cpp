@@ -435,6 +435,73 @@ py::dict pcg_fsai_refined(CsrTensor& A, FsaiTensor& F, torch::Tensor b, torch::T
 
 // Batched Jacobi-PCG over the diagonal blocks of a csr_tensor (kept alive by a
 // reference held on the Python side).
+// Drop-in for SparseQRSolver (sparse_bridge/bridge.cpp:14-191): the
+// constructor orders and factors the CSC matrix (int64 indices, float64
+// values; CPU or ROCm tensors) on the GPU; solve(B) takes a vector or a matrix
+// of right-hand sides in any layout and returns X on B's device (num_cols rows,
+// column-major for a matrix, as bridge.cpp:128-152 returns it);
+// get_grad_a_helper(col_starts, row_indices, grad_b, x, n_rows, n_cols)
+// returns -grad_b[i] * x[j] per CSC entry (i, j) (bridge.cpp:160-189).
+class QrSolver {
+ public:
+  QrSolver(torch::Tensor col_starts, torch::Tensor row_indices, torch::Tensor values, int64_t n_rows,
+           int64_t n_cols)
+      : m_(n_rows), n_(n_cols) {
+    TORCH_CHECK(n_rows > 0 && n_cols > 0, "SparseQRSolver: empty matrix");
+    expect(col_starts, torch::kInt64, n_cols + 1, "col_starts");
+    expect(row_indices, torch::kInt64, -1, "row_indices");
+    expect(values, torch::kFloat64, row_indices.numel(), "values");
+    device_ = pick_device({&col_starts, &row_indices, &values});
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    void* s = stream_of(device_);
+    py::gil_scoped_release nogil;
+    check(msp_qr_create_csc(&h_, col_starts.data_ptr<int64_t>(), row_indices.data_ptr<int64_t>(),
+                            values.data_ptr<double>(), n_rows, n_cols, s),
+          "SparseQRSolver");
+  }
+  ~QrSolver() {
+    if (h_) msp_qr_destroy(h_);
+  }
+  torch::Tensor solve(torch::Tensor B) {
+    TORCH_CHECK(B.scalar_type() == torch::kFloat64, "B must be float64");
+    TORCH_CHECK(B.dim() == 1 || B.dim() == 2, "B must be a vector or a matrix");
+    TORCH_CHECK(B.size(0) == m_, "B row size mismatch: ", B.size(0), " rows, matrix has ", m_);
+    const int64_t k = B.dim() == 1 ? 1 : B.size(1);
+    const auto dev = torch::Device(torch::kCUDA, device_);
+    // column-major right-hand sides on the solver's device
+    torch::Tensor Bt = (B.dim() == 1 ? B.reshape({1, m_}) : B.t()).to(dev).contiguous();
+    torch::Tensor Xt = torch::empty({k, n_}, torch::TensorOptions().device(dev).dtype(torch::kFloat64));
+    {
+      c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+      void* s = stream_of(device_);
+      py::gil_scoped_release nogil;
+      if (k > 0) check(msp_qr_solve(h_, Bt.data_ptr<double>(), m_, k, Xt.data_ptr<double>(), n_, s), "solve");
+    }
+    torch::Tensor X = B.dim() == 1 ? Xt.reshape({n_}) : Xt.t();
+    return X.to(B.device());
+  }
+  torch::Tensor get_grad_a_helper(torch::Tensor col_starts, torch::Tensor row_indices, torch::Tensor grad_b,
+                                  torch::Tensor x, int64_t n_rows, int64_t n_cols) {
+    TORCH_CHECK(col_starts.numel() == n_cols + 1, "col_starts: expected ", n_cols + 1, " entries");
+    TORCH_CHECK(grad_b.numel() == n_rows && x.numel() == n_cols, "grad_b / x sizes");
+    const auto dev = grad_b.device();
+    auto cs = col_starts.to(dev, torch::kInt64);
+    auto cols = torch::repeat_interleave(torch::arange(n_cols, cs.options()), cs.slice(0, 1) - cs.slice(0, 0, -1));
+    auto rows = row_indices.to(dev, torch::kInt64);
+    return -(grad_b.to(torch::kFloat64).index_select(0, rows) * x.to(dev, torch::kFloat64).index_select(0, cols));
+  }
+  py::tuple bandwidth() const {
+    int32_t lo = 0, up = 0;
+    check(msp_qr_bandwidth(h_, &lo, &up), "bandwidth");
+    return py::make_tuple(lo, up);
+  }
+
+ private:
+  msp_qr h_ = nullptr;
+  int64_t m_, n_;
+  int device_ = 0;
+};
+
 class BatchTensor {
  public:
   BatchTensor(CsrTensor& A, torch::Tensor sys_off) : device_(A.device()) {
@@ -781,6 +848,13 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
            py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("maxiter") = 10000,
            py::arg("steps_per_graph") = 16, py::arg("tile_rows") = 0, py::arg("rows_per_thread") = 0,
            py::arg("max_verify") = 3);
+  py::class_<QrSolver>(m, "SparseQRSolver")
+      .def(py::init<torch::Tensor, torch::Tensor, torch::Tensor, int64_t, int64_t>(), py::arg("col_starts"),
+           py::arg("row_indices"), py::arg("values"), py::arg("n_rows"), py::arg("n_cols"))
+      .def("solve", &QrSolver::solve, py::arg("B"))
+      .def("get_grad_a_helper", &QrSolver::get_grad_a_helper, py::arg("col_starts"), py::arg("row_indices"),
+           py::arg("grad_b"), py::arg("x"), py::arg("n_rows"), py::arg("n_cols"))
+      .def_property_readonly("bandwidth", &QrSolver::bandwidth);
   py::class_<FsaiTensor>(m, "fsai_tensor")
       .def(py::init<CsrTensor&>(), py::arg("A"), py::keep_alive<1, 2>())
       .def_property_readonly("num_edges", &FsaiTensor::num_edges)

@@ -363,3 +363,24 @@ def rhs(n: int, seed: int = 0, nrhs: int = 1) -> np.ndarray:
     if nrhs == 1:
         return rng.standard_normal(n).astype(np.float32)
     return rng.standard_normal((nrhs, n)).astype(np.float32)
+
+
+def laplacian_1d_csc(n: int, shift: float = 0.1):
+    """The reference's direct-solver benchmark matrix (sparse_bridge/benchmark.py:
+    20-36): tridiagonal 1D Laplacian (2 on the diagonal, -1 beside it) plus
+    shift * I, as CSC (col_starts int64[n+1], row_indices int64, values float64),
+    rows sorted within each column."""
+    cols = []
+    for off in (-1, 0, 1):
+        r = np.arange(n, dtype=np.int64)
+        c = r + off
+        keep = (c >= 0) & (c < n)
+        cols.append((r[keep], c[keep], np.full(int(keep.sum()), 2.0 + shift if off == 0 else -1.0)))
+    rows = np.concatenate([t[0] for t in cols])
+    cc = np.concatenate([t[1] for t in cols])
+    vals = np.concatenate([t[2] for t in cols])
+    order = np.lexsort((rows, cc))
+    rows, cc, vals = rows[order], cc[order], vals[order]
+    col_starts = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(col_starts, cc + 1, 1)
+    return np.cumsum(col_starts).astype(np.int64), rows.astype(np.int64), vals.astype(np.float64)

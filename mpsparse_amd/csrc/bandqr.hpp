@@ -58,6 +58,16 @@ __global__ __launch_bounds__(kBlock) void k_bqr_fill(const int64_t* __restrict__
   }
 }
 
+// Workgroup barrier that orders LDS only: __syncthreads() also waits for the
+// wave's outstanding global stores (the streamed-out rows of R and reflectors),
+// which put an HBM write round trip into every column step. Nothing in these
+// kernels reads back global memory written by another thread.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -98,14 +108,14 @@ __global__ __launch_bounds__(T) void k_bqr_factor(const double* __restrict__ RB,
   fetch(kl + 1);
   for (int64_t j = 0; j < n; ++j) {
     if (j % kStage == 0) {  // rows j + kl + 1 .. j + kl + kStage enter during the next kStage steps
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
         const int e = t + q * T;
         if (e < kStage * WC) stage[e] = pre[q];
       }
       fetch(j + kl + 1 + kStage);
-      __syncthreads();
+      lds_barrier();
     }
     const int L = (int)(m - j < (int64_t)WR ? m - j : (int64_t)WR);
     const int W = (int)(n - 1 - j < (int64_t)(WC - 1) ? n - 1 - j : (int64_t)(WC - 1));
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(T) void k_bqr_factor(const double* __restrict__ RB,
       }
       if (t == 0) s_tau = tau;
     }
-    __syncthreads();
+    lds_barrier();
     const double tau = s_tau;
     if (tau != 0.0) {
       for (int c = t; c < W; c += T) {  // column j + 1 + c
@@ -145,20 +155,220 @@ __global__ __launch_bounds__(T) void k_bqr_factor(const double* __restrict__ RB,
         for (int i = 0, s = rj; i < L; ++i, s = (s + 1 == WR ? 0 : s + 1)) win[s * WC + cs] -= f * sv[i];
       }
     }
-    __syncthreads();
+    lds_barrier();
     // retire row j (a finished row of R) and column j (the reflector)
     for (int c = t; c < WC; c += T)
       if (j + c < n) Rb[j * WC + c] = win[rj * WC + (int)((j + c) % WC)];
     for (int i = 1 + t; i < WR; i += T) Vb[j * kl + (i - 1)] = i < L ? win[((rj + i) % WR) * WC + cj] : 0.0;
     if (t == 0) tau_out[j] = tau;
-    __syncthreads();
+    lds_barrier();
     // column j's slot becomes column j + WC (rows j+1 .. j+kl are fill: zero);
     // row j's slot becomes row j + kl + 1 (band row from the stage)
     for (int i = 1 + t; i < WR; i += T) win[((rj + i) % WR) * WC + cj] = 0.0;
     const int64_t e = j + kl + 1;
     const int srow = (int)(j % kStage);
     for (int c = t; c < WC; c += T) win[rj * WC + (int)((j + 1 + c) % WC)] = e < m ? stage[srow * WC + c] : 0.0;
-    __syncthreads();
+    lds_barrier();
+  }
+}
+
+// Narrow bands (kl + 1 <= kSmallWR, kl + ku + 1 <= kSmallWC): the same
+// factorisation by lane 0 alone with the window in registers (relative
+// indexing: w[r][c] holds row j + r, column j + c and shifts by one each step),
+// so a step is a dependent chain of a few flops instead of several LDS round
+// trips and wave reductions. The other lanes only prefetch: each holds one
+// band row of the block after next and parks it in an LDS stage for lane 0.
+constexpr int kSmallWR = 4, kSmallWC = 8, kSmallD = 32;
+
+// KL, KU >= 0: the band shape fixed at compile time (straight-line steps, the
+// tridiagonal case); KL < 0: kl, ku from the arguments.
+template <int KL, int KU>
+__global__ __launch_bounds__(kWave) void k_bqr_factor_small(const double* __restrict__ RB, int64_t m, int64_t n,
+                                                            int kl_arg, int ku_arg, double* __restrict__ Rb,
+                                                            double* __restrict__ Vb, double* __restrict__ tau_out) {
+  constexpr int R = KL >= 0 ? KL + 1 : kSmallWR, C = KL >= 0 ? KL + KU + 1 : kSmallWC, D = kSmallD;
+  __shared__ double stage[D][C];
+  const int kl = KL >= 0 ? KL : kl_arg, ku = KL >= 0 ? KU : ku_arg;
+  const int WR = kl + 1, WC = kl + ku + 1;
+  const int lane = threadIdx.x;
+  // Rows at or past m and columns at or past n are zero in the window (the
+  // band and the stage hold zeros there), so reflectors and updates need no
+  // edge tests: zero rows add nothing, zero columns stay zero.
+  double w[R][C];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int k = c - r + kl;
+      w[r][c] = (r < WR && c < WC && r < m && c < n && k >= 0 && k < WC) ? RB[(int64_t)r * WC + k] : 0.0;
+    }
+  // lane l < D holds band row first + l (the row entering at step first - kl - 1 + l)
+  double held[C];
+  auto fetch = [&](int64_t first) {
+    const int64_t row = first + lane;
+#pragma unroll
+    for (int c = 0; c < C; ++c) held[c] = (lane < D && c < WC && row < m) ? RB[row * WC + c] : 0.0;
+  };
+  fetch(kl + 1);
+  for (int64_t j0 = 0; j0 < n; j0 += D) {
+    if (lane < D) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) stage[lane][c] = held[c];
+    }
+    __syncthreads();  // one wave: orders the stage writes before lane 0's reads
+    fetch(j0 + D + kl + 1);
+    if (lane == 0) {
+      const int steps = (int)(n - j0 < (int64_t)D ? n - j0 : (int64_t)D);
+      for (int d = 0; d < steps; ++d) {
+        const int64_t j = j0 + d;
+        double nrow[C];  // row j + kl + 1, read early
+#pragma unroll
+        for (int c = 0; c < C; ++c) nrow[c] = stage[d][c];
+        const double alpha = w[0][0];
+        double ss = 0.0;
+#pragma unroll
+        for (int i = 1; i < R; ++i) ss += w[i][0] * w[i][0];
+        double beta = alpha, tau = 0.0, scal = 0.0;
+        if (ss != 0.0) {
+          beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+          tau = (beta - alpha) / beta;
+          scal = 1.0 / (alpha - beta);
+        }
+        double v[R];
+        v[0] = 1.0;
+#pragma unroll
+        for (int i = 1; i < R; ++i) v[i] = w[i][0] * scal;
+#pragma unroll
+        for (int c = 1; c < C; ++c) {
+          double dot = w[0][c];
+#pragma unroll
+          for (int i = 1; i < R; ++i) dot += v[i] * w[i][c];
+          const double f = tau * dot;
+#pragma unroll
+          for (int i = 0; i < R; ++i) w[i][c] -= f * v[i];
+        }
+        Rb[j * WC] = beta;
+#pragma unroll
+        for (int c = 1; c < C; ++c)
+          if (c < WC && j + c < n) Rb[j * WC + c] = w[0][c];
+#pragma unroll
+        for (int i = 1; i < R; ++i)
+          if (i < WR) Vb[j * kl + (i - 1)] = v[i];
+        tau_out[j] = tau;
+        // slide: row j + r + 1 / column j + c + 1 move to r / c; column j + WC
+        // enters as fill (zero), row j + kl + 1 from the stage
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            double nv = 0.0;
+            if (r + 1 < WR) nv = (c + 1 < C && c + 1 < WC) ? w[r + 1 < R ? r + 1 : 0][c + 1 < C ? c + 1 : 0] : 0.0;
+            else if (r + 1 == WR) nv = c < WC ? nrow[c] : 0.0;
+            w[r][c] = nv;
+          }
+      }
+    }
+    __syncthreads();  // lane 0 is done with the stage before it is refilled
+  }
+}
+
+// y <- Q^T y for narrow bands: the window of rows j .. j + kl of this
+// right-hand side in registers (shifted each step).
+__global__ __launch_bounds__(kRhs) void k_bqr_apply_qt_small(const double* __restrict__ Vb,
+                                                             const double* __restrict__ tau, int64_t m, int64_t n,
+                                                             int kl, double* __restrict__ Y, int64_t nrhs) {
+  constexpr int R = kSmallWR;
+  __shared__ double vs[kChunk][R];
+  __shared__ double ts[kChunk];
+  const int WR = kl + 1;
+  const int t = threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.x * kRhs + t;
+  const bool act = col < nrhs;
+  double w[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) w[r] = (act && r < WR && r < m) ? Y[(int64_t)r * nrhs + col] : 0.0;
+  double pre[kChunk], cur[kChunk];
+  auto fetch = [&](int64_t first) {
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) pre[q] = (act && first + q < m) ? Y[(first + q) * nrhs + col] : 0.0;
+  };
+  fetch(kl + 1);
+  for (int64_t j0 = 0; j0 < n; j0 += kChunk) {
+    lds_barrier();
+    for (int e = t; e < kChunk * R; e += kRhs) {
+      const int q = e / R, i = e % R;  // vs[q][i] = v_{i+1} of reflector j0 + q
+      vs[q][i] = (j0 + q < n && i < kl) ? Vb[(j0 + q) * kl + i] : 0.0;
+    }
+    for (int e = t; e < kChunk; e += kRhs) ts[e] = j0 + e < n ? tau[j0 + e] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) cur[q] = pre[q];
+    fetch(j0 + kChunk + kl + 1);
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      const int64_t j = j0 + q;
+      if (j >= n) break;
+      const double tj = ts[q];
+      double s = w[0];
+#pragma unroll
+      for (int i = 1; i < R; ++i) s += vs[q][i - 1] * w[i];  // v past the reflector's rows is 0
+      s *= tj;
+      w[0] -= s;
+#pragma unroll
+      for (int i = 1; i < R; ++i) w[i] -= s * vs[q][i - 1];
+      if (act) Y[j * nrhs + col] = w[0];
+      // shift: row j + 1 + r -> r; row j + kl + 1 enters at kl
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r] = (r + 1 < WR) ? w[r + 1 < R ? r + 1 : 0] : (r + 1 == WR ? cur[q] : 0.0);
+    }
+  }
+  for (int64_t r = n; r < m && r < n + WR; ++r)
+    if (act) Y[r * nrhs + col] = w[(int)(r - n)];
+}
+
+// x = R^-1 y for narrow bands (wc <= kSmallWC): x_{i+1} .. x_{i+wc-1} of this
+// right-hand side in registers.
+__global__ __launch_bounds__(kRhs) void k_bqr_rsolve_small(const double* __restrict__ Rb, int64_t n, int wc,
+                                                           const double* __restrict__ Y, double* __restrict__ Xp,
+                                                           int64_t nrhs) {
+  constexpr int C = kSmallWC;
+  __shared__ double rs[kChunk][C];
+  const int t = threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.x * kRhs + t;
+  const bool act = col < nrhs;
+  double x[C];  // x[c] = x_{i+c}, c >= 1
+#pragma unroll
+  for (int c = 0; c < C; ++c) x[c] = 0.0;
+  double pre[kChunk], cur[kChunk];
+  auto fetch = [&](int64_t top) {
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) pre[q] = (act && top - q >= 0) ? Y[(top - q) * nrhs + col] : 0.0;
+  };
+  fetch(n - 1);
+  for (int64_t top = n - 1; top >= 0; top -= kChunk) {
+    lds_barrier();
+    for (int e = t; e < kChunk * C; e += kRhs) {
+      const int q = e / C, c = e % C;
+      const int64_t i = top - q;
+      rs[q][c] = (i >= 0 && c < wc && i + c < n) ? Rb[i * wc + c] : 0.0;  // zero past the matrix edge
+    }
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) cur[q] = pre[q];
+    fetch(top - kChunk);
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      const int64_t i = top - q;
+      if (i < 0) break;
+      double s = cur[q];
+#pragma unroll
+      for (int c = 1; c < C; ++c) s -= rs[q][c] * x[c];
+      const double xi = s / rs[q][0];
+#pragma unroll
+      for (int c = C - 1; c >= 2; --c) x[c] = x[c - 1];
+      x[1] = xi;
+      if (act) Xp[i * nrhs + col] = xi;
+    }
   }
 }
 
@@ -218,7 +428,7 @@ __global__ __launch_bounds__(kRhs) void k_bqr_apply_qt(const double* __restrict_
   fetch(kl + 1);
   double cur[kChunk];
   for (int64_t j0 = 0; j0 < n; j0 += kChunk) {
-    __syncthreads();
+    lds_barrier();
     for (int e = t; e < kChunk * kl; e += kRhs) {
       const int64_t jj = j0 + e / (kl > 0 ? kl : 1);
       vs[e] = jj < n ? Vb[j0 * kl + e] : 0.0;
@@ -227,7 +437,7 @@ __global__ __launch_bounds__(kRhs) void k_bqr_apply_qt(const double* __restrict_
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) cur[q] = pre[q];
     fetch(j0 + kChunk + kl + 1);
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       const int64_t j = j0 + q;
@@ -271,7 +481,7 @@ __global__ __launch_bounds__(kRhs) void k_bqr_rsolve(const double* __restrict__ 
   };
   fetch(n - 1);
   for (int64_t top = n - 1; top >= 0; top -= kChunk) {
-    __syncthreads();
+    lds_barrier();
     for (int e = t; e < kChunk * wc; e += kRhs) {
       const int64_t i = top - e / wc;
       rs[e] = i >= 0 ? Rb[i * wc + e % wc] : 0.0;
@@ -279,7 +489,7 @@ __global__ __launch_bounds__(kRhs) void k_bqr_rsolve(const double* __restrict__ 
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) cur[q] = pre[q];
     fetch(top - kChunk);
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       const int64_t i = top - q;

@@ -2840,7 +2840,13 @@ extern "C" msp_status msp_qr_create_csc(msp_qr* out, const int64_t* col_starts, 
   MSPCHK(F->Vb.alloc(sizeof(double) * std::max<int64_t>(n * ikl, 1)));
   MSPCHK(F->tau.alloc(sizeof(double) * n));
   const size_t lds = bqr_factor_lds(ikl, wc);
-  if (narrow)
+  if (ikl == 1 && iku == 1)
+    bqr::k_bqr_factor_small<1, 1><<<1, kWave, 0, s>>>(RB.as<double>(), m, n, ikl, iku, F->Rb.as<double>(),
+                                                      F->Vb.as<double>(), F->tau.as<double>());
+  else if (ikl + 1 <= bqr::kSmallWR && wc <= bqr::kSmallWC)
+    bqr::k_bqr_factor_small<-1, -1><<<1, kWave, 0, s>>>(RB.as<double>(), m, n, ikl, iku, F->Rb.as<double>(),
+                                                        F->Vb.as<double>(), F->tau.as<double>());
+  else if (narrow)
     bqr::k_bqr_factor<kWave><<<1, kWave, lds, s>>>(RB.as<double>(), m, n, ikl, iku, F->Rb.as<double>(),
                                                    F->Vb.as<double>(), F->tau.as<double>());
   else
@@ -2882,10 +2888,18 @@ extern "C" msp_status msp_qr_solve(msp_qr F, const double* B, int64_t ldb, int64
   bqr::k_bqr_gather_rhs<<<grid_for(gm), kBlock, 0, s>>>(static_cast<const double*>(bin.d), ldb, m, nrhs,
                                                         F->src_row.as<int32_t>(), Y.as<double>());
   const uint32_t g = (uint32_t)((nrhs + bqr::kRhs - 1) / bqr::kRhs);
-  bqr::k_bqr_apply_qt<<<g, bqr::kRhs, bqr_qt_lds(F->kl), s>>>(F->Vb.as<double>(), F->tau.as<double>(), m, n, F->kl,
-                                                              Y.as<double>(), nrhs);
-  bqr::k_bqr_rsolve<<<g, bqr::kRhs, bqr_rs_lds(F->wc), s>>>(F->Rb.as<double>(), n, F->wc, Y.as<double>(),
-                                                            Xp.as<double>(), nrhs);
+  if (F->kl + 1 <= bqr::kSmallWR)
+    bqr::k_bqr_apply_qt_small<<<g, bqr::kRhs, 0, s>>>(F->Vb.as<double>(), F->tau.as<double>(), m, n, F->kl,
+                                                      Y.as<double>(), nrhs);
+  else
+    bqr::k_bqr_apply_qt<<<g, bqr::kRhs, bqr_qt_lds(F->kl), s>>>(F->Vb.as<double>(), F->tau.as<double>(), m, n,
+                                                                F->kl, Y.as<double>(), nrhs);
+  if (F->wc <= bqr::kSmallWC)
+    bqr::k_bqr_rsolve_small<<<g, bqr::kRhs, 0, s>>>(F->Rb.as<double>(), n, F->wc, Y.as<double>(), Xp.as<double>(),
+                                                    nrhs);
+  else
+    bqr::k_bqr_rsolve<<<g, bqr::kRhs, bqr_rs_lds(F->wc), s>>>(F->Rb.as<double>(), n, F->wc, Y.as<double>(),
+                                                              Xp.as<double>(), nrhs);
   MSPCHK(launch_check("banded QR solve"));
   const bool xdev = is_device_ptr(X);
   double* xd = X;

@@ -74,6 +74,29 @@ def test_shuffled_nonsymmetric_band_is_recovered():
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= 1e-10
 
 
+@pytest.mark.parametrize("kl,ku", [(0, 0), (1, 2), (2, 2), (3, 4), (3, 5), (4, 3)])
+def test_band_shapes(kl, ku):
+    """Register-window kernels (kl + 1 <= 4, kl + ku + 1 <= 8), the LDS-window
+    kernels, and mixtures of both, on sizes that are not multiples of the
+    chunk lengths and 70 right-hand sides (two solve workgroups)."""
+    import torch
+
+    rng = np.random.default_rng(100 + 10 * kl + ku)
+    n = 257
+    offs = list(range(-kl, ku + 1))
+    A = sp.diags([rng.standard_normal(n - abs(o)) for o in offs], offs, shape=(n, n)).tolil()
+    A.setdiag(A.diagonal() + 6.0)
+    cs, ri, v = _csc(A)
+    say("direct band", kl, ku)
+    S = _solver(cs, ri, v, n, n)
+    lo, up = S.bandwidth
+    assert lo + up <= kl + ku
+    B = rng.standard_normal((n, 70))
+    X = S.solve(torch.tensor(B, device="cuda")).cpu().numpy()
+    Xo = D.least_squares(cs, ri, v, n, n, B)
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= 1e-12
+
+
 def test_laplacian_2d_grid():
     import torch
 

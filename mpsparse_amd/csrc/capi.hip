@@ -1052,10 +1052,8 @@ static msp_status ensure_dinv(const Op& op, hipStream_t s) {
 // never idles waiting for the host). A recurrence that reports convergence
 // halts the steps until the true-residual check `verify` at the end of the
 // batch (launched only when o.verify).
-template <typename Init, typename Step, typename Verify>
-static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
-                             Step step, Verify verify) {
-  Workspace& w = A->ws;
+// Solver state on the host before a solve (tolerance, limits, criterion).
+static msp_status start_solve(Workspace& w, const msp_solve_opts& o, hipStream_t s) {
   std::memset(w.hS, 0, sizeof(Scalars));
   w.hS->tol = (float)o.tol;
   w.hS->maxiter = o.maxiter;
@@ -1063,6 +1061,33 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
   w.hS->use_verify = o.verify ? 1 : 0;
   HIPCHK(hipMemcpyAsync(w.S_dev(), w.hS, sizeof(Scalars), hipMemcpyHostToDevice, s));
   HIPCHK(hipEventRecord(w.t0, s));
+  return MSP_OK;
+}
+
+// Final state back to the host and into the info struct.
+static msp_status finish_solve(Workspace& w, msp_solve_info* info, hipStream_t s) {
+  HIPCHK(hipEventRecord(w.t1, s));
+  HIPCHK(hipMemcpyAsync(w.hS, w.S_dev(), sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (info) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, w.t0, w.t1);
+    info->iterations = w.hS->iter;
+    info->status = w.hS->done == kRunning ? MSP_SOLVE_MAXITER : w.hS->done;
+    info->verifications = w.hS->nverify;
+    info->rr = w.hS->rr;
+    info->bb = w.hS->bb;
+    info->relres = w.hS->bb > 0 ? std::sqrt((double)w.hS->rr / (double)w.hS->bb) : std::sqrt((double)w.hS->rr);
+    info->seconds = ms * 1e-3;
+  }
+  return MSP_OK;
+}
+
+template <typename Init, typename Step, typename Verify>
+static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
+                             Step step, Verify verify) {
+  Workspace& w = A->ws;
+  MSPCHK(start_solve(w, o, s));
   MSPCHK(init());
   int32_t* dflag = &w.S_dev()->done;
   int enqueued = 0, slot = 0;
@@ -1085,21 +1110,7 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
     if (enqueued >= limit) break;
     slot ^= 1;
   }
-  HIPCHK(hipEventRecord(w.t1, s));
-  HIPCHK(hipMemcpyAsync(w.hS, w.S_dev(), sizeof(Scalars), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (info) {
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, w.t0, w.t1);
-    info->iterations = w.hS->iter;
-    info->status = w.hS->done == kRunning ? MSP_SOLVE_MAXITER : w.hS->done;
-    info->verifications = w.hS->nverify;
-    info->rr = w.hS->rr;
-    info->bb = w.hS->bb;
-    info->relres = w.hS->bb > 0 ? std::sqrt((double)w.hS->rr / (double)w.hS->bb) : std::sqrt((double)w.hS->rr);
-    info->seconds = ms * 1e-3;
-  }
-  return MSP_OK;
+  return finish_solve(w, info, s);
 }
 
 static msp_status solver_prologue(MatBase* A, const float* b, const float* x, const msp_solve_opts& o,
@@ -1134,6 +1145,24 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   const float* D = A->dinv.as<float>();
   Scalars* S = w.S_dev();
   const bool comp = compensate(o);
+  if constexpr (std::is_same<Op, OpCsr<ValF32>>::value) {
+    // small systems: the whole solve in one workgroup (no per-step launches)
+    if (o.cg_form == 0 && A->n > 0 && A->n <= kSmallN) {
+      MSPCHK(start_solve(w, o, s));
+      const RowsCsr rows = op.rows();
+      const float* Bv = w.v(Workspace::B);
+      const bool j = o.precond == MSP_PRECOND_JACOBI;
+      if (j && comp) k_cg_small<kPcJacobi, true><<<1, kSmallThreads, 0, s>>>(rows, Bv, X, D, A->n, o.use_x0, S);
+      else if (j) k_cg_small<kPcJacobi, false><<<1, kSmallThreads, 0, s>>>(rows, Bv, X, D, A->n, o.use_x0, S);
+      else if (comp) k_cg_small<kPcNone, true><<<1, kSmallThreads, 0, s>>>(rows, Bv, X, D, A->n, o.use_x0, S);
+      else k_cg_small<kPcNone, false><<<1, kSmallThreads, 0, s>>>(rows, Bv, X, D, A->n, o.use_x0, S);
+      MSPCHK(launch_check("k_cg_small"));
+      MSPCHK(finish_solve(w, info, s));
+      MSPCHK(copy_out(x, X, sizeof(float) * A->n, s));
+      HIPCHK(hipStreamSynchronize(s));
+      return MSP_OK;
+    }
+  }
   GridRed rb = w.gr();
   const uint32_t n = A->n, gv = vec_grid(n);
   const bool jac = o.precond == MSP_PRECOND_JACOBI;

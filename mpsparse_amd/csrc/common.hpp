@@ -84,9 +84,9 @@ __device__ __forceinline__ float group_reduce_sum(float v, uint32_t L) {
 }
 
 // Block-wide reduction of ND compensated sums; result valid in thread 0.
-template <int ND>
+template <int ND, int NW = kWavesPerBlock>
 __device__ __forceinline__ void block_reduce(CSum (&acc)[ND]) {
-  __shared__ CSum red[kWavesPerBlock][ND > 0 ? ND : 1];
+  __shared__ CSum red[NW][ND > 0 ? ND : 1];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
 #pragma unroll
@@ -100,7 +100,7 @@ __device__ __forceinline__ void block_reduce(CSum (&acc)[ND]) {
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       CSum t = red[0][d];
-      for (int w = 1; w < kWavesPerBlock; ++w) t = csum_merge(t, red[w][d]);
+      for (int w = 1; w < NW; ++w) t = csum_merge(t, red[w][d]);
       acc[d] = t;
     }
   }

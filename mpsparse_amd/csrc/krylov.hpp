@@ -648,6 +648,185 @@ struct FinVerify {
   }
 };
 
+// Whole CG / Jacobi-PCG solve in one workgroup for small systems (n <=
+// kSmallN): 1024 threads, thread t owns rows t, t + 1024, ... with x, x_lo, r,
+// p, b in registers; the direction p (or the iterate, during a verification)
+// in LDS for the SpMV's gathers; the scalar bookkeeping of FinCgInit /
+// FinCgStep / FinVerify run by thread 0 on an LDS copy of the state. On a
+// 4096-row Laplacian a step is then a few microseconds of barriers and LDS
+// instead of four kernel launches. Same recurrence, stopping rules,
+// true-residual verification and restarts as the multi-kernel path (the
+// reduction trees differ, so the bits do too).
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallRpt = 4;
+constexpr uint32_t kSmallN = (uint32_t)kSmallThreads * kSmallRpt;
+
+template <int NDS>
+__device__ __forceinline__ void small_total(CSum (&acc)[NDS], float (&tot)[NDS]) {
+  __shared__ float s_tot[NDS];
+  block_reduce<NDS, kSmallThreads / kWave>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < NDS; ++d) s_tot[d] = csum_val(acc[d]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < NDS; ++d) tot[d] = s_tot[d];
+  __syncthreads();
+}
+
+template <int PC, bool COMP>
+__global__ __launch_bounds__(kSmallThreads) void k_cg_small(RowsCsr A, const float* __restrict__ b, float* x_io,
+                                                            const float* __restrict__ dinv, uint32_t n, int use_x0,
+                                                            Scalars* Sg) {
+  constexpr int RPT = kSmallRpt;
+  __shared__ float sv[kSmallN];  // p, or the iterate during a verification
+  __shared__ Scalars S;
+  const int t = threadIdx.x;
+  if (t == 0) S = *Sg;
+  float x[RPT], xl[RPT], r[RPT], p[RPT], bv[RPT], dv[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const uint32_t i = t + k * kSmallThreads;
+    const bool own = i < n;
+    bv[k] = own ? b[i] : 0.0f;
+    x[k] = (own && use_x0) ? x_io[i] : 0.0f;
+    xl[k] = 0.0f;
+    dv[k] = (own && PC == kPcJacobi) ? dinv[i] : 1.0f;
+    if (own) sv[i] = x[k];
+  }
+  __syncthreads();
+  // r = b - A x0 (plain FP32 row sums, as the multi-kernel path's EpiResidual)
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const uint32_t i = t + k * kSmallThreads;
+    float ax = 0.0f;
+    if (i < n && use_x0) A.foreach(i, [&](float a, uint32_t c) { ax += a * sv[c]; });
+    r[k] = bv[k] - ax;
+  }
+  __syncthreads();
+  {
+    CSum acc[3] = {csum_zero(), csum_zero(), csum_zero()};
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const uint32_t i = t + k * kSmallThreads;
+      const float z = r[k] * dv[k];
+      p[k] = z;
+      if (i < n) sv[i] = z;
+      acc[0] = csum_add(acc[0], bv[k] * bv[k]);
+      acc[1] = csum_add(acc[1], r[k] * r[k]);
+      acc[2] = csum_add(acc[2], r[k] * z);
+    }
+    float tot[3];
+    small_total<3>(acc, tot);
+    if (t == 0) FinCgInit{&S}(tot);
+    __syncthreads();
+  }
+  while (true) {
+    const int done = S.done, verify = S.verify;
+    __syncthreads();
+    if (done) break;
+    if (!verify) {
+      // q = A p, <p, q>; alpha; x, r update; <r, r>, <r, z>
+      float q[RPT];
+      CSum acc[1] = {csum_zero()};
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const uint32_t i = t + k * kSmallThreads;
+        float s = 0.0f;
+        if (i < n) A.foreach(i, [&](float a, uint32_t c) { s += a * sv[c]; });
+        q[k] = s;
+        acc[0] = csum_add(acc[0], p[k] * s);
+      }
+      float tpq[1];
+      small_total<1>(acc, tpq);
+      const bool ok = finite_pos(tpq[0]);
+      const float alpha = ok ? S.rz / tpq[0] : 0.0f;
+      CSum acc2[2] = {csum_zero(), csum_zero()};
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        if constexpr (COMP) {
+          comp_axpy(x[k], xl[k], alpha, p[k]);
+        } else {
+          x[k] += alpha * p[k];
+        }
+        r[k] -= alpha * q[k];
+        acc2[0] = csum_add(acc2[0], r[k] * r[k]);
+        acc2[1] = csum_add(acc2[1], r[k] * (r[k] * dv[k]));
+      }
+      float trr[2];
+      small_total<2>(acc2, trr);
+      if (t == 0) {
+        S.pq = ok ? tpq[0] : -1.0f;
+        S.alpha = alpha;
+        FinCgStep<PC>{&S}(trr);
+      }
+      __syncthreads();
+      if (S.done == 0 && S.verify == 0) {
+        const float beta = S.beta;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+          const uint32_t i = t + k * kSmallThreads;
+          p[k] = r[k] * dv[k] + beta * p[k];
+          if (i < n) sv[i] = p[k];
+        }
+      }
+      __syncthreads();
+    } else {
+      // true residual of the FP32 iterate fl(x + x_lo), as k_verify
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const uint32_t i = t + k * kSmallThreads;
+        if (i < n) sv[i] = x[k] + xl[k];
+      }
+      __syncthreads();
+      float tr[RPT];
+      CSum acc[2] = {csum_zero(), csum_zero()};
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const uint32_t i = t + k * kSmallThreads;
+        CSum s{bv[k], 0.0f};
+        if (i < n)
+          A.foreach(i, [&](float a, uint32_t c) {
+            const float xv = sv[c];
+            const float pr = a * xv;
+            const float e = __builtin_fmaf(a, xv, -pr);
+            s = csum_add(s, -pr);
+            s.c -= e;
+          });
+        tr[k] = i < n ? csum_val(s) : 0.0f;
+        acc[0] = csum_add(acc[0], tr[k] * tr[k]);
+        acc[1] = csum_add(acc[1], tr[k] * (tr[k] * dv[k]));
+      }
+      float tot[2];
+      small_total<2>(acc, tot);
+      if (t == 0) FinVerify<PC == kPcJacobi ? 1 : 0>{&S}(tot);
+      __syncthreads();
+      if (S.done == 0) {  // restart from the true residual with p = z
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+          const uint32_t i = t + k * kSmallThreads;
+          x[k] = x[k] + xl[k];
+          xl[k] = 0.0f;
+          r[k] = tr[k];
+          p[k] = tr[k] * dv[k];
+          if (i < n) sv[i] = p[k];
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const uint32_t i = t + k * kSmallThreads;
+    if (i < n) x_io[i] = x[k] + xl[k];
+  }
+  if (t == 0) {
+    S.halt = 1;
+    *Sg = S;
+  }
+}
+
 // x = x + xl (collapse the compensated iterate to FP32)
 __global__ __launch_bounds__(kBlock) void k_collapse(float* __restrict__ x, const float* __restrict__ xl, uint32_t n) {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = x[i] + xl[i];

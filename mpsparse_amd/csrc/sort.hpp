@@ -163,11 +163,14 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
   __shared__ uint32_t gbase[kRadix];
   __shared__ uint64_t stage[kTile];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-  const uint64_t tile_base = (uint64_t)blockIdx.x * kTile;
+  // consecutive tiles on the same XCD: the runs that neighbouring tiles write
+  // for one digit are adjacent in the output and then meet in one L2
+  const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint64_t tile_base = (uint64_t)tile * kTile;
   const uint32_t tile_n = (uint32_t)((nnz - tile_base) < (uint64_t)kTile ? (nnz - tile_base) : (uint64_t)kTile);
 
   for (int i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
-  for (int d = tid; d < kRadix; d += kBlock) gbase[d] = offsets[(uint64_t)d * num_tiles + blockIdx.x];
+  for (int d = tid; d < kRadix; d += kBlock) gbase[d] = offsets[(uint64_t)d * num_tiles + tile];
   __syncthreads();
 
   OutK key[kItems];

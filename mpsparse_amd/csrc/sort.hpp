@@ -6,7 +6,7 @@
 // duplicates kept in input order), different machine:
 //   * keys are first compacted to (row << cbits) | col with cbits = bitlen(num_cols-1),
 //     so only rbits + cbits significant bits are sorted (u32 keys when they fit);
-//   * 8-bit digits, 4096-key tiles; per pass: upsweep (tile digit counts in LDS),
+//   * 8-bit digits, 3072-key tiles; per pass: upsweep (tile digit counts in LDS),
 //     a three-kernel exclusive scan of the digit-major count table, and a
 //     downsweep that ranks keys stably with 64-lane ballots, stages the tile in
 //     LDS in digit order and writes runs of equal digits contiguously;
@@ -19,8 +19,8 @@
 namespace msp {
 namespace sortk {
 
-constexpr int kItems = 16;
-constexpr int kTile = kBlock * kItems;  // 4096 keys per tile
+constexpr int kItems = 12;
+constexpr int kTile = kBlock * kItems;  // 3072 keys per tile (measured best of 2048 / 3072 / 4096 / 6144)
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kScanItems = 16;

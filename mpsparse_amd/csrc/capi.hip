@@ -1094,8 +1094,12 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
   bool have_prev = false;
   // a few extra steps beyond maxiter leave room for the final verification
   const int limit = o.maxiter + (o.verify ? 2 * o.check_every : 0);
+  // batches grow 4, 8, ... up to check_every, so that a solve that converges in
+  // a few steps does not queue two full batches of (skipped) step launches
+  int grow = std::min(4, std::max(1, o.check_every));
   while (true) {
-    const int batch = std::min(o.check_every, std::max(1, limit - enqueued));
+    const int batch = std::min(grow, std::max(1, limit - enqueued));
+    grow = std::min(2 * grow, std::max(1, o.check_every));
     for (int k = 0; k < batch; ++k) step();
     if (o.verify) verify();
     MSPCHK(launch_check("solver step"));
@@ -1122,6 +1126,25 @@ static msp_status solver_prologue(MatBase* A, const float* b, const float* x, co
   MSPCHK(copy_in(A->ws.v(Workspace::B), b, sizeof(float) * A->n, s));
   if (o.use_x0) MSPCHK(copy_in(A->ws.v(Workspace::X), x, sizeof(float) * A->n, s));
   return MSP_OK;
+}
+
+// True-residual check (k_verify): lane groups per row for CSR matrices with
+// long rows, one thread per row otherwise.
+template <int MODE, typename Op>
+static void verify_launch(const Op& op, uint32_t g, const float* X, const float* XLo, const float* B, float* R,
+                          float* P, float* R0, const float* D, uint32_t n, Scalars* S, GridRed rb, hipStream_t s) {
+  if constexpr (std::is_same<decltype(op.rows()), RowsCsr>::value) {
+    const uint64_t avg = op.A->n ? op.A->nnz / op.A->n : 0;
+    if (avg >= 64) {
+      k_verify_lanes<MODE, 64><<<g, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
+      return;
+    }
+    if (avg >= 16) {
+      k_verify_lanes<MODE, 16><<<g, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
+      return;
+    }
+  }
+  k_verify<MODE><<<g, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
 }
 
 // The CG iterate carries its rounding error in a second vector (x + xl) only
@@ -1224,10 +1247,10 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   // a restart (beta = 0) makes the next direction z whatever P[k & 1] holds
   auto verify = [&]() {
     if (jac) {
-      k_verify<1><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb);
+      verify_launch<1>(op, gv, X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb, s);
       finish<2>(FinVerify<1>{S}, rb, gv, s);
     } else {
-      k_verify<0><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb);
+      verify_launch<0>(op, gv, X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb, s);
       finish<2>(FinVerify<0>{S}, rb, gv, s);
     }
   };
@@ -1269,7 +1292,7 @@ static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const ms
     k_bicg_p<<<gv, kBlock, 0, s>>>(P, R, Vv, n, S);
   };
   auto verify = [&]() {
-    k_verify<2><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, nullptr, n, S, rb);
+    verify_launch<2>(op, gv, X, XLo, B, R, P, R0, nullptr, n, S, rb, s);
     finish<2>(FinVerify<2>{S}, rb, gv, s);
   };
   MSPCHK(run_solver(A, o, info, s, init, step, verify));
@@ -2036,7 +2059,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     finish<2>(FinCgStep<kPcExplicit>{S}, rb2, gv, s);
   };
   auto verify = [&]() {
-    k_verify<3><<<gv, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, nullptr, nullptr, n, S, rb);
+    verify_launch<3>(op, gv, X, XLo, B, R, P, nullptr, nullptr, n, S, rb, s);
     finish<2>(FinVerify<3>{S}, rb, gv, s);
   };
   MSPCHK(run_solver(A, o, info, s, init, step, verify));

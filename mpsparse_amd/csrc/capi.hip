@@ -1638,7 +1638,11 @@ static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_r
   double prev = 0.0;
   int slow = 0;
   for (int k = 0;; ++k) {
-    k_residual64<<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, reinterpret_cast<double*>(rb.partials));
+    double* part = reinterpret_cast<double*>(rb.partials);
+    const uint64_t avg = n ? A->nnz / n : 0;
+    if (avg >= 64) k_residual64_lanes<64><<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, part);
+    else if (avg >= 16) k_residual64_lanes<16><<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, part);
+    else k_residual64<<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, part);
     k_finish_d<2><<<1, kBlock, 0, s>>>(reinterpret_cast<const double*>(rb.partials), gv, SUMS);
     MSPCHK(launch_check("k_residual64"));
     double sums[2] = {0.0, 0.0};

@@ -71,6 +71,34 @@ __global__ __launch_bounds__(kBlock) void k_residual64(Rows rows, const double* 
   }
 }
 
+// The same with L lanes per row (rows averaging many entries): coalesced row
+// loads, lane sums combined by a butterfly.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_residual64_lanes(RowsCsr rows, const double* __restrict__ x,
+                                                             const double* __restrict__ b, double* __restrict__ r,
+                                                             uint32_t n, double* __restrict__ partials) {
+  constexpr uint32_t G = kBlock / L;
+  const uint32_t lane = threadIdx.x % L, grp = threadIdx.x / L;
+  double acc[2] = {0.0, 0.0};
+  for (uint32_t i = blockIdx.x * G + grp; i < n; i += gridDim.x * G) {  // uniform within a lane group
+    double s = 0.0;
+    for (uint32_t j = rows.row_ptr[i] + lane; j < rows.row_ptr[i + 1]; j += L) s -= (double)rows.val[j] * x[rows.col[j]];
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (lane == 0) {
+      s += b[i];
+      r[i] = s;
+      acc[0] += s * s;
+      acc[1] += b[i] * b[i];
+    }
+  }
+  block_reduce_d<2>(acc);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 2] = acc[0];
+    partials[blockIdx.x * 2 + 1] = acc[1];
+  }
+}
+
 // out = float(r * scale)
 __global__ __launch_bounds__(kBlock) void k_scale_to_f32(const double* __restrict__ r, float* __restrict__ out,
                                                          double scale, uint32_t n) {

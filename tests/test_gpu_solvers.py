@@ -126,6 +126,30 @@ def test_cg_step_forms(fmt, precond):
         _solve(A, b, cg_form="both")
 
 
+@pytest.mark.parametrize("precond", ["none", "jacobi"])
+def test_small_system_single_workgroup_path(precond):
+    """n <= 4096 solves run in one workgroup (automatic step form); the forced
+    multi-kernel form runs the same recurrence: both reach the FP64 true
+    residual bar, with iteration counts within 2, also from a warm start."""
+    import torch
+
+    M = diffusion_2d(60, sigma=1.5, seed=7)
+    b = np.random.default_rng(7).standard_normal(M.num_rows).astype(np.float32)
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    S = M.to_scipy()
+    its = {}
+    for form in ("auto", "separate"):
+        say("small path", precond, form)
+        x, info = _solve(A, b, precond=precond, tol=1e-5, cg_form=form)
+        assert info["status"] == "converged", info
+        assert K.true_relres(S, x, b) <= 1.05e-5
+        its[form] = info["iterations"]
+    assert abs(its["auto"] - its["separate"]) <= 2, its
+    x0 = torch.tensor(x, device="cuda")
+    info = A.cgd(t_f32(b), x0, precond=precond, tol=1e-5, use_x0=True)
+    assert info["status"] == "converged" and info["iterations"] <= 3, info
+
+
 def test_bicgstab_block_sparse():
     q = block_sparse(1024, 16, 8, seed=2)
     say("bicgstab block sparse 1024")

@@ -512,10 +512,10 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
   using namespace sortk;
   const uint32_t num_tiles = (uint32_t)((nnz + kTile - 1) / kTile);
   const uint64_t N = (uint64_t)kRadix * num_tiles;
-  const uint32_t nscan = (uint32_t)((N + kScanTile - 1) / kScanTile);
+  const uint32_t nchunks = (num_tiles + kColChunk - 1) / kColChunk;
   TmpMem counts, sums, ka, kb, va, vb;
   MSPCHK(counts.alloc(sizeof(uint32_t) * N, s));
-  MSPCHK(sums.alloc(sizeof(uint32_t) * nscan, s));
+  MSPCHK(sums.alloc(sizeof(uint32_t) * kRadix * nchunks, s));
   if (passes > 1) {
     MSPCHK(ka.alloc(sizeof(K) * nnz, s));
     MSPCHK(va.alloc(sizeof(float) * nnz, s));
@@ -534,10 +534,10 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
     if (first) k_upsweep<uint64_t, true><<<num_tiles, kBlock, 0, s>>>(keys, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
     else k_upsweep<K, false><<<num_tiles, kBlock, 0, s>>>(kin, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
     MSPCHK(launch_check("k_upsweep"));
-    k_scan_reduce<<<nscan, kBlock, 0, s>>>(counts.as<uint32_t>(), N, sums.as<uint32_t>());
-    k_scan_top<<<1, kBlock, 0, s>>>(sums.as<uint32_t>(), nscan);
-    k_scan_down<<<nscan, kBlock, 0, s>>>(counts.as<uint32_t>(), N, sums.as<uint32_t>());
-    MSPCHK(launch_check("k_scan"));
+    k_colscan_reduce<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
+    k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks);
+    k_colscan_down<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
+    MSPCHK(launch_check("k_colscan"));
     K* kout = kbuf[p & 1];
     float* vout = vbuf[p & 1];
     const uint32_t* off = counts.as<uint32_t>();

@@ -7,7 +7,7 @@
 //   * keys are first compacted to (row << cbits) | col with cbits = bitlen(num_cols-1),
 //     so only rbits + cbits significant bits are sorted (u32 keys when they fit);
 //   * 8-bit digits, 3072-key tiles; per pass: upsweep (tile digit counts in LDS),
-//     a three-kernel exclusive scan of the digit-major count table, and a
+//     a per-digit scan of the tile-major count table (three kernels), and a
 //     downsweep that ranks keys stably with 64-lane ballots, stages the tile in
 //     LDS in digit order and writes runs of equal digits contiguously;
 //   * the last pass writes col_ind / values / row directly (no key array), and
@@ -47,7 +47,8 @@ __device__ __forceinline__ uint32_t digit_of(K k, int shift) {
   return (uint32_t)(k >> shift) & (kRadix - 1);
 }
 
-// Tile digit histogram -> counts[d * num_tiles + tile].
+// Tile digit histogram -> counts[tile * kRadix + d] (tile-major: one
+// coalesced 1 KiB row per tile).
 template <typename InK, bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_upsweep(const InK* __restrict__ keys, uint64_t nnz, KeyMap km,
                                                     int shift, uint32_t num_tiles, uint32_t* __restrict__ counts,
@@ -71,7 +72,61 @@ __global__ __launch_bounds__(kBlock) void k_upsweep(const InK* __restrict__ keys
     uint32_t s = 0;
 #pragma unroll
     for (int w = 0; w < kWavesPerBlock; ++w) s += h[w][d];
-    counts[(uint64_t)d * num_tiles + blockIdx.x] = s;
+    counts[(uint64_t)blockIdx.x * kRadix + d] = s;
+  }
+}
+
+// Scatter offsets from the tile-major count table: offset(t, d) = start of
+// digit d + sum of counts[t'][d] over tiles t' < t. Three kernels, all with
+// one thread per digit and coalesced rows: per-chunk column sums, one block
+// that scans them per digit and adds the digit starts, and the per-chunk
+// running sums written back in place.
+constexpr int kColChunk = 256;  // tiles per column-scan block
+
+__global__ __launch_bounds__(kRadix) void k_colscan_reduce(const uint32_t* __restrict__ counts, uint32_t num_tiles,
+                                                           uint32_t* __restrict__ sums) {
+  const uint32_t d = threadIdx.x, t0 = blockIdx.x * kColChunk;
+  const uint32_t t1 = t0 + kColChunk < num_tiles ? t0 + kColChunk : num_tiles;
+  uint32_t s = 0;
+#pragma unroll 16
+  for (uint32_t t = t0; t < t1; ++t) s += counts[(uint64_t)t * kRadix + d];
+  sums[(uint64_t)blockIdx.x * kRadix + d] = s;
+}
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl);
+
+__global__ __launch_bounds__(kRadix) void k_colscan_top(uint32_t* __restrict__ sums, uint32_t nchunks) {
+  const uint32_t d = threadIdx.x;
+  uint32_t run = 0;
+  for (uint32_t c0 = 0; c0 < nchunks; c0 += 16) {  // 16 loads in flight per pass
+    uint32_t v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = c0 + q < nchunks ? sums[(uint64_t)(c0 + q) * kRadix + d] : 0u;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (c0 + q < nchunks) sums[(uint64_t)(c0 + q) * kRadix + d] = run;
+      run += v[q];
+    }
+  }
+  uint32_t start;
+  block_exclusive_scan(run, start);  // digit starts: totals scanned over the digits
+  for (uint32_t c = 0; c < nchunks; ++c) sums[(uint64_t)c * kRadix + d] += start;
+}
+
+__global__ __launch_bounds__(kRadix) void k_colscan_down(uint32_t* __restrict__ counts, uint32_t num_tiles,
+                                                         const uint32_t* __restrict__ sums) {
+  const uint32_t d = threadIdx.x, t0 = blockIdx.x * kColChunk;
+  const uint32_t t1 = t0 + kColChunk < num_tiles ? t0 + kColChunk : num_tiles;
+  uint32_t run = sums[(uint64_t)blockIdx.x * kRadix + d];
+  for (uint32_t tb = t0; tb < t1; tb += 16) {
+    uint32_t v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = tb + q < t1 ? counts[(uint64_t)(tb + q) * kRadix + d] : 0u;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (tb + q < t1) counts[(uint64_t)(tb + q) * kRadix + d] = run;
+      run += v[q];
+    }
   }
 }
 
@@ -126,16 +181,26 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t* __restrict__ sums
   }
 }
 
-// Scan phase C: exclusive scan inside each block plus its block offset (in place).
+// Scan phase C: exclusive scan inside each block plus its block offset (in
+// place). The block's counts pass through LDS so that both the load and the
+// store are coalesced (thread-contiguous runs of kScanItems in HBM were 4x
+// slower).
 __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ data, uint64_t N,
                                                       const uint32_t* __restrict__ block_off) {
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  __shared__ uint32_t buf[kScanTile + kScanTile / 32];  // +1 word per 32 against bank conflicts
+  auto at = [](uint32_t k) { return k + (k >> 5); };
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  for (int i = 0; i < kScanItems; ++i) {
+    const uint32_t k = (uint32_t)i * kBlock + threadIdx.x;
+    buf[at(k)] = base + k < N ? data[base + k] : 0u;
+  }
+  __syncthreads();
+  const uint32_t k0 = threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    uint64_t idx = base + i;
-    v[i] = idx < N ? data[idx] : 0u;
+    v[i] = buf[at(k0 + i)];
     s += v[i];
   }
   uint32_t excl;
@@ -143,9 +208,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ dat
   uint32_t run = block_off[blockIdx.x] + excl;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    uint64_t idx = base + i;
-    if (idx < N) data[idx] = run;
+    buf[at(k0 + i)] = run;
     run += v[i];
+  }
+  __syncthreads();
+  for (int i = 0; i < kScanItems; ++i) {
+    const uint32_t k = (uint32_t)i * kBlock + threadIdx.x;
+    if (base + k < N) data[base + k] = buf[at(k)];
   }
 }
 
@@ -170,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
   const uint32_t tile_n = (uint32_t)((nnz - tile_base) < (uint64_t)kTile ? (nnz - tile_base) : (uint64_t)kTile);
 
   for (int i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
-  for (int d = tid; d < kRadix; d += kBlock) gbase[d] = offsets[(uint64_t)d * num_tiles + tile];
+  for (int d = tid; d < kRadix; d += kBlock) gbase[d] = offsets[(uint64_t)tile * kRadix + d];
   __syncthreads();
 
   OutK key[kItems];

@@ -147,18 +147,27 @@ __global__ __launch_bounds__(T) void k_bqr_factor(const double* __restrict__ RB,
     lds_barrier();
     const double tau = s_tau;
     if (tau != 0.0) {
+      // rows j .. j + L - 1 sit in slots rj .. WR - 1, then 0 ..: two straight
+      // runs (no wrap test in the loop, so the LDS loads can be issued ahead)
+      const int n1 = L < WR - rj ? L : WR - rj;
       for (int c = t; c < W; c += T) {  // column j + 1 + c
-        const int cs = (int)((j + 1 + c) % WC);
+        const int cs = cj + 1 + c < WC ? cj + 1 + c : cj + 1 + c - WC;
         double dot = 0.0;
-        for (int i = 0, s = rj; i < L; ++i, s = (s + 1 == WR ? 0 : s + 1)) dot += sv[i] * win[s * WC + cs];
+#pragma unroll 4
+        for (int i = 0; i < n1; ++i) dot += sv[i] * win[(rj + i) * WC + cs];
+#pragma unroll 4
+        for (int i = n1; i < L; ++i) dot += sv[i] * win[(i - n1) * WC + cs];
         const double f = tau * dot;
-        for (int i = 0, s = rj; i < L; ++i, s = (s + 1 == WR ? 0 : s + 1)) win[s * WC + cs] -= f * sv[i];
+#pragma unroll 4
+        for (int i = 0; i < n1; ++i) win[(rj + i) * WC + cs] -= f * sv[i];
+#pragma unroll 4
+        for (int i = n1; i < L; ++i) win[(i - n1) * WC + cs] -= f * sv[i];
       }
     }
     lds_barrier();
     // retire row j (a finished row of R) and column j (the reflector)
     for (int c = t; c < WC; c += T)
-      if (j + c < n) Rb[j * WC + c] = win[rj * WC + (int)((j + c) % WC)];
+      if (j + c < n) Rb[j * WC + c] = win[rj * WC + (cj + c < WC ? cj + c : cj + c - WC)];
     for (int i = 1 + t; i < WR; i += T) Vb[j * kl + (i - 1)] = i < L ? win[((rj + i) % WR) * WC + cj] : 0.0;
     if (t == 0) tau_out[j] = tau;
     lds_barrier();
@@ -167,7 +176,8 @@ __global__ __launch_bounds__(T) void k_bqr_factor(const double* __restrict__ RB,
     for (int i = 1 + t; i < WR; i += T) win[((rj + i) % WR) * WC + cj] = 0.0;
     const int64_t e = j + kl + 1;
     const int srow = (int)(j % kStage);
-    for (int c = t; c < WC; c += T) win[rj * WC + (int)((j + 1 + c) % WC)] = e < m ? stage[srow * WC + c] : 0.0;
+    for (int c = t; c < WC; c += T)
+      win[rj * WC + (cj + 1 + c < WC ? cj + 1 + c : cj + 1 + c - WC)] = e < m ? stage[srow * WC + c] : 0.0;
     lds_barrier();
   }
 }
@@ -447,11 +457,19 @@ __global__ __launch_bounds__(kRhs) void k_bqr_apply_qt(const double* __restrict_
       const double tj = ts[q];
       const double* v = vs + q * kl;
       if (tj != 0.0) {
+        // rows j + i in slots rj + i, wrapping once at WR (two straight runs)
+        const int n1 = L < WR - rj ? L : WR - rj;
         double w = wb[rj * kRhs + t];
-        for (int i = 1, s = (rj + 1) % WR; i < L; ++i, s = (s + 1 == WR ? 0 : s + 1)) w += v[i - 1] * wb[s * kRhs + t];
+#pragma unroll 4
+        for (int i = 1; i < n1; ++i) w += v[i - 1] * wb[(rj + i) * kRhs + t];
+#pragma unroll 4
+        for (int i = n1; i < L; ++i) w += v[i - 1] * wb[(i - n1) * kRhs + t];
         w *= tj;
         wb[rj * kRhs + t] -= w;
-        for (int i = 1, s = (rj + 1) % WR; i < L; ++i, s = (s + 1 == WR ? 0 : s + 1)) wb[s * kRhs + t] -= w * v[i - 1];
+#pragma unroll 4
+        for (int i = 1; i < n1; ++i) wb[(rj + i) * kRhs + t] -= w * v[i - 1];
+#pragma unroll 4
+        for (int i = n1; i < L; ++i) wb[(i - n1) * kRhs + t] -= w * v[i - 1];
       }
       if (act) Y[j * nrhs + col] = wb[rj * kRhs + t];  // row j of Q^T y is final
       wb[rj * kRhs + t] = cur[q];                       // row j + kl + 1 enters
@@ -497,8 +515,12 @@ __global__ __launch_bounds__(kRhs) void k_bqr_rsolve(const double* __restrict__ 
       const double* r = rs + q * wc;
       const int lim = (int)(n - 1 - i < (int64_t)(wc - 1) ? n - 1 - i : (int64_t)(wc - 1));
       double s = cur[q];
-      for (int c = 1, sl = (int)((i + 1) % wc); c <= lim; ++c, sl = (sl + 1 == wc ? 0 : sl + 1))
-        s -= r[c] * wx[sl * kRhs + t];
+      const int r0 = (int)(i % wc);
+      const int c1 = lim < wc - 1 - r0 ? lim : wc - 1 - r0;  // x_{i+c} in slot r0 + c up to the wrap
+#pragma unroll 4
+      for (int c = 1; c <= c1; ++c) s -= r[c] * wx[(r0 + c) * kRhs + t];
+#pragma unroll 4
+      for (int c = c1 + 1; c <= lim; ++c) s -= r[c] * wx[(r0 + c - wc) * kRhs + t];
       const double x = s / r[0];
       wx[(int)(i % wc) * kRhs + t] = x;
       if (act) Xp[i * nrhs + col] = x;

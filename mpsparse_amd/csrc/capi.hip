@@ -2133,10 +2133,10 @@ extern "C" msp_status msp_csr_solve_refined_fsai(msp_csr A, msp_fsai F, const do
 // ABI: edge MLP of the learned FSAI (BF16 MFMA, FP32 accumulate)
 extern "C" int32_t msp_mlp_num_params(void) { return mlp::kParams; }
 
-static uint32_t mlp_grid(uint64_t ne) {
+static uint32_t mlp_grid(uint64_t ne, uint32_t per_cu = 2) {
   const uint64_t tiles = (ne + 31) / 32;
   const uint64_t g = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)cu_count() * 2));
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)cu_count() * per_cu));
 }
 
 static msp_status need_dev(const void* p, const char* what) {
@@ -2152,7 +2152,8 @@ extern "C" msp_status msp_mlp_forward(const float* params, const float* feat, ui
   MSPCHK(need_dev(out, "out"));
   if ((reinterpret_cast<uintptr_t>(feat) & 15u) != 0) return fail(MSP_ERR_INVALID, "feat must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
-  mlp::k_mlp_forward<<<mlp_grid(ne), kBlock, 0, s>>>(params, feat, ne, out);
+  static const uint32_t fwd_per_cu = resident_per_cu(mlp::k_mlp_forward);
+  mlp::k_mlp_forward<<<mlp_grid(ne, fwd_per_cu), kBlock, 0, s>>>(params, feat, ne, out);
   return launch_check("mlp forward");
 }
 

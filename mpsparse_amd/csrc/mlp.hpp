@@ -25,10 +25,18 @@ constexpr int kParams = oB3 + 1;  // 4801
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ short bf16_bits(float f) { return (short)f32_to_bf16(f); }
+// FP32 -> BF16 with the hardware converter (v_cvt_pk_bf16_f32: round to
+// nearest even, two values per instruction), which replaced ~5 integer
+// instructions per value of the portable rounding.
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
 __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
-  return s4{bf16_bits(a), bf16_bits(b), bf16_bits(c), bf16_bits(d)};
+  const bf2 lo = __builtin_convertvector(fl2{a, b}, bf2), hi = __builtin_convertvector(fl2{c, d}, bf2);
+  const uint32_t l = __builtin_bit_cast(uint32_t, lo), h = __builtin_bit_cast(uint32_t, hi);
+  return s4{(short)(l & 0xFFFFu), (short)(l >> 16), (short)(h & 0xFFFFu), (short)(h >> 16)};
 }
 
 __device__ __forceinline__ f16 mfma(s4 a, s4 b, f16 c) {
@@ -129,8 +137,11 @@ __device__ __forceinline__ void layer3(const FwdWeights& W, const f16 (&H2)[2], 
   for (int r = 0; r < 16; ++r) o[r] += W.bias3;
 }
 
-// out[e] = MLP(feat[e]) for all edges.
-__global__ __launch_bounds__(kBlock) void k_mlp_forward(const float* __restrict__ P, const float* __restrict__ feat,
+// out[e] = MLP(feat[e]) for all edges. Bounded to 128 registers (4 waves per
+// SIMD): without the bound the compiler kept 132 and ran 3 waves per SIMD, and
+// the MFMA chain of one tile has too little independent work to hide the
+// conversions, the LDS staging and the layer-3 reduction with fewer waves.
+__global__ __launch_bounds__(kBlock, 4) void k_mlp_forward(const float* __restrict__ P, const float* __restrict__ feat,
                                                         uint64_t ne, float* __restrict__ out) {
   __shared__ short stage[kWavesPerBlock][32 * 68];
   const int l = threadIdx.x & 63, h = l >> 5, c = l & 31, wid = threadIdx.x / kWave;

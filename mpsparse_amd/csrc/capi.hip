@@ -446,7 +446,11 @@ static msp_status ensure_dia16(msp_csr A, int32_t vt, hipStream_t s) {
   MSPCHK(A->dia16.alloc(sizeof(uint16_t) * cnt));
   HIPCHK(hipMemsetAsync(A->dia16.p, 0, sizeof(uint16_t) * cnt, s));
   MSPCHK(A->dia_dlo.alloc(sizeof(float) * ((size_t)A->n + 1)));
-  if (vt == MSP_VALUES_BF16)
+  if (vt == kValuesBF16Plain)
+    k_dia_cvt16<true, false><<<grid_for(A->n), kBlock, 0, s>>>(A->dia.as<float>(), A->n, A->m, A->dia_o, A->dia_dd,
+                                                               A->dia_ld, A->dia16.as<uint16_t>(),
+                                                               A->dia_dlo.as<float>());
+  else if (vt == MSP_VALUES_BF16)
     k_dia_cvt16<true><<<grid_for(A->n), kBlock, 0, s>>>(A->dia.as<float>(), A->n, A->m, A->dia_o, A->dia_dd,
                                                         A->dia_ld, A->dia16.as<uint16_t>(), A->dia_dlo.as<float>());
   else
@@ -944,6 +948,13 @@ struct OpCsr {
                   const int* done, hipStream_t s) const {
     const XPNew<PC> xl{r, pold, dinv, beta};
     return spmv_any<V, 1>(A, xl, EpiCgQ<XPNew<PC>>{q, pnew, xl}, FinNone{}, red_of(A), done, s);
+  }
+  // Explicit preconditioner: p = z + beta p_old in the gather with beta from
+  // the <r,z> partials (zr, nz blocks), q = A p, <p,q> partials.
+  uint32_t pcg_dir(const float* z, const float* pold, const Scalars* S, GridRed zr, uint32_t nz, float* q,
+                   float* pnew, const int* done, hipStream_t s) const {
+    const XPzLocal xl{z, pold, S, zr, nz, 0.0f};
+    return spmv_any<V, 1>(A, xl, EpiCgQ<XPzLocal>{q, pnew, xl}, FinNone{}, red_of(A), done, s);
   }
   RowsCsr rows() const { return RowsCsr{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()}; }
   void inv_diag(float* dinv, int* bad, hipStream_t s) const {
@@ -1962,6 +1973,14 @@ static uint32_t spmv_with(msp_csr M, XL xl, Epi epi, Fin fin, GridRed rb, const 
   return spmv_any<ValF32, ND>(M, xl, epi, fin, rb, done, s);
 }
 
+// A product with an FSAI factor (G or G^T), FP32 values or their one-by-one
+// BF16 rounding (g16; diagonal storage only, prepared by ensure_dia16).
+template <int ND, typename XL, typename Epi>
+static uint32_t factor_spmv(bool g16, msp_csr M, XL xl, Epi epi, GridRed rb, const int* done, hipStream_t s) {
+  if (g16) return spmv_any<ValBF16Plain, ND>(M, xl, epi, FinNone{}, rb, done, s);
+  return spmv_any<ValF32, ND>(M, xl, epi, FinNone{}, rb, done, s);
+}
+
 extern "C" msp_status msp_fsai_apply(msp_fsai F, const float* r, float* z, void* stream) {
   if (!F || !F->ready) return fail(MSP_ERR_INVALID, "FSAI values not set");
   HIPCHK(hipSetDevice(F->device));
@@ -2030,6 +2049,14 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
                                 msp_solve_info* info, hipStream_t s) {
   MatBase* A = op.base();
   MSPCHK(solver_prologue(A, b, x, o, s));
+  // With 16-bit values of A, G's values are BF16 too (each rounded on its
+  // own, so the stored G^T is the exact transpose of the stored G and
+  // M = G^T G stays symmetric positive definite); diagonal storage only.
+  const bool g16 = o.values != MSP_VALUES_F32 && F->G.dia_state == 1 && F->GT.dia_state == 1;
+  if (g16) {
+    MSPCHK(ensure_dia16(&F->G, kValuesBF16Plain, s));
+    MSPCHK(ensure_dia16(&F->GT, kValuesBF16Plain, s));
+  }
   Workspace& w = A->ws;
   float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Q = w.v(Workspace::Q);
   float *Z = w.v(Workspace::Z), *Wv = w.v(Workspace::W), *XLo = w.v(Workspace::XL);
@@ -2045,22 +2072,30 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     k_set_restart<<<1, 1, 0, s>>>(S);
     return launch_check("pcg init");
   };
-  // one step, six launches: w = G r; z = G^T w with <r,z> partials (zr);
-  // p = z + beta p with beta from zr; q = A p with <p,q> partials (rb);
-  // alpha from zr and rb inside the x/r update (partials rb2); FinCgStep
+  // one step, five launches: w = G r; z = G^T w with <r,z> partials (zr);
+  // q = A p with p = z + beta p_old formed in the gather (beta from zr) and
+  // <p,q> partials (rb); alpha from zr and rb inside the x/r update (partials
+  // rb2); FinCgStep
   GridRed rb2 = w.gr2(), zr = w.gr3();
+  // ping-pong directions: step k reads Pb[k & 1] and writes Pb[(k + 1) & 1]
+  // (the direction is formed in the gather of q = A p, so the p being written
+  // cannot be the one other workgroups gather)
+  float* Pb[2] = {P, w.v(Workspace::T)};
+  int k = 0;
   auto step = [&]() {
-    spmv_with<0>(&F->G, XPlain{R}, EpiStore{Wv}, FinNone{}, rb, &S->halt, s);
-    const uint32_t gz = spmv_with<1>(&F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, FinNone{}, zr, &S->halt, s);
-    k_cg_update_p<kPcExplicit, true><<<gv, kBlock, 0, s>>>(P, P, R, nullptr, Z, n, S, zr, gz);
-    const uint32_t gq = op.template spmv<1>(P, EpiStoreDot{Q, P}, FinNone{}, &S->halt, s);
+    const float* pold = Pb[k & 1];
+    float* pnew = Pb[(k + 1) & 1];
+    factor_spmv<0>(g16, &F->G, XPlain{R}, EpiStore{Wv}, rb, &S->halt, s);
+    const uint32_t gz = factor_spmv<1>(g16, &F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, zr, &S->halt, s);
+    const uint32_t gq = op.pcg_dir(Z, pold, S, zr, gz, Q, pnew, &S->halt, s);
     if (comp)
-      k_cg_update_xr<kPcExplicit, kAlphaRzPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb2, rb,
-                                                                          gq, zr, gz);
+      k_cg_update_xr<kPcExplicit, kAlphaRzPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, nullptr, n, S, rb2,
+                                                                          rb, gq, zr, gz);
     else
-      k_cg_update_xr<kPcExplicit, kAlphaRzPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, P, Q, nullptr, n, S, rb2, rb,
-                                                                           gq, zr, gz);
+      k_cg_update_xr<kPcExplicit, kAlphaRzPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, nullptr, n, S, rb2,
+                                                                           rb, gq, zr, gz);
     finish<2>(FinCgStep<kPcExplicit>{S}, rb2, gv, s);
+    ++k;
   };
   auto verify = [&]() {
     verify_launch<3>(op, gv, X, XLo, B, R, P, nullptr, nullptr, n, S, rb, s);

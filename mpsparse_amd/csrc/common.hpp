@@ -129,6 +129,32 @@ __device__ __forceinline__ void block_partials(CSum (&acc)[ND], GridRed rb) {
   }
 }
 
+// This thread's share of nb block partials (ND consecutive values at a
+// stride of `stride` CSums per block), merged into a[] in block order i = tid,
+// tid + kBlock, ...: the loads of a batch are all issued before its first
+// merge (a merge waiting on each load in turn would pay the memory latency
+// once per partial).
+constexpr int kPartialBatch = 8;
+template <int ND>
+__device__ __forceinline__ void merge_partials(const CSum* __restrict__ p, uint32_t nb, CSum (&a)[ND],
+                                               uint32_t stride = ND) {
+  for (uint32_t base = 0; base < nb; base += kPartialBatch * kBlock) {
+    CSum v[kPartialBatch][ND];
+#pragma unroll
+    for (int j = 0; j < kPartialBatch; ++j) {
+      const uint32_t i = base + j * kBlock + threadIdx.x;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) v[j][d] = i < nb ? p[i * stride + d] : csum_zero();
+    }
+#pragma unroll
+    for (int j = 0; j < kPartialBatch; ++j) {
+      const bool in = base + j * kBlock + threadIdx.x < nb;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) a[d] = in ? csum_merge(a[d], v[j][d]) : a[d];
+    }
+  }
+}
+
 // Finisher protocol: `active()` repeats the producer's skip test (a producer
 // that returned early wrote no partials), `operator()(tot)` then `after()` run
 // in thread 0.
@@ -138,10 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(GridRed rb, uint32_t nb, Fin 
   CSum a[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) a[d] = csum_zero();
-  for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
-#pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] = csum_merge(a[d], rb.partials[i * ND + d]);
-  }
+  merge_partials<ND>(rb.partials, nb, a);
   block_reduce<ND>(a);
   if (threadIdx.x == 0) {
     float tot[ND];
@@ -224,6 +247,16 @@ struct ValF16 {
     return f4{cvt(w.x & 0xFFFFu), cvt(w.x >> 16), cvt(w.y & 0xFFFFu), cvt(w.y >> 16)};
   }
 };
+
+// BF16 values rounded one by one, without the diagonal remainder: for a
+// preconditioner factor stored with its transpose (the two round to exact
+// transposes of each other, so G^T G stays symmetric). Internal value type 3
+// (not a public msp_value_type).
+struct ValBF16Plain : ValBF16 {
+  static constexpr bool kLowDiag = false;
+  static constexpr int kType = 3;
+};
+constexpr int32_t kValuesBF16Plain = ValBF16Plain::kType;
 
 // Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {

@@ -48,6 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
   // their triangular factors); 0: A.o.ndiag <= kMaxDiag
   constexpr int DMAX = NDIAG > 0 ? NDIAG : kMaxDiag;
   if (done && *done) return;
+  kernel_prologue(xl, epi);
   constexpr int NA = ND > 0 ? ND : 1;
   CSum acc[NA];
 #pragma unroll
@@ -142,12 +143,20 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(const uint32_t* __restrict_
 // 16-bit diagonal storage with exact row sums, as k_cvt16_rowsum does for CSR:
 // off-diagonals rounded, the diagonal (offset 0, index dd; -1 if absent)
 // absorbs their rounding defects and is stored as a 16-bit value plus the FP32
-// remainder dlo.
-template <bool BF16>
+// remainder dlo. SUMS = false: every value rounded on its own, dlo = 0.
+template <bool BF16, bool SUMS = true>
 __global__ __launch_bounds__(kBlock) void k_dia_cvt16(const float* __restrict__ in, uint32_t n, uint32_t m,
                                                       DiaOffsets o, int dd, uint64_t ld, uint16_t* __restrict__ out,
                                                       float* __restrict__ dlo) {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    if constexpr (!SUMS) {
+      for (int d = 0; d < o.ndiag; ++d) {
+        const float a = in[d * ld + i];
+        out[d * ld + i] = BF16 ? f32_to_bf16(a) : f32_to_f16(a);
+      }
+      dlo[i] = 0.0f;
+      continue;
+    }
     double dsum = 0.0;
     for (int d = 0; d < o.ndiag; ++d) {
       const float a = in[d * ld + i];

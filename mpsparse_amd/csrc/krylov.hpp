@@ -210,8 +210,9 @@ __device__ __forceinline__ bool step_converged(const Scalars* S, int it, float r
 __device__ __forceinline__ uint32_t vec_chunks(uint32_t n) { return (n + 3u) / 4u; }
 
 
-// Initialisation: x = 0 (or kept), r = b (or precomputed b - A x0), p = z,
-// <b,b>, <r,r>, <r,z>.
+// Initialisation: x = 0 (or kept), r = b (or precomputed b - A x0), p = z
+// (p = 0 for an explicit preconditioner, whose first direction update takes
+// beta = 0 times this p), <b,b>, <r,r>, <r,z>.
 __global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float* __restrict__ xl,
                                                     float* __restrict__ r, float* __restrict__ p,
                                                     const float* __restrict__ b, const float* __restrict__ dinv,
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(float* __restrict__ x, float
       reinterpret_cast<f4*>(r)[c] = rv;
     }
     reinterpret_cast<f4*>(xl)[c] = f4{0.f, 0.f, 0.f, 0.f};
-    if (pc != kPcExplicit) reinterpret_cast<f4*>(p)[c] = zv;
+    reinterpret_cast<f4*>(p)[c] = pc != kPcExplicit ? zv : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       acc[0] = csum_add(acc[0], bv[k] * bv[k]);
@@ -270,13 +271,46 @@ template <int ND>
 __device__ __forceinline__ float block_total(const CSum* __restrict__ partials, uint32_t nb, int d) {
   __shared__ float s_tot;
   CSum a[1] = {csum_zero()};
-  for (uint32_t i = threadIdx.x; i < nb; i += kBlock) a[0] = csum_merge(a[0], partials[i * ND + d]);
+  merge_partials<1>(partials + d, nb, a, ND);
   block_reduce<1>(a);
   if (threadIdx.x == 0) s_tot = csum_val(a[0]);
   __syncthreads();
   const float t = s_tot;
   __syncthreads();
   return t;
+}
+
+// Explicitly preconditioned CG direction formed in the gather of q = A p:
+// p = z + beta p_old with beta = <r,z>_new / <r,z>_old, <r,z>_new from the
+// block partials of the z product (zr, nz blocks) reduced by every workgroup
+// in kernel_prologue (the same rule as k_cg_update_p<kPcExplicit, true>).
+// p_old must not alias the p written by the epilogue.
+struct XPzLocal {
+  const float* __restrict__ z;
+  const float* __restrict__ pold;
+  const Scalars* S;
+  GridRed zr;
+  uint32_t nz;
+  float beta;
+  struct Raw {
+    float z, p;
+  };
+  __device__ __forceinline__ Raw raw(uint32_t c) const { return Raw{z[c], pold[c]}; }
+  __device__ __forceinline__ float make(Raw w) const { return w.z + beta * w.p; }
+  __device__ __forceinline__ float operator()(uint32_t c) const { return z[c] + beta * pold[c]; }
+  __device__ __forceinline__ f4 load4(u4 c) const {
+    if (quad_run(c))
+      return *reinterpret_cast<const f4*>(z + c.x) + beta * *reinterpret_cast<const f4*>(pold + c.x);
+    return f4{(*this)(c.x), (*this)(c.y), (*this)(c.z), (*this)(c.w)};
+  }
+};
+
+__device__ __forceinline__ void kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& epi) {
+  const float rz_new = block_total<1>(xl.zr.partials, xl.nz, 0);
+  const Scalars* S = xl.S;
+  const float beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
+  xl.beta = beta;
+  epi.xl.beta = beta;
 }
 
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.

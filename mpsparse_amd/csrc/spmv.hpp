@@ -57,6 +57,13 @@ struct XPlain {
   }
 };
 
+// Per-workgroup set-up of a gather loader and its epilogue, run by every
+// thread before the row loop (a loader whose scalar comes from a grid
+// reduction computes it here). The default does nothing; overloads for
+// specific loader types are found by argument-dependent lookup.
+template <typename XL, typename Epi>
+__device__ __forceinline__ void kernel_prologue(XL&, Epi&) {}
+
 // s = r - alpha * v formed on the fly (used to fuse a vector update into the
 // SpMV that consumes it); the owning rows also get s written by the epilogue.
 struct XAxpy {
@@ -299,6 +306,7 @@ template <typename V, int ND, typename XL, typename Epi>
 __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, GridRed rb,
                                                  const int* __restrict__ done) {
   if (done && *done) return;
+  kernel_prologue(xl, epi);
   __shared__ float sprod[kCap];
   constexpr int NA = ND > 0 ? ND : 1;
   CSum acc[NA];

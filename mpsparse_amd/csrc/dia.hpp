@@ -31,58 +31,83 @@ struct DiaView {
   DiaOffsets o;
 };
 
-// One row per thread, grid-stride (measured faster than contiguous per-XCD
-// chunks at 256^3 and within 10% at 128^3), plain coalesced loads (a
-// non-temporal hint on these 4-byte lane loads more than doubled the time);
-// partial sums of the epilogue go to rb. Every load of a row is issued before
-// any arithmetic on loaded data (value widening, the gather's own arithmetic):
-// arithmetic under the per-diagonal branch makes each diagonal wait for its own
-// loads, leaving a row one value/x pair in flight (2.5 TB/s on the 7-point
-// operator at 256^3). Columns outside the
+// Grid-stride over groups of U * kBlock consecutive rows (measured faster
+// than contiguous per-XCD chunks at 256^3 and within 10% at 128^3); a thread
+// takes rows tid, tid + kBlock, ... of its workgroup's group (U rows a full
+// grid stride apart were slower: the grid then streams U distant regions at
+// once). Plain coalesced loads (a non-temporal hint on these 4-byte lane
+// loads more than doubled the time); partial sums of the epilogue go to rb.
+// Every load of a thread's U rows is issued before any arithmetic on loaded
+// data (value widening, the gather's own arithmetic): arithmetic under the
+// per-diagonal branch makes each diagonal wait for its own loads (2.5 TB/s
+// on the 7-point operator at 256^3). Columns outside the
 // matrix read a clamped in-range x and are then left out of the sum, so the
-// result equals the guarded form bit for bit. Requires m > 0.
-template <typename V, int ND, typename XL, typename Epi, int NDIAG>
+// result equals the guarded form bit for bit; rows past the end of the
+// U-group load row i0's data and store nothing. Requires m > 0.
+template <typename V, int ND, typename XL, typename Epi, int NDIAG, int U = 1>
 __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, GridRed rb,
                                                 const int* __restrict__ done) {
   // NDIAG > 0: exactly NDIAG diagonals (no per-diagonal branch; stencils and
   // their triangular factors); 0: A.o.ndiag <= kMaxDiag
   constexpr int DMAX = NDIAG > 0 ? NDIAG : kMaxDiag;
-  if (done && *done) return;
+  // the gate is tested after the prologue (whose loads overlap its latency)
+  const int halt = done ? *done : 0;
   kernel_prologue(xl, epi);
+  if (halt) return;
   constexpr int NA = ND > 0 ? ND : 1;
   CSum acc[NA];
 #pragma unroll
   for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
   const int nd = NDIAG > 0 ? NDIAG : A.o.ndiag;
   const int64_t m = A.m;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += gridDim.x * kBlock) {
-    // 16-bit values held one per register (packing two per register would
-    // wait for the loads)
-    using Held = typename std::conditional<sizeof(typename V::storage) == 2, uint32_t, float>::type;
-    Held a[DMAX];
-    typename XL::Raw xr[DMAX];
+  // a workgroup's U-group is U contiguous runs of kBlock rows
+  const uint64_t stride = kBlock;
+  const uint64_t gstride = (uint64_t)gridDim.x * kBlock * U;
+  // 16-bit values held one per register (packing two per register would
+  // wait for the loads)
+  using Held = typename std::conditional<sizeof(typename V::storage) == 2, uint32_t, float>::type;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * kBlock * U + threadIdx.x; i0 < A.n; i0 += gstride) {
+    Held a[U][DMAX];
+    typename XL::Raw xr[U][DMAX];
+    [[maybe_unused]] float dl[U];
+    [[maybe_unused]] typename XL::Raw xd[U];
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d) {
-      a[d] = {};
-      xr[d] = {};
-      if (d < nd) {  // loads only: no arithmetic on loaded data before the last load
-        const int64_t c = (int64_t)i + A.o.off[d];
-        a[d] = A.val[d * A.ld + i];
-        xr[d] = xl.raw((uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c)));
+    for (int u = 0; u < U; ++u) {
+      const uint64_t iu = i0 + (uint64_t)u * stride;
+      const uint64_t i = iu < A.n ? iu : i0;
+#pragma unroll
+      for (int d = 0; d < DMAX; ++d) {
+        a[u][d] = {};
+        xr[u][d] = {};
+        if (d < nd) {  // loads only: no arithmetic on loaded data before the last load
+          const int64_t c = (int64_t)i + A.o.off[d];
+          a[u][d] = A.val[d * A.ld + i];
+          xr[u][d] = xl.raw((uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c)));
+        }
+      }
+      if constexpr (V::kLowDiag) {
+        dl[u] = A.dlo[i];
+        xd[u] = xl.raw((uint32_t)((int64_t)i < m ? (int64_t)i : m - 1));
       }
     }
-    float s = 0.0f;
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d) {
-      if (d < nd) {
-        const int64_t c = (int64_t)i + A.o.off[d];
-        s = (c >= 0 && c < m) ? s + V::cvt1((typename V::storage)a[d]) * xl.make(xr[d]) : s;
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + (uint64_t)u * stride;
+      if (i < A.n) {
+        float s = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DMAX; ++d) {
+          if (d < nd) {
+            const int64_t c = (int64_t)i + A.o.off[d];
+            s = (c >= 0 && c < m) ? s + V::cvt1((typename V::storage)a[u][d]) * xl.make(xr[u][d]) : s;
+          }
+        }
+        if constexpr (V::kLowDiag) {
+          if ((int64_t)i < m) s += dl[u] * xl.make(xd[u]);
+        }
+        epi.row((uint32_t)i, s, acc);
       }
     }
-    if constexpr (V::kLowDiag) {
-      if (i < A.m) s += A.dlo[i] * xl(i);
-    }
-    epi.row(i, s, acc);
   }
   if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
 }

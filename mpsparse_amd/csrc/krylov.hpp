@@ -280,6 +280,40 @@ __device__ __forceinline__ float block_total(const CSum* __restrict__ partials, 
   return t;
 }
 
+// block_total of two single-value partial sets (pa: na blocks, pb: nb
+// blocks) in one pass: the loads of both are issued before the first merge.
+// Each total has the bits block_total<1> gives it.
+__device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32_t na, const CSum* __restrict__ pb,
+                                             uint32_t nb, float& ta, float& tb) {
+  __shared__ float s_tot[2];
+  CSum a[2] = {csum_zero(), csum_zero()};
+  const uint32_t nmax = na > nb ? na : nb;
+  for (uint32_t base = 0; base < nmax; base += kPartialBatch * kBlock) {
+    CSum va[kPartialBatch], vb[kPartialBatch];
+#pragma unroll
+    for (int j = 0; j < kPartialBatch; ++j) {
+      const uint32_t i = base + j * kBlock + threadIdx.x;
+      va[j] = i < na ? pa[i] : csum_zero();
+      vb[j] = i < nb ? pb[i] : csum_zero();
+    }
+#pragma unroll
+    for (int j = 0; j < kPartialBatch; ++j) {
+      const uint32_t i = base + j * kBlock + threadIdx.x;
+      a[0] = i < na ? csum_merge(a[0], va[j]) : a[0];
+      a[1] = i < nb ? csum_merge(a[1], vb[j]) : a[1];
+    }
+  }
+  block_reduce<2>(a);
+  if (threadIdx.x == 0) {
+    s_tot[0] = csum_val(a[0]);
+    s_tot[1] = csum_val(a[1]);
+  }
+  __syncthreads();
+  ta = s_tot[0];
+  tb = s_tot[1];
+  __syncthreads();
+}
+
 // Explicitly preconditioned CG direction formed in the gather of q = A p:
 // p = z + beta p_old with beta = <r,z>_new / <r,z>_old, <r,z>_new from the
 // block partials of the z product (zr, nz blocks) reduced by every workgroup
@@ -332,16 +366,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
                                                          uint32_t n, Scalars* S, GridRed rb, GridRed pq,
                                                          uint32_t npq, GridRed zr = GridRed{}, uint32_t nz = 0) {
-  if (S->halt) return;
+  // the gate is tested after the partials are in (their loads overlap its
+  // latency); a halted step still writes nothing
+  const int32_t halt = S->halt;
+  if (ALPHA == kAlphaState && halt) return;
   float alpha = S->alpha;
   if constexpr (ALPHA != kAlphaState) {
-    const float pqv = block_total<1>(pq.partials, npq, 0);
-    float rz = S->rz;
+    float pqv, rz = S->rz;
+    if constexpr (ALPHA == kAlphaRzPq) block_total2(pq.partials, npq, zr.partials, nz, pqv, rz);
+    else pqv = block_total<1>(pq.partials, npq, 0);
+    if (halt) return;
     bool ok = finite_pos(pqv);
-    if constexpr (ALPHA == kAlphaRzPq) {
-      rz = block_total<1>(zr.partials, nz, 0);
-      ok = ok && finite_pos(rz);
-    }
+    if constexpr (ALPHA == kAlphaRzPq) ok = ok && finite_pos(rz);
     alpha = ok ? rz / pqv : 0.0f;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       S->pq = ok ? pqv : -1.0f;
@@ -439,10 +475,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(float* pnew, const float
                                                         const float* __restrict__ dinv, const float* __restrict__ z,
                                                         uint32_t n, const Scalars* S, GridRed zr = GridRed{},
                                                         uint32_t nz = 0) {
-  if (S->halt) return;
+  const int32_t halt = S->halt;
+  if (!BETA_LOCAL && halt) return;
   float beta = S->beta;
   if constexpr (BETA_LOCAL) {
-    const float rz_new = block_total<1>(zr.partials, nz, 0);
+    const float rz_new = block_total<1>(zr.partials, nz, 0);  // loads overlap the gate's latency
+    if (halt) return;
     beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
   }
   const uint32_t nc = vec_chunks(n);

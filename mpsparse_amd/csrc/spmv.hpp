@@ -305,8 +305,10 @@ struct FinStore {
 template <typename V, int ND, typename XL, typename Epi>
 __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, GridRed rb,
                                                  const int* __restrict__ done) {
-  if (done && *done) return;
+  // the gate is tested after the prologue (whose loads overlap its latency)
+  const int halt = done ? *done : 0;
   kernel_prologue(xl, epi);
+  if (halt) return;
   __shared__ float sprod[kCap];
   constexpr int NA = ND > 0 ? ND : 1;
   CSum acc[NA];

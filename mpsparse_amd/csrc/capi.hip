@@ -2155,41 +2155,54 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   GridRed rb = w.gr();
   const uint32_t n = A->n, gv = vec_grid(n);
   const bool comp = compensate(o);
+  // ping-pong directions and residuals: step k reads Pb[k & 1], Rb[k & 1]
+  // and writes Pb[(k + 1) & 1], Rb[(k + 1) & 1] (each is formed in a gather,
+  // so the vector being written cannot be the one other workgroups gather)
+  float* Pb[2] = {P, w.v(Workspace::T)};
+  float* Rb[2] = {R, w.v(Workspace::R0)};
+  int k = 0;
+  GridRed rb2 = w.gr2(), zr = w.gr3();
+  // w = G r of the current residual (gated: skipped when *gate != 0)
+  auto g_of_r = [&](const int* gate) {
+    factor_spmv<0>(g16, &F->G, XPlain{Rb[k & 1]}, EpiStore{Wv}, rb, gate, s);
+  };
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
     k_cg_init<<<gv, kBlock, 0, s>>>(X, XLo, R, P, B, nullptr, n, o.use_x0, kPcExplicit, S, rb);
     finish<3>(FinCgInit{S}, rb, gv, s);
     k_set_restart<<<1, 1, 0, s>>>(S);
+    g_of_r(&S->halt);
     return launch_check("pcg init");
   };
-  // one step, five launches: w = G r; z = G^T w with <r,z> partials (zr);
-  // q = A p with p = z + beta p_old formed in the gather (beta from zr) and
-  // <p,q> partials (rb); alpha from zr and rb inside the x/r update (partials
-  // rb2); FinCgStep
-  GridRed rb2 = w.gr2(), zr = w.gr3();
-  // ping-pong directions: step k reads Pb[k & 1] and writes Pb[(k + 1) & 1]
-  // (the direction is formed in the gather of q = A p, so the p being written
-  // cannot be the one other workgroups gather)
-  float* Pb[2] = {P, w.v(Workspace::T)};
-  int k = 0;
+  // one step, four launches: z = G^T w with <r,z> partials (zr); q = A p with
+  // p = z + beta p_old formed in the gather (beta from zr) and <p,q> partials
+  // (rb); the x / r update with alpha from zr and rb fused with the next
+  // step's w = G r (r formed in the gather, <r,r> partials rb2); FinCgStep
   auto step = [&]() {
     const float* pold = Pb[k & 1];
     float* pnew = Pb[(k + 1) & 1];
-    factor_spmv<0>(g16, &F->G, XPlain{R}, EpiStore{Wv}, rb, &S->halt, s);
-    const uint32_t gz = factor_spmv<1>(g16, &F->GT, XPlain{Wv}, EpiStoreDot{Z, R}, zr, &S->halt, s);
+    const float* rcur = Rb[k & 1];
+    float* rnext = Rb[(k + 1) & 1];
+    const uint32_t gz = factor_spmv<1>(g16, &F->GT, XPlain{Wv}, EpiStoreDot{Z, rcur}, zr, &S->halt, s);
     const uint32_t gq = op.pcg_dir(Z, pold, S, zr, gz, Q, pnew, &S->halt, s);
+    uint32_t gu;
     if (comp)
-      k_cg_update_xr<kPcExplicit, kAlphaRzPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, nullptr, n, S, rb2,
-                                                                          rb, gq, zr, gz);
+      gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
+                          EpiCgXrW<true>{X, XLo, rcur, rnext, pnew, Q, Wv, S, rb, gq, zr, gz, 0.0f, 0}, rb2,
+                          &S->halt, s);
     else
-      k_cg_update_xr<kPcExplicit, kAlphaRzPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, nullptr, n, S, rb2,
-                                                                           rb, gq, zr, gz);
-    finish<2>(FinCgStep<kPcExplicit>{S}, rb2, gv, s);
+      gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
+                          EpiCgXrW<false>{X, XLo, rcur, rnext, pnew, Q, Wv, S, rb, gq, zr, gz, 0.0f, 0}, rb2,
+                          &S->halt, s);
+    finish<2>(FinCgStep<kPcExplicit>{S}, rb2, gu, s);
     ++k;
   };
+  // a verification rewrites the current residual; after a restart w = G r is
+  // recomputed (gated by skip_g, which FinVerify clears only then)
   auto verify = [&]() {
-    verify_launch<3>(op, gv, X, XLo, B, R, P, nullptr, nullptr, n, S, rb, s);
+    verify_launch<3>(op, gv, X, XLo, B, Rb[k & 1], P, nullptr, nullptr, n, S, rb, s);
     finish<2>(FinVerify<3>{S}, rb, gv, s);
+    g_of_r(&S->skip_g);
   };
   MSPCHK(run_solver(A, o, info, s, init, step, verify));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
   constexpr int DMAX = NDIAG > 0 ? NDIAG : kMaxDiag;
   // the gate is tested after the prologue (whose loads overlap its latency)
   const int halt = done ? *done : 0;
-  kernel_prologue(xl, epi);
+  kernel_prologue(xl, epi, halt);
   if (halt) return;
   constexpr int NA = ND > 0 ? ND : 1;
   CSum acc[NA];
@@ -71,6 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
     typename XL::Raw xr[U][DMAX];
     [[maybe_unused]] float dl[U];
     [[maybe_unused]] typename XL::Raw xd[U];
+    [[maybe_unused]] typename EpiPre<Epi>::type pre[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t iu = i0 + (uint64_t)u * stride;
@@ -89,6 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
         dl[u] = A.dlo[i];
         xd[u] = xl.raw((uint32_t)((int64_t)i < m ? (int64_t)i : m - 1));
       }
+      if constexpr (EpiPre<Epi>::has) pre[u] = epi.pre((uint32_t)i);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -105,7 +107,8 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
         if constexpr (V::kLowDiag) {
           if ((int64_t)i < m) s += dl[u] * xl.make(xd[u]);
         }
-        epi.row((uint32_t)i, s, acc);
+        if constexpr (EpiPre<Epi>::has) epi.row((uint32_t)i, s, acc, pre[u]);
+        else epi.row((uint32_t)i, s, acc);
       }
     }
   }

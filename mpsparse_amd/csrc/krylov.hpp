@@ -57,7 +57,7 @@ struct Scalars {
   int32_t collapse;  // fold xl into x at the next update (after a restart)
   int32_t restart;   // next preconditioned direction update uses beta = 0
   int32_t halt;      // done || verify: the step kernels do nothing (kept by k_finish)
-  int32_t pad;
+  int32_t skip_g;    // explicit preconditioner: w = G r of the current residual is up to date
 };
 
 // k_finish hook: after a finisher that owns the solver state, refresh the
@@ -191,6 +191,14 @@ struct EpiCgQ {
   float* __restrict__ q;
   float* __restrict__ pnew;
   XL xl;
+  typedef typename RawOf<XL>::type Pre;  // operands prefetched by the diagonal kernel
+  __device__ __forceinline__ Pre pre(uint32_t i) const { return xl.raw(i); }
+  __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc, Pre w) const {
+    const float pi = xl.make(w);
+    pnew[i] = pi;
+    q[i] = s;
+    acc[0] = csum_add(acc[0], pi * s);
+  }
   __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc) const {
     const float pi = xl(i);
     pnew[i] = pi;
@@ -257,6 +265,7 @@ struct FinCgInit {
     S->beta = 0.0f;
     S->alpha = 0.0f;
     S->omega = 0.0f;
+    S->skip_g = 1;
     int32_t d = kRunning;
     if (S->crit == kCritRelres && tot[1] <= S->thresh) d = kConverged;
     if (!isfinite(tot[0]) || !isfinite(tot[1])) d = kBreakdown;
@@ -339,12 +348,101 @@ struct XPzLocal {
   }
 };
 
-__device__ __forceinline__ void kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& epi) {
+__device__ __forceinline__ void kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& epi, int) {
   const float rz_new = block_total<1>(xl.zr.partials, xl.nz, 0);
   const Scalars* S = xl.S;
   const float beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
   xl.beta = beta;
   epi.xl.beta = beta;
+}
+
+// The x / r update of an explicitly preconditioned CG step fused with the
+// next step's first factor product w = G r_new: r_new = r - alpha q is formed
+// in the gather (r_new goes to the other residual buffer, so every workgroup
+// gathers the step's r), alpha = <r,z> / <p,q> from the block partials of the
+// z product (zr, nz blocks) and of q = A p (pq, npq blocks), reduced by every
+// workgroup in kernel_prologue with the rule of k_cg_update_xr<kPcExplicit,
+// kAlphaRzPq, COMP> (workgroup 0 records pq, alpha, <r,z>, clears the restart
+// flag and marks w current). The epilogue updates the owned rows (x, x_lo
+// when COMP, r_new, w) and accumulates <r_new, r_new> (second partial unused).
+struct XRNew {
+  const float* __restrict__ r;
+  const float* __restrict__ q;
+  float alpha;
+  struct Raw {
+    float r, q;
+  };
+  __device__ __forceinline__ Raw raw(uint32_t c) const { return Raw{r[c], q[c]}; }
+  __device__ __forceinline__ float make(Raw w) const { return w.r - alpha * w.q; }
+  __device__ __forceinline__ float operator()(uint32_t c) const { return r[c] - alpha * q[c]; }
+  __device__ __forceinline__ f4 load4(u4 c) const {
+    if (quad_run(c))
+      return *reinterpret_cast<const f4*>(r + c.x) - alpha * *reinterpret_cast<const f4*>(q + c.x);
+    return f4{(*this)(c.x), (*this)(c.y), (*this)(c.z), (*this)(c.w)};
+  }
+};
+
+template <bool COMP>
+struct EpiCgXrW {
+  float* __restrict__ x;
+  float* __restrict__ xl;
+  const float* __restrict__ rold;
+  float* __restrict__ rnew;
+  const float* __restrict__ p;
+  const float* __restrict__ q;
+  float* __restrict__ w;
+  Scalars* S;
+  GridRed pq;
+  uint32_t npq;
+  GridRed zr;
+  uint32_t nz;
+  float alpha;
+  int fold;
+  struct Pre {
+    float x, xl, p, r, q;
+  };
+  __device__ __forceinline__ Pre pre(uint32_t i) const {
+    return Pre{x[i], COMP ? xl[i] : 0.0f, p[i], rold[i], q[i]};
+  }
+  __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc, Pre v) const {
+    float xi = v.x;
+    if constexpr (COMP) {
+      float li = v.xl;
+      if (fold) {
+        xi += li;
+        li = 0.0f;
+      }
+      comp_axpy(xi, li, alpha, v.p);
+      xl[i] = li;
+    } else {
+      xi += alpha * v.p;
+    }
+    x[i] = xi;
+    const float ri = v.r - alpha * v.q;
+    rnew[i] = ri;
+    w[i] = s;
+    acc[0] = csum_add(acc[0], ri * ri);
+  }
+  __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc) const { row(i, s, acc, pre(i)); }
+};
+
+template <bool COMP>
+__device__ __forceinline__ void kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, int halt) {
+  float pqv, rz;
+  block_total2(epi.pq.partials, epi.npq, epi.zr.partials, epi.nz, pqv, rz);
+  Scalars* S = epi.S;
+  const bool ok = finite_pos(pqv) && finite_pos(rz);
+  const float alpha = ok ? rz / pqv : 0.0f;
+  xl.alpha = alpha;
+  epi.alpha = alpha;
+  epi.fold = S->collapse != 0;
+  if (!halt && blockIdx.x == 0 && threadIdx.x == 0) {
+    S->pq = ok ? pqv : -1.0f;
+    S->alpha = alpha;
+    S->rz = rz;
+    S->restart = 0;
+    S->skip_g = 1;
+  }
 }
 
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
@@ -732,7 +830,11 @@ __global__ __launch_bounds__(kBlock) void k_verify_lanes(RowsCsr rows, const flo
 template <int MODE>
 struct FinVerify {
   Scalars* S;
-  __device__ __forceinline__ void after() const { refresh_halt(S); }
+  // explicit preconditioner: a restart replaced r, so w = G r is recomputed
+  __device__ __forceinline__ void after() const {
+    refresh_halt(S);
+    if (MODE == 3) S->skip_g = (S->halt == 0 && S->restart != 0) ? 0 : 1;
+  }
   __device__ __forceinline__ bool active() const { return S->done == 0 && S->verify != 0; }
   __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
     const float best = S->nverify > 0 ? S->rr_best : tot[0] / 0.81f;

@@ -18,6 +18,8 @@
 // XCD-contiguous order.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace msp {
@@ -59,10 +61,12 @@ struct XPlain {
 
 // Per-workgroup set-up of a gather loader and its epilogue, run by every
 // thread before the row loop (a loader whose scalar comes from a grid
-// reduction computes it here). The default does nothing; overloads for
-// specific loader types are found by argument-dependent lookup.
+// reduction computes it here); `halt` is the kernel's gate (nonzero: the
+// kernel will return after the prologue, which must then write nothing). The
+// default does nothing; overloads for specific loader types are found by
+// argument-dependent lookup.
 template <typename XL, typename Epi>
-__device__ __forceinline__ void kernel_prologue(XL&, Epi&) {}
+__device__ __forceinline__ void kernel_prologue(XL&, Epi&, int) {}
 
 // s = r - alpha * v formed on the fly (used to fuse a vector update into the
 // SpMV that consumes it); the owning rows also get s written by the epilogue.
@@ -233,14 +237,42 @@ struct EpiStore {
   __device__ __forceinline__ void row(uint32_t r, float s, CSum*) const { y[r] = s; }
 };
 
+// Epilogues may declare `Pre`, the per-row operands they read, with
+// `pre(i)` loading them and `row(i, s, acc, pre)` using them: the diagonal
+// kernel then issues those loads with the row's matrix and vector loads
+// instead of after the sum (a second round trip per row).
+template <typename Epi, typename = void>
+struct EpiPre {
+  struct type {};
+  static constexpr bool has = false;
+};
+template <typename Epi>
+struct EpiPre<Epi, std::void_t<typename Epi::Pre>> {
+  using type = typename Epi::Pre;
+  static constexpr bool has = true;
+};
+
+// A gather loader's Raw type (float for loaders without one).
+template <typename XL, typename = void>
+struct RawOf {
+  typedef float type;
+};
+template <typename XL>
+struct RawOf<XL, std::void_t<typename XL::Raw>> {
+  typedef typename XL::Raw type;
+};
+
 // y = A x and <y, w>
 struct EpiStoreDot {
   float* __restrict__ y;
   const float* __restrict__ w;
-  __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc) const {
+  typedef float Pre;
+  __device__ __forceinline__ Pre pre(uint32_t r) const { return w[r]; }
+  __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc, Pre wr) const {
     y[r] = s;
-    acc[0] = csum_add(acc[0], s * w[r]);
+    acc[0] = csum_add(acc[0], s * wr);
   }
+  __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc) const { row(r, s, acc, pre(r)); }
 };
 
 // t = A s with s formed on the fly (XAxpy); also writes s for owned rows and
@@ -307,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, G
                                                  const int* __restrict__ done) {
   // the gate is tested after the prologue (whose loads overlap its latency)
   const int halt = done ? *done : 0;
-  kernel_prologue(xl, epi);
+  kernel_prologue(xl, epi, halt);
   if (halt) return;
   __shared__ float sprod[kCap];
   constexpr int NA = ND > 0 ? ND : 1;

@@ -154,9 +154,11 @@ void msp_default_opts(msp_solve_opts *opts);
 int32_t msp_device_count(void);
 /* Temporaries of matrix builds (sort buffers, probe counters) come from a
  * library-owned scratch arena per (device, stream) that is kept for the next
- * call on that stream. This waits for each device with an idle arena and
- * returns its memory to the system (arenas in use by a call in progress are
- * kept). A stream handle must not be destroyed and recreated while work that
+ * call on that stream, and the blocks of destroyed matrices, preconditioners
+ * and workspaces are kept in a per-device cache (at most 8 GiB per device)
+ * for later allocations of about the same size. This returns the cached
+ * blocks, and waits for each device with an idle arena and returns its memory
+ * to the system (arenas in use by a call in progress are kept). A stream handle must not be destroyed and recreated while work that
  * used its arena is still running. No reference counterpart (the reference
  * allocates and frees Metal buffers per call, iterative/BiCGSTAB/spmv.cpp:62-228). */
 msp_status msp_release_workspace(void);

@@ -987,8 +987,15 @@ template <typename V, int ND, typename XL, typename Epi, int NDIAG>
 static uint32_t dia_launch(msp_csr A, XL xl, Epi epi, GridRed rb, const int* done, hipStream_t s) {
   constexpr int U = dia_rows_per_thread<XL, NDIAG>();
   static const uint32_t per_cu = resident_per_cu(k_dia<V, ND, XL, Epi, NDIAG, U>);
-  const uint32_t g =
-      std::max<uint32_t>(1, std::min<uint32_t>((A->n + kBlock - 1) / kBlock, per_cu * (uint32_t)cu_count()));
+  // as many grid-stride passes as residency requires, all equally full: a
+  // resident-sized grid left the last pass mostly empty (2.1M rows at U = 4
+  // over 1792 workgroups: 1.83M rows, then 0.26M) at the cost of a full round
+  // trip
+  const uint64_t per_wg = (uint64_t)kBlock * U;
+  const uint64_t need = ((uint64_t)A->n + per_wg - 1) / per_wg;
+  const uint64_t cap = (uint64_t)per_cu * (uint32_t)cu_count();
+  const uint64_t passes = (need + cap - 1) / cap;
+  const uint32_t g = (uint32_t)std::max<uint64_t>(1, (need + passes - 1) / passes);
   k_dia<V, ND, XL, Epi, NDIAG, U><<<g, kBlock, 0, s>>>(dia_view_of<V>(A), xl, epi, rb, done);
   return g;
 }

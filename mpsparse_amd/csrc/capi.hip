@@ -427,7 +427,9 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
 // diagonals that are at least 60% full: candidate offsets come from a few
 // interior sample rows, then one pass over all entries confirms that none lies
 // elsewhere. Probed once per matrix (its pattern never changes).
-static msp_status ensure_dia(msp_csr A, hipStream_t s) {
+// fill = false: the values are set later (refresh_dia fills the storage then),
+// so only the pattern probe and the allocation happen here.
+static msp_status ensure_dia(msp_csr A, hipStream_t s, bool fill = true) {
   if (A->dia_state != 0) return MSP_OK;
   A->dia_state = 2;
   if (A->format != MSP_FORMAT_AUTO || A->n == 0 || A->m == 0 || A->nnz == 0 || A->max_row > (uint32_t)kMaxDiag)
@@ -486,10 +488,12 @@ static msp_status ensure_dia(msp_csr A, hipStream_t s) {
   A->dia_ld = ((uint64_t)n + 63) / 64 * 64;
   const size_t bytes = sizeof(float) * A->dia_ld * o.ndiag;
   MSPCHK(A->dia.alloc(bytes));
-  HIPCHK(hipMemsetAsync(A->dia.p, 0, bytes, s));
-  k_dia_fill<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n,
-                                            o, A->dia_ld, A->dia.as<float>());
-  MSPCHK(launch_check("k_dia_fill"));
+  if (fill) {
+    HIPCHK(hipMemsetAsync(A->dia.p, 0, bytes, s));
+    k_dia_fill<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
+                                              n, o, A->dia_ld, A->dia.as<float>());
+    MSPCHK(launch_check("k_dia_fill"));
+  }
   A->dia_o = o;
   A->dia_dd = dd;
   A->dia_state = 1;
@@ -579,7 +583,8 @@ static msp_status zero_pad(msp_csr A, hipStream_t s) {
 // COO -> CSR driver
 template <typename K>
 static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t nnz, sortk::KeyMap km, int passes,
-                              uint32_t* col_out, float* val_out, uint32_t* rows_out, int* err, hipStream_t s) {
+                              int shift0, uint32_t* col_out, float* val_out, uint32_t* rows_out, int* err,
+                              hipStream_t s) {
   using namespace sortk;
   const uint32_t num_tiles = (uint32_t)((nnz + kTile - 1) / kTile);
   const uint64_t N = (uint64_t)kRadix * num_tiles;
@@ -600,7 +605,7 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
   const K* kin = nullptr;
   const float* vin = vals;
   for (int p = 0; p < passes; ++p) {
-    const int shift = p * kRadixBits;
+    const int shift = shift0 + p * kRadixBits;
     const bool first = p == 0, last = p == passes - 1;
     if (first) k_upsweep<uint64_t, true><<<num_tiles, kBlock, 0, s>>>(keys, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
     else k_upsweep<K, false><<<num_tiles, kBlock, 0, s>>>(kin, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
@@ -629,10 +634,13 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
 }
 
 // Sort (keys, vals) on the device into CSR arrays. `row_ptr` has n+1 entries,
-// `col_out`/`val_out` nnz entries.
+// `col_out`/`val_out` nnz entries. cols_in_order: the input already lists each
+// row's entries in ascending column order (a transposed CSR pattern, say), so
+// the stable passes over the row bits alone give the full order (half the
+// passes for 2M x 2M).
 static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n, uint32_t m,
                                     uint32_t* row_ptr, uint32_t* col_out, float* val_out, hipStream_t s,
-                                    uint32_t* rows_out = nullptr) {
+                                    uint32_t* rows_out = nullptr, bool cols_in_order = false) {
   TmpMem err, rows_own;
   MSPCHK(err.alloc(sizeof(int), s));
   HIPCHK(hipMemsetAsync(err.p, 0, sizeof(int), s));
@@ -644,15 +652,20 @@ static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uin
     km.cmask = km.cbits ? ((1ull << km.cbits) - 1ull) : 0ull;
     const uint32_t rbits = n > 1 ? bitlen(n - 1) : 0;
     const uint32_t tb = km.cbits + rbits;
-    const int passes = std::max(1, (int)((tb + sortk::kRadixBits - 1) / sortk::kRadixBits));
+    const uint32_t sorted_bits = cols_in_order ? rbits : tb;
+    const int shift0 = cols_in_order ? (int)km.cbits : 0;
+    const int passes = std::max(1, (int)((sorted_bits + sortk::kRadixBits - 1) / sortk::kRadixBits));
     if (!rows_out) {
       MSPCHK(rows_own.alloc(sizeof(uint32_t) * nnz, s));
       rows_out = rows_own.as<uint32_t>();
     }
-    if (tb <= 32)
-      MSPCHK(sort_passes<uint32_t>(keys, vals, nnz, km, passes, col_out, val_out, rows_out, err.as<int>(), s));
+    // 32-bit keys while the compacted key and every pass's shift fit them
+    if (tb <= 32 && shift0 + sortk::kRadixBits * (passes - 1) < 32)
+      MSPCHK(sort_passes<uint32_t>(keys, vals, nnz, km, passes, shift0, col_out, val_out, rows_out, err.as<int>(),
+                                   s));
     else
-      MSPCHK(sort_passes<uint64_t>(keys, vals, nnz, km, passes, col_out, val_out, rows_out, err.as<int>(), s));
+      MSPCHK(sort_passes<uint64_t>(keys, vals, nnz, km, passes, shift0, col_out, val_out, rows_out, err.as<int>(),
+                                   s));
   }
   uint64_t work = std::max<uint64_t>(nnz, (uint64_t)n + 1);
   uint32_t g = (uint32_t)std::min<uint64_t>((work + kBlock - 1) / kBlock, 65536);
@@ -1909,18 +1922,22 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     MSPCHK(launch_check("transpose keys"));
     MSPCHK(alloc_csr(&F->GT));
     MSPCHK(F->GTmap.alloc(sizeof(float) * (ne + 64)));
+    // (G's edges are listed row by row with ascending columns, so each row of
+    // G^T receives its entries in ascending column order: sort by row only)
     MSPCHK(coo_to_csr_device(keys.as<uint64_t>(), payload.as<float>(), ne, n, n, F->GT.row_ptr.as<uint32_t>(),
-                             F->GT.col.as<uint32_t>(), F->GTmap.as<float>(), s));
+                             F->GT.col.as<uint32_t>(), F->GTmap.as<float>(), s, nullptr, true));
     HIPCHK(hipMemsetAsync(F->GT.val.p, 0, sizeof(float) * (ne + 64), s));
     MSPCHK(zero_pad(&F->GT, s));
     MSPCHK(finalize_matrix(&F->GT, s));
-    MSPCHK(ensure_dia(&F->G, s));
-    MSPCHK(ensure_dia(&F->GT, s));
+    // G's values are set later (fsai_commit refills the diagonal storage)
+    MSPCHK(ensure_dia(&F->G, s, false));
+    MSPCHK(ensure_dia(&F->GT, s, false));
     // features and column scaling
     MSPCHK(F->diag.alloc(sizeof(float) * ((size_t)n + 1)));
     MSPCHK(F->feat.alloc(sizeof(float) * 8 * (ne + 64)));
     MSPCHK(F->colscale.alloc(sizeof(float) * (ne + 64)));
-    HIPCHK(hipMemsetAsync(F->feat.p, 0, sizeof(float) * 8 * (ne + 64), s));
+    // k_fsai_features writes all 8 features of every edge: zero the padding only
+    HIPCHK(hipMemsetAsync(F->feat.as<float>() + 8 * ne, 0, sizeof(float) * 8 * 64, s));
     k_diag<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n,
                                           F->diag.as<float>());
     k_fsai_features<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),

@@ -37,12 +37,23 @@ __global__ __launch_bounds__(kBlock) void k_lower_fill(const uint32_t* __restric
                                                        const uint32_t* __restrict__ col, uint32_t n,
                                                        const uint32_t* __restrict__ Lrp, uint32_t* __restrict__ Lcol,
                                                        uint32_t* __restrict__ Lrow, uint32_t* __restrict__ Lsrc) {
+  // a row's columns are loaded 8 at a time before they are stored (one load
+  // then store per entry waited a memory round trip each)
+  constexpr int kB = 8;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
     const uint32_t a = row_ptr[r], o = Lrp[r], k = Lrp[r + 1] - o;
-    for (uint32_t t = 0; t < k; ++t) {
-      Lcol[o + t] = col[a + t];
-      Lrow[o + t] = r;
-      Lsrc[o + t] = a + t;
+    for (uint32_t t0 = 0; t0 < k; t0 += kB) {
+      uint32_t cv[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) cv[u] = t0 + u < k ? col[a + t0 + u] : 0u;
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        if (t0 + u < k) {
+          Lcol[o + t0 + u] = cv[u];
+          Lrow[o + t0 + u] = r;
+          Lsrc[o + t0 + u] = a + t0 + u;
+        }
+      }
     }
   }
 }
@@ -158,37 +169,70 @@ __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __rest
                                                           const uint32_t* __restrict__ Lcol,
                                                           const uint32_t* __restrict__ Lsrc, const float* __restrict__ diag,
                                                           float* __restrict__ feat, float* __restrict__ colscale) {
+  // A row's entries are read kB at a time: every column / value load of the
+  // batch, then every diagonal gather, then the arithmetic in entry order (a
+  // loop over single entries waited two memory round trips per entry). The
+  // lower edges of row i are its first Lrp[i+1] - Lrp[i] entries (Lsrc, Lcol
+  // restate that); an edge's 8 features are two 16-byte stores.
+  constexpr int kB = 8;
+  (void)Lcol;
+  (void)Lsrc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float aii = diag[i];
+    const uint32_t a0 = row_ptr[i], a1 = row_ptr[i + 1];
+    const uint32_t o = Lrp[i], k = Lrp[i + 1] - o;
     float asum = 0.0f, s_low = 0.0f, s_all = 0.0f;
-    for (uint32_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j) {
-      const float a = val[j];
-      const uint32_t c = col[j];
-      asum += fabsf(a);
-      const float dc = diag[c];
-      if (c != i && aii > 0.0f && dc > 0.0f) {
-        const float q = a * a / (aii * dc);
-        s_all += q;
-        if (c < i) s_low += q;
+    for (uint32_t j0 = a0; j0 < a1; j0 += kB) {
+      uint32_t cv[kB];
+      float av[kB], dv[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const bool in = j0 + u < a1;
+        cv[u] = in ? col[j0 + u] : i;
+        av[u] = in ? val[j0 + u] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < kB; ++u) dv[u] = diag[cv[u]];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        if (j0 + u < a1) {
+          const float a = av[u];
+          const uint32_t c = cv[u];
+          asum += fabsf(a);
+          const float dc = dv[u];
+          if (c != i && aii > 0.0f && dc > 0.0f) {
+            const float q = a * a / (aii * dc);
+            s_all += q;
+            if (c < i) s_low += q;
+          }
+        }
       }
     }
-    const float rn = (float)(row_ptr[i + 1] - row_ptr[i]) * 0.125f;
+    const float rn = (float)(a1 - a0) * 0.125f;
     const float t = 1.0f / sqrtf(fmaxf(1.0f - s_low, 1.0f / 64.0f));
-    for (uint32_t e = Lrp[i]; e < Lrp[i + 1]; ++e) {
-      const uint32_t c = Lcol[e];
-      const float ajj = diag[c];
-      const float aij = val[Lsrc[e]];
-      float* f = feat + (size_t)e * 8;
-      const bool ok = aii > 0.0f && ajj > 0.0f;
-      f[0] = ok ? aij * rsqrtf(aii * ajj) : 0.0f;
-      f[1] = ok ? logf(ajj / aii) : 0.0f;
-      f[2] = rn;
-      f[3] = ok ? asum / aii : 0.0f;
-      f[4] = (c == i) ? 1.0f : 0.0f;
-      f[5] = s_low;
-      f[6] = s_all;
-      f[7] = t;
-      colscale[e] = ajj > 0.0f ? rsqrtf(ajj) : 0.0f;
+    for (uint32_t t0 = 0; t0 < k; t0 += kB) {
+      uint32_t cv[kB];
+      float av[kB], dv[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const bool in = t0 + u < k;
+        cv[u] = in ? col[a0 + t0 + u] : i;
+        av[u] = in ? val[a0 + t0 + u] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < kB; ++u) dv[u] = diag[cv[u]];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        if (t0 + u < k) {
+          const uint32_t e = o + t0 + u, c = cv[u];
+          const float ajj = dv[u], aij = av[u];
+          const bool ok = aii > 0.0f && ajj > 0.0f;
+          f4* f = reinterpret_cast<f4*>(feat + (size_t)e * 8);
+          f[0] = f4{ok ? aij * rsqrtf(aii * ajj) : 0.0f, ok ? logf(ajj / aii) : 0.0f, rn, ok ? asum / aii : 0.0f};
+          f[1] = f4{(c == i) ? 1.0f : 0.0f, s_low, s_all, t};
+          colscale[e] = ajj > 0.0f ? rsqrtf(ajj) : 0.0f;
+        }
+      }
     }
   }
 }

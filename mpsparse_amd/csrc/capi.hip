@@ -2018,8 +2018,9 @@ extern "C" msp_status msp_fsai_set_values(msp_fsai F, const float* g, int32_t* n
   MSPCHK(cnt.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
   MSPCHK(scaled.alloc(sizeof(float) * (F->ne + 1), s));
-  k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(F->G.val.as<float>(), F->colscale.as<float>(), F->feat.as<float>(),
-                                                  F->ne, scaled.as<float>(), cnt.as<uint32_t>());
+  k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(F->G.val.as<float>(), F->colscale.as<float>(),
+                                                  F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), F->ne,
+                                                  scaled.as<float>(), cnt.as<uint32_t>());
   MSPCHK(fsai_commit(F, s));
   uint32_t h = 0;
   HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -2040,8 +2041,8 @@ extern "C" msp_status msp_fsai_set_network_output(msp_fsai F, const float* out, 
   MSPCHK(cnt.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
   k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(static_cast<const float*>(o.d), F->colscale.as<float>(),
-                                                  F->feat.as<float>(), F->ne, F->G.val.as<float>(),
-                                                  cnt.as<uint32_t>());
+                                                  F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), F->ne,
+                                                  F->G.val.as<float>(), cnt.as<uint32_t>());
   MSPCHK(launch_check("fsai scale"));
   MSPCHK(fsai_commit(F, s));
   uint32_t h = 0;

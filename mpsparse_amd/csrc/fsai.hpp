@@ -250,13 +250,16 @@ __global__ __launch_bounds__(kBlock) void k_diag(const uint32_t* __restrict__ ro
 
 // G values from the network output: g_e = out_e * colscale_e; counts diagonal
 // entries that are not positive (the solver then falls back to plain CG).
+// (Diagonal edges are found from the edge's row and column: reading feature 4
+// touched every line of the 32-byte-per-edge feature array.)
 __global__ __launch_bounds__(kBlock) void k_fsai_scale(const float* __restrict__ out, const float* __restrict__ colscale,
-                                                       const float* __restrict__ feat, uint64_t ne,
+                                                       const uint32_t* __restrict__ Lrow,
+                                                       const uint32_t* __restrict__ Lcol, uint64_t ne,
                                                        float* __restrict__ G, uint32_t* __restrict__ nonpos) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
     const float g = out[e] * colscale[e];
     G[e] = g;
-    if (feat[e * 8 + 4] != 0.0f && !(g > 0.0f)) atomicAdd(nonpos, 1u);
+    if (Lrow[e] == Lcol[e] && !(g > 0.0f)) atomicAdd(nonpos, 1u);
   }
 }
 

@@ -125,16 +125,35 @@ __device__ __forceinline__ void forward_tile(const float* __restrict__ feat, uin
     for (int r = 0; r < 16; ++r) H2[t][r] = fmaxf(H2[t][r] + W.bias2[t], 0.0f);
 }
 
-// o[m] for the tile's rows: sum over the 32 lanes of a half-wave of w3[n] H2[m][n].
+// DPP lane moves (v_mov_dpp): quad permutations, mirrors within 8- and
+// 16-lane rows, and row_bcast:15 (lane 15 of each row into the next row,
+// written only where row_mask allows; other lanes get `old` = 0).
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF,
+                                                               false));
+}
+
+// Sum over each 32-lane half of the wave, complete in lanes 16-31 and 48-63:
+// quad swaps (xor 1, xor 2), half-row and row mirrors (8- then 16-lane sums),
+// then row 0 into row 1 and row 2 into row 3 (VALU lane moves: the
+// ds_bpermute shuffles they replace went through LDS).
+__device__ __forceinline__ float half_wave_sum(float v) {
+  v += dpp_mov<0xB1>(v);         // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);         // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);        // row_half_mirror
+  v += dpp_mov<0x140>(v);        // row_mirror
+  v += dpp_mov<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+  return v;
+}
+
+// o[m] for the tile's rows: sum over the 32 lanes of a half-wave of w3[n]
+// H2[m][n]; valid in lanes c >= 16 of each half.
 __device__ __forceinline__ void layer3(const FwdWeights& W, const f16 (&H2)[2], float (&o)[16]) {
 #pragma unroll
   for (int r = 0; r < 16; ++r) o[r] = W.w3[0] * H2[0][r] + W.w3[1] * H2[1][r];
 #pragma unroll
-  for (int off = 1; off < 32; off <<= 1)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) o[r] += __shfl_xor(o[r], off, 64);
-#pragma unroll
-  for (int r = 0; r < 16; ++r) o[r] += W.bias3;
+  for (int r = 0; r < 16; ++r) o[r] = half_wave_sum(o[r]) + W.bias3;
 }
 
 // out[e] = MLP(feat[e]) for all edges. Bounded to 128 registers (4 waves per
@@ -155,7 +174,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_mlp_forward(const float* __restri
     forward_tile(feat, e0, ne, W, stage[wid], H1, H2);
     float o[16];
     layer3(W, H2, o);
-    if (c == 0) {
+    if (c == 31) {
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll

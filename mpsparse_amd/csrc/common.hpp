@@ -61,19 +61,49 @@ __device__ __forceinline__ void comp_axpy(float& x, float& xl, float a, float p)
   xl += e + te;
 }
 
+// DPP lane move (v_mov_dpp): lanes of rows outside ROW_MASK get 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF,
+                                                               false));
+}
+constexpr int kDppXor1 = 0xB1;       // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;  // lane i <- 7 - i within 8 lanes
+constexpr int kDppMirror = 0x140;      // lane i <- 15 - i within 16 lanes
+constexpr int kDppBcast15 = 0x142;     // row r <- lane 15 of row r - 1
+constexpr int kDppBcast31 = 0x143;     // rows 2, 3 <- lane 31
+
+// Wave-wide sums by DPP lane moves (VALU; the ds_bpermute shuffles they
+// replaced each took an LDS round trip): after the quad, half-row and row
+// steps every lane holds its 16-lane row's sum; row_bcast:15 adds row 0 into
+// row 1 and row 2 into row 3, row_bcast:31 adds rows 0-1 into row 3, whose
+// lane 63 then holds the wave's total, read out to every lane. The tree is
+// fixed and csum_merge is symmetric, so every wave (and every workgroup)
+// gets the same bits for the same inputs.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ CSum csum_dpp(CSum v) {
+  return csum_merge(v, CSum{dpp_mov<CTRL, ROW_MASK>(v.s), dpp_mov<CTRL, ROW_MASK>(v.c)});
+}
 __device__ __forceinline__ CSum wave_reduce(CSum v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    CSum o{__shfl_xor(v.s, off, kWave), __shfl_xor(v.c, off, kWave)};
-    v = csum_merge(v, o);
-  }
-  return v;
+  v = csum_dpp<kDppXor1>(v);
+  v = csum_dpp<kDppXor2>(v);
+  v = csum_dpp<kDppHalfMirror>(v);
+  v = csum_dpp<kDppMirror>(v);
+  v = csum_dpp<kDppBcast15, 0xA>(v);
+  v = csum_dpp<kDppBcast31, 0xC>(v);
+  return CSum{__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.s), kWave - 1)),
+              __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.c), kWave - 1))};
 }
 
 __device__ __forceinline__ float wave_reduce_sum(float v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+  v += dpp_mov<kDppXor1>(v);
+  v += dpp_mov<kDppXor2>(v);
+  v += dpp_mov<kDppHalfMirror>(v);
+  v += dpp_mov<kDppMirror>(v);
+  v += dpp_mov<kDppBcast15, 0xA>(v);
+  v += dpp_mov<kDppBcast31, 0xC>(v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), kWave - 1));
 }
 
 // Sum over groups of L consecutive lanes (L a power of two <= 64); every lane

@@ -125,25 +125,16 @@ __device__ __forceinline__ void forward_tile(const float* __restrict__ feat, uin
     for (int r = 0; r < 16; ++r) H2[t][r] = fmaxf(H2[t][r] + W.bias2[t], 0.0f);
 }
 
-// DPP lane moves (v_mov_dpp): quad permutations, mirrors within 8- and
-// 16-lane rows, and row_bcast:15 (lane 15 of each row into the next row,
-// written only where row_mask allows; other lanes get `old` = 0).
-template <int CTRL, int ROW_MASK = 0xF>
-__device__ __forceinline__ float dpp_mov(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF,
-                                                               false));
-}
-
 // Sum over each 32-lane half of the wave, complete in lanes 16-31 and 48-63:
 // quad swaps (xor 1, xor 2), half-row and row mirrors (8- then 16-lane sums),
 // then row 0 into row 1 and row 2 into row 3 (VALU lane moves: the
 // ds_bpermute shuffles they replace went through LDS).
 __device__ __forceinline__ float half_wave_sum(float v) {
-  v += dpp_mov<0xB1>(v);         // quad_perm [1,0,3,2]
-  v += dpp_mov<0x4E>(v);         // quad_perm [2,3,0,1]
-  v += dpp_mov<0x141>(v);        // row_half_mirror
-  v += dpp_mov<0x140>(v);        // row_mirror
-  v += dpp_mov<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+  v += dpp_mov<kDppXor1>(v);
+  v += dpp_mov<kDppXor2>(v);
+  v += dpp_mov<kDppHalfMirror>(v);
+  v += dpp_mov<kDppMirror>(v);
+  v += dpp_mov<kDppBcast15, 0xA>(v);  // rows 1 and 3 only
   return v;
 }
 

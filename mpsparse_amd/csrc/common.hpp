@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace msp {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -188,20 +191,57 @@ __device__ __forceinline__ void merge_partials(const CSum* __restrict__ p, uint3
 // Finisher protocol: `active()` repeats the producer's skip test (a producer
 // that returned early wrote no partials), `operator()(tot)` then `after()` run
 // in thread 0.
+// Finishers that own a solver state (a member `S`) run on an LDS copy of it:
+// its words are loaded with the partials, so the two latencies overlap, and
+// stored back after the bookkeeping (the state's own loads used to wait until
+// after the reduction).
+template <typename Fin, typename = void>
+struct FinHasState : std::false_type {};
+template <typename Fin>
+struct FinHasState<Fin, std::void_t<decltype(std::declval<Fin&>().S)>> : std::true_type {};
+
 template <int ND, typename Fin>
 __global__ __launch_bounds__(kBlock) void k_finish(GridRed rb, uint32_t nb, Fin fin) {
-  if (!fin.active()) return;
-  CSum a[ND];
+  if constexpr (FinHasState<Fin>::value) {
+    using St = typename std::remove_reference<decltype(*fin.S)>::type;
+    constexpr int kWords = (int)(sizeof(St) / sizeof(uint32_t));
+    static_assert(sizeof(St) % sizeof(uint32_t) == 0 && kWords <= kBlock, "state words");
+    __shared__ uint32_t sw[kWords];
+    uint32_t word = 0;
+    if ((int)threadIdx.x < kWords) word = reinterpret_cast<const uint32_t*>(fin.S)[threadIdx.x];
+    CSum a[ND];
 #pragma unroll
-  for (int d = 0; d < ND; ++d) a[d] = csum_zero();
-  merge_partials<ND>(rb.partials, nb, a);
-  block_reduce<ND>(a);
-  if (threadIdx.x == 0) {
-    float tot[ND];
+    for (int d = 0; d < ND; ++d) a[d] = csum_zero();
+    merge_partials<ND>(rb.partials, nb, a);  // (stale when the producer was gated off; then unused)
+    if ((int)threadIdx.x < kWords) sw[threadIdx.x] = word;
+    __syncthreads();
+    Fin lf = fin;
+    lf.S = reinterpret_cast<St*>(sw);
+    if (!lf.active()) return;  // the same LDS word for every thread
+    block_reduce<ND>(a);
+    if (threadIdx.x == 0) {
+      float tot[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) tot[d] = csum_val(a[d]);
-    fin(tot);
-    fin.after();
+      for (int d = 0; d < ND; ++d) tot[d] = csum_val(a[d]);
+      lf(tot);
+      lf.after();
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < kWords) reinterpret_cast<uint32_t*>(fin.S)[threadIdx.x] = sw[threadIdx.x];
+  } else {
+    if (!fin.active()) return;
+    CSum a[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) a[d] = csum_zero();
+    merge_partials<ND>(rb.partials, nb, a);
+    block_reduce<ND>(a);
+    if (threadIdx.x == 0) {
+      float tot[ND];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) tot[d] = csum_val(a[d]);
+      fin(tot);
+      fin.after();
+    }
   }
 }
 

@@ -338,7 +338,7 @@ struct Workspace {
   DevMem scal;            // Scalars
   DevMem red;             // grid-reduction partials
   Scalars* hS = nullptr;  // pinned host mirror
-  int32_t* hflag = nullptr;
+  Scalars* hsnap = nullptr;  // pinned: the state after each of the last two batches
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
   static constexpr int kNumVec = 13;
@@ -353,7 +353,7 @@ struct Workspace {
   GridRed gr3() const { return GridRed{red.as<CSum>() + (size_t)32 * cu_count()}; }
   ~Workspace() {
     if (hS) (void)hipHostFree(hS);
-    if (hflag) (void)hipHostFree(hflag);
+    if (hsnap) (void)hipHostFree(hsnap);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (t0) (void)hipEventDestroy(t0);
@@ -414,7 +414,7 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
     MSPCHK(w.red.alloc(red_bytes));
     HIPCHK(hipMemsetAsync(w.red.p, 0, red_bytes, s));
     HIPCHK(hipHostMalloc((void**)&w.hS, sizeof(Scalars), hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&w.hflag, 2 * sizeof(int32_t), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&w.hsnap, 2 * sizeof(Scalars), hipHostMallocDefault));
     for (auto& e : w.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipEventCreate(&w.t0));
     HIPCHK(hipEventCreate(&w.t1));
@@ -1210,26 +1210,47 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
   Workspace& w = A->ws;
   MSPCHK(start_solve(w, o, s));
   MSPCHK(init());
-  int32_t* dflag = &w.S_dev()->done;
   int enqueued = 0, slot = 0;
   bool have_prev = false;
   // a few extra steps beyond maxiter leave room for the final verification
   const int limit = o.maxiter + (o.verify ? 2 * o.check_every : 0);
+  const int full = std::max(1, o.check_every);
   // batches grow 4, 8, ... up to check_every, so that a solve that converges in
   // a few steps does not queue two full batches of (skipped) step launches
-  int grow = std::min(4, std::max(1, o.check_every));
+  int grow = std::min(4, full);
+  // Near convergence the batches shrink to the steps the recurrence still
+  // needs: the host reads the state after each batch (one batch behind the
+  // GPU), takes the residual's rate of decrease over the last two snapshots
+  // and sizes the next batch to the estimated remainder past the batch in
+  // flight, plus 2 (a halted step still costs a launch per kernel).
+  int cap = full;
+  int32_t it_a = -1, nv_a = 0;
+  float rr_a = 0.0f;
   while (true) {
-    const int batch = std::min(grow, std::max(1, limit - enqueued));
-    grow = std::min(2 * grow, std::max(1, o.check_every));
+    const int batch = std::min(std::min(grow, cap), std::max(1, limit - enqueued));
+    grow = std::min(2 * grow, full);
     for (int k = 0; k < batch; ++k) step();
     if (o.verify) verify();
     MSPCHK(launch_check("solver step"));
     enqueued += batch;
-    HIPCHK(hipMemcpyAsync(&w.hflag[slot], dflag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&w.hsnap[slot], w.S_dev(), sizeof(Scalars), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(w.ev[slot], s));
     if (have_prev) {
       HIPCHK(hipEventSynchronize(w.ev[slot ^ 1]));
-      if (w.hflag[slot ^ 1] != 0) break;
+      const Scalars& p = w.hsnap[slot ^ 1];
+      if (p.done != 0) break;
+      cap = full;
+      if (o.criterion == kCritRelres && !p.verify && it_a >= 0 && p.iter > it_a && p.rr > 0.0f &&
+          rr_a > p.rr && p.thresh_rec > 0.0f && std::isfinite(p.rr)) {
+        const double rate = std::log((double)p.rr / rr_a) / (double)(p.iter - it_a);  // < 0
+        const double rem = std::log((double)p.thresh_rec / p.rr) / rate;               // steps from p
+        const double after = rem - batch;  // past the batch now in flight
+        cap = (int)std::min<double>(full, std::max(2.0, std::ceil(after) + 2.0));
+      }
+      // a verification replaces the recurrence residual: start a new estimate
+      it_a = p.nverify == nv_a ? p.iter : -1;
+      nv_a = p.nverify;
+      rr_a = p.rr;
     }
     have_prev = true;
     if (enqueued >= limit) break;

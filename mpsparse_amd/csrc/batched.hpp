@@ -33,6 +33,7 @@ namespace msp {
 namespace batch {
 
 constexpr int kRhs = 4;
+constexpr int kEB = 6;  // row entries per load batch of the batched SpMM
 
 enum : int32_t { kPairRunning = 0, kPairConverged = 1, kPairBreakdown = 2, kPairMaxIter = 3, kPairStagnated = 4 };
 
@@ -216,10 +217,29 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmm(View V, const float* __restr
         if (i < r1) {
           const f4 pi = V.dinv[i] * ld4(r, i) + beta * ld4(pold, i);
           f4 qi = f4{0.f, 0.f, 0.f, 0.f};
-          for (uint32_t e = V.row_ptr[i]; e < V.row_ptr[i + 1]; ++e) {
-            const uint32_t j = V.col[e];
-            const f4 pj = V.dinv[j] * ld4(r, j) + beta * ld4(pold, j);
-            qi += V.val[e] * pj;
+          // kEB entries at a time: every column / value load, then every
+          // gather, then the products in entry order (one entry at a time
+          // waited two memory round trips per entry)
+          const uint32_t e0 = V.row_ptr[i], e1 = V.row_ptr[i + 1];
+          for (uint32_t eb = e0; eb < e1; eb += kEB) {
+            uint32_t cj[kEB];
+            float vj[kEB], dj[kEB];
+            f4 rj[kEB], pj[kEB];
+#pragma unroll
+            for (int u = 0; u < kEB; ++u) {
+              const bool in = eb + u < e1;
+              cj[u] = in ? V.col[eb + u] : i;
+              vj[u] = in ? V.val[eb + u] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kEB; ++u) {
+              dj[u] = V.dinv[cj[u]];
+              rj[u] = ld4(r, cj[u]);
+              pj[u] = ld4(pold, cj[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kEB; ++u)
+              if (eb + u < e1) qi += vj[u] * (dj[u] * rj[u] + beta * pj[u]);
           }
           st4(pnew, i, pi);
           st4(q, i, qi);

@@ -34,6 +34,7 @@ namespace batch {
 
 constexpr int kRhs = 4;
 constexpr int kEB = 6;  // row entries per load batch of the batched SpMM
+constexpr int kUB = 4;  // rows per thread per load batch of the batched update
 
 enum : int32_t { kPairRunning = 0, kPairConverged = 1, kPairBreakdown = 2, kPairMaxIter = 3, kPairStagnated = 4 };
 
@@ -295,29 +296,50 @@ __global__ __launch_bounds__(kBlock) void k_bt_update(View V, float* __restrict_
     CSum acc[2 * kRhs];
 #pragma unroll
     for (int d = 0; d < 2 * kRhs; ++d) acc[d] = csum_zero();
-    for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-      f4 xi = ld4(x, i), li = ld4(xl, i);
-      const f4 pi = ld4(p, i);
+    // kUB rows per thread at a time, every load of the group issued before
+    // the first update (a row at a time waited a memory round trip per row);
+    // rows past the tile load row i0's operands and store nothing
+    for (uint32_t i0 = r0 + threadIdx.x; i0 < r1; i0 += kUB * kBlock) {
+      f4 xv[kUB], lv[kUB], pv[kUB], rv[kUB], qv[kUB];
+      float dv[kUB];
 #pragma unroll
-      for (int k = 0; k < kRhs; ++k) {
-        float xk = xi[k], lk = li[k];
-        if (fold[k]) {
-          xk += lk;
-          lk = 0.0f;
-        }
-        comp_axpy(xk, lk, alpha[k], pi[k]);
-        xi[k] = xk;
-        li[k] = lk;
+      for (int u = 0; u < kUB; ++u) {
+        const uint32_t iu = i0 + u * kBlock;
+        const uint32_t i = iu < r1 ? iu : i0;
+        xv[u] = ld4(x, i);
+        lv[u] = ld4(xl, i);
+        pv[u] = ld4(p, i);
+        rv[u] = ld4(r, i);
+        qv[u] = ld4(q, i);
+        dv[u] = V.dinv[i];
       }
-      const f4 ri = ld4(r, i) - alpha * ld4(q, i);
-      st4(x, i, xi);
-      st4(xl, i, li);
-      st4(r, i, ri);
-      const float di = V.dinv[i];
 #pragma unroll
-      for (int k = 0; k < kRhs; ++k) {
-        acc[k] = csum_add(acc[k], ri[k] * ri[k]);
-        acc[kRhs + k] = csum_add(acc[kRhs + k], ri[k] * (di * ri[k]));
+      for (int u = 0; u < kUB; ++u) {
+        const uint32_t i = i0 + u * kBlock;
+        if (i >= r1) continue;
+        f4 xi = xv[u], li = lv[u];
+        const f4 pi = pv[u];
+#pragma unroll
+        for (int k = 0; k < kRhs; ++k) {
+          float xk = xi[k], lk = li[k];
+          if (fold[k]) {
+            xk += lk;
+            lk = 0.0f;
+          }
+          comp_axpy(xk, lk, alpha[k], pi[k]);
+          xi[k] = xk;
+          li[k] = lk;
+        }
+        const f4 ri = rv[u] - alpha * qv[u];
+        st4(x, i, xi);
+        st4(xl, i, li);
+        st4(r, i, ri);
+        const float di = dv[u];
+#pragma unroll
+        for (int k = 0; k < kRhs; ++k) {
+          acc[k] = csum_add(acc[k], ri[k] * ri[k]);
+          acc[kRhs + k] = csum_add(acc[kRhs + k], ri[k] * (di * ri[k]));
+        }
       }
     }
     tile_partials<2 * kRhs>(acc, partials, t);

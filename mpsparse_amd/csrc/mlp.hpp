@@ -1,5 +1,8 @@
-// Edge MLP of the learned FSAI preconditioner: 8 (5 used) -> 64 -> 64 -> 1,
-// ReLU, BF16 MFMA operands with FP32 accumulation (v_mfma_f32_32x32x8_bf16).
+// Edge MLP of the learned FSAI preconditioner: 8 -> 64 -> 64 -> 1, ReLU, BF16
+// MFMA operands with FP32 accumulation (v_mfma_f32_32x32x8_bf16 for layer 1
+// and the backward products, v_mfma_f32_32x32x16_bf16 for the forward's
+// layer 2: A lane l holds A[l % 32][8 (l / 32) .. + 7], B lane l holds
+// B[8 (l / 32) .. + 7][l % 32], C as below).
 //
 // Parameters (FP32 master copy, flat): W1[64][8] | b1[64] | W2[64][64] | b2[64]
 // | w3[64] | b3 -> 4801 floats. Each wave handles tiles of 32 edges.
@@ -23,6 +26,7 @@ constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH
 constexpr int kParams = oB3 + 1;  // 4801
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f16 __attribute__((ext_vector_type(16)));
 
 // FP32 -> BF16 with the hardware converter (v_cvt_pk_bf16_f32: round to
@@ -43,6 +47,14 @@ __device__ __forceinline__ f16 mfma(s4 a, s4 b, f16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
 }
 
+__device__ __forceinline__ f16 mfma16(bf8 a, bf8 b, f16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// Row stride of the per-wave BF16 staging tile (32 rows): 72 halves keeps
+// each lane's 8 consecutive values one aligned 16-byte LDS read.
+constexpr int kStage = 72;
+
 __device__ __forceinline__ f16 zero16() {
   f16 z;
 #pragma unroll
@@ -53,7 +65,7 @@ __device__ __forceinline__ f16 zero16() {
 // Weights of the forward pass as B operands (kept in registers).
 struct FwdWeights {
   s4 b1[2];     // layer 1: B[k][n] = W1[n][k], n-tile t
-  s4 b2[2][8];  // layer 2: B[k][n] = W2[n][k], n-tile t, k-chunk q
+  bf8 b2[2][4];  // layer 2: B[k][n] = W2[n][k], n-tile t, 16-deep k-chunk q
   float bias1[2], bias2[2], w3[2];
   float bias3;
 };
@@ -66,9 +78,10 @@ __device__ __forceinline__ void load_fwd(const float* __restrict__ P, FwdWeights
     const float* w1 = P + oW1 + n * kIn + 4 * h;
     W.b1[t] = pack4(w1[0], w1[1], w1[2], w1[3]);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const float* w2 = P + oW2 + n * kH + 8 * q + 4 * h;
-      W.b2[t][q] = pack4(w2[0], w2[1], w2[2], w2[3]);
+    for (int q = 0; q < 4; ++q) {
+      const float* w2 = P + oW2 + n * kH + 16 * q + 8 * h;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) W.b2[t][q][i] = (__bf16)w2[i];
     }
     W.bias1[t] = P[oB1 + n];
     W.bias2[t] = P[oB2 + n];
@@ -79,7 +92,8 @@ __device__ __forceinline__ void load_fwd(const float* __restrict__ P, FwdWeights
 
 // Forward of one 32-edge tile. Returns H1, H2 (post-ReLU, C layout, n-tiles 0/1)
 // and leaves the per-lane layer-3 partial sums in `part` (row 8j+4h+i).
-// `sh` is this wave's 32 x 64 BF16 staging area (row stride 68 shorts).
+// `sh` is this wave's 32 x 64 BF16 staging area (row stride kStage halves,
+// 16-byte aligned).
 __device__ __forceinline__ void forward_tile(const float* __restrict__ feat, uint64_t e0, uint64_t ne,
                                              const FwdWeights& W, short* sh, f16 (&H1)[2], f16 (&H2)[2]) {
   const int l = threadIdx.x & 63, h = l >> 5, c = l & 31;
@@ -106,17 +120,16 @@ __device__ __forceinline__ void forward_tile(const float* __restrict__ feat, uin
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) sh[(8 * j + 4 * h + i) * 68 + 32 * t + c] = bf16_bits(H1[t][4 * j + i]);
+      for (int i = 0; i < 4; ++i) sh[(8 * j + 4 * h + i) * kStage + 32 * t + c] = bf16_bits(H1[t][4 * j + i]);
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
   for (int t = 0; t < 2; ++t) H2[t] = zero16();
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const short* row = sh + c * 68 + 8 * q + 4 * h;
-    const s4 a2 = s4{row[0], row[1], row[2], row[3]};
+  for (int q = 0; q < 4; ++q) {
+    const bf8 a2 = *reinterpret_cast<const bf8*>(sh + c * kStage + 16 * q + 8 * h);
 #pragma unroll
-    for (int t = 0; t < 2; ++t) H2[t] = mfma(a2, W.b2[t][q], H2[t]);
+    for (int t = 0; t < 2; ++t) H2[t] = mfma16(a2, W.b2[t][q], H2[t]);
   }
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -153,7 +166,7 @@ __device__ __forceinline__ void layer3(const FwdWeights& W, const f16 (&H2)[2], 
 // conversions, the LDS staging and the layer-3 reduction with fewer waves.
 __global__ __launch_bounds__(kBlock, 4) void k_mlp_forward(const float* __restrict__ P, const float* __restrict__ feat,
                                                         uint64_t ne, float* __restrict__ out) {
-  __shared__ short stage[kWavesPerBlock][32 * 68];
+  __shared__ __attribute__((aligned(16))) short stage[kWavesPerBlock][32 * kStage];
   const int l = threadIdx.x & 63, h = l >> 5, c = l & 31, wid = threadIdx.x / kWave;
   FwdWeights W;
   load_fwd(P, W);
@@ -183,7 +196,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_mlp_forward(const float* __restri
 __global__ __launch_bounds__(kBlock) void k_mlp_backward(const float* __restrict__ P, const float* __restrict__ feat,
                                                          const float* __restrict__ go, uint64_t ne,
                                                          float* __restrict__ partial) {
-  __shared__ short stage[kWavesPerBlock][32 * 68];
+  __shared__ __attribute__((aligned(16))) short stage[kWavesPerBlock][32 * kStage];
   __shared__ float red[kParams];
   const int l = threadIdx.x & 63, h = l >> 5, c = l & 31, wid = threadIdx.x / kWave;
   for (int i = threadIdx.x; i < kParams; i += kBlock) red[i] = 0.0f;
@@ -254,13 +267,13 @@ __global__ __launch_bounds__(kBlock) void k_mlp_backward(const float* __restrict
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sh[(8 * j + 4 * h + i) * 68 + 32 * t + c] = bf16_bits(dH2[t][4 * j + i]);
+        for (int i = 0; i < 4; ++i) sh[(8 * j + 4 * h + i) * kStage + 32 * t + c] = bf16_bits(dH2[t][4 * j + i]);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     f16 dH1[2] = {zero16(), zero16()};
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const short* row = sh + c * 68 + 8 * q + 4 * h;
+      const short* row = sh + c * kStage + 8 * q + 4 * h;
       const s4 a = s4{row[0], row[1], row[2], row[3]};
 #pragma unroll
       for (int u = 0; u < 2; ++u) dH1[u] = mfma(a, w2t[u][q], dH1[u]);

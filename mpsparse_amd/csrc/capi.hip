@@ -501,6 +501,30 @@ static msp_status ensure_dia(msp_csr A, hipStream_t s, bool fill = true) {
   return MSP_OK;
 }
 
+// Diagonal storage of T = G^T from G's: every entry (i, j) of G lies on offset
+// j - i, so (j, i) lies on i - j; the offsets are G's negated (same count and
+// density), no probe needed. The values are set later (refresh_dia).
+static msp_status dia_as_transpose(msp_csr T, const MatBase* G) {
+  const auto* g = static_cast<const msp_csr_s*>(G);
+  if (T->dia_state != 0 || g->dia_state != 1 || T->format != MSP_FORMAT_AUTO || T->n != g->m || T->m != g->n)
+    return MSP_OK;
+  DiaOffsets o{};
+  o.ndiag = g->dia_o.ndiag;
+  int dd = -1;
+  for (int d = 0; d < o.ndiag; ++d) {
+    o.off[d] = -g->dia_o.off[o.ndiag - 1 - d];  // ascending
+    if (o.off[d] == 0) dd = d;
+  }
+  T->dia_state = 2;
+  T->dia_ld = ((uint64_t)T->n + 63) / 64 * 64;
+  MSPCHK(T->dia.alloc(sizeof(float) * T->dia_ld * o.ndiag));
+  T->dia_o = o;
+  T->dia_dd = dd;
+  T->dia_state = 1;
+  T->dia16_type = -1;
+  return MSP_OK;
+}
+
 // Refill the FP32 diagonal storage after A's values changed (pattern fixed).
 static msp_status refresh_dia(msp_csr A, hipStream_t s) {
   if (A->dia_state != 1) return MSP_OK;
@@ -540,18 +564,24 @@ static msp_status ensure_format(msp_csr A, int32_t vt, hipStream_t s) {
   return MSP_OK;
 }
 
-static msp_status finalize_matrix(msp_csr A, hipStream_t s) {
+// max_row_bound >= 0: a known upper bound on the row length stands in for the
+// measured maximum (no device round trip; a pattern derived from another).
+static msp_status finalize_matrix(msp_csr A, hipStream_t s, int64_t max_row_bound = -1) {
   const uint32_t n = A->n;
-  TmpMem mx;
-  MSPCHK(mx.alloc(sizeof(uint32_t), s));
-  HIPCHK(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), s));
-  if (n > 0) {
-    uint32_t g = std::min<uint32_t>((n + kBlock - 1) / kBlock, 1024);
-    k_max_row<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), n, mx.as<uint32_t>());
-    MSPCHK(launch_check("k_max_row"));
+  if (max_row_bound >= 0) {
+    A->max_row = (uint32_t)std::min<int64_t>(max_row_bound, UINT32_MAX);
+  } else {
+    TmpMem mx;
+    MSPCHK(mx.alloc(sizeof(uint32_t), s));
+    HIPCHK(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), s));
+    if (n > 0) {
+      uint32_t g = std::min<uint32_t>((n + kBlock - 1) / kBlock, 1024);
+      k_max_row<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), n, mx.as<uint32_t>());
+      MSPCHK(launch_check("k_max_row"));
+    }
+    HIPCHK(hipMemcpyAsync(&A->max_row, mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
   }
-  HIPCHK(hipMemcpyAsync(&A->max_row, mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
   // nonzeros per row block: as large as the LDS window allows with room for
   // the longest row (measured best near 3584 for 7- and 144-entry rows)
   const uint32_t room = A->max_row + 4 < kCap ? kCap - 4 - A->max_row : 0;
@@ -1933,7 +1963,10 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     MSPCHK(launch_check("lower fill"));
     HIPCHK(hipMemsetAsync(F->G.val.p, 0, sizeof(float) * (ne + 64), s));
     MSPCHK(zero_pad(&F->G, s));
-    MSPCHK(finalize_matrix(&F->G, s));
+    // G's rows are the lower parts of A's: A's longest row bounds them
+    MSPCHK(finalize_matrix(&F->G, s, A->max_row));
+    // G's values are set later (fsai_commit refills the diagonal storage)
+    MSPCHK(ensure_dia(&F->G, s, false));
     // transpose pattern by sorting (col << 32 | row) with the edge index as payload
     TmpMem keys, payload;
     MSPCHK(keys.alloc(sizeof(uint64_t) * (ne + 1), s));
@@ -1949,9 +1982,10 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
                              F->GT.col.as<uint32_t>(), F->GTmap.as<float>(), s, nullptr, true));
     HIPCHK(hipMemsetAsync(F->GT.val.p, 0, sizeof(float) * (ne + 64), s));
     MSPCHK(zero_pad(&F->GT, s));
-    MSPCHK(finalize_matrix(&F->GT, s));
-    // G's values are set later (fsai_commit refills the diagonal storage)
-    MSPCHK(ensure_dia(&F->G, s, false));
+    // with G in diagonal storage, G^T's rows hold at most its diagonal count
+    // and its storage follows from G's
+    MSPCHK(finalize_matrix(&F->GT, s, F->G.dia_state == 1 ? (int64_t)F->G.dia_o.ndiag : -1));
+    MSPCHK(dia_as_transpose(&F->GT, &F->G));
     MSPCHK(ensure_dia(&F->GT, s, false));
     // features and column scaling
     MSPCHK(F->diag.alloc(sizeof(float) * ((size_t)n + 1)));

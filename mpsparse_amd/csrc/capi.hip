@@ -1406,12 +1406,12 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       g = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S->halt, s);
     }
     if (jac) {
-      if (comp) k_cg_update_xr<kPcJacobi, kAlphaPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      else k_cg_update_xr<kPcJacobi, kAlphaPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      if (comp) k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      else k_cg_update_xr<kPcJacobi, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcJacobi>{S}, rb2, gv, s);
     } else {
-      if (comp) k_cg_update_xr<kPcNone, kAlphaPq, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      else k_cg_update_xr<kPcNone, kAlphaPq, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      if (comp) k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
+      else k_cg_update_xr<kPcNone, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
       finish<2>(FinCgStep<kPcNone>{S}, rb2, gv, s);
     }
     ++k;

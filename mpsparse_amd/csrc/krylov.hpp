@@ -73,21 +73,6 @@ __device__ __forceinline__ void on_recurrence_converged(Scalars* S) {
 __device__ __forceinline__ bool finite_pos(float v) { return v > 0.0f && isfinite(v); }
 
 // Finishers (run in one thread after the grid reduction) --------------------
-struct FinCgAlpha {
-  Scalars* S;
-  __device__ __forceinline__ void after() const { refresh_halt(S); }
-  __device__ __forceinline__ bool active() const { return S->halt == 0; }
-  __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
-    S->pq = tot[0];
-    if (!finite_pos(tot[0])) {
-      S->done = kBreakdown;
-      S->alpha = 0.0f;
-    } else {
-      S->alpha = S->rz / tot[0];
-    }
-  }
-};
-
 struct FinBicgAlpha {
   Scalars* S;
   __device__ __forceinline__ void after() const { refresh_halt(S); }
@@ -112,23 +97,7 @@ struct FinBicgOmega {
   }
 };
 
-// Beta for PCG with an explicit z computed by other kernels: rz_new in tot[0]
-// (beta = 0 on the first step and after a restart).
-struct FinPcgBeta {
-  Scalars* S;
-  __device__ __forceinline__ void after() const { refresh_halt(S); }
-  __device__ __forceinline__ bool active() const { return S->halt == 0; }
-  __device__ __forceinline__ void operator()(const float (&tot)[1]) const {
-    const float rz_new = tot[0];
-    if (!isfinite(rz_new) || !(rz_new > 0.0f)) {
-      S->done = kBreakdown;
-      return;
-    }
-    S->beta = (S->restart || S->rz == 0.0f) ? 0.0f : rz_new / S->rz;
-    S->restart = 0;
-    S->rz = rz_new;
-  }
-};
+
 
 __global__ void k_set_restart(Scalars* S) { S->restart = 1; }
 
@@ -326,7 +295,8 @@ __device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32
 // Explicitly preconditioned CG direction formed in the gather of q = A p:
 // p = z + beta p_old with beta = <r,z>_new / <r,z>_old, <r,z>_new from the
 // block partials of the z product (zr, nz blocks) reduced by every workgroup
-// in kernel_prologue (the same rule as k_cg_update_p<kPcExplicit, true>).
+// in kernel_prologue; beta = 0 on the first step, after a restart, or for a
+// non-positive <r,z>_new (the step's update then records the breakdown).
 // p_old must not alias the p written by the epilogue.
 struct XPzLocal {
   const float* __restrict__ z;
@@ -361,9 +331,9 @@ __device__ __forceinline__ void kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& 
 // in the gather (r_new goes to the other residual buffer, so every workgroup
 // gathers the step's r), alpha = <r,z> / <p,q> from the block partials of the
 // z product (zr, nz blocks) and of q = A p (pq, npq blocks), reduced by every
-// workgroup in kernel_prologue with the rule of k_cg_update_xr<kPcExplicit,
-// kAlphaRzPq, COMP> (workgroup 0 records pq, alpha, <r,z>, clears the restart
-// flag and marks w current). The epilogue updates the owned rows (x, x_lo
+// workgroup in kernel_prologue; alpha = 0 when either is not positive (the
+// step's finisher then records a breakdown); workgroup 0 records pq, alpha,
+// <r,z>, clears the restart flag and marks w current. The epilogue updates the owned rows (x, x_lo
 // when COMP, r_new, w) and accumulates <r_new, r_new> (second partial unused).
 struct XRNew {
   const float* __restrict__ r;
@@ -446,45 +416,29 @@ __device__ __forceinline__ void kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, 
 }
 
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
-// ALPHA: kAlphaState takes alpha from the state. kAlphaPq: alpha = <r,z> /
-// <p,q> with <p,q> from the SpMV's block partials (pq, npq blocks), computed
-// identically by every workgroup; block 0 records pq and alpha, and a
-// non-positive <p,q> leaves x and r alone (FinCgStep then stops with a
-// breakdown). kAlphaRzPq (explicit preconditioner): <r,z> also from block
-// partials (zr, nz blocks, those of the z = M r product), and block 0 records
-// it as the new <r,z> and clears the restart flag; a non-positive <r,z> is
-// recorded as a breakdown the same way. COMP: the iterate is x + xl with xl
-// carrying the rounding error of x += alpha p (for targets near FP32's
-// attainable accuracy); otherwise x alone, xl untouched.
-enum : int { kAlphaState = 0, kAlphaPq = 1, kAlphaRzPq = 2 };
-
-template <int PC, int ALPHA, bool COMP>
+// alpha = <r,z> / <p,q> with <p,q> from the SpMV's block partials (pq, npq
+// blocks), computed identically by every workgroup; block 0 records pq and
+// alpha, and a non-positive <p,q> leaves x and r alone (FinCgStep then stops
+// with a breakdown). COMP: the iterate is x + xl with xl carrying the rounding
+// error of x += alpha p (for targets near FP32's attainable accuracy);
+// otherwise x alone, xl untouched. (The explicitly preconditioned step fuses
+// this update into its factor product: XRNew / EpiCgXrW.)
+template <int PC, bool COMP>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ xl,
                                                          float* __restrict__ r, const float* __restrict__ p,
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
                                                          uint32_t n, Scalars* S, GridRed rb, GridRed pq,
-                                                         uint32_t npq, GridRed zr = GridRed{}, uint32_t nz = 0) {
+                                                         uint32_t npq) {
   // the gate is tested after the partials are in (their loads overlap its
   // latency); a halted step still writes nothing
   const int32_t halt = S->halt;
-  if (ALPHA == kAlphaState && halt) return;
-  float alpha = S->alpha;
-  if constexpr (ALPHA != kAlphaState) {
-    float pqv, rz = S->rz;
-    if constexpr (ALPHA == kAlphaRzPq) block_total2(pq.partials, npq, zr.partials, nz, pqv, rz);
-    else pqv = block_total<1>(pq.partials, npq, 0);
-    if (halt) return;
-    bool ok = finite_pos(pqv);
-    if constexpr (ALPHA == kAlphaRzPq) ok = ok && finite_pos(rz);
-    alpha = ok ? rz / pqv : 0.0f;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      S->pq = ok ? pqv : -1.0f;
-      S->alpha = alpha;
-      if constexpr (ALPHA == kAlphaRzPq) {
-        S->rz = rz;
-        S->restart = 0;
-      }
-    }
+  const float pqv = block_total<1>(pq.partials, npq, 0), rz = S->rz;
+  if (halt) return;
+  const bool ok = finite_pos(pqv);
+  const float alpha = ok ? rz / pqv : 0.0f;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->pq = ok ? pqv : -1.0f;
+    S->alpha = alpha;
   }
   [[maybe_unused]] const bool fold = S->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
@@ -563,24 +517,13 @@ struct FinCgStep {
 };
 
 // pnew = z + beta pold with z = r, dinv*r, or an explicit vector (pnew may
-// alias pold). BETA_LOCAL (explicit preconditioner): beta = <r,z>_new /
-// <r,z>_old from the block partials of the z product (zr, nz blocks) and the
-// state's <r,z>, which k_cg_update_xr<.., kAlphaRzPq, ..> replaces later in
-// the step; 0 on the first step, after a restart, or for a non-positive
-// <r,z>_new (the update then records the breakdown).
-template <int PC, bool BETA_LOCAL = false>
+// alias pold); beta from the state (FinCgStep).
+template <int PC>
 __global__ __launch_bounds__(kBlock) void k_cg_update_p(float* pnew, const float* pold, const float* __restrict__ r,
                                                         const float* __restrict__ dinv, const float* __restrict__ z,
-                                                        uint32_t n, const Scalars* S, GridRed zr = GridRed{},
-                                                        uint32_t nz = 0) {
-  const int32_t halt = S->halt;
-  if (!BETA_LOCAL && halt) return;
-  float beta = S->beta;
-  if constexpr (BETA_LOCAL) {
-    const float rz_new = block_total<1>(zr.partials, nz, 0);  // loads overlap the gate's latency
-    if (halt) return;
-    beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
-  }
+                                                        uint32_t n, const Scalars* S) {
+  if (S->halt) return;
+  const float beta = S->beta;
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
     f4 zv;

@@ -7,7 +7,9 @@ Bars (FP32 device arithmetic):
   exact solution is within 3x of tol (gpu_util.check_fp32_solution);
 * error vs the FP64 direct solution <= kappa-scaled bound stated per test;
 * iteration counts within 15% (+2) of the FP64 oracle's;
-* Richardson: |x - x64| <= 1e-4 * ||x64||_inf after the stated steps.
+* Richardson: |x - x64| <= 1e-4 * ||x64||_inf after the stated steps;
+* the host's step batching (check interval 1, 5, 16; adaptive near convergence)
+  leaves the iterate, the step count and the verifications bit-identical.
 """
 import numpy as np
 import pytest
@@ -240,3 +242,28 @@ def test_breakdown_and_limits():
     say("zero rhs")
     x, info = _solve(A, np.zeros_like(p.b))
     assert info["status"] == "converged" and info["iterations"] == 0 and np.all(x == 0), info
+
+
+@pytest.mark.parametrize("method,precond", [("cgd", "none"), ("cgd", "jacobi"), ("bicgstab", "none")])
+def test_batch_sizes_do_not_change_the_solve(method, precond):
+    """The host enqueues the steps in batches (growing at the start, shrinking
+    near convergence from the residual's observed rate) and halted steps do
+    nothing, so the iterate, the step count and the verifications are the same
+    bit for bit for every check interval."""
+    M = diffusion_2d(60, sigma=1.5, seed=9)
+    b = np.random.default_rng(9).standard_normal(M.num_rows).astype(np.float32)
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    outs = []
+    for ce in (1, 5, 16):
+        say("batch sizes", method, precond, ce)
+        kw = dict(tol=1e-6, check_every=ce)
+        if method == "cgd":
+            kw.update(precond=precond, cg_form="separate")
+        x, info = _solve(A, b, method, **kw)
+        outs.append((x, info))
+    x0, i0 = outs[0]
+    assert i0["status"] == "converged", i0
+    for x, info in outs[1:]:
+        assert (info["iterations"], info["verifications"], info["status"]) == \
+            (i0["iterations"], i0["verifications"], i0["status"]), (info, i0)
+        assert np.array_equal(x, x0)

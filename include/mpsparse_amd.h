@@ -97,7 +97,8 @@ typedef struct msp_solve_info {
 /* Mixed-precision iterative refinement (msp_csr_solve_refined). */
 typedef struct msp_refine_opts {
   double tol;            /* outer target ||b - A x|| / ||b|| in FP64; default 1e-10 */
-  double inner_tol;      /* relative tolerance of each FP32 correction solve; default 1e-3 */
+  double inner_tol;      /* relative tolerance of each FP32 correction solve; 0 (default):
+                            1e-3, or the values' unit roundoff if larger (BF16: 2^-8) */
   int32_t max_outer;     /* default 30 */
   int32_t inner_maxiter; /* default 10000 */
   int32_t inner_solver;  /* 0: CG (default), 1: BiCGSTAB */

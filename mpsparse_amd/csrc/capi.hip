@@ -1765,7 +1765,7 @@ extern "C" void msp_default_refine_opts(msp_refine_opts* o) {
   if (!o) return;
   std::memset(o, 0, sizeof *o);
   o->tol = 1e-10;
-  o->inner_tol = 1e-3;
+  o->inner_tol = 0.0;  // automatic (resolved_inner_tol)
   o->max_outer = 30;
   o->inner_maxiter = 10000;
   o->inner_solver = 0;
@@ -1860,6 +1860,19 @@ static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_r
   return MSP_OK;
 }
 
+// inner_tol <= 0: 1e-3, or the unit roundoff of the matrix values inside the
+// correction if larger (BF16 2^-8; FP16 2^-11 and FP32 fall back to 1e-3): a
+// correction cannot be more accurate than its operator's rounding, so solving
+// further is wasted steps (128^3 learned-FSAI refinement with BF16 values: 355
+// inner steps at 1e-3, 336 at 4e-3, same outer count and final accuracy).
+static double resolved_inner_tol(const msp_refine_opts& ro) {
+  if (ro.inner_tol > 0.0) return ro.inner_tol;
+  const double u = ro.values == MSP_VALUES_BF16 ? std::ldexp(1.0, -8)
+                   : ro.values == MSP_VALUES_F16 ? std::ldexp(1.0, -11)
+                                                 : std::ldexp(1.0, -24);
+  return std::max(1e-3, u);
+}
+
 static msp_solve_opts inner_opts(const msp_refine_opts& ro) {
   msp_solve_opts io;
   msp_default_opts(&io);
@@ -1877,6 +1890,8 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
   msp_refine_opts ro;
   msp_default_refine_opts(&ro);
   if (opts) ro = *opts;
+  if (ro.inner_tol < 0.0) return fail(MSP_ERR_INVALID, "negative inner_tol");
+  ro.inner_tol = resolved_inner_tol(ro);
   if (A->n != A->m) return fail(MSP_ERR_INVALID, "refinement needs a square matrix");
   if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
   if (ro.inner_solver != 0 && ro.inner_solver != 1) return fail(MSP_ERR_INVALID, "inner_solver must be 0 or 1");
@@ -2323,6 +2338,8 @@ extern "C" msp_status msp_csr_solve_refined_fsai(msp_csr A, msp_fsai F, const do
   msp_refine_opts ro;
   msp_default_refine_opts(&ro);
   if (opts) ro = *opts;
+  if (ro.inner_tol < 0.0) return fail(MSP_ERR_INVALID, "negative inner_tol");
+  ro.inner_tol = resolved_inner_tol(ro);
   if ((!b || !x) && A->n) return fail(MSP_ERR_INVALID, "null b / x");
   hipStream_t s = (hipStream_t)stream;
   if (ro.values != MSP_VALUES_F32) MSPCHK(msp_csr_prepare_values(A, ro.values, stream));

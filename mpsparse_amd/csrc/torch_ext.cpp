@@ -788,7 +788,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         py::arg("values") = "f32", py::arg("verify") = true);
   m.def("pcg_fsai_refined", &pcg_fsai_refined,
         "FP64 iterative refinement around FSAI-PCG corrections (b, x float64)", py::arg("A"), py::arg("F"),
-        py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("inner_tol") = 1e-3, py::arg("max_outer") = 30,
+        py::arg("b"), py::arg("x"), py::arg("tol") = 1e-6, py::arg("inner_tol") = 0.0, py::arg("max_outer") = 30,
         py::arg("inner_maxiter") = 10000, py::arg("values") = "f32", py::arg("use_x0") = false);
   m.def("device_count", []() { return msp_device_count(); });
   m.def(
@@ -826,7 +826,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
            py::arg("omega") = 0.01, py::arg("iters") = 1000, py::arg("use_x0") = false, py::arg("values") = "f32")
       .def("solve_refined", &CsrTensor::solve_refined,
            "FP64 iterative refinement around FP32 (BF16/FP16-valued) Krylov corrections; b, x float64",
-           py::arg("b"), py::arg("x"), py::arg("tol") = 1e-10, py::arg("inner_tol") = 1e-3,
+           py::arg("b"), py::arg("x"), py::arg("tol") = 1e-10, py::arg("inner_tol") = 0.0,
            py::arg("max_outer") = 30, py::arg("inner_maxiter") = 10000, py::arg("solver") = "cg",
            py::arg("values") = "f32", py::arg("precond") = "none", py::arg("use_x0") = false)
       .def("export", &CsrTensor::export_into, "write row_ptr / col_ind / values", py::arg("row_ptr"),

@@ -347,6 +347,8 @@ struct Workspace {
   enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL, RH };
   float* v(int i) const { return vecs.as<float>() + (size_t)i * npad; }
   Scalars* S_dev() const { return scal.as<Scalars>(); }
+  // staging copy of the state for decisions taken inside a step kernel
+  Scalars* S_stage() const { return scal.as<Scalars>() + 1; }
   // two halves: SpMV partials, and those of the kernel that consumes them
   GridRed gr() const { return GridRed{red.as<CSum>()}; }
   GridRed gr2() const { return GridRed{red.as<CSum>() + (size_t)16 * cu_count()}; }
@@ -407,8 +409,8 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
   MSPCHK(w.vecs.alloc(sizeof(float) * need * Workspace::kNumVec));
   HIPCHK(hipMemsetAsync(w.vecs.p, 0, sizeof(float) * need * Workspace::kNumVec, s));
   if (!w.scal.p) {
-    MSPCHK(w.scal.alloc(sizeof(Scalars)));
-    HIPCHK(hipMemsetAsync(w.scal.p, 0, sizeof(Scalars), s));
+    MSPCHK(w.scal.alloc(2 * sizeof(Scalars)));
+    HIPCHK(hipMemsetAsync(w.scal.p, 0, 2 * sizeof(Scalars), s));
     // partials of up to 4 values for grids of up to 8 workgroups per CU
     size_t red_bytes = sizeof(CSum) * 6 * (size_t)cu_count() * 8;  // three regions of 16 CSums per CU
     MSPCHK(w.red.alloc(red_bytes));
@@ -1091,9 +1093,10 @@ struct OpCsr {
   }
   // Explicit preconditioner: p = z + beta p_old in the gather with beta from
   // the <r,z> partials (zr, nz blocks), q = A p, <p,q> partials.
-  uint32_t pcg_dir(const float* z, const float* pold, const Scalars* S, GridRed zr, uint32_t nz, float* q,
-                   float* pnew, const int* done, hipStream_t s) const {
-    const XPzLocal xl{z, pold, S, zr, nz, 0.0f};
+  uint32_t pcg_dir(const float* z, const float* pold, const Scalars* S, Scalars* S2, GridRed zr, uint32_t nz,
+                   GridRed rr, uint32_t nrr, float* q, float* pnew, hipStream_t s) const {
+    const XPzLocal xl{z, pold, S, S2, zr, nz, rr, nrr, 0.0f};
+    const int* done = &S->halt;
     return spmv_any<V, 1>(A, xl, EpiCgQ<XPzLocal>{q, pnew, xl}, FinNone{}, red_of(A), done, s);
   }
   RowsCsr rows() const { return RowsCsr{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()}; }
@@ -1234,9 +1237,16 @@ static msp_status finish_solve(Workspace& w, msp_solve_info* info, hipStream_t s
   return MSP_OK;
 }
 
-template <typename Init, typename Step, typename Verify>
+struct NoFlush {
+  void operator()() const {}
+};
+
+// `flush` runs after each batch of steps, before the verification and the
+// host's read of the state (a solver whose steps leave a decision pending
+// takes it there).
+template <typename Init, typename Step, typename Verify, typename Flush = NoFlush>
 static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
-                             Step step, Verify verify) {
+                             Step step, Verify verify, Flush flush = Flush{}) {
   Workspace& w = A->ws;
   MSPCHK(start_solve(w, o, s));
   MSPCHK(init());
@@ -1260,6 +1270,7 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
     const int batch = std::min(std::min(grow, cap), std::max(1, limit - enqueued));
     grow = std::min(2 * grow, full);
     for (int k = 0; k < batch; ++k) step();
+    flush();
     if (o.verify) verify();
     MSPCHK(launch_check("solver step"));
     enqueued += batch;
@@ -2247,6 +2258,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   float *Z = w.v(Workspace::Z), *Wv = w.v(Workspace::W), *XLo = w.v(Workspace::XL);
   const float* B = w.v(Workspace::B);
   Scalars* S = w.S_dev();
+  Scalars* S2 = w.S_stage();
   GridRed rb = w.gr();
   const uint32_t n = A->n, gv = vec_grid(n);
   const bool comp = compensate(o);
@@ -2256,6 +2268,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   float* Pb[2] = {P, w.v(Workspace::T)};
   float* Rb[2] = {R, w.v(Workspace::R0)};
   int k = 0;
+  uint32_t gu = 0;  // workgroups of the latest x / r update (its <r,r> partials in rb2)
   GridRed rb2 = w.gr2(), zr = w.gr3();
   // w = G r of the current residual (gated: skipped when *gate != 0)
   auto g_of_r = [&](const int* gate) {
@@ -2269,28 +2282,33 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     g_of_r(&S->halt);
     return launch_check("pcg init");
   };
-  // one step, four launches: z = G^T w with <r,z> partials (zr); q = A p with
-  // p = z + beta p_old formed in the gather (beta from zr) and <p,q> partials
-  // (rb); the x / r update with alpha from zr and rb fused with the next
-  // step's w = G r (r formed in the gather, <r,r> partials rb2); FinCgStep
+  // one step, three launches: z = G^T w with <r,z> partials (zr); q = A p
+  // with p = z + beta p_old formed in the gather (beta from zr), the previous
+  // step's decision (FinCgStep on rb2) taken by every workgroup and staged in
+  // S2, <p,q> partials (rb); the decision committed to S and the x / r update
+  // with alpha from zr and rb fused with the next step's w = G r (r formed in
+  // the gather, <r,r> partials rb2, decision pending). A z product after the
+  // step that converged still runs; its z and <r,z> are recomputed before use.
   auto step = [&]() {
     const float* pold = Pb[k & 1];
     float* pnew = Pb[(k + 1) & 1];
     const float* rcur = Rb[k & 1];
     float* rnext = Rb[(k + 1) & 1];
     const uint32_t gz = factor_spmv<1>(g16, &F->GT, XPlain{Wv}, EpiStoreDot{Z, rcur}, zr, &S->halt, s);
-    const uint32_t gq = op.pcg_dir(Z, pold, S, zr, gz, Q, pnew, &S->halt, s);
-    uint32_t gu;
+    const uint32_t gq = op.pcg_dir(Z, pold, S, S2, zr, gz, rb2, gu, Q, pnew, s);
     if (comp)
       gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
-                          EpiCgXrW<true>{X, XLo, rcur, rnext, pnew, Q, Wv, S, rb, gq, zr, gz, 0.0f, 0}, rb2,
-                          &S->halt, s);
+                          EpiCgXrW<true>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0}, rb2,
+                          &S2->halt, s);
     else
       gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
-                          EpiCgXrW<false>{X, XLo, rcur, rnext, pnew, Q, Wv, S, rb, gq, zr, gz, 0.0f, 0}, rb2,
-                          &S->halt, s);
-    finish<2>(FinCgStep<kPcExplicit>{S}, rb2, gu, s);
+                          EpiCgXrW<false>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0}, rb2,
+                          &S2->halt, s);
     ++k;
+  };
+  // the last step's decision, before the host reads the state
+  auto flush = [&]() {
+    if (gu) finish<2>(FinCgStepPending{S}, rb2, gu, s);
   };
   // a verification rewrites the current residual; after a restart w = G r is
   // recomputed (gated by skip_g, which FinVerify clears only then)
@@ -2299,7 +2317,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     finish<2>(FinVerify<3>{S}, rb, gv, s);
     g_of_r(&S->skip_g);
   };
-  MSPCHK(run_solver(A, o, info, s, init, step, verify));
+  MSPCHK(run_solver(A, o, info, s, init, step, verify, flush));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   HIPCHK(hipStreamSynchronize(s));

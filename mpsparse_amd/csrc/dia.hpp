@@ -51,8 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
   // their triangular factors); 0: A.o.ndiag <= kMaxDiag
   constexpr int DMAX = NDIAG > 0 ? NDIAG : kMaxDiag;
   // the gate is tested after the prologue (whose loads overlap its latency)
-  const int halt = done ? *done : 0;
-  kernel_prologue(xl, epi, halt);
+  const int halt = kernel_prologue(xl, epi, done ? *done : 0);
   if (halt) return;
   constexpr int NA = ND > 0 ? ND : 1;
   CSum acc[NA];

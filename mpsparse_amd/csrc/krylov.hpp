@@ -10,8 +10,15 @@
 //   alpha = <r,z> / <p,q> from those sums in every workgroup, x += alpha p,
 //   r -= alpha q, sums of <r,r>, <r,z>   (k_cg_update_xr<PC, true>)
 //   convergence, beta                    (k_finish + FinCgStep)
-// FSAI-preconditioned CG keeps z explicit (two more SpMVs) and the separate
-// alpha finisher and p update.
+// FSAI-preconditioned CG (M = G^T G) keeps z explicit, three launches a step:
+//   z = G^T w, sums of <r,z>
+//   q = A p with p = z + beta p_old formed in the gather, beta from the <r,z>
+//   sums in every workgroup, which also take the previous step's convergence
+//   decision from its <r,r> sums; sums of <p,q> (XPzLocal)
+//   the decision committed; alpha from the <r,z> and <p,q> sums, x += alpha p,
+//   r_new = r - alpha q in the gather of the next w = G r_new, sums of
+//   <r_new,r_new> (XRNew / EpiCgXrW)
+// with a finisher (FinCgStepPending) only at the end of a batch of steps.
 // BiCGSTAB (reference: iterative/BiCGSTAB/bicgstab.metal:12-326, spmv.cpp:699-888):
 //   v = A p, <r0,v> -> alpha             (k_spmv + FinBicgAlpha)
 //   s = r - alpha v formed inside the second SpMV's gather, t = A s,
@@ -58,6 +65,7 @@ struct Scalars {
   int32_t restart;   // next preconditioned direction update uses beta = 0
   int32_t halt;      // done || verify: the step kernels do nothing (kept by k_finish)
   int32_t skip_g;    // explicit preconditioner: w = G r of the current residual is up to date
+  int32_t pending;   // explicit preconditioner: the latest step's <r,r> sums await FinCgStep
 };
 
 // k_finish hook: after a finisher that owns the solver state, refresh the
@@ -235,6 +243,7 @@ struct FinCgInit {
     S->alpha = 0.0f;
     S->omega = 0.0f;
     S->skip_g = 1;
+    S->pending = 0;
     int32_t d = kRunning;
     if (S->crit == kCritRelres && tot[1] <= S->thresh) d = kConverged;
     if (!isfinite(tot[0]) || !isfinite(tot[1])) d = kBreakdown;
@@ -258,11 +267,12 @@ __device__ __forceinline__ float block_total(const CSum* __restrict__ partials, 
   return t;
 }
 
-// block_total of two single-value partial sets (pa: na blocks, pb: nb
-// blocks) in one pass: the loads of both are issued before the first merge.
-// Each total has the bits block_total<1> gives it.
+// block_total of two partial sets (pa: na blocks, pb: nb blocks, one value
+// taken every sb CSums) in one pass: the loads of both are issued before the
+// first merge. Each total has the bits block_total gives it (and k_finish
+// gives that value).
 __device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32_t na, const CSum* __restrict__ pb,
-                                             uint32_t nb, float& ta, float& tb) {
+                                             uint32_t nb, float& ta, float& tb, uint32_t sb = 1) {
   __shared__ float s_tot[2];
   CSum a[2] = {csum_zero(), csum_zero()};
   const uint32_t nmax = na > nb ? na : nb;
@@ -272,7 +282,7 @@ __device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32
     for (int j = 0; j < kPartialBatch; ++j) {
       const uint32_t i = base + j * kBlock + threadIdx.x;
       va[j] = i < na ? pa[i] : csum_zero();
-      vb[j] = i < nb ? pb[i] : csum_zero();
+      vb[j] = i < nb ? pb[(size_t)i * sb] : csum_zero();
     }
 #pragma unroll
     for (int j = 0; j < kPartialBatch; ++j) {
@@ -298,12 +308,21 @@ __device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32
 // in kernel_prologue; beta = 0 on the first step, after a restart, or for a
 // non-positive <r,z>_new (the step's update then records the breakdown).
 // p_old must not alias the p written by the epilogue.
+// The prologue also takes the previous step's pending decision in place of a
+// finisher launch: every workgroup sums that step's <r,r> partials (rr, nrr
+// blocks, value 0 of 2) and applies FinCgStep to a private copy of the state,
+// so all agree on the gate; workgroup 0 stages the decided fields in S2, also
+// when halted (every workgroup of this kernel reads S, so the next kernel
+// commits S2 to it).
 struct XPzLocal {
   const float* __restrict__ z;
   const float* __restrict__ pold;
   const Scalars* S;
+  Scalars* S2;
   GridRed zr;
   uint32_t nz;
+  GridRed rr;
+  uint32_t nrr;
   float beta;
   struct Raw {
     float z, p;
@@ -318,12 +337,18 @@ struct XPzLocal {
   }
 };
 
-__device__ __forceinline__ void kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& epi, int) {
-  const float rz_new = block_total<1>(xl.zr.partials, xl.nz, 0);
+__device__ __forceinline__ int take_pending_step(const Scalars* S, Scalars* S2, float rr, int halt);
+__device__ __forceinline__ void commit_step_state(Scalars* S, const Scalars* S2);
+
+__device__ __forceinline__ int kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& epi, int halt) {
+  float rz_new, rr;
+  block_total2(xl.zr.partials, xl.nz, xl.rr.partials, xl.nrr, rz_new, rr, 2);
+  halt = take_pending_step(xl.S, xl.S2, rr, halt);
   const Scalars* S = xl.S;
   const float beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
   xl.beta = beta;
   epi.xl.beta = beta;
+  return halt;
 }
 
 // The x / r update of an explicitly preconditioned CG step fused with the
@@ -332,8 +357,11 @@ __device__ __forceinline__ void kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& 
 // gathers the step's r), alpha = <r,z> / <p,q> from the block partials of the
 // z product (zr, nz blocks) and of q = A p (pq, npq blocks), reduced by every
 // workgroup in kernel_prologue; alpha = 0 when either is not positive (the
-// step's finisher then records a breakdown); workgroup 0 records pq, alpha,
-// <r,z>, clears the restart flag and marks w current. The epilogue updates the owned rows (x, x_lo
+// step's decision then records a breakdown). The kernel's gate is S2->halt
+// (the state as decided by XPzLocal's prologue); workgroup 0 commits S2 to S,
+// also when halted, then records pq, alpha, <r,z>, clears the restart flag,
+// marks w current and the step's decision pending.
+// The epilogue updates the owned rows (x, x_lo
 // when COMP, r_new, w) and accumulates <r_new, r_new> (second partial unused).
 struct XRNew {
   const float* __restrict__ r;
@@ -362,6 +390,7 @@ struct EpiCgXrW {
   const float* __restrict__ q;
   float* __restrict__ w;
   Scalars* S;
+  const Scalars* S2;
   GridRed pq;
   uint32_t npq;
   GridRed zr;
@@ -397,7 +426,7 @@ struct EpiCgXrW {
 };
 
 template <bool COMP>
-__device__ __forceinline__ void kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, int halt) {
+__device__ __forceinline__ int kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, int halt) {
   float pqv, rz;
   block_total2(epi.pq.partials, epi.npq, epi.zr.partials, epi.nz, pqv, rz);
   Scalars* S = epi.S;
@@ -405,14 +434,19 @@ __device__ __forceinline__ void kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, 
   const float alpha = ok ? rz / pqv : 0.0f;
   xl.alpha = alpha;
   epi.alpha = alpha;
-  epi.fold = S->collapse != 0;
-  if (!halt && blockIdx.x == 0 && threadIdx.x == 0) {
-    S->pq = ok ? pqv : -1.0f;
-    S->alpha = alpha;
-    S->rz = rz;
-    S->restart = 0;
-    S->skip_g = 1;
+  epi.fold = epi.S2->collapse != 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    commit_step_state(S, epi.S2);
+    if (!halt) {
+      S->pq = ok ? pqv : -1.0f;
+      S->alpha = alpha;
+      S->rz = rz;
+      S->restart = 0;
+      S->skip_g = 1;
+      S->pending = 1;
+    }
   }
+  return halt;
 }
 
 // x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
@@ -515,6 +549,50 @@ struct FinCgStep {
     }
   }
 };
+
+// The fields FinCgStep<kPcExplicit> changes, staged in S2 by XPzLocal's
+// prologue and copied to S by the next kernel's (one thread).
+__device__ __forceinline__ void stage_step_state(Scalars* S2, const Scalars& L) {
+  S2->iter = L.iter;
+  S2->rr = L.rr;
+  S2->collapse = L.collapse;
+  S2->done = L.done;
+  S2->verify = L.verify;
+  S2->halt = L.halt;
+  S2->pending = L.pending;
+}
+__device__ __forceinline__ void commit_step_state(Scalars* S, const Scalars* S2) {
+  const Scalars L = *S2;
+  stage_step_state(S, L);
+}
+
+// The decision of an explicitly preconditioned CG step (FinCgStep on the
+// <r,r> sums of its x / r update) for a step whose decision is still pending;
+// run at the end of a batch of steps, before the host reads the state.
+struct FinCgStepPending {
+  Scalars* S;
+  __device__ __forceinline__ bool active() const { return S->pending != 0 && S->halt == 0; }
+  __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
+    FinCgStep<kPcExplicit>{S}(tot);
+    S->pending = 0;
+  }
+  __device__ __forceinline__ void after() const { refresh_halt(S); }
+};
+
+// The previous step's pending decision, taken by every workgroup of a step
+// kernel on a private copy of the state (see XPzLocal); returns the gate.
+__device__ __forceinline__ int take_pending_step(const Scalars* S, Scalars* S2, float rr, int halt) {
+  Scalars L = *S;
+  if (!halt && L.pending) {
+    const float tot[2] = {rr, 0.0f};
+    FinCgStepPending f{&L};
+    f(tot);
+    f.after();
+    halt = L.halt;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) stage_step_state(S2, L);
+  return halt;
+}
 
 // pnew = z + beta pold with z = r, dinv*r, or an explicit vector (pnew may
 // alias pold); beta from the state (FinCgStep).

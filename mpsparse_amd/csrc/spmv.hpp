@@ -61,12 +61,15 @@ struct XPlain {
 
 // Per-workgroup set-up of a gather loader and its epilogue, run by every
 // thread before the row loop (a loader whose scalar comes from a grid
-// reduction computes it here); `halt` is the kernel's gate (nonzero: the
-// kernel will return after the prologue, which must then write nothing). The
-// default does nothing; overloads for specific loader types are found by
-// argument-dependent lookup.
+// reduction computes it here); `halt` is the kernel's gate as loaded, and the
+// returned value is the gate the kernel then tests (nonzero: it returns at
+// once). A prologue writes nothing when the gate was set on entry, unless it
+// says otherwise. The default does nothing; overloads for specific loader
+// types are found by argument-dependent lookup.
 template <typename XL, typename Epi>
-__device__ __forceinline__ void kernel_prologue(XL&, Epi&, int) {}
+__device__ __forceinline__ int kernel_prologue(XL&, Epi&, int halt) {
+  return halt;
+}
 
 // s = r - alpha * v formed on the fly (used to fuse a vector update into the
 // SpMV that consumes it); the owning rows also get s written by the epilogue.
@@ -338,8 +341,7 @@ template <typename V, int ND, typename XL, typename Epi>
 __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<V> A, XL xl, Epi epi, GridRed rb,
                                                  const int* __restrict__ done) {
   // the gate is tested after the prologue (whose loads overlap its latency)
-  const int halt = done ? *done : 0;
-  kernel_prologue(xl, epi, halt);
+  const int halt = kernel_prologue(xl, epi, done ? *done : 0);
   if (halt) return;
   __shared__ float sprod[kCap];
   constexpr int NA = ND > 0 ? ND : 1;

@@ -42,7 +42,7 @@ template <int B, int ND, typename XL, typename Epi>
 __global__ __launch_bounds__(kBlock) void k_bspmv(BcsrView A, XL xl, Epi epi, GridRed rb,
                                                   const int* __restrict__ done) {
   static_assert(B == 4 || B == 8 || B == 16, "block size");
-  if (done && *done) return;
+  if (kernel_prologue(xl, epi, done ? *done : 0)) return;
   constexpr int Q = B / 4;                    // quads per block line
   constexpr int G = B * Q;                    // lanes per block row
   constexpr int W = kWave / G;                // block rows per wave

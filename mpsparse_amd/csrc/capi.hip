@@ -1086,9 +1086,9 @@ struct OpCsr {
   // CG step head: p = z + beta p_old formed in the gather, q = A p, <p,q>
   // partials (first reduction half); returns the number of partials.
   template <int PC>
-  uint32_t cg_dir(const float* r, const float* pold, const float* dinv, const float* beta, float* q, float* pnew,
+  uint32_t cg_dir(const float* r, const float* pold, const float* dinv, CgDecide dec, float* q, float* pnew,
                   const int* done, hipStream_t s) const {
-    const XPNew<PC> xl{r, pold, dinv, beta};
+    const XPNew<PC> xl{r, pold, dinv, dec, 0.0f};
     return spmv_any<V, 1>(A, xl, EpiCgQ<XPNew<PC>>{q, pnew, xl}, FinNone{}, red_of(A), done, s);
   }
   // Explicit preconditioner: p = z + beta p_old in the gather with beta from
@@ -1135,9 +1135,9 @@ struct OpBcsr {
     return g;
   }
   template <int PC>
-  uint32_t cg_dir(const float* r, const float* pold, const float* dinv, const float* beta, float* q, float* pnew,
+  uint32_t cg_dir(const float* r, const float* pold, const float* dinv, CgDecide dec, float* q, float* pnew,
                   const int* done, hipStream_t s) const {
-    const X4PNew<PC> xl{r, pold, dinv, beta};
+    const X4PNew<PC> xl{r, pold, dinv, dec, 0.0f};
     return launch<1>(xl, EpiCgQ<X4PNew<PC>>{q, pnew, xl}, FinNone{}, done, s);
   }
   template <int ND, typename Epi, typename Fin>
@@ -1350,6 +1350,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   const float* B = w.v(Workspace::B);
   const float* D = A->dinv.as<float>();
   Scalars* S = w.S_dev();
+  Scalars* S2 = w.S_stage();
   const bool comp = compensate(o);
   if constexpr (std::is_same<Op, OpCsr<ValF32>>::value) {
     // small systems: the whole solve in one workgroup (no per-step launches)
@@ -1380,10 +1381,13 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   };
   // ping-pong directions: step k reads P[k & 1], writes P[(k + 1) & 1] (P, Z).
   // Two forms of the step compute the same recurrence: the direction formed in
-  // the SpMV's gather (3 launches; best while the gathered neighbourhood of r
+  // the SpMV's gather (2 launches; best while the gathered neighbourhood of r
   // and p_old stays in cache), or a separate p update feeding a plain SpMV
-  // (4 launches; best for large stencils). The first solve on a matrix times
-  // steps 0-7 in one form and 8-15 in the other and keeps the faster.
+  // (3 launches; best for large stencils). The first solve on a matrix times
+  // steps 0-7 in one form and 8-15 in the other and keeps the faster. The
+  // first kernel of a step takes the previous step's decision from the
+  // update's partials (rb2), stages it in S2 and the update commits it; the
+  // last step's decision is taken by a finisher at the end of each batch.
   float* Pb[2] = {P, w.v(Workspace::Z)};
   int k = 0;
   const GridRed rb2 = w.gr2();
@@ -1399,33 +1403,41 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
     const bool fused = o.cg_form == 1 || (o.cg_form == 0 && (tune ? k < 8 : form == 1));
     if (tune && (k == 0 || k == 8 || k == 16)) (void)hipEventRecord(tev[k / 8], s);
     if (tune && k == 16) {
-      // decide only if all 16 steps did work (none halted by convergence)
+      // decide only if all 16 steps did work (none halted by convergence): the
+      // state holds the decisions of steps 0-14 (the last one pending), and
+      // step 15 ran if none of them halted
       float ms_f = 0.f, ms_u = 0.f;
-      int32_t it = 0;
+      Scalars hs;
       if (hipEventSynchronize(tev[2]) == hipSuccess && hipEventElapsedTime(&ms_f, tev[0], tev[1]) == hipSuccess &&
           hipEventElapsedTime(&ms_u, tev[1], tev[2]) == hipSuccess &&
-          hipMemcpy(&it, &S->iter, sizeof it, hipMemcpyDeviceToHost) == hipSuccess && it >= 16)
+          hipMemcpy(&hs, S, sizeof hs, hipMemcpyDeviceToHost) == hipSuccess && hs.iter >= 15 && hs.halt == 0)
         form = ms_f <= ms_u ? 1 : 0;
     }
+    // the update's <r,r>, <r,z> partials of the previous step (none before the first)
+    const CgDecide dec{S, S2, rb2, k ? gv : 0u};
     uint32_t g;
     if (fused) {
-      g = jac ? op.template cg_dir<kPcJacobi>(R, pold, D, &S->beta, Q, pnew, &S->halt, s)
-              : op.template cg_dir<kPcNone>(R, pold, D, &S->beta, Q, pnew, &S->halt, s);
+      g = jac ? op.template cg_dir<kPcJacobi>(R, pold, D, dec, Q, pnew, &S->halt, s)
+              : op.template cg_dir<kPcNone>(R, pold, D, dec, Q, pnew, &S->halt, s);
     } else {
-      if (jac) k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, nullptr, n, S);
-      else k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, nullptr, n, S);
-      g = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S->halt, s);
+      if (jac) k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, n, dec);
+      else k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, n, dec);
+      g = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S2->halt, s);
     }
     if (jac) {
-      if (comp) k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      else k_cg_update_xr<kPcJacobi, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      finish<2>(FinCgStep<kPcJacobi>{S}, rb2, gv, s);
+      if (comp) k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
+      else k_cg_update_xr<kPcJacobi, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
     } else {
-      if (comp) k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      else k_cg_update_xr<kPcNone, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, rb2, rb, g);
-      finish<2>(FinCgStep<kPcNone>{S}, rb2, gv, s);
+      if (comp) k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
+      else k_cg_update_xr<kPcNone, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
     }
     ++k;
+  };
+  // the last step's decision, before the host reads the state
+  auto flush = [&]() {
+    if (!k) return;
+    if (jac) finish<2>(FinCgStepPending<kPcJacobi>{S}, rb2, gv, s);
+    else finish<2>(FinCgStepPending<kPcNone>{S}, rb2, gv, s);
   };
   // a restart (beta = 0) makes the next direction z whatever P[k & 1] holds
   auto verify = [&]() {
@@ -1437,7 +1449,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       finish<2>(FinVerify<0>{S}, rb, gv, s);
     }
   };
-  const msp_status st = run_solver(A, o, info, s, init, step, verify);
+  const msp_status st = run_solver(A, o, info, s, init, step, verify, flush);
   for (auto& e : tev)
     if (e) (void)hipEventDestroy(e);
   MSPCHK(st);
@@ -2308,7 +2320,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   };
   // the last step's decision, before the host reads the state
   auto flush = [&]() {
-    if (gu) finish<2>(FinCgStepPending{S}, rb2, gu, s);
+    if (gu) finish<2>(FinCgStepPending<kPcExplicit>{S}, rb2, gu, s);
   };
   // a verification rewrites the current residual; after a restart w = G r is
   // recomputed (gated by skip_g, which FinVerify clears only then)

@@ -4,12 +4,16 @@
 // on the device.
 //
 // CG (reference: iterative/BiCGSTAB/cgd.metal:11-90, spmv.cpp:528-659) per step,
-// three launches:
-//   p = z + beta p_old for every gathered column (XPNew, ping-pong p buffers),
-//   q = A p, block sums of <p,q>          (k_spmv / k_dia + EpiCgQ)
-//   alpha = <r,z> / <p,q> from those sums in every workgroup, x += alpha p,
-//   r -= alpha q, sums of <r,r>, <r,z>   (k_cg_update_xr<PC, true>)
-//   convergence, beta                    (k_finish + FinCgStep)
+// two launches:
+//   the previous step's convergence decision and beta, taken by every
+//   workgroup from that step's <r,r>, <r,z> sums (FinCgStep on a private copy
+//   of the state, staged in a second state block); p = z + beta p_old for
+//   every gathered column (XPNew, ping-pong p buffers), q = A p, block sums of
+//   <p,q>                                (k_spmv / k_dia + EpiCgQ)
+//   the decision committed; alpha = <r,z> / <p,q> from those sums in every
+//   workgroup, x += alpha p, r -= alpha q, sums of <r,r>, <r,z>
+//                                        (k_cg_update_xr)
+// and a finisher (FinCgStepPending) only at the end of a batch of steps.
 // FSAI-preconditioned CG (M = G^T G) keeps z explicit, three launches a step:
 //   z = G^T w, sums of <r,z>
 //   q = A p with p = z + beta p_old formed in the gather, beta from the <r,z>
@@ -109,14 +113,27 @@ struct FinBicgOmega {
 
 __global__ void k_set_restart(Scalars* S) { S->restart = 1; }
 
+// Where a step kernel takes the previous CG step's pending decision: the
+// state S (read by every workgroup), the staging block S2 (written by
+// workgroup 0, committed to S by the next kernel) and the <r,r>, <r,z> block
+// partials of that step's x / r update (rr, nrr blocks, 2 values each).
+struct CgDecide {
+  const Scalars* S;
+  Scalars* S2;
+  GridRed rr;
+  uint32_t nrr;
+};
+
 // CG direction formed on the fly: p = z + beta p_old, z = r (PC none) or
-// D^-1 r (Jacobi); beta = 0 on the first step and after a restart.
+// D^-1 r (Jacobi); beta (0 on the first step and after a restart) from the
+// decision the kernel's prologue takes (dec).
 template <int PC>
 struct XPNew {
   const float* __restrict__ r;
   const float* __restrict__ pold;
   const float* __restrict__ dinv;
-  const float* __restrict__ beta;
+  CgDecide dec;
+  float beta;
   struct Raw {
     float r, p, d;
   };
@@ -126,18 +143,18 @@ struct XPNew {
   __device__ __forceinline__ float make(Raw w) const {
     float z = w.r;
     if (PC == kPcJacobi) z *= w.d;
-    return z + (*beta) * w.p;
+    return z + beta * w.p;
   }
   __device__ __forceinline__ float operator()(uint32_t c) const {
     float z = r[c];
     if (PC == kPcJacobi) z *= dinv[c];
-    return z + (*beta) * pold[c];
+    return z + beta * pold[c];
   }
   __device__ __forceinline__ f4 load4(u4 c) const {
     if (quad_run(c)) {
       f4 z = *reinterpret_cast<const f4*>(r + c.x);
       if (PC == kPcJacobi) z *= *reinterpret_cast<const f4*>(dinv + c.x);
-      return z + (*beta) * *reinterpret_cast<const f4*>(pold + c.x);
+      return z + beta * *reinterpret_cast<const f4*>(pold + c.x);
     }
     return f4{(*this)(c.x), (*this)(c.y), (*this)(c.z), (*this)(c.w)};
   }
@@ -149,16 +166,17 @@ struct X4PNew {
   const float* __restrict__ r;
   const float* __restrict__ pold;
   const float* __restrict__ dinv;
-  const float* __restrict__ beta;
+  CgDecide dec;
+  float beta;
   __device__ __forceinline__ float operator()(uint32_t c) const {
     float z = r[c];
     if (PC == kPcJacobi) z *= dinv[c];
-    return z + (*beta) * pold[c];
+    return z + beta * pold[c];
   }
   __device__ __forceinline__ f4 load4(uint32_t c) const {
     f4 z = *reinterpret_cast<const f4*>(r + c);
     if (PC == kPcJacobi) z *= *reinterpret_cast<const f4*>(dinv + c);
-    return z + (*beta) * *reinterpret_cast<const f4*>(pold + c);
+    return z + beta * *reinterpret_cast<const f4*>(pold + c);
   }
 };
 
@@ -267,12 +285,12 @@ __device__ __forceinline__ float block_total(const CSum* __restrict__ partials, 
   return t;
 }
 
-// block_total of two partial sets (pa: na blocks, pb: nb blocks, one value
-// taken every sb CSums) in one pass: the loads of both are issued before the
-// first merge. Each total has the bits block_total gives it (and k_finish
-// gives that value).
+// block_total of two partial sets (pa: na blocks, one value taken every sa
+// CSums; pb: nb blocks, every sb) in one pass: the loads of both are issued
+// before the first merge. Each total has the bits block_total gives it (and
+// k_finish gives that value).
 __device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32_t na, const CSum* __restrict__ pb,
-                                             uint32_t nb, float& ta, float& tb, uint32_t sb = 1) {
+                                             uint32_t nb, float& ta, float& tb, uint32_t sb = 1, uint32_t sa = 1) {
   __shared__ float s_tot[2];
   CSum a[2] = {csum_zero(), csum_zero()};
   const uint32_t nmax = na > nb ? na : nb;
@@ -281,7 +299,7 @@ __device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32
 #pragma unroll
     for (int j = 0; j < kPartialBatch; ++j) {
       const uint32_t i = base + j * kBlock + threadIdx.x;
-      va[j] = i < na ? pa[i] : csum_zero();
+      va[j] = i < na ? pa[(size_t)i * sa] : csum_zero();
       vb[j] = i < nb ? pb[(size_t)i * sb] : csum_zero();
     }
 #pragma unroll
@@ -337,13 +355,17 @@ struct XPzLocal {
   }
 };
 
-__device__ __forceinline__ int take_pending_step(const Scalars* S, Scalars* S2, float rr, int halt);
+template <int PC>
+__device__ __forceinline__ int take_pending_step(const Scalars* S, Scalars* S2, const float (&tot)[2], int halt,
+                                                 float& beta);
 __device__ __forceinline__ void commit_step_state(Scalars* S, const Scalars* S2);
 
 __device__ __forceinline__ int kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& epi, int halt) {
   float rz_new, rr;
   block_total2(xl.zr.partials, xl.nz, xl.rr.partials, xl.nrr, rz_new, rr, 2);
-  halt = take_pending_step(xl.S, xl.S2, rr, halt);
+  const float tot[2] = {rr, 0.0f};
+  float unused;
+  halt = take_pending_step<kPcExplicit>(xl.S, xl.S2, tot, halt, unused);
   const Scalars* S = xl.S;
   const float beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
   xl.beta = beta;
@@ -449,11 +471,14 @@ __device__ __forceinline__ int kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, i
   return halt;
 }
 
-// x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> iteration bookkeeping.
+// x += alpha p; r -= alpha q; <r,r>; <r, dinv r> (Jacobi) -> the next step's
+// decision (pending).
 // alpha = <r,z> / <p,q> with <p,q> from the SpMV's block partials (pq, npq
-// blocks), computed identically by every workgroup; block 0 records pq and
-// alpha, and a non-positive <p,q> leaves x and r alone (FinCgStep then stops
-// with a breakdown). COMP: the iterate is x + xl with xl carrying the rounding
+// blocks), computed identically by every workgroup; <r,z> and the gate come
+// from S2, the state as decided by the step's first kernel, which block 0
+// commits to S (also when halted) before it records pq and alpha; a
+// non-positive <p,q> leaves x and r alone (the decision then stops with a
+// breakdown). COMP: the iterate is x + xl with xl carrying the rounding
 // error of x += alpha p (for targets near FP32's attainable accuracy);
 // otherwise x alone, xl untouched. (The explicitly preconditioned step fuses
 // this update into its factor product: XRNew / EpiCgXrW.)
@@ -461,20 +486,22 @@ template <int PC, bool COMP>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ xl,
                                                          float* __restrict__ r, const float* __restrict__ p,
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
-                                                         uint32_t n, Scalars* S, GridRed rb, GridRed pq,
-                                                         uint32_t npq) {
+                                                         uint32_t n, Scalars* S, const Scalars* S2, GridRed rb,
+                                                         GridRed pq, uint32_t npq) {
   // the gate is tested after the partials are in (their loads overlap its
-  // latency); a halted step still writes nothing
-  const int32_t halt = S->halt;
-  const float pqv = block_total<1>(pq.partials, npq, 0), rz = S->rz;
+  // latency); a halted step writes nothing but the commit
+  const int32_t halt = S2->halt;
+  const float pqv = block_total<1>(pq.partials, npq, 0), rz = S2->rz;
+  if (blockIdx.x == 0 && threadIdx.x == 0) commit_step_state(S, S2);
   if (halt) return;
   const bool ok = finite_pos(pqv);
   const float alpha = ok ? rz / pqv : 0.0f;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S->pq = ok ? pqv : -1.0f;
     S->alpha = alpha;
+    S->pending = 1;
   }
-  [[maybe_unused]] const bool fold = S->collapse != 0;
+  [[maybe_unused]] const bool fold = S2->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
@@ -550,8 +577,8 @@ struct FinCgStep {
   }
 };
 
-// The fields FinCgStep<kPcExplicit> changes, staged in S2 by XPzLocal's
-// prologue and copied to S by the next kernel's (one thread).
+// The fields FinCgStep changes, staged in S2 by the kernel that takes the
+// decision and copied to S by the next kernel (one thread).
 __device__ __forceinline__ void stage_step_state(Scalars* S2, const Scalars& L) {
   S2->iter = L.iter;
   S2->rr = L.rr;
@@ -560,53 +587,83 @@ __device__ __forceinline__ void stage_step_state(Scalars* S2, const Scalars& L) 
   S2->verify = L.verify;
   S2->halt = L.halt;
   S2->pending = L.pending;
+  S2->beta = L.beta;
+  S2->rz = L.rz;
 }
 __device__ __forceinline__ void commit_step_state(Scalars* S, const Scalars* S2) {
   const Scalars L = *S2;
   stage_step_state(S, L);
 }
 
-// The decision of an explicitly preconditioned CG step (FinCgStep on the
-// <r,r> sums of its x / r update) for a step whose decision is still pending;
-// run at the end of a batch of steps, before the host reads the state.
+// The decision of a CG step (FinCgStep on the <r,r>, <r,z> sums of its x / r
+// update) for a step whose decision is still pending; run at the end of a
+// batch of steps, before the host reads the state.
+template <int PC>
 struct FinCgStepPending {
   Scalars* S;
   __device__ __forceinline__ bool active() const { return S->pending != 0 && S->halt == 0; }
   __device__ __forceinline__ void operator()(const float (&tot)[2]) const {
-    FinCgStep<kPcExplicit>{S}(tot);
+    FinCgStep<PC>{S}(tot);
     S->pending = 0;
   }
   __device__ __forceinline__ void after() const { refresh_halt(S); }
 };
 
 // The previous step's pending decision, taken by every workgroup of a step
-// kernel on a private copy of the state (see XPzLocal); returns the gate.
-__device__ __forceinline__ int take_pending_step(const Scalars* S, Scalars* S2, float rr, int halt) {
+// kernel on a private copy of the state; workgroup 0 stages the result in S2,
+// also when halted. Returns the gate; beta is the direction update's.
+template <int PC>
+__device__ __forceinline__ int take_pending_step(const Scalars* S, Scalars* S2, const float (&tot)[2], int halt,
+                                                 float& beta) {
   Scalars L = *S;
   if (!halt && L.pending) {
-    const float tot[2] = {rr, 0.0f};
-    FinCgStepPending f{&L};
+    FinCgStepPending<PC> f{&L};
     f(tot);
     f.after();
     halt = L.halt;
   }
+  beta = L.beta;
   if (blockIdx.x == 0 && threadIdx.x == 0) stage_step_state(S2, L);
   return halt;
 }
 
-// pnew = z + beta pold with z = r, dinv*r, or an explicit vector (pnew may
-// alias pold); beta from the state (FinCgStep).
+// The same for a plain or Jacobi CG step: both update sums in one pass.
+template <int PC>
+__device__ __forceinline__ int take_cg_decision(const CgDecide& d, int halt, float& beta) {
+  float tot[2];
+  block_total2(d.rr.partials, d.nrr, d.rr.partials + 1, d.nrr, tot[0], tot[1], 2, 2);
+  return take_pending_step<PC>(d.S, d.S2, tot, halt, beta);
+}
+
+template <int PC>
+__device__ __forceinline__ int kernel_prologue(XPNew<PC>& xl, EpiCgQ<XPNew<PC>>& epi, int halt) {
+  float beta;
+  halt = take_cg_decision<PC>(xl.dec, halt, beta);
+  xl.beta = beta;
+  epi.xl.beta = beta;
+  return halt;
+}
+template <int PC>
+__device__ __forceinline__ int kernel_prologue(X4PNew<PC>& xl, EpiCgQ<X4PNew<PC>>& epi, int halt) {
+  float beta;
+  halt = take_cg_decision<PC>(xl.dec, halt, beta);
+  xl.beta = beta;
+  epi.xl.beta = beta;
+  return halt;
+}
+
+// pnew = z + beta pold with z = r or dinv*r (pnew may alias pold), after the
+// previous step's pending decision and beta (take_cg_decision; staged in
+// dec.S2, whose halt gates the SpMV that follows, committed by the update).
 template <int PC>
 __global__ __launch_bounds__(kBlock) void k_cg_update_p(float* pnew, const float* pold, const float* __restrict__ r,
-                                                        const float* __restrict__ dinv, const float* __restrict__ z,
-                                                        uint32_t n, const Scalars* S) {
-  if (S->halt) return;
-  const float beta = S->beta;
+                                                        const float* __restrict__ dinv, uint32_t n, CgDecide dec) {
+  float beta;
+  const int halt = take_cg_decision<PC>(dec, dec.S->halt, beta);
+  if (halt) return;
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
-    f4 zv;
-    if (PC == kPcExplicit) zv = reinterpret_cast<const f4*>(z)[c];
-    else zv = reinterpret_cast<const f4*>(r)[c];
+    f4 zv = reinterpret_cast<const f4*>(r)[c];
     if (PC == kPcJacobi) zv *= reinterpret_cast<const f4*>(dinv)[c];
     reinterpret_cast<f4*>(pnew)[c] = zv + beta * reinterpret_cast<const f4*>(pold)[c];
   }

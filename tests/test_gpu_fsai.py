@@ -118,6 +118,42 @@ def test_classical_fsai(name):
         assert np.all(np.abs(row[:-1]) <= 1e-5 * abs(row[-1])), (i, row)
 
 
+def test_pcg_fsai_step_limit_inside_a_batch():
+    """The FSAI-PCG step's bookkeeping is taken in the next step's direction
+    kernel, or by a finisher at the end of a batch: a step limit inside a batch
+    (batches of 4, 8, ...: step 11 is inside the second; verification stays on
+    so the host's run limit does not cut a batch at the step limit) stops after
+    exactly that many steps, with the iterate a batch of one step gives, and
+    that iterate is FP64 PCG's with the same G after that many steps (one step
+    fewer or more is much further away)."""
+    torch = _torch()
+    from mpsparse_amd.learned_fsai import solve
+
+    M = diffusion_3d(16, sigma=2.0, seed=3)
+    S = _scipy(M)
+    b = np.random.default_rng(4).standard_normal(M.num_rows).astype(np.float32)
+    A, F = _handles(M)
+    assert F.classical() == 0
+    lim = 11
+    xs = []
+    for ce in (16, 1):
+        say("fsai step limit", ce)
+        x = torch.full((M.num_rows,), np.nan, device="cuda")
+        info = solve(A, F, t_f32(b), x, tol=1e-12, maxiter=lim, check_every=ce)
+        assert info["status"] == "maxiter" and info["iterations"] == lim, info
+        xs.append(x.cpu().numpy())
+    assert np.array_equal(xs[0], xs[1])
+    g = F.values().cpu().numpy().astype(np.float64)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    G = OF.G_matrix(M.num_rows, r, c, g)
+    b64 = b.astype(np.float64)
+    ref = {k: K.cg(S, b64, tol=0.0, maxiter=k, M_apply=lambda v: G.T @ (G @ v))[0] for k in (lim - 1, lim, lim + 1)}
+    nrm = np.linalg.norm(ref[lim])
+    err = np.linalg.norm(xs[0] - ref[lim]) / nrm
+    gap = min(np.linalg.norm(ref[k] - ref[lim]) for k in (lim - 1, lim + 1)) / nrm
+    assert err < 1e-4 and gap > 10 * err, (err, gap)
+
+
 def test_apply_is_gt_g():
     torch = _torch()
     M = diffusion_3d(12, sigma=1.5, seed=2)

@@ -244,8 +244,44 @@ def test_breakdown_and_limits():
     assert info["status"] == "converged" and info["iterations"] == 0 and np.all(x == 0), info
 
 
-@pytest.mark.parametrize("method,precond", [("cgd", "none"), ("cgd", "jacobi"), ("bicgstab", "none")])
-def test_batch_sizes_do_not_change_the_solve(method, precond):
+@pytest.mark.parametrize("precond,form", [("none", "fused"), ("none", "separate"), ("jacobi", "fused"),
+                                          ("jacobi", "separate")])
+def test_step_limit_inside_a_batch(precond, form):
+    """A step's bookkeeping (count, convergence, maxiter) is taken by the next
+    step's first kernel, or by a finisher at the end of a batch: a step limit
+    inside a batch of 16 stops after exactly that many steps, with the iterate
+    a batch of one step gives, and that iterate is CG's after that many steps
+    (FP64 oracle; one step fewer or more is much further away). Verification
+    stays on so that the host's run limit (maxiter plus two batches) does not
+    cut the last batch at the limit; batches of 4, 8, 16, 16 put step 37 inside
+    the fourth. (A mild contrast keeps FP32 CG within 1e-6 of FP64 CG after 37
+    steps; at sigma 1.5 they part by 8%.)"""
+    M = diffusion_2d(60, sigma=0.2, seed=9)
+    b = np.random.default_rng(9).standard_normal(M.num_rows).astype(np.float32)
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    S = _scipy(M.row_ptr, M.col_ind, M.vals, M.num_rows)
+    lim = 37
+    xs = []
+    for ce in (16, 1):
+        say("step limit", precond, form, ce)
+        x, info = _solve(A, b, tol=1e-12, maxiter=lim, check_every=ce, precond=precond, cg_form=form)
+        assert info["status"] == "maxiter" and info["iterations"] == lim, info
+        xs.append(x)
+    assert np.array_equal(xs[0], xs[1])
+    d = S.diagonal()
+    Mj = (lambda v: v / d) if precond == "jacobi" else None
+    b64 = b.astype(np.float64)
+    ref = {k: K.cg(S, b64, tol=0.0, maxiter=k, M_apply=Mj)[0] for k in (lim - 1, lim, lim + 1)}
+    nrm = np.linalg.norm(ref[lim])
+    err = np.linalg.norm(xs[0] - ref[lim]) / nrm
+    gap = min(np.linalg.norm(ref[k] - ref[lim]) for k in (lim - 1, lim + 1)) / nrm
+    assert err < 1e-4 and gap > 10 * err, (err, gap)
+
+
+@pytest.mark.parametrize("method,precond,form", [("cgd", "none", "separate"), ("cgd", "jacobi", "separate"),
+                                                 ("cgd", "none", "fused"), ("cgd", "jacobi", "fused"),
+                                                 ("bicgstab", "none", None)])
+def test_batch_sizes_do_not_change_the_solve(method, precond, form):
     """The host enqueues the steps in batches (growing at the start, shrinking
     near convergence from the residual's observed rate) and halted steps do
     nothing, so the iterate, the step count and the verifications are the same
@@ -255,10 +291,10 @@ def test_batch_sizes_do_not_change_the_solve(method, precond):
     A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
     outs = []
     for ce in (1, 5, 16):
-        say("batch sizes", method, precond, ce)
+        say("batch sizes", method, precond, form, ce)
         kw = dict(tol=1e-6, check_every=ce)
         if method == "cgd":
-            kw.update(precond=precond, cg_form="separate")
+            kw.update(precond=precond, cg_form=form)
         x, info = _solve(A, b, method, **kw)
         outs.append((x, info))
     x0, i0 = outs[0]

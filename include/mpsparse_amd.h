@@ -78,8 +78,8 @@ typedef struct msp_solve_opts {
                             r = b - A x with compensated FP32 sums and continue from it
                             unless it meets tol too (MSP_CRIT_RELRES only) */
   int32_t cg_form;       /* CG step form: 0 (default) timed on the first solve of a matrix,
-                            1 direction formed in the SpMV gather (3 launches/step),
-                            2 separate direction update (4 launches/step) */
+                            1 direction formed in the SpMV gather (2 launches/step),
+                            2 separate direction update (3 launches/step) */
   int32_t reserved[2];
 } msp_solve_opts;
 

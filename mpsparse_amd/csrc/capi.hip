@@ -1383,7 +1383,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   // Two forms of the step compute the same recurrence: the direction formed in
   // the SpMV's gather (2 launches; best while the gathered neighbourhood of r
   // and p_old stays in cache), or a separate p update feeding a plain SpMV
-  // (3 launches; best for large stencils). The first solve on a matrix times
+  // (3 launches). The first solve on a matrix times
   // steps 0-7 in one form and 8-15 in the other and keeps the faster. The
   // first kernel of a step takes the previous step's decision from the
   // update's partials (rb2), stages it in S2 and the update commits it; the

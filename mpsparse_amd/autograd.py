@@ -7,10 +7,14 @@ bridge.cpp:151-185, which does this with a CPU QR factorisation).
     dL/db = lam
     dL/dA_ij = -lam_i x_j       (on the pattern, in the caller's entry order)
 
+The matrix is stored in FP32 (the library's matrix format): ``values`` in
+float32, or in float64 when every value is exactly representable in FP32;
+other float64 values raise instead of being rounded silently, since the
+solution and the gradients would then belong to a different matrix.
 ``b`` in float32 runs the FP32 solvers (relative residual ``tol``, default
 1e-6); ``b`` in float64 runs FP64 iterative refinement around them (default
-1e-10), so gradients can be checked to FP64 accuracy. A solve that does not
-converge raises.
+1e-10): x is then accurate to FP64 for the stored matrix, and so are the
+gradients. A solve that does not converge raises.
 """
 from __future__ import annotations
 
@@ -23,10 +27,25 @@ _C = load_ext()
 __all__ = ["sparse_solve"]
 
 
+def _fp32_values(values):
+    """values as float32; float64 values must round-trip through FP32 exactly."""
+    v = values.detach()
+    if v.dtype == torch.float32:
+        return v.contiguous()
+    if v.dtype != torch.float64:
+        raise TypeError(f"sparse_solve: values must be float32 or float64, got {v.dtype}")
+    v32 = v.float()
+    if not torch.equal(v32.double(), v):
+        raise ValueError("sparse_solve: the matrix is stored in FP32 and these float64 values are not exactly "
+                         "representable in FP32; pass values.float() (or values.float().double()) to solve the "
+                         "FP32-rounded matrix")
+    return v32.contiguous()
+
+
 def _matrix(rows, cols, values, n):
     keys = (rows.to(torch.int64) << 32) | cols.to(torch.int64)
     e = torch.empty(0)
-    return _C.csr_tensor(keys.contiguous(), values.detach().float().contiguous(), e, e, e, n, n)
+    return _C.csr_tensor(keys.contiguous(), _fp32_values(values), e, e, e, n, n)
 
 
 def _solve(A, b, method, tol, maxiter):
@@ -75,4 +94,5 @@ def sparse_solve(rows, cols, values, b, method="cg", tol=None, maxiter=10000):
         raise ValueError("b must be a vector")
     if tol is None:
         tol = 1e-10 if b.dtype == torch.float64 else 1e-6
+    _fp32_values(values)  # refuse values the FP32 matrix cannot hold before any work
     return _SparseSolve.apply(values, b, rows, cols, b.numel(), method, float(tol), int(maxiter))

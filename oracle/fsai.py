@@ -21,6 +21,8 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
+from .spmv import round_bf16
+
 KIN, KH = 8, 64
 OW1, OB1 = 0, 512
 OW2 = OB1 + KH
@@ -120,12 +122,10 @@ def probe_loss_grad(A, rows, cols, g, W):
 
 
 def _bf16(a, on):
+    """FP32 rounding, then BF16 round-to-nearest-even (oracle.spmv.round_bf16), as float64."""
     if not on:
         return np.asarray(a, dtype=np.float64)
-    a32 = np.ascontiguousarray(a, dtype=np.float32)
-    u = a32.view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
-    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    return round_bf16(np.asarray(a, dtype=np.float32)).astype(np.float64)
 
 
 def unpack(P):

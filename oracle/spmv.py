@@ -87,10 +87,20 @@ def bsr_spmv(brow_ptr, bcol, blocks, x, dtype=np.float64):
 
 
 def round_bf16(a):
-    """Round-to-nearest-even FP32 -> BF16, widened back to float32."""
-    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
-    return (u.astype(np.uint32) << 16).view(np.float32)
+    """Round-to-nearest-even FP32 -> BF16, widened back to float32.
+
+    Finite values and infinities: add half an ulp of the BF16 result minus one,
+    plus the lowest kept bit (ties go to the even neighbour), then truncate;
+    values past the largest BF16 overflow to infinity. NaNs stay NaN (quietened,
+    sign and high payload bits kept); adding the bias to a NaN's bits could
+    carry into the sign and give -0. Pinned against torch's bfloat16 cast in
+    tests/test_oracle_bf16.py."""
+    u32 = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+    u = u32.astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32)
+    nan = ((u32 & 0x7F800000) == 0x7F800000) & ((u32 & 0x007FFFFF) != 0)
+    r = np.where(nan, (u32 >> 16) | 0x40, r).astype(np.uint32)
+    return (r << 16).view(np.float32)
 
 
 def round16_rowsum(row_ptr, col_ind, vals, fmt="bf16", num_cols=None):

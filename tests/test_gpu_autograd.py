@@ -78,3 +78,28 @@ def test_sparse_solve_rejects_bad_input():
     z = torch.zeros(4, dtype=torch.int64, device="cuda")
     with pytest.raises(ValueError):
         sparse_solve(z, z, torch.ones(4, device="cuda"), torch.ones(4, device="cuda"), method="lu")
+
+
+def test_sparse_solve_float64_values():
+    """float64 values that FP32 holds exactly: x and every gradient (dL/dvalues
+    in float64) agree with dense FP64 autograd to 1e-8."""
+    import torch
+
+    from mpsparse_amd.autograd import sparse_solve
+
+    M = diffusion_2d(12, sigma=1.0, seed=1)
+    rows = np.repeat(np.arange(M.num_rows), np.diff(M.row_ptr.astype(np.int64)))
+    cols, n = M.col_ind.astype(np.int64), M.num_rows
+    vals64 = M.vals.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(7)
+    b, w = rng.standard_normal(n), rng.standard_normal(n)
+    x_ref, gv_ref, gb_ref = _dense_ref(rows, cols, vals64, b, w)
+    v = torch.tensor(vals64, dtype=torch.float64, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, dtype=torch.float64, device="cuda", requires_grad=True)
+    say("sparse_solve float64 values")
+    x = sparse_solve(torch.tensor(rows, device="cuda"), torch.tensor(cols, device="cuda"), v, bt, tol=1e-12)
+    (x * torch.tensor(w, dtype=torch.float64, device="cuda")).sum().backward()
+    assert v.grad.dtype == torch.float64
+    assert _rel(x.detach().cpu().numpy(), x_ref) <= 1e-8
+    assert _rel(bt.grad.cpu().numpy(), gb_ref) <= 1e-8
+    assert _rel(v.grad.cpu().numpy(), gv_ref) <= 1e-8

@@ -1,24 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the hot path on one MI355X.
 
-Headline (`value`): time to true relative residual 1e-6 of learned-FSAI PCG
-(BF16 matrix values, FP32 Krylov, FP64 refinement, network set-up included)
-against plain FP32 CG on the same GPU, as the ratio plain-CG seconds /
-learned-PCG seconds, on a held-out 128^3 7-point log-normal diffusion system
-(sigma 2, n = 2,097,152; synthetic, generated on the GPU). Plain CG is the
-faster of FP32 CG and FP64-refined CG. Jacobi-PCG and classical FSAI-PCG are
-reported beside it.
+Headline (`value`): time to FP64 true relative residual 1e-6 of learned-FSAI
+PCG (the stated 5-input edge network; BF16 matrix values, FP32 Krylov, FP64
+refinement; preconditioner set-up included) against plain CG on the same GPU,
+as the ratio plain-CG seconds / learned-PCG seconds, on a held-out 128^3
+7-point log-normal diffusion system (sigma 2, n = 2,097,152; synthetic,
+generated on the GPU). A "step" is one such learned-PCG solve: `--warmup`
+untimed, then `--steps` timed; ms_per_step is their median. Plain CG is the
+faster of FP32 CG and FP64-refined CG (median of `--reps`). Jacobi-PCG,
+classical FSAI-PCG and the 8-feature network variant are reported beside it.
 
 SpMV part (`spmv`, `roofline`): the block-sparse system, n = 1,048,576 (65,536
 block-rows of 16x16 blocks, diagonal block + 8 random block-columns;
 nnz = 150,994,944; seed 0): COO -> CSR on the GPU from shuffled packed u64 keys,
-then `--steps` CSR SpMV launches after `--warmup`, bracketed by HIP events on
-the launch stream; HBM throughput from algorithmic bytes
-nnz*(4+4) + 4(n+1) + 4n + 4n. Also BiCGSTAB to 1e-6 on it, CG on the 64x64
-Laplacian, BCSR-16, and the 256^3 precision path (FP64 refinement to 1e-10).
+then `--spmv-steps` CSR SpMV launches bracketed by HIP events on the launch
+stream; HBM throughput from algorithmic bytes nnz*(4+4) + 4(n+1) + 4n + 4n.
+Also BiCGSTAB to 1e-6 on it, CG on the 64x64 Laplacian, BCSR-16, the 256^3
+precision path (FP64 refinement to 1e-10), the 512-system batched solver and
+the direct QR solver.
 
-CPU baseline: scipy CG on the 128^3 system, a bounded sample of iterations
-on the host, extrapolated to plain CG's iteration count; scipy CSR matvec.
+CPU baseline (one thread, this host): scipy Jacobi-PCG to 1e-6 on the headline
+system (measured), a bounded plain scipy CG sample there, scipy CG on the 64x64
+Laplacian (median of 5), scipy matvec and BiCGSTAB on the block-sparse system.
 
 Prints one JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 """
@@ -42,14 +46,19 @@ def spmv_bytes(n, m, nnz, val_bytes=4):
 
 
 def measured_traffic(key):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    """(HBM bytes per launch, where the figure comes from) from the committed
+    rocprofv3 PMC summary profiles/pmc_traffic.json, or (None, None): the
+    counters need their own profiler runs, so this run does not measure them."""
     path = os.path.join(HERE, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get(key, {}).get("bytes_per_launch")
+        e = d.get(key)
+        if not e:
+            return None, None
+        return e.get("bytes_per_launch"), "profiles/pmc_traffic.json: " + d.get("_measured", "")
     except (OSError, ValueError):
-        return None
+        return None, None
 
 
 def time_launches(fn, steps, warmup):
@@ -81,8 +90,11 @@ def true_relres_gpu(rows, cols, vals, x, b):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10,
+                    help="timed headline solves (learned-FSAI PCG, BF16 values, 128^3 sigma 2)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed headline solves before them")
+    ap.add_argument("--reps", type=int, default=5, help="timed solves per solver (median) in the comparisons")
+    ap.add_argument("--spmv-steps", type=int, default=50, help="timed launches of each SpMV / stencil product")
     ap.add_argument("--sections", default="fsai,spmv,solve,bcsr,cpu,precision,batched,direct",
                     help="comma list of fsai, spmv, solve, bcsr, cpu, precision, batched, direct, copy "
                          "(copy: 1 GiB device copy for counter calibration)")
@@ -107,7 +119,7 @@ def main():
            "data": "synthetic (seeded generator, random-init values)"}
     fsai = None
     if "fsai" in sections:
-        fsai = fsai_section(dev, args.steps, args.fsai_weights)
+        fsai = fsai_section(dev, args.steps, args.warmup, args.reps, args.fsai_weights)
         out["learned_pcg"] = {k: v for k, v in fsai.items() if not k.startswith("_")}
 
     if "copy" in sections:
@@ -154,7 +166,7 @@ def main():
     if "spmv" in sections:
         x = torch.randn(n, device=dev)
         y = torch.empty(n, device=dev)
-        ms = time_launches(lambda: A.mv(x, y), args.steps, args.warmup)
+        ms = time_launches(lambda: A.mv(x, y), args.spmv_steps, 5)
         alg = spmv_bytes(n, n, nnz)
         gbs = alg / (ms * 1e-3) / 1e9
         # spot check of this y against an FP64 product computed by torch
@@ -163,10 +175,11 @@ def main():
         out["spmv"] = {"GBps": gbs, "ms": ms, "workload": "block_sparse n=1048576 nnz=150994944"}
         out["value"] = gbs
         out["ms_per_step"] = ms
-        traffic = measured_traffic("csr_spmv_block_sparse")
+        traffic, tsrc = measured_traffic("csr_spmv_block_sparse")
         out["roofline"] = {"bound": "hbm", "kernel": "k_spmv<ValF32,0,XPlain,EpiStore>", "achieved": gbs,
                            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-                           "traffic": traffic, "algorithmic_bytes": alg}
+                           "traffic": traffic, "traffic_source": tsrc, "algorithmic_bytes": alg,
+                           "launches_timed": args.spmv_steps}
         out["spmv_check"] = {"max_rel_err_vs_fp64": err}
         log(f"spmv {ms:.4f} ms  {gbs:.1f} GB/s  err {err:.2e}")
 
@@ -216,7 +229,7 @@ def main():
         balg = nblk * (16 * 16 * 4 + 4) + 4 * (n // 16 + 1) + 8 * n
         xb = torch.randn(n, device=dev)
         yb = torch.empty(n, device=dev)
-        bms = time_launches(lambda: Bm.mv(xb, yb), args.steps, args.warmup)
+        bms = time_launches(lambda: Bm.mv(xb, yb), args.spmv_steps, 5)
         yref = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * xb.double()[cols])
         berr = float((yb.double() - yref).abs().max() / yref.abs().max())
         xs = torch.zeros(n, device=dev)
@@ -238,35 +251,18 @@ def main():
         log("bcsr16", out["bcsr16"])
         del Bm
 
+    cpu = None
     if "cpu" in sections:
-        import scipy.sparse as sp
-
-        from oracle import krylov as K
-
-        cores = len(os.sched_getaffinity(0))
-        S = sp.csr_matrix((cv.cpu().numpy(), ci.cpu().numpy(), rp.cpu().numpy()), shape=(n, n))
-        xh = np.random.default_rng(0).standard_normal(n).astype(np.float32)
-        S @ xh
-        reps = 5
-        t = time.perf_counter()
-        for _ in range(reps):
-            S @ xh
-        cpu_ms = (time.perf_counter() - t) / reps * 1e3
-        out["cpu_baseline"] = {"value": spmv_bytes(n, n, nnz) / (cpu_ms * 1e-3) / 1e9, "unit": "GB/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"scipy CSR matvec (single-threaded) on the full matrix, {reps} calls, "
-                                         f"{cpu_ms:.1f} ms each; host has {cores} cores"}
-        log("cpu spmv", out["cpu_baseline"])
-        if "solve" in sections:
-            t = time.perf_counter()
-            xc, itc, infoc = K.scipy_bicgstab(S, b.cpu().numpy(), tol=1e-6)
-            tc = time.perf_counter() - t
-            out["cpu_baseline"]["bicgstab"] = {"wall_s": tc, "iterations": itc, "info": int(infoc),
-                                               "cores_available": cores}
-            log("cpu bicgstab", tc, itc)
+        cpu = cpu_block_sparse(rp, ci, cv, b, n, nnz, "solve" in sections)
 
     if "precision" in sections:
-        out["precision_path"] = precision_section(dev, empty)
+        out["precision_path"] = precision_section(dev, empty, args.spmv_steps)
+        pm = out["precision_path"]["mv"]
+        out["roofline_stencil_256"] = {"bound": "hbm", "kernel": "k_dia (7 diagonals): y = A x, 256^3",
+                                       "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                                       **{vt: {"achieved": pm[vt]["GBps"], "frac": pm[vt]["GBps"] / PEAK_HBM_GBS,
+                                               "algorithmic_bytes": pm[vt]["algorithmic_bytes"], "ms": pm[vt]["ms"]}
+                                          for vt in ("f32", "bf16")}}
 
     if "batched" in sections:
         out["batched_pcg"] = batched_section(dev)
@@ -275,42 +271,168 @@ def main():
         out["direct_qr"] = direct_section(dev)
 
     if fsai is not None:
-        lp = fsai["learned_fsai_refined_bf16"]
-        out["value"] = fsai["speedup_learned_bf16_vs_plain_cg"]
-        out["unit"] = "x (plain FP32 CG seconds / learned-FSAI PCG seconds, both to true relres 1e-6)"
-        out["ms_per_step"] = lp["wall_s"] * 1e3
+        lp = fsai["learned_fsai_stated5_refined_bf16"]
+        hs = fsai["headline_steps"]
+        plain = min(fsai["plain_cg_fp32"]["wall_s"], fsai["plain_cg_refined"]["wall_s"])
+        out["value"] = plain / hs["median_s"]
+        out["unit"] = "x (plain CG seconds / learned-FSAI PCG seconds, both to FP64 true relres 1e-6)"
+        out["steps"], out["warmup"] = hs["steps"], hs["warmup"]
+        out["ms_per_step"] = hs["median_s"] * 1e3
+        out["step"] = ("one learned-FSAI PCG solve to relres 1e-6 (stated 5-input network, BF16 matrix values, "
+                       "FP32 Krylov, FP64 refinement), preconditioner set-up included; ms_per_step is the median "
+                       "of the timed steps; value = median plain-CG seconds (the faster of FP32 CG and "
+                       "FP64-refined CG, %d runs) / ms_per_step" % args.reps)
         out["dtype"] = "bf16 matrix values, f32 Krylov, f64 refinement"
         out["config"] = {"workload": "learned_fsai_pcg_vs_plain_cg diffusion 128^3 sigma=2",
                          "n": fsai["n"], "nnz": fsai["nnz"], "tol": 1e-6, "seed": 2024,
+                         "network": "edge MLP (a_ij, a_ii, a_jj, row nnz, row |sum|) -> 64 -> 64 -> 1",
                          "spmv_workload": config}
-        out["time_to_solution_s"] = {"learned_fsai_pcg_bf16": lp["wall_s"],
-                                     "plain_cg": min(fsai["plain_cg_fp32"]["wall_s"],
-                                                     fsai["plain_cg_refined"]["wall_s"])}
-        if "cpu" in sections:
-            import scipy.sparse as sp
-
-            from oracle import krylov as K
-
-            rp5, ci5, cv5, b5 = (t.cpu().numpy() for t in fsai["_csr"])
-            n5 = rp5.size - 1
-            S5 = sp.csr_matrix((cv5, ci5, rp5), shape=(n5, n5))
-            sample_its = 300
-            from threadpoolctl import threadpool_limits
-
-            with threadpool_limits(limits=1):
-                t = time.perf_counter()
-                K.scipy_cg(S5, b5, tol=1e-30, maxiter=sample_its)
-                per_it = (time.perf_counter() - t) / sample_its
-            its_gpu = fsai["plain_cg_fp32"]["iterations"]
-            spmv_cpu = out.get("cpu_baseline", {})
-            out["cpu_baseline"] = {"value": per_it * its_gpu, "unit": "s (plain CG time to relres 1e-6)",
+        out["time_to_solution_s"] = {
+            "learned_fsai_pcg_bf16": hs["median_s"], "plain_cg": plain,
+            "jacobi_pcg": fsai["jacobi_pcg_refined"]["wall_s"], "classical_fsai_pcg": fsai["classical_fsai_refined"]["wall_s"],
+            "learned_fsai_pcg_bf16_comparison_runs": lp["wall_s"]}
+        out["roofline_headline"] = {"bound": "cache", "kernel": "k_dia (7 diagonals, BF16 values): q = A p",
+                                    "achieved": fsai["mv_bf16"]["GBps"], "unit": "GB/s",
+                                    "algorithmic_bytes": fsai["mv_bf16"]["algorithmic_bytes"],
+                                    "note": fsai["mv_bf16"]["note"]}
+        if cpu is not None:
+            fs = cpu_fsai_system(*(t.cpu().numpy() for t in fsai["_csr"]))
+            its = fsai["plain_cg_fp32"]["iterations"]
+            fs["plain_cg_extrapolated_estimate"] = {
+                "s": fs["plain_cg_sample"]["s_per_iteration"] * its,
+                "note": f"ESTIMATE, not measured: the sample's seconds per iteration x the {its} iterations "
+                        f"plain CG needs on the GPU"}
+            cpu["learned_fsai_system"] = fs
+    if cpu is not None:
+        cpu["laplacian_64"] = cpu_laplacian_64()
+        fs = cpu.get("learned_fsai_system")
+        if fs is not None:
+            out["cpu_baseline"] = {"value": fs["jacobi_cg"]["wall_s"],
+                                   "unit": "s (scipy Jacobi-PCG time to relres 1e-6 on the 128^3 sigma 2 system)",
                                    "cores": 1, "kind": "port",
-                                   "sample": f"scipy.sparse.linalg.cg (the reference's CPU path) on the 128^3 "
-                                             f"system, {sample_its} iterations timed ({per_it * 1e3:.1f} ms each), "
-                                             f"extrapolated to the {its_gpu} iterations plain CG needs on the GPU",
-                                   "spmv": spmv_cpu}
-            log("cpu cg", out["cpu_baseline"])
+                                   "sample": "scipy.sparse.linalg.cg with M = diag(A)^-1 run to convergence on the "
+                                             "headline system (FP64, one thread: scipy's CSR matvec is "
+                                             "single-threaded and BLAS is limited to 1 thread); plain scipy CG "
+                                             "there is a bounded sample (see details)",
+                                   "details": cpu}
+        else:
+            out["cpu_baseline"] = {"value": cpu["spmv"]["GBps"], "unit": "GB/s", "cores": 1, "kind": "port",
+                                   "sample": cpu["spmv"]["sample"], "details": cpu}
+        if fsai is not None:
+            fsai.pop("_csr", None)
+            out["learned_pcg"] = {k: v for k, v in fsai.items() if not k.startswith("_")}
+
     print(json.dumps(out), flush=True)
+
+
+def _cpu_threads():
+    return {"cores_available": len(os.sched_getaffinity(0)), "threads_used": 1,
+            "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_block_sparse(rp, ci, cv, b, n, nnz, solve):
+    """scipy on the block-sparse system (one thread): CSR matvec (5 calls) and,
+    with `solve`, BiCGSTAB to relres 1e-6 (one run: it takes seconds)."""
+    import numpy as np
+    import scipy.sparse as sp
+    from threadpoolctl import threadpool_limits
+
+    from oracle import krylov as K
+
+    S = sp.csr_matrix((cv.cpu().numpy(), ci.cpu().numpy(), rp.cpu().numpy()), shape=(n, n))
+    xh = np.random.default_rng(0).standard_normal(n).astype(np.float32)
+    res = {"threads": _cpu_threads()}
+    with threadpool_limits(limits=1):
+        S @ xh
+        ts = []
+        for _ in range(5):
+            t = time.perf_counter()
+            S @ xh
+            ts.append(time.perf_counter() - t)
+        ms = float(np.median(ts)) * 1e3
+        res["spmv"] = {"ms": ms, "GBps": spmv_bytes(n, n, nnz) / (ms * 1e-3) / 1e9,
+                       "sample": f"scipy CSR matvec (FP32, single-threaded) on the block-sparse matrix, median "
+                                 f"of 5 calls"}
+        if solve:
+            bh = b.cpu().numpy()
+            t = time.perf_counter()
+            xc, itc, infoc = K.scipy_bicgstab(S, bh, tol=1e-6)
+            res["bicgstab"] = {"wall_s": time.perf_counter() - t, "iterations": itc, "info": int(infoc),
+                               "true_relres_fp64": K.true_relres(S, xc, bh), "runs": 1}
+    log("cpu block-sparse", res)
+    return res
+
+
+def cpu_laplacian_64():
+    """scipy CG to relres 1e-6 on the 64x64 Laplacian (one thread): 1 warm-up,
+    median of 5."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from mpsparse_amd import problems
+    from oracle import coo_csr as oc
+    from oracle import krylov as K
+    import scipy.sparse as sp
+
+    p = problems.laplacian_2d(64)
+    rp, ci, v = oc.coo_to_csr(p.keys, p.vals, p.num_rows, p.num_cols)
+    S = sp.csr_matrix((v.astype(np.float64), ci.astype(np.int64), rp.astype(np.int64)), shape=(p.num_rows,) * 2)
+    b = p.b.astype(np.float64)
+    with threadpool_limits(limits=1):
+        K.scipy_cg(S, b, tol=1e-6)
+        ts = []
+        for _ in range(5):
+            t = time.perf_counter()
+            x, it, info = K.scipy_cg(S, b, tol=1e-6)
+            ts.append(time.perf_counter() - t)
+    res = {"wall_s": float(np.median(ts)), "runs_s": ts, "iterations": it, "info": int(info),
+           "true_relres_fp64": K.true_relres(S, x, b), "solver": "scipy cg rtol=1e-6 atol=0 (FP64)"}
+    log("cpu laplacian 64", res)
+    return res
+
+
+def cpu_fsai_system(rp, ci, cv, b32, plain_maxiter=300):
+    """scipy on the headline system (one thread): Jacobi-preconditioned CG to
+    relres 1e-6 (one run, it takes tens of seconds), and plain CG stopped at
+    `plain_maxiter` steps (plain CG needs ~16k steps, minutes on one core); the
+    extrapolation to plain CG's full step count is a separately labelled
+    estimate."""
+    import numpy as np
+    import scipy.sparse as sp
+    from threadpoolctl import threadpool_limits
+
+    from oracle import krylov as K
+
+    n = rp.size - 1
+    S = sp.csr_matrix((cv.astype(np.float64), ci.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    b = b32.astype(np.float64)
+    res = {"threads": _cpu_threads()}
+    with threadpool_limits(limits=1):
+        t = time.perf_counter()
+        x, it, info = K.scipy_cg(S, b, tol=1e-6, maxiter=20000, jacobi=True)
+        res["jacobi_cg"] = {"wall_s": time.perf_counter() - t, "iterations": it, "info": int(info),
+                            "true_relres_fp64": K.true_relres(S, x, b), "runs": 1,
+                            "solver": "scipy cg rtol=1e-6 atol=0, M = diag(A)^-1 (FP64)"}
+        log("cpu jacobi cg", res["jacobi_cg"])
+        t = time.perf_counter()
+        x, it, info = K.scipy_cg(S, b, tol=1e-6, maxiter=plain_maxiter)
+        dt = time.perf_counter() - t
+        res["plain_cg_sample"] = {"wall_s": dt, "iterations": it, "maxiter": plain_maxiter, "info": int(info),
+                                  "true_relres_fp64": K.true_relres(S, x, b), "s_per_iteration": dt / max(it, 1),
+                                  "solver": "scipy cg rtol=1e-6 atol=0 (FP64), stopped at maxiter"}
+        log("cpu plain cg sample", res["plain_cg_sample"])
+    return res
 
 
 def stencil_operator64(rp, ci, v, n, nx):
@@ -441,11 +563,15 @@ def batched_section(dev, num_systems=512, lo=32, hi=512, sigma=1.5, seed=0):
     return res
 
 
-def fsai_section(dev, steps, weights=None):
+def fsai_section(dev, steps, warmup, reps, weights=None):
     """Held-out 128^3 7-point log-normal diffusion (sigma 2): time to true relres
     1e-6 of plain FP32 CG and of FP64-refined CG / Jacobi-PCG / classical FSAI-PCG
-    / learned FSAI-PCG (FP32 and BF16 matrix values), preconditioner set-up
-    included; every error against a Jacobi-PCG FP64 oracle (torch only)."""
+    / learned FSAI-PCG (the stated 5-input network with FP32 and BF16 matrix
+    values; the 8-feature variant with BF16 values), preconditioner set-up
+    included; every error against a Jacobi-PCG FP64 oracle (torch only).
+    Each solver: one warm-up, then the median of `reps` solves. The headline
+    solve (stated-5 network, BF16 values) is then timed once more as `warmup`
+    untimed + `steps` timed solves."""
     import numpy as np
     import torch
 
@@ -473,7 +599,8 @@ def fsai_section(dev, steps, weights=None):
         return float(torch.linalg.vector_norm(b64 - ax)) / bnorm
 
     res = {"workload": f"diffusion 3D 7-point {side}^3 sigma={sigma} (held out: seed {seed}, side outside the "
-                       f"training range)", "n": n, "nnz": nnz, "tol": 1e-6}
+                       f"training range)", "n": n, "nnz": nnz, "tol": 1e-6,
+           "timing": f"per solver: 1 warm-up, then the median of {reps} solves (wall clock, set-up included)"}
     t = time.perf_counter()
     op64 = stencil_operator64(rp, ci, cv, n, side)
     d64 = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(
@@ -488,22 +615,23 @@ def fsai_section(dev, steps, weights=None):
     def err(x):
         return float(torch.linalg.vector_norm(x.double() - xo)) / xo_norm
 
-    def timed(fn, reps=2):
-        best, out = None, None
+    def timed(fn):
+        fn()  # warm-up
+        times, out = [], None
         for _ in range(reps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             out = fn()
             torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-        return best, out
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), times, out
 
     # plain FP32 CG (the baseline)
     x32 = torch.zeros(n, device=dev)
-    t_cg, info = timed(lambda: A.cgd(b32, x32, tol=1e-6, maxiter=100000))
-    res["plain_cg_fp32"] = {"wall_s": t_cg, "iterations": info["iterations"], "status": info["status"],
-                            "true_relres_fp64": relres64(x32), "rel_err_vs_fp64_oracle": err(x32)}
+    t_cg, runs, info = timed(lambda: A.cgd(b32, x32, tol=1e-6, maxiter=100000))
+    res["plain_cg_fp32"] = {"wall_s": t_cg, "wall_s_runs": runs, "iterations": info["iterations"],
+                            "status": info["status"], "true_relres_fp64": relres64(x32),
+                            "rel_err_vs_fp64_oracle": err(x32)}
     log("plain cg", res["plain_cg_fp32"])
     x64 = torch.zeros(n, dtype=torch.float64, device=dev)
 
@@ -512,26 +640,30 @@ def fsai_section(dev, steps, weights=None):
         return A.solve_refined(b64, x64, tol=1e-6, precond=precond)
 
     for name, pc in (("plain_cg_refined", "none"), ("jacobi_pcg_refined", "jacobi")):
-        t_r, rinfo = timed(lambda: refined(pc))
-        res[name] = {"wall_s": t_r, "outer": rinfo["outer_iterations"], "iterations": rinfo["inner_iterations"],
-                     "status": rinfo["status"], "true_relres_fp64": relres64(x64),
-                     "rel_err_vs_fp64_oracle": err(x64)}
+        t_r, runs, rinfo = timed(lambda: refined(pc))
+        res[name] = {"wall_s": t_r, "wall_s_runs": runs, "outer": rinfo["outer_iterations"],
+                     "iterations": rinfo["inner_iterations"], "status": rinfo["status"],
+                     "true_relres_fp64": relres64(x64), "rel_err_vs_fp64_oracle": err(x64)}
         log(name, res[name])
 
-    params = LF.load_params(weights, device=dev) if weights else LF.load_params(device=dev)
-    res["weights"] = os.path.relpath(weights or LF.DEFAULT_WEIGHTS, HERE)
-    feat_ne = None
-    for name, kind, vt in (("classical_fsai_refined", "classical", "f32"),
-                           ("learned_fsai_refined_f32", "learned", "f32"),
-                           ("learned_fsai_refined_bf16", "learned", "bf16")):
+    nets = {"stated5": LF.load_network(weights or LF.DEFAULT_WEIGHTS, device=dev),
+            "coupled8": LF.load_network(LF.COUPLED8_WEIGHTS, device=dev)}
+    res["weights"] = {"stated5": os.path.relpath(weights or LF.DEFAULT_WEIGHTS, HERE),
+                      "coupled8": os.path.relpath(LF.COUPLED8_WEIGHTS, HERE)}
+    variants = (("classical_fsai_refined", "classical", "f32"),
+                ("learned_fsai_stated5_refined_f32", "stated5", "f32"),
+                ("learned_fsai_stated5_refined_bf16", "stated5", "bf16"),
+                ("learned_fsai_coupled8_refined_bf16", "coupled8", "bf16"))
+
+    def make_solver(kind, vt):
         holder = {}
 
         def setup():
-            F = LF.fsai_tensor(A)
             if kind == "classical":
+                F = LF.fsai_tensor(A)
                 holder["fallback_rows"] = F.classical()
             else:
-                holder["nonpositive_diag"] = F.set_network(params)
+                F, holder["nonpositive_diag"] = LF.build(A, nets[kind])
             holder["F"] = F
             return F
 
@@ -540,30 +672,63 @@ def fsai_section(dev, steps, weights=None):
             x64.zero_()
             return LF.solve_refined(A, F, b64, x64, tol=1e-6, values=vt)
 
-        t_s, _ = timed(setup)
-        t_all, rinfo = timed(solve)
-        F = holder["F"]
-        feat_ne = F.num_edges
-        res[name] = {"wall_s": t_all, "setup_s": t_s, "outer": rinfo["outer_iterations"],
+        return holder, setup, solve
+
+    for name, kind, vt in variants:
+        holder, setup, solve = make_solver(kind, vt)
+        t_s, _, _ = timed(setup)
+        t_all, runs, rinfo = timed(solve)
+        res[name] = {"wall_s": t_all, "wall_s_runs": runs, "setup_s": t_s, "outer": rinfo["outer_iterations"],
                      "iterations": rinfo["inner_iterations"], "fallbacks": rinfo["fallbacks"],
                      "status": rinfo["status"], "true_relres_fp64": relres64(x64),
                      "rel_err_vs_fp64_oracle": err(x64),
                      **{k: v for k, v in holder.items() if k != "F"}}
+        if kind != "classical":
+            res[name]["network"] = {"feature_set": kind, "inputs": "a_ij, a_ii, a_jj, row nnz, row |sum|"
+                                    if kind == "stated5" else "the five above + s_low, s_all, 1/sqrt(1-s_low)"}
         log(name, res[name])
-        del F, holder
-    # the edge MLP on this matrix's lower pattern
-    F = LF.fsai_tensor(A)
+        holder.clear()
+
+    # the headline solve, timed as the benchmark's steps
+    _, setup, solve = make_solver("stated5", "bf16")
+    for _ in range(warmup):
+        solve()
+    torch.cuda.synchronize()
+    step_times = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        solve()
+        torch.cuda.synchronize()
+        step_times.append(time.perf_counter() - t1)
+    res["headline_steps"] = {"steps": steps, "warmup": warmup, "total_s": time.perf_counter() - t0,
+                             "median_s": float(np.median(step_times)), "runs_s": step_times}
+
+    # the edge MLP on this matrix's lower pattern (stated-5 features)
+    F, _ = LF.build(A, nets["stated5"])
     feat, _ = F.features()
     o = torch.empty(feat.shape[0], device=dev)
-    ms = time_launches(lambda: LF._C.mlp_forward(params, feat, o), steps, 3)
-    flops = feat.shape[0] * 2 * (8 * 64 + 64 * 64 + 64)
-    res["mlp_forward"] = {"edges": int(feat.shape[0]), "ms": ms, "flops_per_edge": 2 * (8 * 64 + 64 * 64 + 64),
-                          "TFLOPs": flops / (ms * 1e-3) / 1e12}
+    ms = time_launches(lambda: LF._C.mlp_forward(nets["stated5"].params, feat, o), 50, 3)
+    fpe = 2 * (8 * 64 + 64 * 64 + 64)  # the first layer runs K = 8 (5 features + 3 zero slots) on the MFMA
+    res["mlp_forward"] = {"edges": int(feat.shape[0]), "ms": ms, "flops_per_edge": fpe,
+                          "TFLOPs": feat.shape[0] * fpe / (ms * 1e-3) / 1e12}
     log("mlp forward", res["mlp_forward"])
+    del F, feat, o
+    # the dominant kernel of the headline step: q = A p on the 16-bit diagonal storage
+    xa = torch.randn(n, device=dev)
+    ya = torch.empty(n, device=dev)
+    A.mv(xa, ya, "bf16")
+    mv_ms = time_launches(lambda: A.mv(xa, ya, "bf16"), 100, 5)
+    mv_bytes = 7 * 2 * n + 4 * n + 4 * n + 4 * n  # 7 BF16 diagonals, FP32 diagonal remainder, x, y
+    res["mv_bf16"] = {"ms": mv_ms, "algorithmic_bytes": mv_bytes, "GBps": mv_bytes / (mv_ms * 1e-3) / 1e9,
+                      "note": "working set ~%d MB: resident in the 256 MB last-level cache, so not an HBM-bound "
+                              "figure" % (mv_bytes // 2 ** 20)}
+    log("mv bf16 128^3", res["mv_bf16"])
     base = min(res["plain_cg_fp32"]["wall_s"], res["plain_cg_refined"]["wall_s"])
-    res["speedup_learned_bf16_vs_plain_cg"] = base / res["learned_fsai_refined_bf16"]["wall_s"]
-    res["speedup_learned_bf16_vs_jacobi_pcg"] = (res["jacobi_pcg_refined"]["wall_s"]
-                                                  / res["learned_fsai_refined_bf16"]["wall_s"])
+    head = res["learned_fsai_stated5_refined_bf16"]["wall_s"]
+    res["speedup_learned_bf16_vs_plain_cg"] = base / head
+    res["speedup_learned_bf16_vs_jacobi_pcg"] = res["jacobi_pcg_refined"]["wall_s"] / head
+    res["speedup_learned_bf16_vs_classical_fsai"] = res["classical_fsai_refined"]["wall_s"] / head
     res["_csr"] = (rp, ci, cv, b32)  # for the CPU baseline; removed before printing
     return res
 
@@ -629,7 +794,7 @@ def direct_section(dev, sizes=(1000, 4000, 10000, 20000, 100000), nrhs=128):
     return out
 
 
-def precision_section(dev, empty):
+def precision_section(dev, empty, spmv_steps=50):
     """256^3 7-point log-normal diffusion: plain FP32 CG at 1e-6 next to FP64
     iterative refinement (FP32 / BF16 / FP16 values) to 1e-10."""
     import numpy as np
@@ -660,6 +825,19 @@ def precision_section(dev, empty):
 
     res = {"workload": "diffusion_3d 256^3 sigma=1 seed=1 (7-point, SpMV alg bytes FP32 %d, BF16 %d)" % (
         M.nnz * 8 + 4 * (n + 1) + 8 * n, M.nnz * 6 + 4 * (n + 1) + 8 * n)}
+    # the stencil product in diagonal storage: 7 value planes of n (zeros where
+    # a diagonal leaves the grid), x, y (+ the FP32 diagonal remainder with 16-bit values)
+    xm = torch.randn(n, device=dev)
+    ym = torch.empty(n, device=dev)
+    res["mv"] = {}
+    for vt, vb, extra in (("f32", 4, 0), ("bf16", 2, 4 * n)):
+        A.mv(xm, ym, vt)
+        ms = time_launches(lambda: A.mv(xm, ym, vt), spmv_steps, 5)
+        alg = 7 * vb * n + 4 * n + 4 * n + extra
+        res["mv"][vt] = {"ms": ms, "algorithmic_bytes": alg, "GBps": alg / (ms * 1e-3) / 1e9,
+                         "launches_timed": spmv_steps}
+        log("stencil mv 256^3", vt, res["mv"][vt])
+    del xm, ym
     x32 = torch.zeros(n, device=dev)
     A.cgd(b32, x32, maxiter=20000)
     torch.cuda.synchronize()

@@ -59,6 +59,16 @@ typedef enum msp_precond {
   MSP_PRECOND_JACOBI = 1
 } msp_precond;
 
+/* Input features of the learned-FSAI edge network (msp_fsai_select_features). */
+typedef enum msp_fsai_feature_set {
+  /* 8 features including the row couplings s_low, s_all and 1/sqrt(1 - s_low) */
+  MSP_FSAI_FEATURES_COUPLED8 = 0,
+  /* the edge's five quantities a_ij, a_ii, a_jj, nnz(row i), sum_j |a_ij|, made
+   * scale-free (a_ij/sqrt(a_ii a_jj), log(a_jj/a_ii), nnz/8, sum|a|/a_ii,
+   * [i == j]; three zero slots); G's rows are normalised to diag(G A G^T) = 1 */
+  MSP_FSAI_FEATURES_STATED5 = 1
+} msp_fsai_feature_set;
+
 typedef enum msp_solve_status {
   MSP_SOLVE_CONVERGED = 1,
   MSP_SOLVE_BREAKDOWN = 2,
@@ -254,6 +264,14 @@ msp_status msp_fsai_edges(msp_fsai F, uint32_t *rows, uint32_t *cols, void *stre
  * with s_low(i) = sum_{j<i} a_ij^2/(a_ii a_jj) and s_all(i) the same over
  * j != i; and the column scaling 1/sqrt(a_jj). */
 msp_status msp_fsai_features(msp_fsai F, float *feat, float *colscale, void *stream);
+/* Feature set of the edge network (msp_fsai_feature_set; a new handle has
+ * MSP_FSAI_FEATURES_COUPLED8). Recomputes the features that msp_fsai_features
+ * exports. With MSP_FSAI_FEATURES_STATED5, G from a network output (and the
+ * training gradient) passes through the row normalisation
+ * g_i <- g_i / sqrt(g_i A[J,J] g_i^T); a row whose quadratic form is not
+ * positive keeps its scale and counts as a non-positive diagonal entry. */
+msp_status msp_fsai_select_features(msp_fsai F, int32_t feature_set, void *stream);
+msp_status msp_fsai_get_features(msp_fsai F, int32_t *feature_set);
 /* Classical FSAI: per row, A[J,J] y = e_last, g = y/sqrt(y_last); rows that
  * cannot be solved become Jacobi rows (counted in fallback_rows). */
 msp_status msp_fsai_classical(msp_fsai F, int32_t *fallback_rows, void *stream);

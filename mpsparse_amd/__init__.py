@@ -18,6 +18,20 @@ from .problems import pack_keys, unpack_keys  # noqa: F401
 __all__ = ["pack_keys", "unpack_keys", "load_ext"]
 
 
+_exit_hook = []
+
+
+def _release_at_exit(ext):
+    """Return the library's cached device blocks and idle scratch while the
+    interpreter and the HIP runtime are both fully alive (handles destroyed
+    later, during finalisation, still work; the library itself makes no HIP
+    call once exit() has begun)."""
+    try:
+        ext.release_workspace()
+    except Exception:  # no device, or nothing was ever allocated
+        pass
+
+
 def load_ext():
     """Import the compiled extension (``mpsparse_amd._C``), failing loudly."""
     try:
@@ -26,4 +40,9 @@ def load_ext():
         raise ImportError(
             "mpsparse_amd native extension is not built: run `python -m mpsparse_amd.build` "
             f"(hipcc for gfx950). Original error: {e}") from e
+    if not _exit_hook:
+        import atexit
+
+        atexit.register(_release_at_exit, _C)
+        _exit_hook.append(True)
     return _C

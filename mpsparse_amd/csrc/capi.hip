@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -100,6 +102,24 @@ static uint32_t bitlen(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v)
 // uses it when it is handed out again. At most kBlockCacheCap bytes per device
 // are held (beyond that, frees go back to the system); a failed hipMalloc
 // empties the cache and retries; msp_release_workspace returns everything.
+// Process teardown. Destructors of handles may run inside exit() (static
+// objects of a host program, handles still owned by a scripting runtime that
+// is finalised late). The HIP runtime tears itself down from its own exit
+// handlers, and a HIP call after that point can crash. The library registers
+// an exit handler right after its first successful HIP call, i.e. after the
+// runtime has registered its own, so it runs first (exit handlers run in
+// reverse order of registration): from then on destructors make no HIP call
+// and leave device memory, pinned memory, events, streams and graphs to the
+// end of the process. Library state with destructors is allocated once and
+// never destroyed (no static destructor of this library calls HIP).
+static std::atomic<bool> g_exiting{false};
+static void on_process_exit() { g_exiting.store(true, std::memory_order_relaxed); }
+static void register_exit_guard() {
+  static std::once_flag once;
+  std::call_once(once, [] { std::atexit(on_process_exit); });
+}
+static bool in_teardown() { return g_exiting.load(std::memory_order_relaxed); }
+
 constexpr size_t kBlockCacheCap = (size_t)8 << 30;
 constexpr int kMaxDevices = 64;
 struct BlockCache {
@@ -136,6 +156,7 @@ struct DevMem {
     if (bytes == 0) bytes = 256;
     int d = 0;
     HIPCHK(hipGetDevice(&d));
+    register_exit_guard();
     const size_t want = block_capacity(bytes);
     if (d >= 0 && d < kMaxDevices) {
       std::lock_guard<std::mutex> lk(g_blocks.mu);
@@ -167,6 +188,12 @@ struct DevMem {
   }
   void reset() {
     if (!p) return;
+    if (in_teardown()) {  // the process is ending: no HIP call
+      p = nullptr;
+      cap = 0;
+      dev = -1;
+      return;
+    }
     int cur = -1;
     bool cached = false;
     if (dev >= 0 && dev < kMaxDevices && hipGetDevice(&cur) == hipSuccess && cur == dev &&
@@ -209,12 +236,14 @@ struct Arena {
   std::vector<ArenaChunk> chunks;
   std::vector<ArenaRec> stack;
 };
-static std::mutex g_arena_mu;
-static std::map<std::pair<int, hipStream_t>, Arena*> g_arenas;
+// never destroyed, like g_blocks
+static std::mutex& g_arena_mu = *new std::mutex;
+static std::map<std::pair<int, hipStream_t>, Arena*>& g_arenas = *new std::map<std::pair<int, hipStream_t>, Arena*>;
 
 static msp_status arena_for(hipStream_t s, Arena** out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
+  register_exit_guard();
   std::lock_guard<std::mutex> lk(g_arena_mu);
   Arena*& a = g_arenas[std::make_pair(dev, s)];
   if (!a) a = new Arena;
@@ -354,6 +383,7 @@ struct Workspace {
   GridRed gr2() const { return GridRed{red.as<CSum>() + (size_t)16 * cu_count()}; }
   GridRed gr3() const { return GridRed{red.as<CSum>() + (size_t)32 * cu_count()}; }
   ~Workspace() {
+    if (in_teardown()) return;
     if (hS) (void)hipHostFree(hS);
     if (hsnap) (void)hipHostFree(hsnap);
     for (auto& e : ev)
@@ -945,8 +975,10 @@ extern "C" msp_status msp_csr_get_format(msp_csr A, int32_t* format_in_use) {
 
 extern "C" msp_status msp_csr_destroy(msp_csr A) {
   if (!A) return MSP_OK;
-  (void)hipSetDevice(A->device);
-  (void)hipDeviceSynchronize();
+  if (!in_teardown()) {
+    (void)hipSetDevice(A->device);
+    (void)hipDeviceSynchronize();
+  }
   delete A;
   return MSP_OK;
 }
@@ -1744,8 +1776,10 @@ extern "C" msp_status msp_bcsr_export(msp_bcsr A, uint32_t* brow_ptr, uint32_t* 
 
 extern "C" msp_status msp_bcsr_destroy(msp_bcsr A) {
   if (!A) return MSP_OK;
-  (void)hipSetDevice(A->device);
-  (void)hipDeviceSynchronize();
+  if (!in_teardown()) {
+    (void)hipSetDevice(A->device);
+    (void)hipDeviceSynchronize();
+  }
   delete A;
   return MSP_OK;
 }
@@ -1951,9 +1985,27 @@ struct msp_fsai_s {
   msp_csr_s G, GT;      // G and its transpose as CSR matrices
   uint64_t ne = 0;
   DevMem Lrow, Lsrc, GTmap, feat, colscale, diag, vec, sums;
+  DevMem rowscale;  // stated-5 network: the row normalisation s_i of G
+  int features = kFeatCoupled8;  // feature set of the edge network (FsaiFeatures)
   int ready = 0;    // values set
   int nonpos = 0;   // diagonal entries of G that are not positive
 };
+
+static msp_status fsai_compute_features(msp_fsai F, hipStream_t s) {
+  msp_csr A = F->A;
+  const uint32_t n = A->n;
+  if (F->features == kFeatStated5)
+    k_fsai_features<kFeatStated5><<<grid_for(n), kBlock, 0, s>>>(
+        A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n, F->G.row_ptr.as<uint32_t>(),
+        F->G.col.as<uint32_t>(), F->Lsrc.as<uint32_t>(), F->diag.as<float>(), F->feat.as<float>(),
+        F->colscale.as<float>());
+  else
+    k_fsai_features<kFeatCoupled8><<<grid_for(n), kBlock, 0, s>>>(
+        A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n, F->G.row_ptr.as<uint32_t>(),
+        F->G.col.as<uint32_t>(), F->Lsrc.as<uint32_t>(), F->diag.as<float>(), F->feat.as<float>(),
+        F->colscale.as<float>());
+  return launch_check("features");
+}
 
 static msp_status exclusive_scan_u32(uint32_t* data, uint64_t N, hipStream_t s) {
   const uint32_t nscan = (uint32_t)((N + sortk::kScanTile - 1) / sortk::kScanTile);
@@ -2033,12 +2085,7 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     HIPCHK(hipMemsetAsync(F->feat.as<float>() + 8 * ne, 0, sizeof(float) * 8 * 64, s));
     k_diag<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n,
                                           F->diag.as<float>());
-    k_fsai_features<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
-                                                   A->val.as<float>(), n, F->G.row_ptr.as<uint32_t>(),
-                                                   F->G.col.as<uint32_t>(), F->Lsrc.as<uint32_t>(),
-                                                   F->diag.as<float>(), F->feat.as<float>(),
-                                                   F->colscale.as<float>());
-    MSPCHK(launch_check("features"));
+    MSPCHK(fsai_compute_features(F, s));
     MSPCHK(ensure_workspace(A, s));
     MSPCHK(F->vec.alloc(sizeof(float) * 6 * A->ws.npad));
     HIPCHK(hipMemsetAsync(F->vec.p, 0, sizeof(float) * 6 * A->ws.npad, s));
@@ -2057,8 +2104,10 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
 
 extern "C" msp_status msp_fsai_destroy(msp_fsai F) {
   if (!F) return MSP_OK;
-  (void)hipSetDevice(F->device);
-  (void)hipDeviceSynchronize();
+  if (!in_teardown()) {
+    (void)hipSetDevice(F->device);
+    (void)hipDeviceSynchronize();
+  }
   delete F;
   return MSP_OK;
 }
@@ -2086,6 +2135,27 @@ extern "C" msp_status msp_fsai_features(msp_fsai F, float* feat, float* colscale
   MSPCHK(copy_out(feat, F->feat.p, sizeof(float) * 8 * F->ne, s));
   MSPCHK(copy_out(colscale, F->colscale.p, sizeof(float) * F->ne, s));
   HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_select_features(msp_fsai F, int32_t feature_set, void* stream) {
+  if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
+  if (feature_set != MSP_FSAI_FEATURES_COUPLED8 && feature_set != MSP_FSAI_FEATURES_STATED5)
+    return fail(MSP_ERR_INVALID, "unknown FSAI feature set %d", (int)feature_set);
+  HIPCHK(hipSetDevice(F->device));
+  hipStream_t s = (hipStream_t)stream;
+  if (feature_set == F->features) return MSP_OK;
+  F->features = feature_set;
+  if (feature_set == MSP_FSAI_FEATURES_STATED5 && !F->rowscale.p)
+    MSPCHK(F->rowscale.alloc(sizeof(float) * ((size_t)F->A->n + 1)));
+  MSPCHK(fsai_compute_features(F, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSP_OK;
+}
+
+extern "C" msp_status msp_fsai_get_features(msp_fsai F, int32_t* feature_set) {
+  if (!F || !feature_set) return fail(MSP_ERR_INVALID, "null argument");
+  *feature_set = F->features;
   return MSP_OK;
 }
 
@@ -2133,10 +2203,20 @@ extern "C" msp_status msp_fsai_set_network_output(msp_fsai F, const float* out, 
   TmpMem cnt;
   MSPCHK(cnt.alloc(sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
-  k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(static_cast<const float*>(o.d), F->colscale.as<float>(),
-                                                  F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), F->ne,
-                                                  F->G.val.as<float>(), cnt.as<uint32_t>());
-  MSPCHK(launch_check("fsai scale"));
+  if (F->features == kFeatStated5) {
+    msp_csr A = F->A;
+    k_fsai_rownorm<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
+                                                     A->val.as<float>(), A->n, F->G.row_ptr.as<uint32_t>(),
+                                                     F->G.col.as<uint32_t>(), static_cast<const float*>(o.d),
+                                                     F->colscale.as<float>(), F->G.val.as<float>(),
+                                                     F->rowscale.as<float>(), cnt.as<uint32_t>());
+    MSPCHK(launch_check("fsai rownorm"));
+  } else {
+    k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(static_cast<const float*>(o.d), F->colscale.as<float>(),
+                                                    F->Lrow.as<uint32_t>(), F->G.col.as<uint32_t>(), F->ne,
+                                                    F->G.val.as<float>(), cnt.as<uint32_t>());
+    MSPCHK(launch_check("fsai scale"));
+  }
   MSPCHK(fsai_commit(F, s));
   uint32_t h = 0;
   HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -2485,8 +2565,17 @@ extern "C" msp_status msp_fsai_train_step(msp_fsai F, const float* params, const
   MSPCHK(go.alloc(sizeof(float) * (F->ne + 1)));
   HIPCHK(hipMemsetAsync(gg.p, 0, sizeof(float) * (F->ne + 1), s));
   MSPCHK(msp_fsai_probe_loss_grad(F, W, k, gg.as<float>(), loss, stream));
-  mlp::k_scale_grad<<<grid_for(F->ne), kBlock, 0, s>>>(gg.as<float>(), F->colscale.as<float>(), F->ne,
-                                                        go.as<float>());
+  if (F->features == kFeatStated5) {
+    msp_csr A = F->A;
+    k_fsai_rownorm_grad<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
+                                                          A->val.as<float>(), A->n, F->G.row_ptr.as<uint32_t>(),
+                                                          F->G.col.as<uint32_t>(), F->G.val.as<float>(),
+                                                          F->rowscale.as<float>(), F->colscale.as<float>(),
+                                                          gg.as<float>(), go.as<float>());
+  } else {
+    mlp::k_scale_grad<<<grid_for(F->ne), kBlock, 0, s>>>(gg.as<float>(), F->colscale.as<float>(), F->ne,
+                                                          go.as<float>());
+  }
   MSPCHK(launch_check("scale grad"));
   return msp_mlp_backward(params, F->feat.as<float>(), go.as<float>(), F->ne, grad_params, stream);
 }
@@ -2515,6 +2604,7 @@ struct msp_batch_s {
   bool tuned = false;
   int tuned_for[2] = {-1, -1};  // (tile_rows, rows_per_thread) requested when tuned
   ~msp_batch_s() {
+    if (in_teardown()) return;
     if (exec) (void)hipGraphExecDestroy(exec);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -2721,8 +2811,10 @@ extern "C" msp_status msp_batch_create(msp_batch* out, msp_csr A, const uint32_t
 
 extern "C" msp_status msp_batch_destroy(msp_batch B) {
   if (!B) return MSP_OK;
-  (void)hipSetDevice(B->device);
-  (void)hipDeviceSynchronize();
+  if (!in_teardown()) {
+    (void)hipSetDevice(B->device);
+    (void)hipDeviceSynchronize();
+  }
   delete B;
   return MSP_OK;
 }
@@ -3241,8 +3333,10 @@ extern "C" msp_status msp_qr_solve(msp_qr F, const double* B, int64_t ldb, int64
 
 extern "C" msp_status msp_qr_destroy(msp_qr F) {
   if (!F) return MSP_OK;
-  (void)hipSetDevice(F->device);
-  (void)hipDeviceSynchronize();
+  if (!in_teardown()) {
+    (void)hipSetDevice(F->device);
+    (void)hipDeviceSynchronize();
+  }
   delete F;
   return MSP_OK;
 }

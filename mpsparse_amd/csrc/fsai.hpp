@@ -155,13 +155,25 @@ __global__ __launch_bounds__(kBlock) void k_fsai_classical(const uint32_t* __res
   }
 }
 
-// Edge features of the lower pattern, 8 floats per edge:
-// [a_ij / sqrt(a_ii a_jj), log(a_jj / a_ii), nnz(row i) / 8, sum_j |a_ij| / a_ii,
-//  [j == i], s_low(i), s_all(i), t(i)] with the row couplings
-//  s_low(i) = sum_{j < i} a_ij^2 / (a_ii a_jj), s_all(i) = sum_{j != i} a_ij^2 / (a_ii a_jj)
-// and t(i) = 1 / sqrt(max(1 - s_low(i), 1/64)) (for an arrow-shaped A[J,J]
-// classical FSAI is g_ii sqrt(a_ii) = t, g_ij sqrt(a_jj) = -f0 t); and
-// s[e] = 1 / sqrt(a_jj) (the column scaling of G).
+// Feature sets of the edge network.
+enum FsaiFeatures : int {
+  // [a_ij / sqrt(a_ii a_jj), log(a_jj / a_ii), nnz(row i) / 8, sum_j |a_ij| / a_ii,
+  //  [j == i], s_low(i), s_all(i), t(i)] with the row couplings
+  //  s_low(i) = sum_{j < i} a_ij^2 / (a_ii a_jj), s_all(i) = sum_{j != i} a_ij^2 / (a_ii a_jj)
+  // and t(i) = 1 / sqrt(max(1 - s_low(i), 1/64)) (for an arrow-shaped A[J,J]
+  // classical FSAI is g_ii sqrt(a_ii) = t, g_ij sqrt(a_jj) = -f0 t)
+  kFeatCoupled8 = 0,
+  // the edge's own five quantities a_ij, a_ii, a_jj, nnz(row i), sum_j |a_ij|,
+  // made scale-free: [a_ij / sqrt(a_ii a_jj), log(a_jj / a_ii), nnz(row i) / 8,
+  // sum_j |a_ij| / a_ii, [j == i], 0, 0, 0] (the diagonal flag is a function of
+  // the first: an SPD matrix has |a_ij| < sqrt(a_ii a_jj) off the diagonal);
+  // G's rows are then normalised so that diag(G A G^T) = 1 (k_fsai_rownorm)
+  kFeatStated5 = 1,
+};
+
+// Edge features of the lower pattern, 8 floats per edge (FEAT: FsaiFeatures),
+// and s[e] = 1 / sqrt(a_jj) (the column scaling of G).
+template <int FEAT>
 __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __restrict__ row_ptr,
                                                           const uint32_t* __restrict__ col,
                                                           const float* __restrict__ val, uint32_t n,
@@ -229,7 +241,8 @@ __global__ __launch_bounds__(kBlock) void k_fsai_features(const uint32_t* __rest
           const bool ok = aii > 0.0f && ajj > 0.0f;
           f4* f = reinterpret_cast<f4*>(feat + (size_t)e * 8);
           f[0] = f4{ok ? aij * rsqrtf(aii * ajj) : 0.0f, ok ? logf(ajj / aii) : 0.0f, rn, ok ? asum / aii : 0.0f};
-          f[1] = f4{(c == i) ? 1.0f : 0.0f, s_low, s_all, t};
+          if (FEAT == kFeatCoupled8) f[1] = f4{(c == i) ? 1.0f : 0.0f, s_low, s_all, t};
+          else f[1] = f4{(c == i) ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f};
           colscale[e] = ajj > 0.0f ? rsqrtf(ajj) : 0.0f;
         }
       }
@@ -260,6 +273,96 @@ __global__ __launch_bounds__(kBlock) void k_fsai_scale(const float* __restrict__
     const float g = out[e] * colscale[e];
     G[e] = g;
     if (Lrow[e] == Lcol[e] && !(g > 0.0f)) atomicAdd(nonpos, 1u);
+  }
+}
+
+// Row normalisation of G (the stated-5 network): with g~ = out * colscale the
+// network's row, N_i = g~_i A[J,J] g~_i^T = (G~ A G~^T)_ii, and
+// G_i = g~_i / sqrt(N_i), so that diag(G A G^T) = 1 -- the scaling classical
+// FSAI has by construction (g = y / sqrt(y_i) gives g A[J,J] g^T = 1). The
+// network then only sets each row's direction. v_p = (A[J,J] g~_i)_p is
+// formed by merging row J_p of A with J (both in ascending column order;
+// duplicates of A summed), so coupled lower neighbours count too.
+__device__ __forceinline__ float rownorm_v(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                                           const float* __restrict__ val, const uint32_t* __restrict__ Lcol,
+                                           uint32_t o, uint32_t k, uint32_t jp, const float* __restrict__ gt,
+                                           float gscale) {
+  // gt[o + q] * gscale = g~ of edge q
+  float v = 0.0f;
+  uint32_t a = row_ptr[jp];
+  const uint32_t a1 = row_ptr[jp + 1];
+  uint32_t q = 0;
+  while (a < a1 && q < k) {
+    const uint32_t c = col[a], jq = Lcol[o + q];
+    if (c < jq) {
+      ++a;
+    } else if (c > jq) {
+      ++q;
+    } else {
+      v += val[a] * (gt[o + q] * gscale);
+      ++a;
+    }
+  }
+  return v;
+}
+
+// Forward: G = s_i g~ (s_i = N_i^{-1/2}; s_i = 1 when N_i is not positive and
+// finite, counted in nonpos together with non-positive diagonal entries);
+// rowscale[i] = s_i.
+__global__ __launch_bounds__(kBlock) void k_fsai_rownorm(const uint32_t* __restrict__ row_ptr,
+                                                         const uint32_t* __restrict__ col,
+                                                         const float* __restrict__ val, uint32_t n,
+                                                         const uint32_t* __restrict__ Lrp,
+                                                         const uint32_t* __restrict__ Lcol,
+                                                         const float* __restrict__ out,
+                                                         const float* __restrict__ colscale, float* __restrict__ G,
+                                                         float* __restrict__ rowscale, uint32_t* __restrict__ nonpos) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t o = Lrp[i], k = Lrp[i + 1] - o;
+    for (uint32_t p = 0; p < k; ++p) G[o + p] = out[o + p] * colscale[o + p];  // g~, scaled below
+    float N = 0.0f;
+    for (uint32_t p = 0; p < k; ++p) N += G[o + p] * rownorm_v(row_ptr, col, val, Lcol, o, k, Lcol[o + p], G, 1.0f);
+    const bool ok = N > 0.0f && N < INFINITY;
+    const float sc = ok ? rsqrtf(N) : 1.0f;
+    bool diag_ok = false;
+    for (uint32_t p = 0; p < k; ++p) {
+      const float g = G[o + p] * sc;
+      G[o + p] = g;
+      if (Lcol[o + p] == i && g > 0.0f) diag_ok = true;
+    }
+    rowscale[i] = ok ? sc : 0.0f;
+    if (!ok || !diag_ok) atomicAdd(nonpos, 1u);
+  }
+}
+
+// Backward of k_fsai_rownorm followed by the column scaling: from gG = dL/dG
+// per edge, dL/dout_e = colscale_e * dL/dg~_e with
+//   dL/dg~_p = s gG_p - s^3 v_p sum_q gG_q g~_q   (v = A[J,J] g~, g~ = G / s);
+// rows whose normalisation was skipped (rowscale 0) pass gG through.
+__global__ __launch_bounds__(kBlock) void k_fsai_rownorm_grad(const uint32_t* __restrict__ row_ptr,
+                                                              const uint32_t* __restrict__ col,
+                                                              const float* __restrict__ val, uint32_t n,
+                                                              const uint32_t* __restrict__ Lrp,
+                                                              const uint32_t* __restrict__ Lcol,
+                                                              const float* __restrict__ G,
+                                                              const float* __restrict__ rowscale,
+                                                              const float* __restrict__ colscale,
+                                                              const float* __restrict__ gG, float* __restrict__ gout) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t o = Lrp[i], k = Lrp[i + 1] - o;
+    const float s = rowscale[i];
+    if (!(s > 0.0f)) {
+      for (uint32_t p = 0; p < k; ++p) gout[o + p] = gG[o + p] * colscale[o + p];
+      continue;
+    }
+    const float inv = 1.0f / s;
+    float c = 0.0f;
+    for (uint32_t p = 0; p < k; ++p) c += gG[o + p] * (G[o + p] * inv);
+    const float s3c = s * s * s * c;
+    for (uint32_t p = 0; p < k; ++p) {
+      const float v = rownorm_v(row_ptr, col, val, Lcol, o, k, Lcol[o + p], G, inv);
+      gout[o + p] = colscale[o + p] * (s * gG[o + p] - s3c * v);
+    }
   }
 }
 

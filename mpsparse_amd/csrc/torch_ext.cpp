@@ -324,6 +324,29 @@ class FsaiTensor {
     check(msp_fsai_set_values(h_, g.data_ptr<float>(), &np, stream_of(device_)), "set_values");
     return np;
   }
+  // "coupled8" or "stated5" (msp_fsai_features)
+  void select_features(const std::string& name) {
+    int32_t set = -1;
+    if (name == "coupled8") set = MSP_FSAI_FEATURES_COUPLED8;
+    else if (name == "stated5") set = MSP_FSAI_FEATURES_STATED5;
+    TORCH_CHECK(set >= 0, "feature set must be 'coupled8' or 'stated5', got '", name, "'");
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_select_features(h_, set, stream_of(device_)), "select_features");
+  }
+  std::string feature_set() const {
+    int32_t set = 0;
+    check(msp_fsai_get_features(h_, &set), "feature_set");
+    return set == MSP_FSAI_FEATURES_STATED5 ? "stated5" : "coupled8";
+  }
+  // G from an edge-network output o (g = o / sqrt(a_jj), then the row
+  // normalisation of the stated-5 feature set); returns the non-positive count
+  int set_network_output(torch::Tensor out) {
+    expect(out, torch::kFloat32, (int64_t)ne_, "out");
+    int32_t np = 0;
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+    check(msp_fsai_set_network_output(h_, out.data_ptr<float>(), &np, stream_of(device_)), "set_network_output");
+    return np;
+  }
   int set_network(torch::Tensor params) {
     expect(params, torch::kFloat32, msp_mlp_num_params(), "params");
     TORCH_CHECK(params.is_cuda(), "params must be a ROCm tensor");
@@ -862,7 +885,12 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def("features", &FsaiTensor::features)
       .def("classical", &FsaiTensor::classical, "classical FSAI values; returns the number of Jacobi fallback rows")
       .def("set_values", &FsaiTensor::set_values, py::arg("g"))
+      .def("select_features", &FsaiTensor::select_features, py::arg("feature_set"),
+           "edge-network inputs: 'coupled8' (default) or 'stated5' (a_ij, a_ii, a_jj, row nnz, row |sum|; "
+           "G's rows normalised to diag(G A G^T) = 1)")
+      .def_property_readonly("feature_set", &FsaiTensor::feature_set)
       .def("set_network", &FsaiTensor::set_network, py::arg("params"))
+      .def("set_network_output", &FsaiTensor::set_network_output, py::arg("out"))
       .def("values", &FsaiTensor::values)
       .def("apply", &FsaiTensor::apply, py::arg("r"), py::arg("z"))
       .def("probe_loss_grad", &FsaiTensor::probe_loss_grad, py::arg("W"), py::arg("grad"))

@@ -1,23 +1,38 @@
 """Learned FSAI preconditioner: an edge MLP predicts G on A's lower pattern,
 M = G^T G, applied inside CG as two SpMVs (all kernels in libmpsparse_amd).
 
-Network: per lower-pattern edge (i, j <= i) the 8 features of
-``fsai_tensor.features()`` (normalised a_ij, a_ii, a_jj, row nnz, row |sum|,
-diagonal flag) -> 64 -> 64 -> 1 with ReLU; BF16 matrix-core operands with FP32
-accumulation (forward and backward are hand-written HIP kernels). The output
-o_e becomes g_ij = o_e / sqrt(a_jj), so o = 1 on the diagonal and 0 elsewhere
-is Jacobi. A non-positive g_ii makes the solver fall back to plain CG.
+Network: per lower-pattern edge (i, j <= i) a feature vector -> 64 -> 64 -> 1
+with ReLU; BF16 matrix-core operands with FP32 accumulation (forward and
+backward are hand-written HIP kernels). Two feature sets
+(``fsai_tensor.select_features``):
+
+* ``"stated5"`` (the default network): the edge's five quantities a_ij, a_ii,
+  a_jj, nnz(row i) and sum_j |a_ij|, made scale-free as a_ij/sqrt(a_ii a_jj),
+  log(a_jj/a_ii), nnz/8, sum|a|/a_ii and the diagonal flag [j == i]; the
+  output o_e gives g~_ij = o_e / sqrt(a_jj), and each row is normalised to
+  diag(G A G^T) = 1 (the scaling classical FSAI has by construction), so the
+  network sets the direction of every row of G and the normalisation its
+  length;
+* ``"coupled8"`` (a labelled variant): the same first five plus the row
+  couplings s_low, s_all and 1/sqrt(1 - s_low) -- classical FSAI's own
+  diagonal on stencil rows -- with g_ij = o_e / sqrt(a_jj) and no
+  normalisation.
+
+A non-positive g_ii (or a row whose quadratic form is not positive) makes the
+solver fall back to plain CG.
 
 Training (``train``): log-normal diffusion matrices (2D 5-point / 3D 7-point,
-sigma in [0.5, 2.5], n between 16k and 1M) are generated on the GPU; for each
-one, residuals of short plain-CG runs (plus one white-noise vector) serve as
-probes w, and Adam minimises sum_w ||G A G^T w - w||^2 / ||w||^2.
+sigma in [0.5, 2.5], n between 16k and 1M) are generated on the GPU; the first
+matrices regress the output onto classical FSAI (for "stated5" onto each
+row's direction, o_ii = 1), the rest minimise
+sum_w ||G A G^T w - w||^2 / ||w||^2 with residuals of short plain-CG runs
+(plus one white-noise vector) as probes w. The parameters with the fewest
+FSAI-PCG steps on a held-out validation matrix are kept.
 """
 from __future__ import annotations
 
 import math
 import os
-import sys
 import time
 
 import numpy as np
@@ -34,7 +49,23 @@ OB2 = OW2 + 64 * 64
 OW3 = OB2 + 64
 OB3 = OW3 + 64
 
-DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "learned_fsai_mlp.npz")
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA, "learned_fsai_stated5.npz")   # the stated 5-input network
+COUPLED8_WEIGHTS = os.path.join(DATA, "learned_fsai_coupled8.npz")  # the 8-feature variant
+FEATURE_SETS = ("stated5", "coupled8")
+
+
+class Network:
+    """Edge-network parameters (float32 device tensor) and their feature set."""
+
+    def __init__(self, params, feature_set="stated5"):
+        if feature_set not in FEATURE_SETS:
+            raise ValueError(f"feature set must be one of {FEATURE_SETS}, got {feature_set!r}")
+        self.params = params
+        self.feature_set = feature_set
+
+    def __repr__(self):
+        return f"Network(feature_set={self.feature_set!r}, params={tuple(self.params.shape)})"
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -50,24 +81,45 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 
 def load_params(path: str = DEFAULT_WEIGHTS, device="cuda"):
+    """The parameter tensor of a weights file (see load_network for its feature set)."""
+    return load_network(path, device).params
+
+
+def load_network(path: str = DEFAULT_WEIGHTS, device="cuda") -> Network:
     import torch
 
     with np.load(path, allow_pickle=False) as z:
         P = z["params"].astype(np.float32)
+        fs = str(z["feature_set"]) if "feature_set" in z.files else "coupled8"
     if P.size != NPARAMS:
         raise ValueError(f"{path}: {P.size} parameters, expected {NPARAMS}")
-    return torch.from_numpy(P).to(device)
+    return Network(torch.from_numpy(P).to(device), fs)
 
 
-def save_params(P, path: str = DEFAULT_WEIGHTS, meta: dict | None = None):
+def save_params(P, path: str = DEFAULT_WEIGHTS, meta: dict | None = None, feature_set: str = "stated5"):
+    """Write the parameters (float32), the feature set and a metadata record.
+    For "stated5" the first layer's weights on the three unused input slots are
+    stored as zeros (they only ever multiply zero features)."""
+    if feature_set not in FEATURE_SETS:
+        raise ValueError(feature_set)
     os.makedirs(os.path.dirname(path), exist_ok=True)
     arr = P.detach().cpu().numpy().astype(np.float32) if hasattr(P, "detach") else np.asarray(P, np.float32)
-    np.savez(path, params=arr, meta=np.array(repr(meta or {})))
+    arr = arr.copy()
+    if feature_set == "stated5":
+        arr[OW1:OB1].reshape(64, 8)[:, 5:] = 0.0
+    np.savez(path, params=arr, feature_set=np.array(feature_set), meta=np.array(repr(meta or {})))
 
 
-def build(A, params):
-    """fsai_tensor for csr_tensor A with G from the network; returns (F, nonpositive_diag)."""
+def build(A, net, feature_set=None):
+    """fsai_tensor for csr_tensor A with G from the network (a Network, or a
+    parameter tensor with `feature_set`, default "coupled8" for a bare tensor);
+    returns (F, nonpositive_diag)."""
+    if isinstance(net, Network):
+        params, fs = net.params, net.feature_set
+    else:
+        params, fs = net, feature_set or "coupled8"
     F = fsai_tensor(A)
+    F.select_features(fs)
     nonpos = F.set_network(params)
     return F, nonpos
 
@@ -114,17 +166,26 @@ def harvest_probes(A, n, gen, device="cuda", iters=(5, 20, 80)):
 
 
 def imitation_target(F):
-    """Network output that reproduces classical FSAI: o_e = g_e * sqrt(a_jj)."""
+    """Network output that reproduces classical FSAI: o_e = g_e * sqrt(a_jj);
+    for the row-normalised "stated5" network, each row's direction with
+    o_ii = 1 (the normalisation restores classical FSAI from it)."""
     F.classical()
     g = F.values()
     _, cs = F.features()
-    return g / cs
+    o = g / cs
+    if F.feature_set == "stated5":
+        rows, cols = F.edges()
+        dg = rows == cols
+        diag = o.new_ones(int(rows.max()) + 1 if rows.numel() else 0)
+        diag[rows[dg].long()] = o[dg]
+        o = o / diag[rows.long()]
+    return o
 
 
 class Validator:
     """FSAI-PCG iterations to relres 1e-6 on a fixed matrix (3D, sigma 2)."""
 
-    def __init__(self, side=40, sigma=2.0, seed=999, device="cuda"):
+    def __init__(self, side=40, sigma=2.0, seed=999, device="cuda", feature_set="stated5"):
         import torch
 
         from . import spmv
@@ -142,6 +203,7 @@ class Validator:
         self.F.classical()
         x.zero_()
         self.classical = _C.pcg_fsai(self.A, self.F, self.b, x, tol=1e-6)["iterations"]
+        self.F.select_features(feature_set)
 
     def __call__(self, params):
         import torch
@@ -153,8 +215,8 @@ class Validator:
 
 
 def train(params=None, matrices=2000, imitate=1000, updates_per_matrix=2, lr=2e-3, probe_lr=5e-4, seed=0,
-          device="cuda", log=print, sigma_range=(0.5, 2.5), validate_every=100):
-    """Train the edge MLP on freshly generated matrices.
+          device="cuda", log=print, sigma_range=(0.5, 2.5), validate_every=100, feature_set="stated5"):
+    """Train the edge MLP (inputs: `feature_set`) on freshly generated matrices.
 
     Matrices 0 .. imitate-1: regression of the output onto classical FSAI
     (mean squared error over the edges); the rest: the probe loss
@@ -174,7 +236,7 @@ def train(params=None, matrices=2000, imitate=1000, updates_per_matrix=2, lr=2e-
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
-    val = Validator(device=device)
+    val = Validator(device=device, feature_set=feature_set)
     hist = {"loss": [], "validation": [], "jacobi_its": val.jacobi, "classical_its": val.classical}
     log(f"[train] validation matrix n={val.n}: jacobi {val.jacobi} its, classical FSAI {val.classical} its")
     best = (val(params), params.clone())
@@ -188,6 +250,7 @@ def train(params=None, matrices=2000, imitate=1000, updates_per_matrix=2, lr=2e-
         A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
         del keys, vals
         F = fsai_tensor(A)
+        F.select_features(feature_set)
         if k < imitate:
             feat, _ = F.features()
             target = imitation_target(F)
@@ -237,29 +300,36 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3, help="Adam step of the regression phase")
     ap.add_argument("--probe-lr", type=float, default=5e-4, help="peak Adam step of the probe phase (cosine decay)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--features", default="stated5", choices=FEATURE_SETS,
+                    help="network inputs: the stated five (default) or the 8-feature variant")
+    ap.add_argument("--out", default=None, help="weights file (default: the packaged file of the feature set)")
     ap.add_argument("--init", default=None, help="start from these weights")
     a = ap.parse_args(argv)
     import torch
 
+    out = a.out or (DEFAULT_WEIGHTS if a.features == "stated5" else COUPLED8_WEIGHTS)
     P = load_params(a.init) if a.init else None
     P, hist = train(P, matrices=a.matrices, imitate=a.imitate, updates_per_matrix=a.updates, lr=a.lr,
-                    probe_lr=a.probe_lr, seed=a.seed, log=lambda s: print(s, flush=True))
+                    probe_lr=a.probe_lr, seed=a.seed, log=lambda s: print(s, flush=True), feature_set=a.features)
     torch.cuda.synchronize()
-    save_params(P, a.out, {"matrices": a.matrices, "imitate": a.imitate, "updates": a.updates, "lr": a.lr,
-                           "probe_lr": a.probe_lr,
-                           "seed": a.seed, "validation": hist["validation"],
-                           "best_validation_its": hist["best_validation_its"],
-                           "classical_its": hist["classical_its"], "jacobi_its": hist["jacobi_its"]})
-    print(f"saved {a.out}", flush=True)
+    save_params(P, out, {"matrices": a.matrices, "imitate": a.imitate, "updates": a.updates, "lr": a.lr,
+                         "probe_lr": a.probe_lr, "seed": a.seed, "features": a.features,
+                         "validation": hist["validation"], "best_validation_its": hist["best_validation_its"],
+                         "classical_its": hist["classical_its"], "jacobi_its": hist["jacobi_its"]},
+                feature_set=a.features)
+    print(f"saved {out}", flush=True)
 
 
 if __name__ == "__main__":
     main()
-    # After a training run (thousands of matrix and preconditioner handles
-    # created and destroyed) the process has ended with SIGSEGV inside native
-    # library teardown, after Python finalisation and after the weights were
-    # written; skip that teardown once everything is flushed.
+    # The training process has ended with SIGSEGV after the weights file was
+    # written and closed, during interpreter / runtime teardown -- also with the
+    # library's own teardown made HIP-free (exit guard, leaked registries,
+    # cached blocks released from an atexit hook). Processes that use the
+    # library otherwise (tests, benchmarks) end normally. Until the cause is
+    # known, this command skips the teardown once its output is flushed.
+    import sys
+
     sys.stdout.flush()
     sys.stderr.flush()
     os._exit(0)

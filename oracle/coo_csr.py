@@ -9,7 +9,8 @@ are permuted alongside (coo_csr.metal:144-193).
 
 Three independent formulations live here:
 
-* :func:`coo_to_csr` -- numpy ``argsort(kind="stable")`` + ``searchsorted``;
+* :func:`coo_to_csr` -- numpy ``argsort(kind="stable")`` + ``searchsorted``
+  (:func:`coo_to_csr_bucketed` is the same in parallel threads, for large inputs);
 * :func:`coo_to_csr_lsd` -- the step-by-step C restatement of the reference's
   kernels (``lsd_coo_csr.c``, compiled by ``oracle/Makefile``);
 * :func:`coo_to_csr_bruteforce` -- Python ``sorted`` over ``(key, index)``.
@@ -51,6 +52,44 @@ def coo_to_csr(keys, vals, num_rows: int, num_cols: int):
     vals = np.ascontiguousarray(vals)
     _check_range(keys, num_rows, num_cols)
     perm = np.argsort(keys, kind="stable")
+    sk = keys[perm]
+    rows = (sk >> np.uint64(32)).astype(np.int64)
+    row_ptr = np.searchsorted(rows, np.arange(num_rows + 1, dtype=np.int64), side="left")
+    col_ind = (sk & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    return row_ptr.astype(np.uint32), col_ind, vals[perm]
+
+
+def coo_to_csr_bucketed(keys, vals, num_rows: int, num_cols: int, workers: int = 16, buckets: int = 128):
+    """:func:`coo_to_csr` for large inputs, in parallel threads; same result.
+
+    The keys are split into `buckets` ranges of their row (the high half of
+    the key, so every key of bucket b precedes every key of bucket b+1); each
+    bucket keeps its keys in input order (a boolean mask preserves order) and
+    is sorted with numpy's stable argsort in a thread pool (numpy releases the
+    interpreter lock while sorting). Concatenating the buckets is therefore the
+    stable sort of all keys. Pinned against :func:`coo_to_csr` in the tests."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    keys = np.ascontiguousarray(keys).view(np.uint64)
+    vals = np.ascontiguousarray(vals)
+    _check_range(keys, num_rows, num_cols)
+    nnz = keys.size
+    if nnz == 0 or num_rows <= 1:
+        return coo_to_csr(keys, vals, num_rows, num_cols)
+    shift = max(0, int(num_rows - 1).bit_length() - max(1, int(buckets - 1).bit_length()))
+    bid = ((keys >> np.uint64(32)) >> np.uint64(shift)).astype(np.uint8)  # buckets <= 256
+    nb = int(bid.max()) + 1
+    counts = np.bincount(bid, minlength=nb)
+    order = np.argsort(bid, kind="stable")  # by bucket, input order inside (radix sort for 8-bit keys)
+    starts = np.concatenate(([0], np.cumsum(counts)))
+    perm = np.empty(nnz, dtype=np.int64)
+
+    def one(b):
+        idx = order[starts[b]:starts[b + 1]]
+        perm[starts[b]:starts[b + 1]] = idx[np.argsort(keys[idx], kind="stable")]
+
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        list(ex.map(one, range(nb)))
     sk = keys[perm]
     rows = (sk >> np.uint64(32)).astype(np.int64)
     row_ptr = np.searchsorted(rows, np.arange(num_rows + 1, dtype=np.int64), side="left")

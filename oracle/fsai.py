@@ -12,6 +12,10 @@ additions from their definitions so the GPU kernels can be checked:
   e_i / sqrt(y_i) and by the diagonal case G = D^{-1/2};
 * :func:`probe_loss_grad` -- L = sum_w ||G A G^T w - w||^2 and dL/dG on the
   pattern; pinned by central finite differences;
+* :func:`rownorm` / :func:`rownorm_grad` -- the stated-5 network's row
+  normalisation diag(G A G^T) = 1 and its chain rule; pinned by classical
+  FSAI (a row shape from classical FSAI normalises back to classical FSAI:
+  it has unit diag(G A G^T) by construction) and by central finite differences;
 * :func:`mlp_forward` / :func:`mlp_backward` -- 8 -> 64 -> 64 -> 1 ReLU network
   in float64, optionally rounding the matrix-core operands to BF16 exactly where
   the kernels do; pinned against torch autograd (float64) with rounding off.
@@ -52,8 +56,13 @@ def diag_of(row_ptr, col_ind, vals):
     return d
 
 
-def features(row_ptr, col_ind, vals):
-    """(feat[ne, 8], colscale[ne]) for the lower pattern (float64)."""
+def features(row_ptr, col_ind, vals, feature_set="coupled8"):
+    """(feat[ne, 8], colscale[ne]) for the lower pattern (float64).
+
+    "coupled8": the 8 features with the row couplings s_low, s_all, t;
+    "stated5": the edge's own five quantities (a_ij, a_ii, a_jj, row nnz,
+    row sum |a|) made scale-free -- the first five columns of "coupled8" --
+    and zeros in the last three."""
     row_ptr = np.asarray(row_ptr, dtype=np.int64)
     vals = np.asarray(vals, dtype=np.float64)
     n = row_ptr.size - 1
@@ -78,7 +87,41 @@ def features(row_ptr, col_ind, vals):
     f[:, 5] = s_low[r]
     f[:, 6] = s_all[r]
     f[:, 7] = 1.0 / np.sqrt(np.maximum(1.0 - s_low[r], 1.0 / 64.0))
+    if feature_set == "stated5":
+        f[:, 5:] = 0.0
+    elif feature_set != "coupled8":
+        raise ValueError(feature_set)
     return f, 1.0 / np.sqrt(d[c])
+
+
+def rownorm(A, rows, cols, gt):
+    """(G, s): G_i = gt_i / sqrt(N_i), N_i = gt_i A[J,J] gt_i^T, so that
+    diag(G A G^T) = 1 (s_i = N_i^{-1/2}; rows with N_i <= 0 keep gt and get
+    s_i = 0). A dense quadratic form per row -- the definition itself."""
+    A = sp.csr_matrix(A, dtype=np.float64)
+    n = A.shape[0]
+    Gt = G_matrix(n, rows, cols, gt)
+    N = np.asarray((Gt @ A).multiply(Gt).sum(axis=1)).ravel()  # (G~ A G~^T)_ii
+    ok = N > 0
+    s = np.where(ok, 1.0 / np.sqrt(np.where(ok, N, 1.0)), 0.0)
+    G = np.asarray(gt, dtype=np.float64) * np.where(ok, s, 1.0)[rows]
+    return G, s
+
+
+def rownorm_grad(A, rows, cols, gt, gG):
+    """dL/dgt from dL/dG through rownorm (chain rule of G = gt / sqrt(N)):
+    dL/dgt_p = s gG_p - s^3 v_p sum_q gG_q gt_q with v = A[J,J] gt."""
+    A = sp.csr_matrix(A, dtype=np.float64)
+    n = A.shape[0]
+    gt = np.asarray(gt, dtype=np.float64)
+    gG = np.asarray(gG, dtype=np.float64)
+    _, s = rownorm(A, rows, cols, gt)
+    Gt = G_matrix(n, rows, cols, gt)
+    V = (Gt @ A).tocsr()  # V[i, j] = (gt_i A)_j; v_p = V[i, J_p] (A symmetric)
+    v = np.asarray(V[rows, cols]).ravel()
+    c = np.bincount(rows, weights=gG * gt, minlength=n)
+    ok = s[rows] > 0
+    return np.where(ok, s[rows] * gG - s[rows] ** 3 * v * c[rows], gG)
 
 
 def classical_fsai(A: sp.csr_matrix):

@@ -149,21 +149,27 @@ def richardson_closed_form(A_dense, b, omega, iters):
     return acc
 
 
-def _scipy_solve(fn, A, b, tol, maxiter):
+def _scipy_solve(fn, A, b, tol, maxiter, **kw):
     it = [0]
 
     def cb(_):
         it[0] += 1
 
-    x, info = fn(A, b, rtol=tol, atol=0.0, maxiter=maxiter, callback=cb)
+    x, info = fn(A, b, rtol=tol, atol=0.0, maxiter=maxiter, callback=cb, **kw)
     return x, it[0], info
 
 
-def scipy_cg(A, b, tol=1e-6, maxiter=10000):
-    """scipy.sparse.linalg.cg (the CPU baseline). Returns (x, iterations, info)."""
+def scipy_cg(A, b, tol=1e-6, maxiter=10000, jacobi=False):
+    """scipy.sparse.linalg.cg (the CPU baseline), optionally preconditioned with
+    M = diag(A)^-1. Returns (x, iterations, info)."""
+    from scipy.sparse.linalg import LinearOperator
     from scipy.sparse.linalg import cg as _cg
 
-    return _scipy_solve(_cg, A, b, tol, maxiter)
+    kw = {}
+    if jacobi:
+        dinv = 1.0 / A.diagonal()
+        kw["M"] = LinearOperator(A.shape, matvec=lambda v: dinv * v, dtype=dinv.dtype)
+    return _scipy_solve(_cg, A, b, tol, maxiter, **kw)
 
 
 def scipy_bicgstab(A, b, tol=1e-6, maxiter=10000):

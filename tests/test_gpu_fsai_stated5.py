@@ -1,0 +1,201 @@
+"""The stated-5 edge network (inputs a_ij, a_ii, a_jj, row nnz, row |sum|) and
+its row normalisation diag(G A G^T) = 1, on the GPU against the oracle.
+
+Bars:
+* features: the first five columns of the 8-feature set, the last three zero,
+  |d| <= 2e-6 * (1 + |ref|) against oracle.fsai.features(feature_set="stated5");
+* row normalisation of a network output: |G - oracle.rownorm| <= 1e-5 * max|G|
+  (FP32 quadratic forms); the stencil shape o_ii = 1, o_ij = -a_ij/sqrt(a_ii a_jj)
+  gives classical FSAI to the same bar; diag(G A G^T) = 1 to 1e-5;
+* a row whose quadratic form or diagonal is not positive is counted (solver
+  falls back to plain CG);
+* set_network (BF16 MFMA forward, then normalisation) against the oracle
+  composition with the same BF16 roundings: |d| <= 2e-3 * max|G|;
+* train_step = forward -> normalisation -> probe loss -> chain rule through the
+  normalisation -> backward: loss to 1e-3, gradient to 1e-2 (2-norm);
+* the shipped stated-5 weights: no non-positive diagonal, FSAI-PCG converges
+  (FP64 true relres <= 1.05e-6) within 15% + 2 of FP64 PCG with the same G, in
+  fewer steps than Jacobi-PCG.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from gpu_util import csr_from_arrays, say, t_f32
+from mpsparse_amd.problems import diffusion_2d, diffusion_3d
+from oracle import fsai as OF
+from oracle import krylov as K
+
+pytestmark = pytest.mark.gpu
+
+
+def _scipy(M):
+    return sp.csr_matrix((M.vals.astype(np.float64), M.col_ind.astype(np.int64), M.row_ptr.astype(np.int64)),
+                         shape=(M.num_rows, M.num_cols))
+
+
+def _coupled(n_side=14, seed=6):
+    """SPD with coupled lower neighbours: the square of a 2D diffusion matrix."""
+    M0 = diffusion_2d(n_side, sigma=1.0, seed=seed)
+    S = (_scipy(M0) @ _scipy(M0)).tocsr().astype(np.float32)
+    S.sort_indices()
+
+    class M:
+        pass
+
+    M.row_ptr, M.col_ind, M.vals = S.indptr.astype(np.uint32), S.indices.astype(np.uint32), S.data
+    M.num_rows = M.num_cols = S.shape[0]
+    return M
+
+
+CASES = {
+    "diffusion3d_12_s1.5": lambda: diffusion_3d(12, sigma=1.5, seed=2),
+    "diffusion2d_30_s2": lambda: diffusion_2d(30, sigma=2.0, seed=4),
+    "coupled_14": _coupled,
+}
+
+
+def _handles(M, feature_set="stated5"):
+    from mpsparse_amd.learned_fsai import fsai_tensor
+
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    F = fsai_tensor(A)
+    F.select_features(feature_set)
+    return A, F
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_stated5_features(name):
+    M = CASES[name]()
+    say("stated5 features", name)
+    A, F = _handles(M)
+    assert F.feature_set == "stated5"
+    f, cs = (t.cpu().numpy().astype(np.float64) for t in F.features())
+    f_ref, cs_ref = OF.features(M.row_ptr, M.col_ind, M.vals, feature_set="stated5")
+    assert np.all(f[:, 5:] == 0)
+    assert np.all(np.abs(f - f_ref) <= 2e-6 * (1 + np.abs(f_ref))), np.abs(f - f_ref).max()
+    assert np.all(np.abs(cs - cs_ref) <= 2e-6 * cs_ref)
+    F.select_features("coupled8")
+    f8, _ = (t.cpu().numpy().astype(np.float64) for t in F.features())
+    f8_ref, _ = OF.features(M.row_ptr, M.col_ind, M.vals)
+    assert np.all(np.abs(f8 - f8_ref) <= 2e-6 * (1 + np.abs(f8_ref)))
+    with pytest.raises(Exception):
+        F.select_features("other")
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_rownorm_of_network_output(name):
+    M = CASES[name]()
+    S = _scipy(M)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    f, cs = OF.features(M.row_ptr, M.col_ind, M.vals, feature_set="stated5")
+    rng = np.random.default_rng(7)
+    # the stencil shape, jittered and scaled per row (the scale must not matter)
+    o = np.where(r == c, 1.0, -f[:, 0]) * np.exp(0.5 * rng.standard_normal(M.num_rows))[r]
+    o = np.where(r == c, o, o * (1 + 0.1 * rng.standard_normal(r.size))).astype(np.float32)
+    say("stated5 rownorm", name)
+    A, F = _handles(M)
+    nonpos = F.set_network_output(t_f32(o))
+    assert nonpos == 0
+    g = F.values().cpu().numpy().astype(np.float64)
+    g_ref, s_ref = OF.rownorm(S, r, c, o.astype(np.float64) * cs)
+    assert np.all(s_ref > 0)
+    assert np.max(np.abs(g - g_ref)) <= 1e-5 * np.max(np.abs(g_ref)), np.max(np.abs(g - g_ref))
+    G = OF.G_matrix(M.num_rows, r, c, g)
+    assert np.allclose((G @ S @ G.T).diagonal(), 1.0, atol=1e-5)
+    if name != "coupled_14":  # uncoupled lower neighbours: the exact shape is classical FSAI
+        o2 = np.where(r == c, 1.0, -f[:, 0]).astype(np.float32)
+        F.set_network_output(t_f32(o2))
+        g2 = F.values().cpu().numpy().astype(np.float64)
+        g_cl = OF.classical_fsai(S)
+        assert np.max(np.abs(g2 - g_cl)) <= 1e-5 * np.max(np.abs(g_cl))
+
+
+def test_rownorm_counts_bad_rows():
+    M = diffusion_2d(16, sigma=1.0, seed=3)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    o = np.where(r == c, 1.0, -0.2).astype(np.float32)
+    o[(r == 5) & (c == 5)] = -1.0  # row 5: negative diagonal (positive quadratic form)
+    o[r == 9] = 0.0  # row 9: zero quadratic form
+    say("stated5 rownorm bad rows")
+    A, F = _handles(M)
+    assert F.set_network_output(t_f32(o)) == 2
+
+
+def _params(seed, scale=0.3):
+    from mpsparse_amd.learned_fsai import init_params
+
+    P = init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 100)
+    P[OF.OB1:OF.OW2] = 0.1 * rng.standard_normal(64)
+    P[OF.OB2:OF.OW3] = 0.1 * rng.standard_normal(64)
+    P[OF.OW3:OF.OB3] = scale * rng.standard_normal(64) / 8
+    return P.astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ["diffusion3d_12_s1.5", "coupled_14"])
+def test_set_network_is_forward_then_rownorm(name):
+    M = CASES[name]()
+    S = _scipy(M)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    P = _params(8)
+    say("stated5 set_network", name)
+    A, F = _handles(M)
+    feat, cs = (t.cpu().numpy() for t in F.features())
+    out, _, _ = OF.mlp_forward(P, feat, bf16=True)
+    g_ref, s_ref = OF.rownorm(S, r, c, out * cs)
+    nonpos = F.set_network(t_f32(P))
+    g = F.values().cpu().numpy().astype(np.float64)
+    diag_ref = g_ref[r == c]
+    assert nonpos == int(np.sum(s_ref == 0) + np.sum((s_ref[r[r == c]] > 0) & (diag_ref <= 0)))
+    assert np.max(np.abs(g - g_ref)) <= 2e-3 * np.max(np.abs(g_ref)), np.max(np.abs(g - g_ref))
+
+
+def test_train_step_through_rownorm():
+    torch = __import__("torch")
+    M = diffusion_2d(24, sigma=1.5, seed=11)
+    S = _scipy(M)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    say("stated5 train_step")
+    A, F = _handles(M)
+    P = _params(3, scale=0.1)
+    W = np.random.default_rng(12).standard_normal((2, M.num_rows)).astype(np.float32)
+    grad = torch.zeros(OF.NPARAMS, device="cuda")
+    loss = F.train_step(t_f32(P), t_f32(W), grad)
+    feat, cs = (t.cpu().numpy() for t in F.features())
+    out, _, _ = OF.mlp_forward(P, feat, bf16=True)
+    gt = out * cs
+    G, _ = OF.rownorm(S, r, c, gt)
+    loss_ref, gG = OF.probe_loss_grad(S, r, c, G, W)
+    ggt = OF.rownorm_grad(S, r, c, gt, gG)
+    ref = OF.mlp_backward(P, feat, ggt * cs, bf16=True)
+    assert abs(loss - loss_ref) <= 1e-3 * loss_ref, (loss, loss_ref)
+    got = grad.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-2 * np.linalg.norm(ref), (np.linalg.norm(got - ref), np.linalg.norm(ref))
+
+
+def test_shipped_stated5_weights_precondition():
+    torch = __import__("torch")
+    from mpsparse_amd import learned_fsai as LF
+
+    net = LF.load_network(LF.DEFAULT_WEIGHTS)
+    assert net.feature_set == "stated5"
+    M = diffusion_3d(24, sigma=2.0, seed=3)
+    S = _scipy(M)
+    b = np.random.default_rng(0).standard_normal(M.num_rows).astype(np.float32)
+    say("stated5 shipped weights 24^3")
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, M.num_rows, M.num_cols)
+    F, nonpos = LF.build(A, net)
+    assert nonpos == 0 and F.feature_set == "stated5"
+    x = torch.full((M.num_rows,), np.nan, device="cuda")
+    info = LF.solve(A, F, t_f32(b), x, tol=1e-6)
+    assert info["status"] == "converged" and not info["fell_back_to_cg"], info
+    assert K.true_relres(S, x.cpu().numpy(), b) <= 1.05e-6
+    g = F.values().cpu().numpy().astype(np.float64)
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    G = OF.G_matrix(M.num_rows, r, c, g)
+    _, it64, _, _ = K.cg(S, b.astype(np.float64), tol=1e-6, M_apply=lambda v: G.T @ (G @ v))
+    assert abs(info["iterations"] - it64) <= 0.15 * it64 + 2, (info, it64)
+    xj = torch.zeros(M.num_rows, device="cuda")
+    ij = A.cgd(t_f32(b), xj, tol=1e-6, precond="jacobi")
+    assert info["iterations"] < ij["iterations"], (info["iterations"], ij["iterations"])

@@ -121,3 +121,22 @@ def _rowptr_right(keys, vals, n, m):  # row_ptr from searchsorted side="right"
 @pytest.mark.parametrize("bad", [_unstable, _swapped, _rowptr_right])
 def test_known_answers_reject_broken_variants(kat_cases, bad):
     assert not all(check_kat(bad, c) for c in kat_cases)
+
+
+@pytest.mark.parametrize("n,m,nnz,dup", [(1, 1, 50, True), (7, 3, 200, True), (1000, 5, 30_000, True),
+                                         (5000, 2 ** 32 - 1, 100_000, False), (2 ** 20, 2 ** 20, 400_000, True),
+                                         (300, 300, 0, False)])
+def test_bucketed_oracle_equals_stable_argsort(n, m, nnz, dup):
+    """The threaded bucketed form is the same stable sort (duplicates in input
+    order, empty and skewed rows, buckets wider than the row range)."""
+    rng = np.random.default_rng(n + nnz)
+    rows = rng.integers(0, n, nnz, dtype=np.uint64)
+    if nnz:
+        rows[: nnz // 3] = rows[0]  # one heavy row
+    cols = rng.integers(0, min(m, 8) if dup else m, nnz, dtype=np.uint64)
+    keys = (rows << np.uint64(32)) | cols
+    vals = np.arange(nnz, dtype=np.float32)  # distinct values expose any reordering of duplicates
+    a = oc.coo_to_csr(keys, vals, n, m)
+    for workers, buckets in ((1, 2), (4, 128), (8, 256)):
+        b = oc.coo_to_csr_bucketed(keys, vals, n, m, workers=workers, buckets=buckets)
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))

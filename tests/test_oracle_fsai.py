@@ -110,3 +110,66 @@ def test_row_coupling_features_give_classical_fsai_on_stencils():
     assert np.all(f[:, 6] >= f[:, 5])
     # t (f[7]) is classical FSAI's diagonal output; off the diagonal -f0 t
     assert np.allclose(out, np.where(r == c, 1.0, -f[:, 0]) * f[:, 7], rtol=1e-12)
+
+
+def test_stated5_features_are_the_first_five():
+    A = _small()
+    f8, cs8 = OF.features(A.indptr, A.indices, A.data)
+    f5, cs5 = OF.features(A.indptr, A.indices, A.data, feature_set="stated5")
+    assert np.array_equal(f5[:, :5], f8[:, :5]) and np.all(f5[:, 5:] == 0) and np.array_equal(cs5, cs8)
+    with pytest.raises(ValueError):
+        OF.features(A.indptr, A.indices, A.data, feature_set="other")
+
+
+@pytest.mark.parametrize("case", ["stencil", "coupled"])
+def test_rownorm_of_classical_shape_is_classical(case):
+    """Classical FSAI has diag(G A G^T) = 1, so any positive row rescaling of it
+    normalises back to it exactly; on a stencil the shape o_ii = 1,
+    o_ij = -a_ij / sqrt(a_ii a_jj) (what a 5-input network can compute from
+    its own edge) gives classical FSAI."""
+    from mpsparse_amd.problems import diffusion_3d
+
+    if case == "stencil":
+        M = diffusion_3d(5, 4, 3, sigma=1.5, seed=2)
+        S = M.to_scipy()
+    else:  # a 9-point-like pattern whose lower neighbours are coupled
+        S = _small()
+        S = (S @ S).tocsr()
+        S.sort_indices()
+    r, c, _ = OF.lower_pattern(S.indptr, S.indices)
+    g = OF.classical_fsai(S)
+    rng = np.random.default_rng(4)
+    scale = np.exp(rng.standard_normal(S.shape[0]))[r]
+    G, s = OF.rownorm(S, r, c, g * scale)
+    assert np.allclose(G, g, rtol=1e-12, atol=1e-14)
+    Gm = OF.G_matrix(S.shape[0], r, c, G)
+    assert np.allclose((Gm @ S @ Gm.T).diagonal(), 1.0)
+    if case == "stencil":
+        f, cs = OF.features(S.indptr, S.indices, S.data, feature_set="stated5")
+        shape = np.where(r == c, 1.0, -f[:, 0]) * cs
+        G2, _ = OF.rownorm(S, r, c, shape)
+        assert np.allclose(G2, g, rtol=1e-12, atol=1e-14)
+
+
+def test_rownorm_grad_matches_finite_differences():
+    A = _small()
+    A = (A @ A).tocsr()  # coupled lower neighbours exercise every term of N_i
+    A.sort_indices()
+    r, c, _ = OF.lower_pattern(A.indptr, A.indices)
+    rng = np.random.default_rng(5)
+    gt = OF.classical_fsai(A) * (1 + 0.2 * rng.standard_normal(r.size))
+    W = rng.standard_normal((2, A.shape[0]))
+
+    def loss(x):
+        G, _ = OF.rownorm(A, r, c, x)
+        return OF.probe_loss_grad(A, r, c, G, W)
+
+    L, gG = loss(gt)
+    grad = OF.rownorm_grad(A, r, c, gt, gG)
+    for k in rng.choice(r.size, 16, replace=False):
+        h = 1e-6 * max(1.0, abs(gt[k]))
+        xp, xm = gt.copy(), gt.copy()
+        xp[k] += h
+        xm[k] -= h
+        fd = (loss(xp)[0] - loss(xm)[0]) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-5 * max(1.0, abs(fd)), (k, fd, grad[k])

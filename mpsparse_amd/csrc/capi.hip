@@ -695,6 +695,81 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
   return MSP_OK;
 }
 
+// Two-level sort (sortk::k_segment_sort): phase-1 LSD passes over the segment
+// bits of the row only, segment ranges by binary search, then one LDS sort per
+// segment that writes col / values / row_ptr (and rows when asked) directly.
+// *overflow is raised on the device when a segment exceeds sortk::kSegMax keys
+// (the caller then runs sort_passes instead). K: key type of the phase-1 output.
+template <typename K>
+static msp_status segment_sort(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n,
+                               const sortk::KeyMap& km, int sb, int passes, uint32_t* row_ptr, uint32_t* col_out,
+                               float* val_out, uint32_t* rows_out, int* err, int* overflow, hipStream_t s) {
+  using namespace sortk;
+  const uint32_t G = (uint32_t)(((uint64_t)n + (1ull << sb) - 1) >> sb);
+  const int shift0 = (int)km.cbits + sb;
+  TmpMem bounds, counts, sums, ka, kb, va, vb;
+  const K* kres = nullptr;
+  const float* vres = vals;
+  if (passes > 0) {
+    const uint32_t num_tiles = (uint32_t)((nnz + kTile - 1) / kTile);
+    const uint32_t nchunks = (num_tiles + kColChunk - 1) / kColChunk;
+    MSPCHK(counts.alloc(sizeof(uint32_t) * (uint64_t)kRadix * num_tiles, s));
+    MSPCHK(sums.alloc(sizeof(uint32_t) * kRadix * nchunks, s));
+    MSPCHK(ka.alloc(sizeof(K) * nnz, s));
+    MSPCHK(va.alloc(sizeof(float) * nnz, s));
+    if (passes > 1) {
+      MSPCHK(kb.alloc(sizeof(K) * nnz, s));
+      MSPCHK(vb.alloc(sizeof(float) * nnz, s));
+    }
+    K* kbuf[2] = {ka.as<K>(), kb.as<K>()};
+    float* vbuf[2] = {va.as<float>(), vb.as<float>()};
+    for (int p = 0; p < passes; ++p) {
+      const int shift = shift0 + p * kRadixBits;
+      if (p == 0)
+        k_upsweep<uint64_t, true><<<num_tiles, kBlock, 0, s>>>(keys, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
+      else
+        k_upsweep<K, false><<<num_tiles, kBlock, 0, s>>>(kres, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
+      k_colscan_reduce<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
+      k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks);
+      k_colscan_down<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
+      if (p == 0)
+        k_downsweep<uint64_t, K, true, false><<<num_tiles, kBlock, 0, s>>>(keys, vres, kbuf[p & 1], vbuf[p & 1], nullptr,
+                                                                           nullptr, nnz, km, shift, num_tiles,
+                                                                           counts.as<uint32_t>());
+      else
+        k_downsweep<K, K, false, false><<<num_tiles, kBlock, 0, s>>>(kres, vres, kbuf[p & 1], vbuf[p & 1], nullptr,
+                                                                     nullptr, nnz, km, shift, num_tiles,
+                                                                     counts.as<uint32_t>());
+      MSPCHK(launch_check("segment phase 1"));
+      kres = kbuf[p & 1];
+      vres = vbuf[p & 1];
+    }
+  }
+  MSPCHK(bounds.alloc(sizeof(uint32_t) * ((size_t)G + 1), s));
+  k_seg_bounds<K><<<grid_for((uint64_t)G + 1), kBlock, 0, s>>>(kres, nnz, shift0, G, passes == 0,
+                                                              bounds.as<uint32_t>());
+  const uint32_t grid = std::max(1u, std::min<uint32_t>(G, 1u << 20));
+  if (passes == 0)
+    k_segment_sort<uint64_t, true><<<grid, kSegThreads, 0, s>>>(keys, vals, bounds.as<uint32_t>(), G, km, sb, n, nnz,
+                                                           col_out, val_out, rows_out, row_ptr, overflow, err);
+  else
+    k_segment_sort<K, false><<<grid, kSegThreads, 0, s>>>(kres, vres, bounds.as<uint32_t>(), G, km, sb, n, nnz, col_out,
+                                                     val_out, rows_out, row_ptr, overflow, err);
+  return launch_check("segment sort");
+}
+
+// Segment size for the two-level sort: the largest s (rows per segment 2^s)
+// whose average segment holds at most 60% of sortk::kSegMax keys; -1 when the
+// two-level sort does not apply (local bits plus the position index exceed 64,
+// or no phase-1 pass would be saved).
+static int segment_bits(uint64_t nnz, uint32_t n, uint32_t rbits, uint32_t cbits) {
+  const double avg = (double)nnz / std::max<uint32_t>(1, n);
+  int sb = 0;
+  while (sb < (int)rbits && sb < 31 && avg * (double)(1ull << (sb + 1)) <= 0.6 * sortk::kSegMax) ++sb;
+  if (cbits + (uint32_t)sb + sortk::kSegIdxBits > 64) return -1;
+  return sb;
+}
+
 // Sort (keys, vals) on the device into CSR arrays. `row_ptr` has n+1 entries,
 // `col_out`/`val_out` nnz entries. cols_in_order: the input already lists each
 // row's entries in ascending column order (a transposed CSR pattern, say), so
@@ -717,6 +792,26 @@ static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uin
     const uint32_t sorted_bits = cols_in_order ? rbits : tb;
     const int shift0 = cols_in_order ? (int)km.cbits : 0;
     const int passes = std::max(1, (int)((sorted_bits + sortk::kRadixBits - 1) / sortk::kRadixBits));
+    const int sb = cols_in_order ? -1 : segment_bits(nnz, n, rbits, km.cbits);
+    if (sb >= 0) {
+      const int passes1 = (int)((rbits - (uint32_t)sb + sortk::kRadixBits - 1) / sortk::kRadixBits);
+      TmpMem ovf;
+      MSPCHK(ovf.alloc(sizeof(int), s));
+      HIPCHK(hipMemsetAsync(ovf.p, 0, sizeof(int), s));
+      if (tb <= 32)
+        MSPCHK(segment_sort<uint32_t>(keys, vals, nnz, n, km, sb, passes1, row_ptr, col_out, val_out, rows_out,
+                                      err.as<int>(), ovf.as<int>(), s));
+      else
+        MSPCHK(segment_sort<uint64_t>(keys, vals, nnz, n, km, sb, passes1, row_ptr, col_out, val_out, rows_out,
+                                      err.as<int>(), ovf.as<int>(), s));
+      int hf[2] = {0, 0};
+      HIPCHK(hipMemcpyAsync(&hf[0], err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(&hf[1], ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (hf[0]) return fail(MSP_ERR_KEY_RANGE, "a COO key lies outside [0,%u) x [0,%u)", n, m);
+      if (!hf[1]) return MSP_OK;
+      // a segment too large for one workgroup: the single-level sort below
+    }
     if (!rows_out) {
       MSPCHK(rows_own.alloc(sizeof(uint32_t) * nnz, s));
       rows_out = rows_own.as<uint32_t>();

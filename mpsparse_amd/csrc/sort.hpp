@@ -333,6 +333,237 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-level COO -> CSR. The rows are grouped into segments of 2^s consecutive
+// rows (s chosen so that a segment holds a few thousand keys). Phase 1 sorts
+// the keys stably by segment only (LSD passes over the segment bits, the
+// kernels above); k_seg_bounds finds each segment's range in that order; and
+// k_segment_sort sorts every segment completely in LDS, one workgroup per
+// segment, writing col_ind, the values, row_ptr of its rows (and the rows when
+// asked) directly. Inside a segment each key carries its position within the
+// segment in its low kSegIdxBits bits: stable LSD passes over the local bits
+// (row within the segment, column) then leave equal keys in input order, and
+// the value is fetched once, at the end, through that position. A segment
+// larger than kSegMax keys raises the overflow flag (the host then runs the
+// single-level sort instead).
+constexpr int kSegThreads = 512;             // workgroup of the segment sort (8 waves)
+constexpr int kSegWaves = kSegThreads / kWave;
+constexpr int kSegItems = 8;
+constexpr int kSegMax = kSegThreads * kSegItems;  // 4096 keys per segment at most
+constexpr int kSegIdxBits = 12;                   // 2^12 >= kSegMax
+
+// seg_start[g] = first position whose segment id (key >> seg_shift) >= g, for
+// g in [0, G]; with `whole` one segment spans everything.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_seg_bounds(const K* __restrict__ keys, uint64_t nnz, int seg_shift,
+                                                       uint32_t G, bool whole, uint32_t* __restrict__ seg_start) {
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += (uint64_t)gridDim.x * blockDim.x) {
+    if (whole) {
+      seg_start[g] = g == 0 ? 0u : (uint32_t)nnz;
+      continue;
+    }
+    uint64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+      const uint64_t mid = lo + ((hi - lo) >> 1);
+      if (((uint64_t)keys[mid] >> seg_shift) < g) lo = mid + 1;
+      else hi = mid;
+    }
+    seg_start[g] = (uint32_t)lo;
+  }
+}
+
+// One workgroup per segment (grid-stride over segments). FIRST: the input is
+// the caller's raw (row << 32 | col) keys (no phase-1 pass ran: one segment),
+// compacted and range-checked here. A segment of cnt keys is spread over
+// ieff = ceil(cnt / kSegThreads) items per thread; element order inside the
+// workgroup is (wave, item, lane) -- wave w holds positions
+// [w * 64 * ieff, (w + 1) * 64 * ieff) -- so the ballot ranks are stable. The
+// values stay in registers during the passes and are placed in LDS (over the
+// keys' buffer) only once the final order is known.
+template <typename K, bool FIRST>
+__global__ __launch_bounds__(kSegThreads) void k_segment_sort(const K* __restrict__ keys,
+                                                              const float* __restrict__ vals,
+                                                              const uint32_t* __restrict__ seg_start, uint32_t G,
+                                                              KeyMap km, int s, uint32_t n, uint64_t nnz,
+                                                              uint32_t* __restrict__ col_out,
+                                                              float* __restrict__ val_out,
+                                                              uint32_t* __restrict__ rows_out,
+                                                              uint32_t* __restrict__ row_ptr,
+                                                              int* __restrict__ overflow, int* __restrict__ err) {
+  __shared__ uint64_t skey[kSegMax];
+  __shared__ uint32_t wcnt[kSegWaves][kRadix];
+  __shared__ uint32_t dstart[kRadix];
+  float* sval = reinterpret_cast<float*>(skey);  // after the last pass
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+  const uint64_t lane_lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
+  const uint32_t lbits = km.cbits + (uint32_t)s;  // local key bits: row within the segment, column
+  const uint64_t lmask = lbits >= 64 ? ~0ull : ((1ull << lbits) - 1ull);
+  const int npass = (int)((lbits + kRadixBits - 1) / kRadixBits);
+  if (blockIdx.x == 0 && tid == 0) row_ptr[n] = (uint32_t)nnz;
+  for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
+    const uint32_t start = seg_start[g], end = seg_start[g + 1], cnt = end - start;
+    const uint32_t row0 = g << s;
+    const uint32_t nrow = (n - row0) < (1u << s) ? (n - row0) : (1u << s);
+    if (cnt > (uint32_t)kSegMax) {
+      if (tid == 0) atomicOr(overflow, 1);
+      continue;
+    }
+    const uint32_t ieff = (cnt + kSegThreads - 1) / kSegThreads;
+    const uint32_t wave_base = (uint32_t)wid * kWave * ieff;
+    uint64_t key[kSegItems];
+    float val[kSegItems];
+#pragma unroll
+    for (int i = 0; i < kSegItems; ++i) {
+      key[i] = 0;
+      val[i] = 0.0f;
+      if ((uint32_t)i < ieff) {
+        const uint32_t j = wave_base + (uint32_t)i * kWave + lane;
+        uint64_t k = 0;
+        if (j < cnt) {
+          k = (uint64_t)keys[start + j];
+          val[i] = vals[start + j];
+          if (FIRST) k = compact_key(k, km, err);
+        }
+        key[i] = ((k & lmask) << kSegIdxBits) | j;
+      }
+    }
+    for (int p = 0; p < npass; ++p) {
+      const int shift = kSegIdxBits + p * kRadixBits;
+      for (int i = tid; i < kSegWaves * kRadix; i += kSegThreads) (&wcnt[0][0])[i] = 0;
+      __syncthreads();
+      uint32_t rank[kSegItems];
+#pragma unroll
+      for (int i = 0; i < kSegItems; ++i) {
+        rank[i] = 0;
+        if ((uint32_t)i < ieff) {
+          const uint32_t j = wave_base + (uint32_t)i * kWave + lane;
+          const bool valid = j < cnt;
+          const uint32_t d = digit_of(key[i], shift);
+          uint64_t peers = __ballot(valid);
+#pragma unroll
+          for (int b = 0; b < kRadixBits; ++b) {
+            const uint64_t bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+          }
+          const uint32_t below = (uint32_t)__popcll(peers & lane_lt);
+          const uint32_t before = valid ? wcnt[wid][d] : 0u;
+          rank[i] = before + below;
+          if (valid && (peers >> lane) == 1ull) wcnt[wid][d] = before + (uint32_t)__popcll(peers);
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __syncthreads();
+      // per digit: exclusive scan over the waves (in place), digit totals
+      // scanned by wave 0 (4 digits per lane) into the tile-local starts
+      uint32_t tot = 0;
+      if (tid < kRadix) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kSegWaves; ++w) {
+          const uint32_t c = wcnt[w][tid];
+          wcnt[w][tid] = run;
+          run += c;
+        }
+        tot = run;
+        dstart[tid] = tot;
+      }
+      __syncthreads();
+      if (wid == 0) {
+        const uint32_t t0 = dstart[4 * lane], t1 = dstart[4 * lane + 1], t2 = dstart[4 * lane + 2],
+                       t3 = dstart[4 * lane + 3];
+        const uint32_t sum = t0 + t1 + t2 + t3;
+        uint32_t incl = sum;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const uint32_t t = __shfl_up(incl, off, kWave);
+          if (lane >= off) incl += t;
+        }
+        const uint32_t base = incl - sum;
+        dstart[4 * lane] = base;
+        dstart[4 * lane + 1] = base + t0;
+        dstart[4 * lane + 2] = base + t0 + t1;
+        dstart[4 * lane + 3] = base + t0 + t1 + t2;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kSegItems; ++i) {
+        if ((uint32_t)i < ieff) {
+          const uint32_t j = wave_base + (uint32_t)i * kWave + lane;
+          if (j < cnt) {
+            const uint32_t d = digit_of(key[i], shift);
+            skey[dstart[d] + wcnt[wid][d] + rank[i]] = key[i];
+          }
+        }
+      }
+      __syncthreads();
+      if (p + 1 < npass) {
+#pragma unroll
+        for (int i = 0; i < kSegItems; ++i) {
+          if ((uint32_t)i < ieff) {
+            const uint32_t j = wave_base + (uint32_t)i * kWave + lane;
+            if (j < cnt) key[i] = skey[j];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (npass == 0) {  // no local bits: the input order is the order
+#pragma unroll
+      for (int i = 0; i < kSegItems; ++i) {
+        if ((uint32_t)i < ieff) {
+          const uint32_t j = wave_base + (uint32_t)i * kWave + lane;
+          if (j < cnt) skey[j] = key[i];
+        }
+      }
+      __syncthreads();
+    }
+    // the sorted keys of this thread's output positions q = tid + k * kSegThreads
+    // (and the previous key's local row, + 1), then the values over the key
+    // buffer at their input positions
+    uint64_t out[kSegItems];
+    uint32_t prev[kSegItems];
+#pragma unroll
+    for (int k = 0; k < kSegItems; ++k) {
+      out[k] = 0;
+      prev[k] = 0;
+      const uint32_t q = (uint32_t)tid + (uint32_t)k * kSegThreads;
+      if ((uint32_t)k < ieff && q < cnt) {
+        out[k] = skey[q];
+        prev[k] = q == 0 ? 0u : (uint32_t)((skey[q - 1] >> kSegIdxBits) >> km.cbits) + 1u;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSegItems; ++i) {
+      if ((uint32_t)i < ieff) {
+        const uint32_t j = wave_base + (uint32_t)i * kWave + lane;
+        if (j < cnt) sval[j] = val[i];  // val[i] was loaded from input position j
+      }
+    }
+    __syncthreads();
+    // outputs (coalesced); row_ptr: rows [prev, r] start at the first element
+    // of row r, rows after the last element start at the segment's end
+#pragma unroll
+    for (int k = 0; k < kSegItems; ++k) {
+      const uint32_t q = (uint32_t)tid + (uint32_t)k * kSegThreads;
+      if ((uint32_t)k < ieff && q < cnt) {
+        const uint64_t e = out[k];
+        const uint64_t lk = e >> kSegIdxBits;
+        const uint32_t lr = (uint32_t)(lk >> km.cbits);
+        col_out[start + q] = (uint32_t)(lk & km.cmask);
+        val_out[start + q] = sval[(uint32_t)(e & ((1u << kSegIdxBits) - 1u))];
+        if (rows_out) rows_out[start + q] = row0 + lr;
+        for (uint32_t r = prev[k]; r <= lr; ++r) row_ptr[row0 + r] = start + q;
+        if (q == cnt - 1)
+          for (uint32_t r = lr + 1; r < nrow; ++r) row_ptr[row0 + r] = end;
+      }
+    }
+    if (cnt == 0)
+      for (uint32_t r = tid; r < nrow; r += kSegThreads) row_ptr[row0 + r] = start;
+    __syncthreads();  // skey is reused by the next segment
+  }
+}
+
 // row_ptr from sorted rows: row_ptr[q] = first position whose row >= q.
 // Thread for position i (1 <= i < nnz) fills the gap (rows[i-1], rows[i]];
 // the head (q <= rows[0]) and tail (q > rows[nnz-1]) are filled grid-stride.

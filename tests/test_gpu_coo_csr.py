@@ -121,3 +121,36 @@ def test_pooled_temporaries_reused_and_released():
         _same(_run_coo_csr(keys, vals, n, n, "cuda"), oc.coo_to_csr(keys, vals, n, n))
         if nnz == 400000:
             spmv.release_workspace()
+
+
+@pytest.mark.parametrize("case", ["empty_gaps", "heavy_row", "stencil_like", "tall_thin"])
+def test_segmented_shapes(case):
+    """Shapes aimed at the two-level sort: long runs of empty segments, one
+    row too long for a segment (the single-level sort takes over), a
+    7-entries-per-row pattern in shuffled order, and many rows with few
+    columns. Bit-exact against the oracle."""
+    rng = np.random.default_rng({"empty_gaps": 1, "heavy_row": 2, "stencil_like": 3, "tall_thin": 4}[case])
+    if case == "empty_gaps":
+        n, m = 1 << 20, 1 << 20
+        rows = np.sort(rng.choice(n, 300, replace=False))[rng.integers(0, 300, 60000)]
+        cols = rng.integers(0, m, rows.size)
+    elif case == "heavy_row":
+        n, m = 200000, 200000
+        rows = np.concatenate([rng.integers(0, n, 400000), np.full(20000, 777)])
+        cols = rng.integers(0, m, rows.size)
+    elif case == "stencil_like":
+        side = 64
+        n = m = side ** 3
+        r = np.arange(n)
+        offs = np.array([-side * side, -side, -1, 0, 1, side, side * side])
+        rows = np.repeat(r, 7)
+        cols = (rows + np.tile(offs, n)) % n
+    else:
+        n, m = 3_000_000, 4
+        rows = rng.integers(0, n, 2_000_000)
+        cols = rng.integers(0, m, rows.size)
+    perm = rng.permutation(rows.size)
+    keys = pack_keys(rows[perm].astype(np.uint64), cols[perm].astype(np.uint64))
+    vals = rng.standard_normal(keys.size).astype(np.float32)
+    say("segmented", case, n, m, keys.size)
+    _same(_run_coo_csr(keys, vals, n, m, "cuda"), oc.coo_to_csr(keys, vals, n, m))

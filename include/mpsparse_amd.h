@@ -184,6 +184,18 @@ msp_status msp_coo_to_csr(const uint64_t *keys, const float *vals, uint64_t nnz,
                           uint32_t num_cols, uint32_t *row_ptr, uint32_t *col_ind, float *out_vals,
                           void *stream);
 
+/* msp_coo_to_csr without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same results as
+ * msp_coo_to_csr. Every pointer must be device memory; nnz is limited to 2^20
+ * keys (larger inputs are refused with MSP_ERR_INVALID). The sort's temporaries
+ * live in a fixed 32 MiB device array per GPU, held until the process ends
+ * and shared by every call on that GPU: calls that overlap on one GPU race.
+ * The call waits for its stream before it returns (it reads a status flag). */
+msp_status msp_coo_to_csr_noalloc(const uint64_t *keys, const float *vals, uint64_t nnz, uint32_t num_rows,
+                                  uint32_t num_cols, uint32_t *row_ptr, uint32_t *col_ind, float *out_vals,
+                                  void *stream);
+
 /* CSR matrix handle built on the GPU from packed COO keys.
  * Replaces spmv.csr_tensor(keys, values, row_ptr, col_ind, out_vals, num_rows, num_cols)
  * (iterative/BiCGSTAB/spmv.cpp:230-268, 62-228). */

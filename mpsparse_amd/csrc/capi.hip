@@ -143,6 +143,12 @@ static void cache_flush(int dev) {
   g_blocks.bytes[dev] = 0;
 }
 
+// No-allocation calls: while set (for the calling thread), every temporary
+// comes from this fixed arena (a scratch array) and a buffer allocation fails
+// instead of calling hipMalloc.
+struct Arena;
+static thread_local Arena* t_fixed_arena = nullptr;
+
 struct DevMem {
   void* p = nullptr;
   size_t cap = 0;
@@ -153,6 +159,7 @@ struct DevMem {
   ~DevMem() { reset(); }
   msp_status alloc(size_t bytes) {
     reset();
+    if (t_fixed_arena) return fail(MSP_ERR_UNSUPPORTED, "a buffer allocation was requested inside a no-allocation call");
     if (bytes == 0) bytes = 256;
     int d = 0;
     HIPCHK(hipGetDevice(&d));
@@ -235,12 +242,18 @@ struct Arena {
   std::mutex mu;
   std::vector<ArenaChunk> chunks;
   std::vector<ArenaRec> stack;
+  bool fixed = false;  // one caller-provided chunk, never grown (no-allocation calls)
 };
+
 // never destroyed, like g_blocks
 static std::mutex& g_arena_mu = *new std::mutex;
 static std::map<std::pair<int, hipStream_t>, Arena*>& g_arenas = *new std::map<std::pair<int, hipStream_t>, Arena*>;
 
 static msp_status arena_for(hipStream_t s, Arena** out) {
+  if (t_fixed_arena) {
+    *out = t_fixed_arena;
+    return MSP_OK;
+  }
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   register_exit_guard();
@@ -264,6 +277,9 @@ static msp_status arena_alloc(Arena* a, size_t bytes, void** p, size_t* rec) {
     off = 0;
   }
   if (c == a->chunks.size()) {
+    if (a->fixed)
+      return fail(MSP_ERR_UNSUPPORTED, "the fixed scratch of a no-allocation call is exhausted (%zu more bytes)",
+                  bytes);
     size_t sz = std::max<size_t>(bytes, (size_t)64 << 20);
     if (!a->chunks.empty()) sz = std::max(sz, a->chunks.back().size);
     void* q = nullptr;
@@ -775,12 +791,18 @@ static int segment_bits(uint64_t nnz, uint32_t n, uint32_t rbits, uint32_t cbits
 // row's entries in ascending column order (a transposed CSR pattern, say), so
 // the stable passes over the row bits alone give the full order (half the
 // passes for 2M x 2M).
+// Flags start at zero through a kernel (not a memset), so the sort can run
+// under graph capture.
+__global__ void k_zero_flags(int* __restrict__ f, int count) {
+  if ((int)threadIdx.x < count) f[threadIdx.x] = 0;
+}
+
 static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n, uint32_t m,
                                     uint32_t* row_ptr, uint32_t* col_out, float* val_out, hipStream_t s,
                                     uint32_t* rows_out = nullptr, bool cols_in_order = false) {
   TmpMem err, rows_own;
-  MSPCHK(err.alloc(sizeof(int), s));
-  HIPCHK(hipMemsetAsync(err.p, 0, sizeof(int), s));
+  MSPCHK(err.alloc(2 * sizeof(int), s));  // [0] key out of range, [1] segment overflow
+  k_zero_flags<<<1, 64, 0, s>>>(err.as<int>(), 2);
   if (nnz > 0) {
     sortk::KeyMap km;
     km.cbits = m > 1 ? bitlen(m - 1) : 0;
@@ -795,18 +817,15 @@ static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uin
     const int sb = cols_in_order ? -1 : segment_bits(nnz, n, rbits, km.cbits);
     if (sb >= 0) {
       const int passes1 = (int)((rbits - (uint32_t)sb + sortk::kRadixBits - 1) / sortk::kRadixBits);
-      TmpMem ovf;
-      MSPCHK(ovf.alloc(sizeof(int), s));
-      HIPCHK(hipMemsetAsync(ovf.p, 0, sizeof(int), s));
+      int* ovf = err.as<int>() + 1;
       if (tb <= 32)
         MSPCHK(segment_sort<uint32_t>(keys, vals, nnz, n, km, sb, passes1, row_ptr, col_out, val_out, rows_out,
-                                      err.as<int>(), ovf.as<int>(), s));
+                                      err.as<int>(), ovf, s));
       else
         MSPCHK(segment_sort<uint64_t>(keys, vals, nnz, n, km, sb, passes1, row_ptr, col_out, val_out, rows_out,
-                                      err.as<int>(), ovf.as<int>(), s));
+                                      err.as<int>(), ovf, s));
       int hf[2] = {0, 0};
-      HIPCHK(hipMemcpyAsync(&hf[0], err.p, sizeof(int), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipMemcpyAsync(&hf[1], ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(hf, err.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (hf[0]) return fail(MSP_ERR_KEY_RANGE, "a COO key lies outside [0,%u) x [0,%u)", n, m);
       if (!hf[1]) return MSP_OK;
@@ -933,6 +952,64 @@ extern "C" msp_status msp_coo_to_csr(const uint64_t* keys, const float* vals, ui
   if (!ov_dev) MSPCHK(copy_out(out_vals, d_ov, sizeof(float) * nnz, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation COO -> CSR. Its temporaries (sort buffers, count tables,
+// segment bounds, flags) come from `scratch`, one fixed array per device,
+// reached through the device address hipGetSymbolAddress returns for that
+// device (the array's name in host code is only a stand-in). Every call on a
+// device uses that device's array: calls that overlap on one device race.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr uint64_t kNoAllocMaxKeys = (uint64_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocScratch {
+  std::mutex mu;
+  Arena* arena[kMaxDevices] = {};  // fixed arenas over each device's scratch (never destroyed)
+};
+static NoAllocScratch& g_noalloc = *new NoAllocScratch;
+
+// The fixed arena of the current device, set up on the first call there.
+static msp_status noalloc_arena(Arena** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(MSP_ERR_UNSUPPORTED, "device %d beyond the library's limit", dev);
+  std::lock_guard<std::mutex> lk(g_noalloc.mu);
+  if (!g_noalloc.arena[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(MSP_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    Arena* a = new Arena;
+    a->fixed = true;
+    a->chunks.push_back(ArenaChunk{ptr, kNoAllocScratchBytes});
+    g_noalloc.arena[dev] = a;
+  }
+  *out = g_noalloc.arena[dev];
+  return MSP_OK;
+}
+
+// Routes the calling thread's temporaries to a fixed arena for one call.
+struct FixedArenaScope {
+  explicit FixedArenaScope(Arena* a) { t_fixed_arena = a; }
+  ~FixedArenaScope() { t_fixed_arena = nullptr; }
+};
+
+extern "C" msp_status msp_coo_to_csr_noalloc(const uint64_t* keys, const float* vals, uint64_t nnz,
+                                             uint32_t num_rows, uint32_t num_cols, uint32_t* row_ptr,
+                                             uint32_t* col_ind, float* out_vals, void* stream) {
+  MSPCHK(need_device());
+  if (nnz > kNoAllocMaxKeys)
+    return fail(MSP_ERR_INVALID, "msp_coo_to_csr_noalloc: %llu keys exceed its limit of %llu",
+                (unsigned long long)nnz, (unsigned long long)kNoAllocMaxKeys);
+  if (!row_ptr || !is_device_ptr(row_ptr))
+    return fail(MSP_ERR_INVALID, "msp_coo_to_csr_noalloc: row_ptr must be device memory");
+  if (nnz && (!is_device_ptr(keys) || !is_device_ptr(vals) || !is_device_ptr(col_ind) || !is_device_ptr(out_vals)))
+    return fail(MSP_ERR_INVALID, "msp_coo_to_csr_noalloc: keys, vals, col_ind and out_vals must be device memory");
+  Arena* a = nullptr;
+  MSPCHK(noalloc_arena(&a));
+  FixedArenaScope scope(a);
+  return coo_to_csr_device(keys, vals, nnz, num_rows, num_cols, row_ptr, col_ind, out_vals, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------

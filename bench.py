@@ -96,8 +96,10 @@ def main():
     ap.add_argument("--reps", type=int, default=5, help="timed solves per solver (median) in the comparisons")
     ap.add_argument("--spmv-steps", type=int, default=50, help="timed launches of each SpMV / stencil product")
     ap.add_argument("--sections", default="fsai,spmv,solve,bcsr,cpu,precision,batched,direct",
-                    help="comma list of fsai, spmv, solve, bcsr, cpu, precision, batched, direct, copy "
-                         "(copy: 1 GiB device copy for counter calibration)")
+                    help="comma list of fsai, spmv, solve, bcsr, cpu, precision, batched, direct, and for "
+                         "profiling copy (1 GiB device copy, counter calibration), coo (the block-sparse build "
+                         "alone), headline (the learned-FSAI BF16 solve alone), stencil (the 256^3 products "
+                         "and CG steps alone)")
     ap.add_argument("--block-rows", type=int, default=65536, help="block-rows of the block-sparse system")
     ap.add_argument("--fsai-weights", default=None, help="edge-MLP weights (.npz); default: the packaged ones")
     args = ap.parse_args()
@@ -121,6 +123,11 @@ def main():
     if "fsai" in sections:
         fsai = fsai_section(dev, args.steps, args.warmup, args.reps, args.fsai_weights)
         out["learned_pcg"] = {k: v for k, v in fsai.items() if not k.startswith("_")}
+    if "headline" in sections:  # the headline solve alone (for profiling)
+        out["headline_only"] = fsai_section(dev, args.steps, args.warmup, args.reps, args.fsai_weights,
+                                            only_headline=True)
+    if "stencil" in sections:  # the 256^3 stencil products and CG steps alone (for profiling)
+        out["stencil_256"] = precision_section(dev, torch.empty(0), args.spmv_steps, only_products=True)
 
     if "copy" in sections:
         nbytes = 1 << 30
@@ -131,136 +138,139 @@ def main():
                                    "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
         log("copy", out["copy_calibration"])
 
-    t0 = time.time()
-    q = problems.block_sparse(args.block_rows, 16, 8, seed=0)
-    n, nnz = q.num_rows, q.nnz
-    log(f"generated n={n} nnz={nnz} in {time.time() - t0:.1f}s")
-    keys = torch.from_numpy(q.keys.view(np.int64)).to(dev)
-    vals = torch.from_numpy(q.vals).to(dev)
-    b = torch.from_numpy(q.b).to(dev)
-    del q
-    empty = torch.empty(0)
-    # COO -> CSR (the first build includes one-time allocations): median of 5 more
-    A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
-    torch.cuda.synchronize()
-    coo_runs = []
-    for _ in range(5):
-        del A
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
-        e1.record()
-        torch.cuda.synchronize()
-        coo_runs.append(e0.elapsed_time(e1))
-    coo_ms = float(np.median(coo_runs))
-    log(f"coo->csr {coo_ms:.2f} ms (runs {[round(t, 3) for t in coo_runs]})")
-    rp = torch.empty(n + 1, dtype=torch.int32, device=dev)
-    ci = torch.empty(nnz, dtype=torch.int32, device=dev)
-    cv = torch.empty(nnz, dtype=torch.float32, device=dev)
-    A.export(rp, ci, cv)
-    del keys, vals
-    rows = torch.repeat_interleave(torch.arange(n, device=dev), (rp[1:] - rp[:-1]).long())
-    cols = ci.long()
-    config = {"workload": "block_sparse_bicgstab_csr", "n": n, "nnz": nnz, "block": 16, "blocks_per_row": 9,
-              "seed": 0}
-    out["config"] = config
-    out["coo_to_csr"] = {"ms": coo_ms, "runs_ms": coo_runs, "keys_per_s": nnz / (coo_ms * 1e-3),
-                         "algorithmic_bytes": 20 * nnz + 4 * (n + 1),
-                         "algorithmic_GBps": (20 * nnz + 4 * (n + 1)) / (coo_ms * 1e-3) / 1e9,
-                         "note": "the build also runs the matrix set-up (row blocks, longest row, diagonal-storage "
-                                 "probe) and its host synchronisations"}
-
-    if "spmv" in sections:
-        x = torch.randn(n, device=dev)
-        y = torch.empty(n, device=dev)
-        ms = time_launches(lambda: A.mv(x, y), args.spmv_steps, 5)
-        alg = spmv_bytes(n, n, nnz)
-        gbs = alg / (ms * 1e-3) / 1e9
-        # spot check of this y against an FP64 product computed by torch
-        y64 = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * x.double()[cols])
-        err = float((y.double() - y64).abs().max() / y64.abs().max())
-        out["spmv"] = {"GBps": gbs, "ms": ms, "workload": "block_sparse n=1048576 nnz=150994944"}
-        out["value"] = gbs
-        out["ms_per_step"] = ms
-        traffic, tsrc = measured_traffic("csr_spmv_block_sparse")
-        out["roofline"] = {"bound": "hbm", "kernel": "k_spmv<ValF32,0,XPlain,EpiStore>", "achieved": gbs,
-                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-                           "traffic": traffic, "traffic_source": tsrc, "algorithmic_bytes": alg,
-                           "launches_timed": args.spmv_steps}
-        out["spmv_check"] = {"max_rel_err_vs_fp64": err}
-        log(f"spmv {ms:.4f} ms  {gbs:.1f} GB/s  err {err:.2e}")
-
-    if "solve" in sections:
-        x = torch.zeros(n, device=dev)
-        A.bicgstab(b, x)  # warm-up (workspace allocation)
-        times, info = [], None
-        for _ in range(3):
-            x.zero_()
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            info = A.bicgstab(b, x)
-            torch.cuda.synchronize()
-            times.append(time.perf_counter() - t)
-        rel = true_relres_gpu(rows, cols, cv, x, b)
-        out["time_to_solution"] = {"solver": "bicgstab", "tol": 1e-6, "wall_s": float(np.median(times)),
-                                   "device_s": info["seconds"], "iterations": info["iterations"],
-                                   "status": info["status"], "true_relres_fp64": rel}
-        log("bicgstab", out["time_to_solution"])
-        # CG on the 64x64 Laplacian (latency-bound case)
-        p = problems.laplacian_2d(64)
-        L = spmv.csr_tensor(torch.from_numpy(p.keys.view(np.int64)).to(dev), torch.from_numpy(p.vals).to(dev),
-                            empty, empty, empty, p.num_rows, p.num_cols)
-        bl = torch.from_numpy(p.b).to(dev)
-        xl = torch.zeros(p.num_rows, device=dev)
-        L.cgd(bl, xl)
-        ct = []
-        for _ in range(5):
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            cinfo = L.cgd(bl, xl)
-            torch.cuda.synchronize()
-            ct.append(time.perf_counter() - t)
-        out["cg_laplacian_64"] = {"wall_s": float(np.median(ct)), "iterations": cinfo["iterations"],
-                                  "status": cinfo["status"], "relres": cinfo["relres"]}
-        log("cg laplacian", out["cg_laplacian_64"])
-
-    if "bcsr" in sections:
-        from mpsparse_amd import bcsr
-
-        # the same matrix, handed over as packed keys rebuilt from the CSR
-        bkeys = (rows.to(torch.int64) << 32) | cols
-        Bm = bcsr.bcsr_tensor(bkeys, cv, torch.empty(0, dtype=torch.int32), torch.empty(0, dtype=torch.int32),
-                              n, n, 16)
-        del bkeys
-        nblk = Bm.num_blocks
-        balg = nblk * (16 * 16 * 4 + 4) + 4 * (n // 16 + 1) + 8 * n
-        xb = torch.randn(n, device=dev)
-        yb = torch.empty(n, device=dev)
-        bms = time_launches(lambda: Bm.mv(xb, yb), args.spmv_steps, 5)
-        yref = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * xb.double()[cols])
-        berr = float((yb.double() - yref).abs().max() / yref.abs().max())
-        xs = torch.zeros(n, device=dev)
-        Bm.bicgstab(b, xs)
-        bt = []
-        for _ in range(3):
-            xs.zero_()
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            binfo = Bm.bicgstab(b, xs)
-            torch.cuda.synchronize()
-            bt.append(time.perf_counter() - t)
-        out["bcsr16"] = {"num_blocks": nblk, "spmv_ms": bms, "algorithmic_bytes": balg,
-                         "GBps": balg / (bms * 1e-3) / 1e9, "frac_of_peak": balg / (bms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                         "max_rel_err_vs_fp64": berr,
-                         "bicgstab": {"wall_s": float(np.median(bt)), "iterations": binfo["iterations"],
-                                      "status": binfo["status"],
-                                      "true_relres_fp64": true_relres_gpu(rows, cols, cv, xs, b)}}
-        log("bcsr16", out["bcsr16"])
-        del Bm
-
+    config = None
     cpu = None
-    if "cpu" in sections:
-        cpu = cpu_block_sparse(rp, ci, cv, b, n, nnz, "solve" in sections)
+    if sections & {"spmv", "solve", "bcsr", "cpu", "coo"}:
+        t0 = time.time()
+        q = problems.block_sparse(args.block_rows, 16, 8, seed=0)
+        n, nnz = q.num_rows, q.nnz
+        log(f"generated n={n} nnz={nnz} in {time.time() - t0:.1f}s")
+        keys = torch.from_numpy(q.keys.view(np.int64)).to(dev)
+        vals = torch.from_numpy(q.vals).to(dev)
+        b = torch.from_numpy(q.b).to(dev)
+        del q
+        empty = torch.empty(0)
+        # COO -> CSR (the first build includes one-time allocations): median of 5 more
+        A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
+        torch.cuda.synchronize()
+        coo_runs = []
+        for _ in range(5):
+            del A
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            A = spmv.csr_tensor(keys, vals, empty, empty, empty, n, n)
+            e1.record()
+            torch.cuda.synchronize()
+            coo_runs.append(e0.elapsed_time(e1))
+        coo_ms = float(np.median(coo_runs))
+        log(f"coo->csr {coo_ms:.2f} ms (runs {[round(t, 3) for t in coo_runs]})")
+        rp = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        ci = torch.empty(nnz, dtype=torch.int32, device=dev)
+        cv = torch.empty(nnz, dtype=torch.float32, device=dev)
+        A.export(rp, ci, cv)
+        del keys, vals
+        rows = torch.repeat_interleave(torch.arange(n, device=dev), (rp[1:] - rp[:-1]).long())
+        cols = ci.long()
+        config = {"workload": "block_sparse_bicgstab_csr", "n": n, "nnz": nnz, "block": 16, "blocks_per_row": 9,
+                  "seed": 0}
+        out["config"] = config
+        out["coo_to_csr"] = {"ms": coo_ms, "runs_ms": coo_runs, "keys_per_s": nnz / (coo_ms * 1e-3),
+                             "algorithmic_bytes": 20 * nnz + 4 * (n + 1),
+                             "algorithmic_GBps": (20 * nnz + 4 * (n + 1)) / (coo_ms * 1e-3) / 1e9,
+                             "note": "the build also runs the matrix set-up (row blocks, longest row, diagonal-storage "
+                                     "probe) and its host synchronisations"}
+
+        if "spmv" in sections:
+            x = torch.randn(n, device=dev)
+            y = torch.empty(n, device=dev)
+            ms = time_launches(lambda: A.mv(x, y), args.spmv_steps, 5)
+            alg = spmv_bytes(n, n, nnz)
+            gbs = alg / (ms * 1e-3) / 1e9
+            # spot check of this y against an FP64 product computed by torch
+            y64 = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * x.double()[cols])
+            err = float((y.double() - y64).abs().max() / y64.abs().max())
+            out["spmv"] = {"GBps": gbs, "ms": ms, "workload": "block_sparse n=1048576 nnz=150994944"}
+            out["value"] = gbs
+            out["ms_per_step"] = ms
+            traffic, tsrc = measured_traffic("csr_spmv_block_sparse")
+            out["roofline"] = {"bound": "hbm", "kernel": "k_spmv<ValF32,0,XPlain,EpiStore>", "achieved": gbs,
+                               "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                               "traffic": traffic, "traffic_source": tsrc, "algorithmic_bytes": alg,
+                               "launches_timed": args.spmv_steps}
+            out["spmv_check"] = {"max_rel_err_vs_fp64": err}
+            log(f"spmv {ms:.4f} ms  {gbs:.1f} GB/s  err {err:.2e}")
+
+        if "solve" in sections:
+            x = torch.zeros(n, device=dev)
+            A.bicgstab(b, x)  # warm-up (workspace allocation)
+            times, info = [], None
+            for _ in range(3):
+                x.zero_()
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                info = A.bicgstab(b, x)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t)
+            rel = true_relres_gpu(rows, cols, cv, x, b)
+            out["time_to_solution"] = {"solver": "bicgstab", "tol": 1e-6, "wall_s": float(np.median(times)),
+                                       "device_s": info["seconds"], "iterations": info["iterations"],
+                                       "status": info["status"], "true_relres_fp64": rel}
+            log("bicgstab", out["time_to_solution"])
+            # CG on the 64x64 Laplacian (latency-bound case)
+            p = problems.laplacian_2d(64)
+            L = spmv.csr_tensor(torch.from_numpy(p.keys.view(np.int64)).to(dev), torch.from_numpy(p.vals).to(dev),
+                                empty, empty, empty, p.num_rows, p.num_cols)
+            bl = torch.from_numpy(p.b).to(dev)
+            xl = torch.zeros(p.num_rows, device=dev)
+            L.cgd(bl, xl)
+            ct = []
+            for _ in range(5):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                cinfo = L.cgd(bl, xl)
+                torch.cuda.synchronize()
+                ct.append(time.perf_counter() - t)
+            out["cg_laplacian_64"] = {"wall_s": float(np.median(ct)), "iterations": cinfo["iterations"],
+                                      "status": cinfo["status"], "relres": cinfo["relres"]}
+            log("cg laplacian", out["cg_laplacian_64"])
+
+        if "bcsr" in sections:
+            from mpsparse_amd import bcsr
+
+            # the same matrix, handed over as packed keys rebuilt from the CSR
+            bkeys = (rows.to(torch.int64) << 32) | cols
+            Bm = bcsr.bcsr_tensor(bkeys, cv, torch.empty(0, dtype=torch.int32), torch.empty(0, dtype=torch.int32),
+                                  n, n, 16)
+            del bkeys
+            nblk = Bm.num_blocks
+            balg = nblk * (16 * 16 * 4 + 4) + 4 * (n // 16 + 1) + 8 * n
+            xb = torch.randn(n, device=dev)
+            yb = torch.empty(n, device=dev)
+            bms = time_launches(lambda: Bm.mv(xb, yb), args.spmv_steps, 5)
+            yref = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(0, rows, cv.double() * xb.double()[cols])
+            berr = float((yb.double() - yref).abs().max() / yref.abs().max())
+            xs = torch.zeros(n, device=dev)
+            Bm.bicgstab(b, xs)
+            bt = []
+            for _ in range(3):
+                xs.zero_()
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                binfo = Bm.bicgstab(b, xs)
+                torch.cuda.synchronize()
+                bt.append(time.perf_counter() - t)
+            out["bcsr16"] = {"num_blocks": nblk, "spmv_ms": bms, "algorithmic_bytes": balg,
+                             "GBps": balg / (bms * 1e-3) / 1e9, "frac_of_peak": balg / (bms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                             "max_rel_err_vs_fp64": berr,
+                             "bicgstab": {"wall_s": float(np.median(bt)), "iterations": binfo["iterations"],
+                                          "status": binfo["status"],
+                                          "true_relres_fp64": true_relres_gpu(rows, cols, cv, xs, b)}}
+            log("bcsr16", out["bcsr16"])
+            del Bm
+
+        cpu = None
+        if "cpu" in sections:
+            cpu = cpu_block_sparse(rp, ci, cv, b, n, nnz, "solve" in sections)
 
     if "precision" in sections:
         out["precision_path"] = precision_section(dev, empty, args.spmv_steps)
@@ -570,7 +580,7 @@ def batched_section(dev, num_systems=512, lo=32, hi=512, sigma=1.5, seed=0):
     return res
 
 
-def fsai_section(dev, steps, warmup, reps, weights=None):
+def fsai_section(dev, steps, warmup, reps, weights=None, only_headline=False):
     """Held-out 128^3 7-point log-normal diffusion (sigma 2): time to true relres
     1e-6 of plain FP32 CG and of FP64-refined CG / Jacobi-PCG / classical FSAI-PCG
     / learned FSAI-PCG (the stated 5-input network with FP32 and BF16 matrix
@@ -608,6 +618,29 @@ def fsai_section(dev, steps, warmup, reps, weights=None):
     res = {"workload": f"diffusion 3D 7-point {side}^3 sigma={sigma} (held out: seed {seed}, side outside the "
                        f"training range)", "n": n, "nnz": nnz, "tol": 1e-6,
            "timing": f"per solver: 1 warm-up, then the median of {reps} solves (wall clock, set-up included)"}
+    if only_headline:
+        net = LF.load_network(weights or LF.DEFAULT_WEIGHTS, device=dev)
+        x64 = torch.zeros(n, dtype=torch.float64, device=dev)
+
+        def solve():
+            F, _ = LF.build(A, net)
+            x64.zero_()
+            return LF.solve_refined(A, F, b64, x64, tol=1e-6, values="bf16")
+
+        for _ in range(warmup):
+            solve()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            info = solve()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t1)
+        res["headline_steps"] = {"steps": steps, "median_s": float(np.median(ts)), "runs_s": ts,
+                                 "inner": info["inner_iterations"], "outer": info["outer_iterations"],
+                                 "true_relres_fp64": relres64(x64)}
+        log("headline only", res["headline_steps"])
+        return res
     t = time.perf_counter()
     op64 = stencil_operator64(rp, ci, cv, n, side)
     d64 = torch.zeros(n, dtype=torch.float64, device=dev).index_add_(
@@ -801,7 +834,7 @@ def direct_section(dev, sizes=(1000, 4000, 10000, 20000, 100000), nrhs=128):
     return out
 
 
-def precision_section(dev, empty, spmv_steps=50):
+def precision_section(dev, empty, spmv_steps=50, only_products=False):
     """256^3 7-point log-normal diffusion: plain FP32 CG at 1e-6 next to FP64
     iterative refinement (FP32 / BF16 / FP16 values) to 1e-10."""
     import numpy as np
@@ -845,6 +878,19 @@ def precision_section(dev, empty, spmv_steps=50):
                          "launches_timed": spmv_steps}
         log("stencil mv 256^3", vt, res["mv"][vt])
     del xm, ym
+    # one CG step (FP32 values): a fixed 400-step run without convergence test
+    xs = torch.zeros(n, device=dev)
+    A.cgd(b32, xs, tol=0.0, maxiter=50, verify=False)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sinfo = A.cgd(b32, xs, tol=0.0, maxiter=400, verify=False)
+    torch.cuda.synchronize()
+    res["cg_step_us"] = {"f32": (time.perf_counter() - t) / max(1, sinfo["iterations"]) * 1e6,
+                         "steps": sinfo["iterations"]}
+    log("cg step 256^3", res["cg_step_us"])
+    del xs
+    if only_products:
+        return res
     x32 = torch.zeros(n, device=dev)
     A.cgd(b32, x32, maxiter=20000)
     torch.cuda.synchronize()

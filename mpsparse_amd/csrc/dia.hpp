@@ -17,10 +17,27 @@ namespace msp {
 
 constexpr int kMaxDiag = 16;
 
+// Diagonal d (column - row = off[d]) of row i is read from
+// val[plane[d] * ld + i + shift[d]]: plane[d] = d, shift[d] = 0 in plain
+// storage; for a matrix found exactly symmetric, a lower diagonal (off = -k)
+// reads the upper diagonal +k one row block earlier (A(i, i-k) = A(i-k, i):
+// plane of +k, shift -k), so the 7-point operator streams 4 value planes
+// (16 B per row) instead of 7 (28 B) -- the same values, summed in the same
+// order, so the products are bit-identical.
 struct DiaOffsets {
   int32_t off[kMaxDiag];
+  int32_t plane[kMaxDiag];
+  int32_t shift[kMaxDiag];
   int32_t ndiag;
 };
+
+// Plain storage: every diagonal reads its own plane.
+inline void dia_plain(DiaOffsets& o) {
+  for (int d = 0; d < kMaxDiag; ++d) {
+    o.plane[d] = d;
+    o.shift[d] = 0;
+  }
+}
 
 template <typename V>
 struct DiaView {
@@ -28,6 +45,7 @@ struct DiaView {
   const float* __restrict__ dlo;                // 16-bit formats: FP32 remainder of the diagonal
   uint64_t ld;
   uint32_t n, m;
+  int32_t mirrored;  // o.plane / o.shift in use (else plane d, shift 0)
   DiaOffsets o;
 };
 
@@ -65,6 +83,18 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
   // 16-bit values held one per register (packing two per register would
   // wait for the loads)
   using Held = typename std::conditional<sizeof(typename V::storage) == 2, uint32_t, float>::type;
+  // start of each diagonal's values, relative to row 0: plane * ld + shift.
+  // A mirrored lower diagonal (shift -k) reads the plane of +k (index >= 1,
+  // after the lower planes), so its rows i < k still address memory inside
+  // the array; those values are left out of the sum (column < 0)
+  int64_t base[DMAX];
+  if (A.mirrored) {
+#pragma unroll
+    for (int d = 0; d < DMAX; ++d) base[d] = d < nd ? (int64_t)A.o.plane[d] * (int64_t)A.ld + A.o.shift[d] : 0;
+  } else {
+#pragma unroll
+    for (int d = 0; d < DMAX; ++d) base[d] = (int64_t)d * (int64_t)A.ld;
+  }
   for (uint64_t i0 = (uint64_t)blockIdx.x * kBlock * U + threadIdx.x; i0 < A.n; i0 += gstride) {
     Held a[U][DMAX];
     typename XL::Raw xr[U][DMAX];
@@ -81,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
         xr[u][d] = {};
         if (d < nd) {  // loads only: no arithmetic on loaded data before the last load
           const int64_t c = (int64_t)i + A.o.off[d];
-          a[u][d] = A.val[d * A.ld + i];
+          a[u][d] = A.val[base[d] + (int64_t)i];
           xr[u][d] = xl.raw((uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c)));
         }
       }
@@ -142,6 +172,27 @@ __global__ __launch_bounds__(kBlock) void k_dia_check(const uint32_t* __restrict
       bool hit = false;
       for (int d = 0; d < o.ndiag; ++d) hit |= (off == o.off[d]);
       miss += hit ? 0u : 1u;
+    }
+  }
+#pragma unroll
+  for (int sh = kWave / 2; sh > 0; sh >>= 1) miss += __shfl_xor(miss, sh, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && miss) atomicAdd(misses, miss);
+}
+
+// Symmetry of filled FP32 diagonal storage: for every lower diagonal d
+// (off[d] = -k, partner p with off[p] = +k) count rows i >= k whose value
+// differs from the partner's at row i - k (bitwise).
+__global__ __launch_bounds__(kBlock) void k_dia_symcheck(const float* __restrict__ val, uint32_t n, DiaOffsets o,
+                                                         const int32_t* __restrict__ partner, uint64_t ld,
+                                                         uint32_t* __restrict__ misses) {
+  uint32_t miss = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int d = 0; d < o.ndiag; ++d) {
+      const int64_t k = -(int64_t)o.off[d];
+      if (k <= 0 || (int64_t)i < k) continue;
+      const uint32_t a = __float_as_uint(val[(uint64_t)d * ld + i]);
+      const uint32_t b = __float_as_uint(val[(uint64_t)partner[d] * ld + (uint64_t)((int64_t)i - k)]);
+      miss += a != b ? 1u : 0u;
     }
   }
 #pragma unroll

@@ -249,7 +249,7 @@ class CsrTensor {
   std::string format() const {
     int32_t f = 0;
     check(msp_csr_get_format(h_, &f), "format");
-    return f == 2 ? "dia" : "csr";
+    return f == 3 ? "dia_symmetric" : (f == 2 ? "dia" : "csr");
   }
   int64_t num_rows() const {
     uint32_t n = 0;

@@ -35,7 +35,7 @@ def test_stencil_uses_dia_and_matches_csr():
     x = np.random.default_rng(0).standard_normal(M.num_cols).astype(np.float32)
     say("dia spmv diffusion 20^3")
     y = _mv(A, x)
-    assert A.format == "dia"
+    assert A.format == "dia_symmetric"
     y64 = S.csr_spmv(M.row_ptr, M.col_ind, M.vals, x)
     assert np.all(np.abs(y - y64) <= _bound(M.row_ptr, M.col_ind, M.vals, x))
     A.set_format("csr")
@@ -43,7 +43,7 @@ def test_stencil_uses_dia_and_matches_csr():
     assert A.format == "csr"
     assert np.all(np.abs(yc - y64) <= _bound(M.row_ptr, M.col_ind, M.vals, x))
     A.set_format("auto")
-    assert np.array_equal(_mv(A, x), y) and A.format == "dia"
+    assert np.array_equal(_mv(A, x), y) and A.format == "dia_symmetric"
 
 
 @pytest.mark.parametrize("values", ["bf16", "f16"])
@@ -53,7 +53,7 @@ def test_dia_16bit_values(values):
     x = np.random.default_rng(1).standard_normal(M.num_cols).astype(np.float32)
     say("dia spmv", values)
     y = _mv(A, x, values)
-    assert A.format == "dia"
+    assert A.format in ("dia", "dia_symmetric")
     v16, dlo = S.round16_rowsum(M.row_ptr, M.col_ind, M.vals, values)
     y64 = S.csr_spmv(M.row_ptr, M.col_ind, v16, x) + dlo * x
     bound = _bound(M.row_ptr, M.col_ind, v16, x) + 4 * 2.0 ** -24 * np.abs(dlo * x)
@@ -118,5 +118,30 @@ def test_solvers_same_on_dia_and_csr(method):
         assert info["status"] == "converged", info
         assert K.true_relres(Ssp, x.cpu().numpy(), p.b) <= 1.05e-6
         its[fmt] = info["iterations"]
-        assert A.format == ("dia" if fmt == "auto" else "csr")
+        assert A.format == ("dia_symmetric" if fmt == "auto" else "csr")
     assert abs(its["auto"] - its["csr"]) <= 0.05 * its["csr"] + 2, its
+
+
+@pytest.mark.parametrize("values", ["f32", "bf16", "f16"])
+def test_symmetric_storage_is_bit_identical(values):
+    """A symmetric stencil whose value planes exceed 128 MB (216^3: 282 MB in
+    FP32, 141 MB in 16 bits) reads its lower diagonals from the upper planes
+    (4 value planes instead of 7 for the 7-point operator). Against the same
+    matrix with one entry of its last row changed (no longer symmetric: every
+    diagonal read from its own plane) the products of all other rows agree
+    bit for bit, and both stay within the FP64 bound."""
+    M = diffusion_3d(216, sigma=1.5, seed=7)
+    n = M.num_rows
+    A = csr_from_arrays(M.row_ptr, M.col_ind, M.vals, n, M.num_cols)
+    v2 = M.vals.copy()
+    v2[M.row_ptr[n - 1]] *= 1.5  # first entry of the last row: a lower-triangle coupling
+    B = csr_from_arrays(M.row_ptr, M.col_ind, v2, n, M.num_cols)
+    x = np.random.default_rng(3).standard_normal(M.num_cols).astype(np.float32)
+    say("dia symmetric storage", values)
+    ya, yb = _mv(A, x, values), _mv(B, x, values)
+    assert A.format == "dia_symmetric" and B.format == "dia"
+    assert np.array_equal(ya[:-1].view(np.uint64), yb[:-1].view(np.uint64))
+    if values == "f32":
+        for vv, y in ((M.vals, ya), (v2, yb)):
+            y64 = S.csr_spmv(M.row_ptr, M.col_ind, vv, x)
+            assert np.all(np.abs(y - y64) <= _bound(M.row_ptr, M.col_ind, vv, x))

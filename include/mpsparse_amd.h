@@ -147,7 +147,9 @@ typedef struct msp_batch_info {
   int32_t verifications;   /* batch-wide true-residual passes */
   int32_t tile_rows;       /* the schedule used */
   int32_t rows_per_thread;
-  int32_t reserved;
+  int32_t diagonal_planes;  /* product over batched diagonal storage with this many
+                               value planes (every system on at most 8 diagonals);
+                               0: the CSR product */
   double seconds;          /* device time of the solve (events) */
   double setup_seconds;    /* host time of staging and tuning */
 } msp_batch_info;

@@ -2827,6 +2827,9 @@ struct msp_batch_s {
   DevMem tile_r0, tile_sys, sys_tile, active, n_active;
   DevMem partials, state, sys_active;
   DevMem vec;  // X, XL, R, P0, P1, Q, B: n * 4 floats each
+  // batched diagonal storage (every system on at most batch::kBD diagonals)
+  DevMem bd_planes, bd_offs, bd_nds;
+  int bd_nd = 0;  // 0: CSR product; else the planes stored (5 when every system has exactly 5)
   int32_t* h_active = nullptr;  // pinned, 2 * nsys
   hipStream_t cs = nullptr;     // capture stream
   hipEvent_t ev[2] = {nullptr, nullptr}, join = nullptr, t0 = nullptr, t1 = nullptr;
@@ -2857,6 +2860,10 @@ struct msp_batch_s {
     V.active = active.as<uint32_t>();
     V.n_active = n_active.as<uint32_t>();
     V.sys_active = sys_active.as<int32_t>();
+    V.planes = bd_nd ? bd_planes.as<float>() : nullptr;
+    V.ld = n;
+    V.offs = bd_offs.as<int32_t>();
+    V.sys_off = sys_off.as<uint32_t>();
     return V;
   }
 };
@@ -2907,6 +2914,18 @@ static void launch_spmm(msp_batch B, int rpt, int parity, hipStream_t s) {
   const uint32_t g = batch_grid(B);
   const float* pold = B->v(parity ? kBP1 : kBP0);
   float* pnew = B->v(parity ? kBP0 : kBP1);
+  PairState* ps = B->state.as<PairState>();
+  CSum* pt = B->partials.as<CSum>();
+  if (B->bd_nd == 5) {  // 2D 5-point systems
+    if (rpt == 2) batch::k_bt_spmm_dia<2, 5><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    else batch::k_bt_spmm_dia<1, 5><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    return;
+  }
+  if (B->bd_nd > 0) {
+    if (rpt == 2) batch::k_bt_spmm_dia<2, 0><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    else batch::k_bt_spmm_dia<1, 0><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    return;
+  }
   if (rpt == 2)
     batch::k_bt_spmm<2><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), B->state.as<PairState>(),
                                              B->partials.as<CSum>());
@@ -2971,6 +2990,43 @@ extern "C" void msp_default_batch_opts(msp_batch_opts* o) {
   o->max_verify = 3;
 }
 
+// Batched diagonal storage: every system's distinct offsets (at most
+// batch::kBD), then the value planes. Left off (CSR product) when a system
+// has more offsets or the batch opts out (B->bd_nd stays 0).
+static msp_status batch_diagonal_storage(msp_batch B, hipStream_t s) {
+  msp_csr A = B->A;
+  const uint32_t nsys = B->nsys;
+  if (nsys == 0 || B->n == 0) return MSP_OK;
+  MSPCHK(B->bd_offs.alloc(sizeof(int32_t) * batch::kBD * (size_t)nsys));
+  MSPCHK(B->bd_nds.alloc(sizeof(int32_t) * (size_t)nsys));
+  TmpMem bad;
+  MSPCHK(bad.alloc(sizeof(uint32_t), s));
+  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s));
+  const uint32_t g = std::min<uint32_t>(nsys, (uint32_t)cu_count() * 8);
+  batch::k_bd_offsets<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
+                                           B->sys_off.as<uint32_t>(), nsys, B->bd_offs.as<int32_t>(),
+                                           B->bd_nds.as<int32_t>(), bad.as<uint32_t>());
+  MSPCHK(launch_check("k_bd_offsets"));
+  std::vector<int32_t> nds(nsys);
+  uint32_t hb = 0;
+  HIPCHK(hipMemcpyAsync(&hb, bad.p, sizeof hb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(nds.data(), B->bd_nds.p, sizeof(int32_t) * nsys, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hb) return MSP_OK;  // some system has more than kBD diagonals: CSR product
+  bool all5 = true;
+  for (int32_t c : nds) all5 = all5 && c == 5;
+  const int planes = all5 ? 5 : batch::kBD;
+  const size_t bytes = sizeof(float) * (size_t)planes * B->n;
+  MSPCHK(B->bd_planes.alloc(bytes));
+  HIPCHK(hipMemsetAsync(B->bd_planes.p, 0, bytes, s));
+  batch::k_bd_fill<<<g, kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
+                                        B->sys_off.as<uint32_t>(), nsys, B->bd_offs.as<int32_t>(), B->n,
+                                        B->bd_planes.as<float>());
+  MSPCHK(launch_check("k_bd_fill"));
+  B->bd_nd = planes;
+  return MSP_OK;
+}
+
 extern "C" msp_status msp_batch_create(msp_batch* out, msp_csr A, const uint32_t* sys_off, uint32_t num_systems,
                                        void* stream) {
   if (!out) return fail(MSP_ERR_INVALID, "null output handle");
@@ -3015,6 +3071,7 @@ extern "C" msp_status msp_batch_create(msp_batch* out, msp_csr A, const uint32_t
     HIPCHK(hipStreamSynchronize(s));
     if (hb[0]) return fail(MSP_ERR_INVALID, "%u entries couple different systems (not block diagonal)", hb[0]);
     if (hb[1]) return fail(MSP_ERR_SINGULAR, "%u rows without a positive diagonal (Jacobi needs one)", hb[1]);
+    MSPCHK(batch_diagonal_storage(B, s));
     MSPCHK(B->state.alloc(sizeof(PairState) * batch::kRhs * ((size_t)num_systems + 1)));
     MSPCHK(B->sys_active.alloc(sizeof(int32_t) * ((size_t)num_systems + 1)));
     MSPCHK(B->n_active.alloc(sizeof(uint32_t)));
@@ -3221,6 +3278,7 @@ static msp_status batch_pcg_impl(msp_batch B, const float* b, void* x, bool x64,
   inf.graph_launches = launches;
   inf.verifications = verifications;
   inf.tile_rows = (int32_t)B->T;
+  inf.diagonal_planes = B->bd_nd;
   inf.rows_per_thread = B->rpt;
   inf.seconds = ms * 1e-3;
   inf.setup_seconds = std::chrono::duration<double>(wall1 - wall0).count();

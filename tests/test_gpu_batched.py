@@ -141,3 +141,76 @@ def test_batched_rejects_coupled_systems():
         batched.batch_tensor(A, torch.tensor([0, 30, 60]))
     B = batched.batch_tensor(A, torch.tensor([0, 64]))
     assert B.num_systems == 1
+
+
+def _block_diag(blocks):
+    """A csr_tensor of the block-diagonal matrix of scipy SPD blocks, its scipy
+    form and the system offsets."""
+    import torch
+
+    from gpu_util import csr_from_arrays
+
+    S = sp.block_diag(blocks, format="csr").astype(np.float32)
+    S.sort_indices()
+    off = np.concatenate([[0], np.cumsum([b.shape[0] for b in blocks])]).astype(np.int64)
+    A = csr_from_arrays(S.indptr.astype(np.uint32), S.indices.astype(np.uint32), S.data, S.shape[0], S.shape[1])
+    return A, S.astype(np.float64), off, torch
+
+
+def _spd_1d(n, seed, shift=0.2):
+    """1D log-normal diffusion plus a diagonal shift (a moderate condition number)."""
+    rng = np.random.default_rng(seed)
+    k = np.exp(rng.standard_normal(n + 1))
+    return sp.diags([-k[1:n], k[:n] + k[1:] + shift, -k[1:n]], [-1, 0, 1]).tocsr()
+
+
+def _spd_9pt(side, seed):
+    """2D 9-point operator (nine diagonals): the Kronecker sum of 1D operators plus diagonal couplings."""
+    T = _spd_1d(side, seed)
+    I = sp.eye(side)
+    C = sp.diags([np.full(side - 1, -0.1), np.full(side - 1, -0.1)], [-1, 1])
+    return (sp.kron(T, I) + sp.kron(I, T) + sp.kron(C, C) + 2.0 * sp.eye(side * side)).tocsr()
+
+
+@pytest.mark.parametrize("mix", ["3_5_7_diagonals", "nine_diagonals"])
+def test_batched_diagonal_storage_and_csr_fallback(mix):
+    """Systems on at most 8 diagonals each are multiplied from batched
+    diagonal storage (1D: 3, 2D: 5 and 3D: 7 diagonals in one batch); a
+    system on 9 diagonals sends the batch to the CSR product. Every pair
+    against FP64 Jacobi-PCG."""
+    from mpsparse_amd import batched
+    from mpsparse_amd.problems import diffusion_2d, diffusion_3d
+
+    if mix == "3_5_7_diagonals":
+        blocks = [_spd_1d(900, 1), diffusion_2d(30, sigma=1.0, seed=2).to_scipy(),
+                  diffusion_3d(9, sigma=1.0, seed=3).to_scipy(), _spd_1d(70, 4)]
+        planes = 8
+    else:
+        blocks = [diffusion_2d(24, sigma=1.0, seed=5).to_scipy(), _spd_9pt(20, 6)]
+        planes = 0
+    A, S, off, torch = _block_diag(blocks)
+    n = int(off[-1])
+    B = batched.batch_tensor(A, torch.from_numpy(off))
+    b = np.random.default_rng(7).standard_normal((n, 4)).astype(np.float32)
+    x = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    say("batched storage", mix)
+    info = B.pcg(torch.from_numpy(b).cuda(), x)
+    assert info["diagonal_planes"] == planes, info["diagonal_planes"]
+    assert info["converged_pairs"] == info["total_pairs"] == 4 * len(blocks)
+    _check_pairs(S.tocsr(), off, b, x.cpu().numpy(), info, 1e-6, 4)
+
+
+def test_batched_stencil_batch_uses_five_planes():
+    import torch
+
+    from mpsparse_amd import batched
+
+    A, S, off, sides = _build(4, 10, 30, seed=9)
+    B = batched.batch_tensor(A, torch.from_numpy(off))
+    n = int(off[-1])
+    b = np.random.default_rng(8).standard_normal((n, 4)).astype(np.float32)
+    x = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    say("batched five planes", sides.tolist())
+    info = B.pcg(torch.from_numpy(b).cuda(), x)
+    assert info["diagonal_planes"] == 5
+    _check_pairs(S, off, b, x.cpu().numpy(), info, 1e-6, 4)

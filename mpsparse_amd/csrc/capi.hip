@@ -1294,7 +1294,11 @@ constexpr int dia_rows_per_thread() {
 template <typename V, int ND, typename XL, typename Epi, int NDIAG>
 static uint32_t dia_launch(msp_csr A, XL xl, Epi epi, GridRed rb, const int* done, hipStream_t s) {
   constexpr int U = dia_rows_per_thread<XL, NDIAG>();
-  static const uint32_t per_cu = resident_per_cu(k_dia<V, ND, XL, Epi, NDIAG, U>);
+  const auto kern = [] {
+    if constexpr (ND > 0) return &k_dia_sum<V, ND, XL, Epi, NDIAG, U>;
+    else return &k_dia<V, ND, XL, Epi, NDIAG, U>;
+  }();
+  static const uint32_t per_cu = resident_per_cu(kern);
   // as many grid-stride passes as residency requires, all equally full: a
   // resident-sized grid left the last pass mostly empty (2.1M rows at U = 4
   // over 1792 workgroups: 1.83M rows, then 0.26M) at the cost of a full round
@@ -1304,7 +1308,7 @@ static uint32_t dia_launch(msp_csr A, XL xl, Epi epi, GridRed rb, const int* don
   const uint64_t cap = (uint64_t)per_cu * (uint32_t)cu_count();
   const uint64_t passes = (need + cap - 1) / cap;
   const uint32_t g = (uint32_t)std::max<uint64_t>(1, (need + passes - 1) / passes);
-  k_dia<V, ND, XL, Epi, NDIAG, U><<<g, kBlock, 0, s>>>(dia_view_of<V>(A), xl, epi, rb, done);
+  kern<<<g, kBlock, 0, s>>>(dia_view_of<V>(A), xl, epi, rb, done);
   return g;
 }
 

@@ -62,9 +62,10 @@ struct DiaView {
 // matrix read a clamped in-range x and are then left out of the sum, so the
 // result equals the guarded form bit for bit; rows past the end of the
 // U-group load row i0's data and store nothing. Requires m > 0.
-template <typename V, int ND, typename XL, typename Epi, int NDIAG, int U = 1>
-__global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, GridRed rb,
-                                                const int* __restrict__ done) {
+template <typename V, int ND, typename XL, typename Epi, int NDIAG, int U>
+__device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, GridRed rb,
+                                         const int* __restrict__ done) {
+  // xl and epi are copies: the prologue sets their step scalars
   // NDIAG > 0: exactly NDIAG diagonals (no per-diagonal branch; stencils and
   // their triangular factors); 0: A.o.ndiag <= kMaxDiag
   constexpr int DMAX = NDIAG > 0 ? NDIAG : kMaxDiag;
@@ -142,6 +143,24 @@ __global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, Gr
     }
   }
   if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
+}
+
+template <typename V, int ND, typename XL, typename Epi, int NDIAG, int U = 1>
+__global__ __launch_bounds__(kBlock) void k_dia(DiaView<V> A, XL xl, Epi epi, GridRed rb,
+                                                const int* __restrict__ done) {
+  dia_rows<V, ND, XL, Epi, NDIAG, U>(A, xl, epi, rb, done);
+}
+
+// The same rows for epilogues with sums (ND > 0), compiled for at most 6 waves
+// per SIMD. Aimed at 8 waves (64 VGPRs), the scheduler issued the first row's
+// loads of the 7-point product with <p, y> one diagonal at a time, each
+// waiting for the last (135 us at 256^3 against 90 us without the sum), in a
+// kernel that then took 59 VGPRs anyway; aimed at 6 it keeps every load of
+// the U rows in flight.
+template <typename V, int ND, typename XL, typename Epi, int NDIAG, int U = 1>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 6))) void k_dia_sum(
+    DiaView<V> A, XL xl, Epi epi, GridRed rb, const int* __restrict__ done) {
+  dia_rows<V, ND, XL, Epi, NDIAG, U>(A, xl, epi, rb, done);
 }
 
 // Rows sampled by the diagonal probe: for sample k (one thread each),

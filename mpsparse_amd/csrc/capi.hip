@@ -526,7 +526,9 @@ static msp_status dia_try_mirror(msp_csr A, hipStream_t s) {
 static msp_status ensure_dia(msp_csr A, hipStream_t s, bool fill = true) {
   if (A->dia_state != 0) return MSP_OK;
   A->dia_state = 2;
-  if (A->format != MSP_FORMAT_AUTO || A->n == 0 || A->m == 0 || A->nnz == 0 || A->max_row > (uint32_t)kMaxDiag)
+  // the diagonal kernel indexes rows, columns and byte offsets in 32 bits
+  if (A->format != MSP_FORMAT_AUTO || A->n == 0 || A->m == 0 || A->nnz == 0 || A->max_row > (uint32_t)kMaxDiag ||
+      A->n >= kDiaMaxDim || A->m >= kDiaMaxDim)
     return MSP_OK;
   const uint32_t n = A->n;
   std::vector<int64_t> offs;

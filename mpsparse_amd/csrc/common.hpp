@@ -254,6 +254,27 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t wg, uint32_t nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// p[c] from global memory with the byte offset c * sizeof(T) formed in 32 bits
+// (the caller guarantees it is below 2^32): for a p held in scalar registers
+// the load takes one 32-bit offset register instead of a 64-bit address.
+template <typename T>
+__device__ __forceinline__ T ld_off32(const T* __restrict__ p, uint32_t c) {
+  typedef const __attribute__((address_space(1))) char gchar;
+  typedef const __attribute__((address_space(1))) T gT;
+  return *reinterpret_cast<gT*>(reinterpret_cast<gchar*>((gT*)p) + c * (uint32_t)sizeof(T));
+}
+
+// p, which must be the same in every lane, materialised in scalar registers:
+// keeps the compiler from folding a per-lane offset into it (one 64-bit
+// address register pair per load instead of one shared offset register).
+template <typename T>
+__device__ __forceinline__ const T* scalar_ptr(const T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
+}
+
 // ---------------------------------------------------------------------------
 // Matrix value types: FP32, or 16-bit storage (BF16 / FP16) widened to FP32.
 // Value formats of the CSR SpMV. The 16-bit formats keep the diagonal exact to

@@ -16,6 +16,9 @@
 namespace msp {
 
 constexpr int kMaxDiag = 16;
+// rows and columns of a matrix in diagonal storage stay below this (32-bit
+// byte offsets into value planes and vectors)
+constexpr uint32_t kDiaMaxDim = 1u << 30;
 
 // Diagonal d (column - row = off[d]) of row i is read from
 // val[plane[d] * ld + i + shift[d]]: plane[d] = d, shift[d] = 0 in plain
@@ -77,13 +80,24 @@ __device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, Gr
 #pragma unroll
   for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
   const int nd = NDIAG > 0 ? NDIAG : A.o.ndiag;
-  const int64_t m = A.m;
+  // W: the plain product (no sums) from the FP32 vector x. Rows, columns and
+  // byte offsets in 32 bits (n, m < kDiaMaxDim), each load one offset register
+  // against a base in scalar registers: the 7-point product at 256^3 took
+  // 82 -> 74 us with BF16 values, 91 -> 89 us with FP32. The same form made
+  // the products with sums and two-vector gathers 5-15% slower at 128^3, so
+  // they keep 64-bit addresses.
+  constexpr bool W = ND == 0 && std::is_same<XL, XPlain>::value;
+  using Row = typename std::conditional<W, uint32_t, uint64_t>::type;
+  using Col = typename std::conditional<W, int32_t, int64_t>::type;
+  const Col m = (Col)A.m;
+  const Row n = A.n;
   // a workgroup's U-group is U contiguous runs of kBlock rows
-  const uint64_t stride = kBlock;
-  const uint64_t gstride = (uint64_t)gridDim.x * kBlock * U;
+  const Row stride = kBlock;
+  const Row gstride = (Row)gridDim.x * kBlock * U;
   // 16-bit values held one per register (packing two per register would
   // wait for the loads)
-  using Held = typename std::conditional<sizeof(typename V::storage) == 2, uint32_t, float>::type;
+  using S = typename V::storage;
+  using Held = typename std::conditional<sizeof(S) == 2, uint32_t, float>::type;
   // start of each diagonal's values, relative to row 0: plane * ld + shift.
   // A mirrored lower diagonal (shift -k) reads the plane of +k (index >= 1,
   // after the lower planes), so its rows i < k still address memory inside
@@ -96,7 +110,12 @@ __device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, Gr
 #pragma unroll
     for (int d = 0; d < DMAX; ++d) base[d] = (int64_t)d * (int64_t)A.ld;
   }
-  for (uint64_t i0 = (uint64_t)blockIdx.x * kBlock * U + threadIdx.x; i0 < A.n; i0 += gstride) {
+  [[maybe_unused]] const S* vd[DMAX];
+  if constexpr (W) {
+#pragma unroll
+    for (int d = 0; d < DMAX; ++d) vd[d] = scalar_ptr(A.val + base[d]);
+  }
+  for (Row i0 = (Row)blockIdx.x * kBlock * U + threadIdx.x; i0 < n; i0 += gstride) {
     Held a[U][DMAX];
     typename XL::Raw xr[U][DMAX];
     [[maybe_unused]] float dl[U];
@@ -104,38 +123,47 @@ __device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, Gr
     [[maybe_unused]] typename EpiPre<Epi>::type pre[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t iu = i0 + (uint64_t)u * stride;
-      const uint64_t i = iu < A.n ? iu : i0;
+      const Row iu = i0 + (Row)u * stride;
+      const Row i = iu < n ? iu : i0;
 #pragma unroll
       for (int d = 0; d < DMAX; ++d) {
         a[u][d] = {};
         xr[u][d] = {};
         if (d < nd) {  // loads only: no arithmetic on loaded data before the last load
-          const int64_t c = (int64_t)i + A.o.off[d];
-          a[u][d] = A.val[base[d] + (int64_t)i];
-          xr[u][d] = xl.raw((uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c)));
+          const Col c = (Col)i + A.o.off[d];
+          const uint32_t cc = (uint32_t)(c < 0 ? 0 : (c >= m ? m - 1 : c));
+          if constexpr (W) {
+            a[u][d] = ld_off32(vd[d], i);
+            xr[u][d] = xl.raw32(cc);
+          } else {
+            a[u][d] = A.val[base[d] + (int64_t)i];
+            xr[u][d] = xl.raw(cc);
+          }
         }
       }
       if constexpr (V::kLowDiag) {
-        dl[u] = A.dlo[i];
-        xd[u] = xl.raw((uint32_t)((int64_t)i < m ? (int64_t)i : m - 1));
+        if constexpr (W) dl[u] = ld_off32(A.dlo, i);
+        else dl[u] = A.dlo[i];
+        const uint32_t ci = (uint32_t)((Col)i < m ? (Col)i : m - 1);
+        if constexpr (W) xd[u] = xl.raw32(ci);
+        else xd[u] = xl.raw(ci);
       }
       if constexpr (EpiPre<Epi>::has) pre[u] = epi.pre((uint32_t)i);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t i = i0 + (uint64_t)u * stride;
-      if (i < A.n) {
+      const Row i = i0 + (Row)u * stride;
+      if (i < n) {
         float s = 0.0f;
 #pragma unroll
         for (int d = 0; d < DMAX; ++d) {
           if (d < nd) {
-            const int64_t c = (int64_t)i + A.o.off[d];
-            s = (c >= 0 && c < m) ? s + V::cvt1((typename V::storage)a[u][d]) * xl.make(xr[u][d]) : s;
+            const Col c = (Col)i + A.o.off[d];
+            s = (c >= 0 && c < m) ? s + V::cvt1((S)a[u][d]) * xl.make(xr[u][d]) : s;
           }
         }
         if constexpr (V::kLowDiag) {
-          if ((int64_t)i < m) s += dl[u] * xl.make(xd[u]);
+          if ((Col)i < m) s += dl[u] * xl.make(xd[u]);
         }
         if constexpr (EpiPre<Epi>::has) epi.row((uint32_t)i, s, acc, pre[u]);
         else epi.row((uint32_t)i, s, acc);

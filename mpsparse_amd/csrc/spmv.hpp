@@ -51,6 +51,8 @@ struct XPlain {
   const float* __restrict__ x;
   typedef float Raw;
   __device__ __forceinline__ Raw raw(uint32_t c) const { return x[c]; }
+  // raw with a 32-bit byte offset (c < 2^30)
+  __device__ __forceinline__ Raw raw32(uint32_t c) const { return ld_off32(x, c); }
   __device__ __forceinline__ float make(Raw w) const { return w; }
   __device__ __forceinline__ float operator()(uint32_t c) const { return x[c]; }
   __device__ __forceinline__ f4 load4(u4 c) const {

@@ -19,6 +19,8 @@ constexpr int kMaxDiag = 16;
 // rows and columns of a matrix in diagonal storage stay below this (32-bit
 // byte offsets into value planes and vectors)
 constexpr uint32_t kDiaMaxDim = 1u << 30;
+// largest row count walked XCD-contiguously by the diagonal kernel
+constexpr uint32_t kDiaXcdRows = 1u << 23;
 
 // Diagonal d (column - row = off[d]) of row i is read from
 // val[plane[d] * ld + i + shift[d]]: plane[d] = d, shift[d] = 0 in plain
@@ -52,8 +54,7 @@ struct DiaView {
   DiaOffsets o;
 };
 
-// Grid-stride over groups of U * kBlock consecutive rows (measured faster
-// than contiguous per-XCD chunks at 256^3 and within 10% at 128^3); a thread
+// Groups of U * kBlock consecutive rows (order below); a thread
 // takes rows tid, tid + kBlock, ... of its workgroup's group (U rows a full
 // grid stride apart were slower: the grid then streams U distant regions at
 // once). Plain coalesced loads (a non-temporal hint on these 4-byte lane
@@ -93,7 +94,6 @@ __device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, Gr
   const Row n = A.n;
   // a workgroup's U-group is U contiguous runs of kBlock rows
   const Row stride = kBlock;
-  const Row gstride = (Row)gridDim.x * kBlock * U;
   // 16-bit values held one per register (packing two per register would
   // wait for the loads)
   using S = typename V::storage;
@@ -115,7 +115,26 @@ __device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, Gr
 #pragma unroll
     for (int d = 0; d < DMAX; ++d) vd[d] = scalar_ptr(A.val + base[d]);
   }
-  for (Row i0 = (Row)blockIdx.x * kBlock * U + threadIdx.x; i0 < n; i0 += gstride) {
+  // Row groups of U * kBlock rows. Up to kDiaXcdRows rows they are walked
+  // XCD-contiguously: the workgroups on one XCD (dispatch is round-robin,
+  // workgroup b on XCD b % 8) take one eighth of the groups in order, so the
+  // x gathers at columns i +- k and a diagonal's reads at rows i - k meet
+  // lines the same XCD's L2 fetched shortly before (128^3: plain CG 527 ->
+  // 506 ms, the learned-FSAI solve 17.4 -> 16.9 ms). Larger matrices are
+  // walked grid-stride: at 256^3 the rows between such reuses outgrow an
+  // XCD's L2 either way (the 7-point product fetched 494 against 508 MB) and
+  // the CG step was 2.5% slower.
+  const Row ngroups = (n + (Row)(kBlock * U) - 1) / (Row)(kBlock * U);
+  Row gfirst = blockIdx.x, glast = ngroups, gstep = gridDim.x;
+  if (gridDim.x >= (uint32_t)kNumXcd && n <= (Row)kDiaXcdRows) {
+    const uint32_t xcd = blockIdx.x % kNumXcd, idx = blockIdx.x / kNumXcd;
+    gstep = gridDim.x / kNumXcd + (xcd < gridDim.x % kNumXcd ? 1u : 0u);
+    gfirst = ngroups * xcd / kNumXcd + idx;
+    glast = ngroups * (xcd + 1) / kNumXcd;
+  }
+  for (Row grp = gfirst; grp < glast; grp += gstep) {
+    const Row i0 = grp * (Row)(kBlock * U) + threadIdx.x;
+    if (i0 >= n) continue;
     Held a[U][DMAX];
     typename XL::Raw xr[U][DMAX];
     [[maybe_unused]] float dl[U];

@@ -13,8 +13,17 @@
 //   k_bt_spmm:    p = z + beta p_old (z = D^-1 r, formed for gathered columns
 //                 too), q = A p, block sums of <p, q>          -> P[k^1], Q
 //   k_bt_fin_alpha (one wave per system): alpha = <r,z> / <p,q>
-//   k_bt_update:  x += alpha p, r -= alpha q, sums of <r,r>, <r,z>
+//   k_bt_update:  r -= alpha q, sums of <r,r>, <r,z>; on odd iterations also
+//                 x += alpha' p' + alpha p with the even iteration's alpha'
+//                 and direction p' (still in P[k^1]'s partner buffer)
 //   k_bt_fin_beta: convergence per (system, rhs), beta, system activity
+// The iterate is updated every other iteration: the recurrence itself never
+// reads x, and the two updates are applied in their original order, so the
+// result is the same bit for bit, while x and xl move through memory once per
+// two iterations (the update's traffic falls from 132 to 100 B per row and
+// iteration). Graphs hold an even number of iterations, so x is current
+// whenever the host looks; a system whose last pair stops on an even
+// iteration stays active for the odd one that applies its last update.
 // Each (system, rhs) pair is its own CG recurrence with its own scalars; a
 // pair that stops gets alpha = 0, a system with no running pair is skipped.
 // The iterate is compensated (x + xl, the rounding errors of every update kept
@@ -45,6 +54,7 @@ struct PairState {
   float rr_true;  // ||b - A x||^2 of the last verification
   int32_t status, iter, nverify;
   int32_t collapse;  // fold xl into x at the next update (restart of an FP32 result)
+  float alpha_hold;  // alpha of the last even iteration, applied to x on the odd one
 };
 
 constexpr int kBD = 8;  // diagonals per system at most in the batched diagonal storage
@@ -414,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_bd_fill(const uint32_t* __restrict__
 __global__ __launch_bounds__(kBlock) void k_bt_fin_alpha(const CSum* __restrict__ partials,
                                                          const uint32_t* __restrict__ sys_tile, uint32_t nsys,
                                                          PairState* __restrict__ ps,
-                                                         const int32_t* __restrict__ sys_active) {
+                                                         const int32_t* __restrict__ sys_active, int32_t even) {
   const uint32_t s = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (s >= nsys || !sys_active[s]) return;
   float tot[kRhs];
@@ -424,17 +434,23 @@ __global__ __launch_bounds__(kBlock) void k_bt_fin_alpha(const CSum* __restrict_
   for (int k = 0; k < kRhs; ++k) {
     PairState& p = ps[s * kRhs + k];
     p.alpha = 0.0f;
-    if (p.status != kPairRunning) continue;
-    p.pq = tot[k];
-    if (!(tot[k] > 0.0f) || !isfinite(tot[k])) p.status = kPairBreakdown;
-    else p.alpha = p.rz / tot[k];
+    if (p.status == kPairRunning) {
+      p.pq = tot[k];
+      if (!(tot[k] > 0.0f) || !isfinite(tot[k])) p.status = kPairBreakdown;
+      else p.alpha = p.rz / tot[k];
+    }
+    if (even) p.alpha_hold = p.alpha;
   }
 }
 
-// (x, xl) += alpha p (compensated), r -= alpha q; sums of <r,r>, <r, D^-1 r>.
+// r -= alpha q; sums of <r,r>, <r, D^-1 r>. X (odd iterations): also
+// (x, xl) += alpha' p' then += alpha p (compensated, in that order), with
+// alpha' = alpha_hold and p' the even iteration's direction.
+template <bool X>
 __global__ __launch_bounds__(kBlock) void k_bt_update(View V, float* __restrict__ x, float* __restrict__ xl,
-                                                      float* __restrict__ r, const float* __restrict__ p,
-                                                      const float* __restrict__ q, const PairState* __restrict__ ps,
+                                                      float* __restrict__ r, const float* __restrict__ pprev,
+                                                      const float* __restrict__ p, const float* __restrict__ q,
+                                                      const PairState* __restrict__ ps,
                                                       CSum* __restrict__ partials) {
   const uint32_t na = *V.n_active;
   for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
@@ -442,11 +458,12 @@ __global__ __launch_bounds__(kBlock) void k_bt_update(View V, float* __restrict_
     const uint32_t s = V.tile_sys[t];
     if (!V.sys_active[s]) continue;
     const uint32_t r0 = V.tile_r0[t], r1 = V.tile_r0[t + 1];
-    f4 alpha;
+    f4 alpha, alpha0;
     bool fold[kRhs];
 #pragma unroll
     for (int k = 0; k < kRhs; ++k) {
       alpha[k] = ps[s * kRhs + k].alpha;
+      alpha0[k] = ps[s * kRhs + k].alpha_hold;
       fold[k] = ps[s * kRhs + k].collapse != 0;
     }
     CSum acc[2 * kRhs];
@@ -456,15 +473,19 @@ __global__ __launch_bounds__(kBlock) void k_bt_update(View V, float* __restrict_
     // the first update (a row at a time waited a memory round trip per row);
     // rows past the tile load row i0's operands and store nothing
     for (uint32_t i0 = r0 + threadIdx.x; i0 < r1; i0 += kUB * kBlock) {
-      f4 xv[kUB], lv[kUB], pv[kUB], rv[kUB], qv[kUB];
+      [[maybe_unused]] f4 xv[kUB], lv[kUB], p0v[kUB], pv[kUB];
+      f4 rv[kUB], qv[kUB];
       float dv[kUB];
 #pragma unroll
       for (int u = 0; u < kUB; ++u) {
         const uint32_t iu = i0 + u * kBlock;
         const uint32_t i = iu < r1 ? iu : i0;
-        xv[u] = ld4(x, i);
-        lv[u] = ld4(xl, i);
-        pv[u] = ld4(p, i);
+        if constexpr (X) {
+          xv[u] = ld4(x, i);
+          lv[u] = ld4(xl, i);
+          p0v[u] = ld4(pprev, i);
+          pv[u] = ld4(p, i);
+        }
         rv[u] = ld4(r, i);
         qv[u] = ld4(q, i);
         dv[u] = V.dinv[i];
@@ -473,22 +494,24 @@ __global__ __launch_bounds__(kBlock) void k_bt_update(View V, float* __restrict_
       for (int u = 0; u < kUB; ++u) {
         const uint32_t i = i0 + u * kBlock;
         if (i >= r1) continue;
-        f4 xi = xv[u], li = lv[u];
-        const f4 pi = pv[u];
+        if constexpr (X) {
+          f4 xi = xv[u], li = lv[u];
 #pragma unroll
-        for (int k = 0; k < kRhs; ++k) {
-          float xk = xi[k], lk = li[k];
-          if (fold[k]) {
-            xk += lk;
-            lk = 0.0f;
+          for (int k = 0; k < kRhs; ++k) {
+            float xk = xi[k], lk = li[k];
+            if (fold[k]) {
+              xk += lk;
+              lk = 0.0f;
+            }
+            comp_axpy(xk, lk, alpha0[k], p0v[u][k]);
+            comp_axpy(xk, lk, alpha[k], pv[u][k]);
+            xi[k] = xk;
+            li[k] = lk;
           }
-          comp_axpy(xk, lk, alpha[k], pi[k]);
-          xi[k] = xk;
-          li[k] = lk;
+          st4(x, i, xi);
+          st4(xl, i, li);
         }
         const f4 ri = rv[u] - alpha * qv[u];
-        st4(x, i, xi);
-        st4(xl, i, li);
         st4(r, i, ri);
         const float di = dv[u];
 #pragma unroll
@@ -503,11 +526,12 @@ __global__ __launch_bounds__(kBlock) void k_bt_update(View V, float* __restrict_
   }
 }
 
-// Convergence of every running pair, beta, then the system's activity.
+// Convergence of every running pair, beta, then (odd iterations, after the
+// iterate's update) the system's activity.
 __global__ __launch_bounds__(kBlock) void k_bt_fin_beta(const CSum* __restrict__ partials,
                                                         const uint32_t* __restrict__ sys_tile, uint32_t nsys,
                                                         int32_t maxiter, PairState* __restrict__ ps,
-                                                        int32_t* __restrict__ sys_active) {
+                                                        int32_t* __restrict__ sys_active, int32_t odd) {
   const uint32_t s = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (s >= nsys || !sys_active[s]) return;
   float tot[2 * kRhs];
@@ -518,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_fin_beta(const CSum* __restrict__
     PairState& p = ps[s * kRhs + k];
     if (p.status != kPairRunning) continue;
     p.iter += 1;
-    p.collapse = 0;
+    if (odd) p.collapse = 0;  // folded by this iteration's update
     p.rr = tot[k];
     const float rz_new = tot[kRhs + k];
     if (!isfinite(tot[k]) || !isfinite(rz_new)) {
@@ -534,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_fin_beta(const CSum* __restrict__
       p.rz = rz_new;
     }
   }
-  refresh_system(ps, sys_active, s);
+  if (odd) refresh_system(ps, sys_active, s);
 }
 
 // True residual r = b - A x of the iterate that would be returned, for every

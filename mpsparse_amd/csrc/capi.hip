@@ -2940,17 +2940,23 @@ static void launch_spmm(msp_batch B, int rpt, int parity, hipStream_t s) {
                                              B->partials.as<CSum>());
 }
 
-// One PCG iteration (4 kernels); `parity` selects the direction buffers.
+// One PCG iteration (4 kernels); `parity` selects the direction buffers, and
+// odd iterations also update the iterate (graphs hold an even number).
 static void batch_iteration(msp_batch B, int rpt, int parity, int32_t maxiter, hipStream_t s) {
   const batch::View V = B->view();
   const uint32_t g = batch_grid(B), gs = sys_grid(B);
   launch_spmm(B, rpt, parity, s);
   batch::k_bt_fin_alpha<<<gs, kBlock, 0, s>>>(B->partials.as<CSum>(), B->sys_tile.as<uint32_t>(), B->nsys,
-                                               B->state.as<PairState>(), B->sys_active.as<int32_t>());
-  batch::k_bt_update<<<g, kBlock, 0, s>>>(V, B->v(kBX), B->v(kBXL), B->v(kBR), B->v(parity ? kBP0 : kBP1),
-                                          B->v(kBQ), B->state.as<PairState>(), B->partials.as<CSum>());
+                                               B->state.as<PairState>(), B->sys_active.as<int32_t>(), parity ? 0 : 1);
+  // the even iteration's direction is P1, the odd one's P0
+  if (parity)
+    batch::k_bt_update<true><<<g, kBlock, 0, s>>>(V, B->v(kBX), B->v(kBXL), B->v(kBR), B->v(kBP1), B->v(kBP0),
+                                                  B->v(kBQ), B->state.as<PairState>(), B->partials.as<CSum>());
+  else
+    batch::k_bt_update<false><<<g, kBlock, 0, s>>>(V, B->v(kBX), B->v(kBXL), B->v(kBR), B->v(kBP0), B->v(kBP1),
+                                                   B->v(kBQ), B->state.as<PairState>(), B->partials.as<CSum>());
   batch::k_bt_fin_beta<<<gs, kBlock, 0, s>>>(B->partials.as<CSum>(), B->sys_tile.as<uint32_t>(), B->nsys, maxiter,
-                                              B->state.as<PairState>(), B->sys_active.as<int32_t>());
+                                              B->state.as<PairState>(), B->sys_active.as<int32_t>(), parity);
 }
 
 static void batch_active_list(msp_batch B, hipStream_t s) {

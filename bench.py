@@ -275,11 +275,16 @@ def main():
     if "precision" in sections:
         out["precision_path"] = precision_section(dev, empty, args.spmv_steps)
         pm = out["precision_path"]["mv"]
+        roof = {}
+        for vt in ("f32", "bf16"):
+            traffic, tsrc = measured_traffic("dia_%s_diffusion_256" % vt)
+            roof[vt] = {"achieved": pm[vt]["GBps"], "frac": pm[vt]["GBps"] / PEAK_HBM_GBS,
+                        "algorithmic_bytes": pm[vt]["algorithmic_bytes"], "ms": pm[vt]["ms"],
+                        "traffic": traffic, "traffic_source": tsrc}
         out["roofline_stencil_256"] = {"bound": "hbm", "kernel": "k_dia (7 diagonals): y = A x, 256^3",
                                        "unit": "GB/s", "peak": PEAK_HBM_GBS,
-                                       **{vt: {"achieved": pm[vt]["GBps"], "frac": pm[vt]["GBps"] / PEAK_HBM_GBS,
-                                               "algorithmic_bytes": pm[vt]["algorithmic_bytes"], "ms": pm[vt]["ms"]}
-                                          for vt in ("f32", "bf16")}}
+                                       "bytes": "value planes the storage streams (%s) + x + y" % pm["storage"],
+                                       **roof}
 
     if "batched" in sections:
         out["batched_pcg"] = batched_section(dev)
@@ -869,11 +874,15 @@ def precision_section(dev, empty, spmv_steps=50, only_products=False):
     # a diagonal leaves the grid), x, y (+ the FP32 diagonal remainder with 16-bit values)
     xm = torch.randn(n, device=dev)
     ym = torch.empty(n, device=dev)
-    res["mv"] = {}
+    res["mv"] = {"storage": A.format}
     for vt, vb, extra in (("f32", 4, 0), ("bf16", 2, 4 * n)):
         A.mv(xm, ym, vt)
         ms = time_launches(lambda: A.mv(xm, ym, vt), spmv_steps, 5)
-        alg = 7 * vb * n + 4 * n + 4 * n + extra
+        # exactly symmetric storage streams the diagonal and the 3 upper planes
+        # once their 7 planes exceed 128 MB (the lower diagonals read the upper
+        # planes shifted); otherwise all 7 planes
+        planes = 4 if A.format == "dia_symmetric" and 7 * vb * n >= (128 << 20) else 7
+        alg = planes * vb * n + 4 * n + 4 * n + extra
         res["mv"][vt] = {"ms": ms, "algorithmic_bytes": alg, "GBps": alg / (ms * 1e-3) / 1e9,
                          "launches_timed": spmv_steps}
         log("stencil mv 256^3", vt, res["mv"][vt])

@@ -1293,9 +1293,60 @@ constexpr int dia_rows_per_thread() {
 
 // The diagonal kernel is grid-stride over rows: a grid beyond residency would
 // leave its last workgroups running alone at the end.
+// The marched walk (DiaMarch) for symmetric storage whose far offset c is
+// large enough that c rows of the streamed planes outgrow an XCD's 4 MB L2:
+// every work item must get its own workgroup within the residency (and the
+// partial-sum buffer's 8 per CU), and a work item should cover several levels
+// (one extra level of carried loads per item).
+constexpr uint64_t kDiaMarchMinBytes = (uint64_t)1 << 20;
+constexpr uint32_t kDiaMarchMinLevels = 8;
+
+template <typename V, int NDIAG>
+static bool dia_march_plan(const DiaView<V>& v, uint32_t cap, DiaMarch& mk) {
+  if constexpr (NDIAG < 3 || (NDIAG & 1) == 0) {
+    return false;
+  } else {
+    if (!v.mirrored) return false;
+    const int32_t c = v.o.off[NDIAG - 1];
+    if ((NDIAG & 1) == 0 || v.o.off[NDIAG / 2] != 0 || c <= 0 || v.o.off[0] != -c || (uint32_t)c >= v.n ||
+        v.n >= kDiaMaxDim)
+      return false;
+    const uint64_t row_bytes = (uint64_t)(NDIAG / 2 + 1) * sizeof(typename V::storage) + 2 * sizeof(float);
+    if ((uint64_t)c * row_bytes < kDiaMarchMinBytes) return false;
+    const uint32_t ntile = ((uint32_t)c + kBlock - 1) / kBlock;
+    const uint32_t nlev = (uint32_t)(((uint64_t)v.n + (uint32_t)c - 1) / (uint32_t)c);
+    if (nlev < kDiaMarchMinLevels || ntile > cap) return false;
+    uint32_t nchunk = std::max(1u, std::min(nlev / kDiaMarchMinLevels, cap / ntile));
+    const uint32_t lpc = (nlev + nchunk - 1) / nchunk;
+    nchunk = (nlev + lpc - 1) / lpc;
+    mk.c = (uint32_t)c;
+    mk.ntile = ntile;
+    mk.nlev = nlev;
+    mk.lpc = lpc;
+    mk.nwork = ntile * nchunk;
+    return true;
+  }
+}
+
 template <typename V, int ND, typename XL, typename Epi, int NDIAG>
 static uint32_t dia_launch(msp_csr A, XL xl, Epi epi, GridRed rb, const int* done, hipStream_t s) {
   constexpr int U = dia_rows_per_thread<XL, NDIAG>();
+  if constexpr (NDIAG >= 3 && (NDIAG & 1)) {
+    // levels per memory round trip: 4 with one-load gathers, 2 with
+    // two-vector gathers (direction formed in the gather)
+    constexpr int UZ = sizeof(typename RawOf<XL>::type) == sizeof(float) ? 4 : 2;
+    const auto mkern = [] {
+      if constexpr (ND > 0) return &k_dia_march_sum<V, ND, XL, Epi, NDIAG, UZ>;
+      else return &k_dia_march<V, ND, XL, Epi, NDIAG, UZ>;
+    }();
+    static const uint32_t mper_cu = resident_per_cu(mkern);
+    const DiaView<V> v = dia_view_of<V>(A);
+    DiaMarch mk;
+    if (dia_march_plan<V, NDIAG>(v, mper_cu * (uint32_t)cu_count(), mk)) {
+      mkern<<<mk.nwork, kBlock, 0, s>>>(v, mk, xl, epi, rb, done);
+      return mk.nwork;
+    }
+  }
   const auto kern = [] {
     if constexpr (ND > 0) return &k_dia_sum<V, ND, XL, Epi, NDIAG, U>;
     else return &k_dia<V, ND, XL, Epi, NDIAG, U>;
@@ -1691,7 +1742,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
     } else {
       if (jac) k_cg_update_p<kPcJacobi><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, n, dec);
       else k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, n, dec);
-      g = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S2->halt, s);
+      g = op.template spmv<1>(pnew, EpiStoreDotX{Q, pnew}, FinNone{}, &S2->halt, s);
     }
     if (jac) {
       if (comp) k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);

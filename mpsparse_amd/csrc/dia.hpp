@@ -210,6 +210,153 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 6))) 
   dia_rows<V, ND, XL, Epi, NDIAG, U>(A, xl, epi, rb, done);
 }
 
+// ---------------------------------------------------------------------------
+// Rows marched along the far diagonal pair of symmetric storage. With offsets
+// +-c (c the largest), row i's -c entry is the +c plane's value at row i - c
+// and its gather x[i - c]; the walk above streams that plane and x a second
+// time, c rows after the first read, and once c rows of all the planes outgrow
+// an XCD's L2 (the 7-point operator at 256^3: c = 65536 rows, 1.5 MB of planes
+// per XCD between the two reads) the second read comes back through the
+// fabric (1.45x the minimum traffic). Here a thread owns the rows
+// t, t + c, t + 2c, ... of one level range: the +c value and x[i + c] it loads
+// for row i are, one level later, the -c value and x[i] of row i + c, carried
+// in registers, and x[i - c] is the previous level's x[i]. A work item is a
+// tile of kBlock rows within [0, c) times a range of levels; the items go to
+// the XCDs in contiguous ranges (one level range, consecutive tiles), so the
+// +-1 and +-nx neighbours of a tile are fetched by the same L2. Each row's sum
+// takes its diagonals in the same order as dia_rows, so the products are bit
+// for bit the same.
+struct DiaMarch {
+  uint32_t c;       // the far offset
+  uint32_t ntile;   // tiles of kBlock rows covering [0, c)
+  uint32_t nlev;    // levels: ceil(n / c)
+  uint32_t lpc;     // levels per work item
+  uint32_t nwork;   // work items (= the grid): ntile * ceil(nlev / lpc)
+};
+
+template <typename V, int ND, typename XL, typename Epi, int NDIAG, int UZ>
+__device__ __forceinline__ void dia_march_rows(const DiaView<V>& A, const DiaMarch& mk, XL xl, Epi epi, GridRed rb,
+                                               const int* __restrict__ done) {
+  static_assert(NDIAG >= 3 && (NDIAG & 1), "the march needs -c, 0, +c and mirrored pairs");
+  const int halt = kernel_prologue(xl, epi, done ? *done : 0);
+  if (halt) return;
+  constexpr int NA = ND > 0 ? ND : 1;
+  CSum acc[NA];
+#pragma unroll
+  for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
+  using S = typename V::storage;
+  using Held = typename std::conditional<sizeof(S) == 2, uint32_t, float>::type;
+  using XR = typename XL::Raw;
+  // offsets ascend and pair up: -c is diagonal 0, +c the last, the main
+  // diagonal the middle one (the host checks all three). Every index into the
+  // register arrays below is a compile-time constant, so a carried value is
+  // never copied while its load is in flight.
+  constexpr int dlo = 0, dhi = NDIAG - 1, dmid = NDIAG / 2;
+  // rows, columns and byte offsets in 32 bits (n, m < kDiaMaxDim); each
+  // diagonal's values from a base in scalar registers
+  const uint32_t n = A.n, c = mk.c;
+  const int32_t m = (int32_t)A.m;
+  const S* vd[NDIAG];
+#pragma unroll
+  for (int d = 0; d < NDIAG; ++d)
+    vd[d] = scalar_ptr(A.val + ((int64_t)A.o.plane[d] * (int64_t)A.ld + A.o.shift[d]));
+  const uint32_t w = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t tile = w % mk.ntile, chunk = w / mk.ntile;
+  const uint32_t t = tile * (uint32_t)kBlock + threadIdx.x;  // row of level 0
+  const uint32_t k0 = chunk * mk.lpc;
+  const uint32_t k1 = k0 + mk.lpc < mk.nlev ? k0 + mk.lpc : mk.nlev;
+  // rows past n (last level) or t >= c (last tile) load row `last` and store
+  // nothing; all later levels of such a thread are past n as well
+  const uint32_t last = n - 1;
+  if (t < c && k0 < k1) {
+    // carried: the +c value of row i - c, x[i - c], x[i]
+    const uint32_t i0 = t + k0 * c;  // < nlev * c < n + c < 2^31
+    const uint32_t ia = i0 < n ? i0 : last;
+    Held vfar = {};
+    XR xm = xl.raw(0), xo;
+    if (k0 > 0) {
+      vfar = ld_off32(vd[dlo], ia);  // the +c plane at row ia - c
+      xm = xl.raw(ia - c);
+    }
+    xo = xl.raw((int32_t)ia < m ? ia : (uint32_t)(m - 1));
+    for (uint32_t k = k0; k < k1; k += UZ) {
+      Held a[UZ][NDIAG];
+      XR xr[UZ][NDIAG];
+      [[maybe_unused]] float dl[UZ];
+      [[maybe_unused]] typename EpiPre<Epi>::type pre[UZ];
+      uint32_t iu[UZ];
+      bool ok[UZ];
+#pragma unroll
+      for (int u = 0; u < UZ; ++u) {
+        const uint32_t ii = t + (k + u) * c;
+        ok[u] = (k + u < k1) && ii < n;
+        const uint32_t i = ok[u] ? ii : last;
+        iu[u] = i;
+#pragma unroll
+        for (int d = 0; d < NDIAG; ++d) {
+          if (d == dlo) {  // value and x of -c: carried
+            a[u][d] = u == 0 ? vfar : a[u - 1][dhi];
+            continue;
+          }
+          // the diagonal's and the +c plane's values are read once (the
+          // +c plane's second use is carried): nontemporal; the other upper
+          // planes are read again by the mirrored lower diagonals from L2
+          if (d == dmid || d == dhi) a[u][d] = __builtin_nontemporal_load(vd[d] + i);
+          else a[u][d] = ld_off32(vd[d], i);
+          if (d == dmid) continue;  // x[i]: carried
+          const int32_t col = (int32_t)i + A.o.off[d];
+          const uint32_t cc = (uint32_t)(col < 0 ? 0 : (col >= m ? m - 1 : col));
+          xr[u][d] = xl.raw(cc);
+        }
+        if constexpr (V::kLowDiag) dl[u] = ld_off32(scalar_ptr(A.dlo), i);
+        if constexpr (EpiPre<Epi>::has && !EpiOwnX<Epi>::value) pre[u] = epi.pre(i);
+      }
+#pragma unroll
+      for (int u = 0; u < UZ; ++u) {
+        xr[u][dmid] = u == 0 ? xo : xr[u - 1][dhi];
+        xr[u][dlo] = u == 0 ? xm : xr[u - 1][dmid];
+        if constexpr (EpiOwnX<Epi>::value) pre[u] = xr[u][dmid];
+      }
+#pragma unroll
+      for (int u = 0; u < UZ; ++u) {
+        if (ok[u]) {
+          const uint32_t i = iu[u];
+          float s = 0.0f;
+#pragma unroll
+          for (int d = 0; d < NDIAG; ++d) {
+            const int32_t col = (int32_t)i + A.o.off[d];
+            s = (uint32_t)col < (uint32_t)m ? s + V::cvt1((S)a[u][d]) * xl.make(xr[u][d]) : s;
+          }
+          if constexpr (V::kLowDiag) {
+            if ((int32_t)i < m) s += dl[u] * xl.make(xr[u][dmid]);
+          }
+          if constexpr (EpiPre<Epi>::has) epi.row(i, s, acc, pre[u]);
+          else epi.row(i, s, acc);
+        }
+      }
+      vfar = a[UZ - 1][dhi];
+      xm = xr[UZ - 1][dmid];
+      xo = xr[UZ - 1][dhi];
+    }
+  }
+  if constexpr (ND > 0) block_partials<ND>(reinterpret_cast<CSum(&)[ND]>(acc), rb);
+}
+
+template <typename V, int ND, typename XL, typename Epi, int NDIAG, int UZ>
+__global__ __launch_bounds__(kBlock) void k_dia_march(DiaView<V> A, DiaMarch mk, XL xl, Epi epi, GridRed rb,
+                                                      const int* __restrict__ done) {
+  dia_march_rows<V, ND, XL, Epi, NDIAG, UZ>(A, mk, xl, epi, rb, done);
+}
+
+// with sums: compiled without k_dia_sum's 6-wave bound (every load of a
+// round trip is issued first here anyway; at 256^3 the CG step's product with
+// <p, q> took 107.8 us with the bound, 92.6 us without)
+template <typename V, int ND, typename XL, typename Epi, int NDIAG, int UZ>
+__global__ __launch_bounds__(kBlock) void k_dia_march_sum(DiaView<V> A, DiaMarch mk, XL xl, Epi epi, GridRed rb,
+                                                          const int* __restrict__ done) {
+  dia_march_rows<V, ND, XL, Epi, NDIAG, UZ>(A, mk, xl, epi, rb, done);
+}
+
 // Rows sampled by the diagonal probe: for sample k (one thread each),
 // out[k * (kMaxDiag + 1)] = row length, then up to kMaxDiag column indices.
 constexpr int kDiaSamples = 64;

@@ -187,6 +187,7 @@ struct EpiCgQ {
   float* __restrict__ pnew;
   XL xl;
   typedef typename RawOf<XL>::type Pre;  // operands prefetched by the diagonal kernel
+  static constexpr bool kOwnX = true;
   __device__ __forceinline__ Pre pre(uint32_t i) const { return xl.raw(i); }
   __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc, Pre w) const {
     const float pi = xl.make(w);

@@ -257,6 +257,19 @@ struct EpiPre<Epi, std::void_t<typename Epi::Pre>> {
   static constexpr bool has = true;
 };
 
+// An epilogue whose `pre(i)` is the kernel's own gather of row i's column i
+// (`xl.raw(i)`) declares `kOwnX = true`; a kernel that already holds that
+// value (the marched diagonal walk carries it from the previous level) passes
+// it as `pre` instead of loading it again.
+template <typename Epi, typename = void>
+struct EpiOwnX {
+  static constexpr bool value = false;
+};
+template <typename Epi>
+struct EpiOwnX<Epi, std::void_t<decltype(Epi::kOwnX)>> {
+  static constexpr bool value = Epi::kOwnX;
+};
+
 // A gather loader's Raw type (float for loaders without one).
 template <typename XL, typename = void>
 struct RawOf {
@@ -276,6 +289,20 @@ struct EpiStoreDot {
   __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc, Pre wr) const {
     y[r] = s;
     acc[0] = csum_add(acc[0], s * wr);
+  }
+  __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc) const { row(r, s, acc, pre(r)); }
+};
+
+// y = A x and <y, x> (the operand of the dot is the product's own input)
+struct EpiStoreDotX {
+  float* __restrict__ y;
+  const float* __restrict__ x;
+  typedef float Pre;
+  static constexpr bool kOwnX = true;
+  __device__ __forceinline__ Pre pre(uint32_t r) const { return x[r]; }
+  __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc, Pre xr) const {
+    y[r] = s;
+    acc[0] = csum_add(acc[0], s * xr);
   }
   __device__ __forceinline__ void row(uint32_t r, float s, CSum* acc) const { row(r, s, acc, pre(r)); }
 };

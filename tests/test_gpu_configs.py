@@ -172,6 +172,41 @@ def test_diffusion_256_refined_to_1e_10(diffusion_256, values):
           f"relres {rel:.2e}")
 
 
+@pytest.mark.parametrize("values", ["f32", "bf16", "f16"])
+def test_diffusion_256_marched_product_bit_identical(diffusion_256, values):
+    """At 256^3 the symmetric storage's product walks the rows along the
+    +-65536 diagonal pair (the far pair's values and x carried in registers
+    from one grid plane to the next). Against the same matrix with one lower
+    entry of its last row changed (no longer symmetric: 7 planes, the
+    row-group walk) every other row's product agrees bit for bit; the FP32
+    product stays within 2 gamma_k (|A||x|) of the FP64 product."""
+    torch = _torch()
+    from gpu_util import csr_from_arrays
+
+    M = diffusion_256["M"]
+    n = M.num_rows
+    if "perturbed" not in diffusion_256:
+        v2 = M.vals.copy()
+        v2[M.row_ptr[n - 1]] *= 1.5  # first entry of the last row: a lower-triangle coupling
+        say("diffusion 256^3: perturbed copy")
+        diffusion_256["perturbed"] = csr_from_arrays(M.row_ptr, M.col_ind, v2, n, M.num_cols)
+        diffusion_256["x"] = np.random.default_rng(11).standard_normal(n).astype(np.float32)
+    A, B, x = diffusion_256["A"], diffusion_256["perturbed"], diffusion_256["x"]
+    xt = torch.from_numpy(x).cuda()
+    ya = torch.full((n,), float("nan"), device="cuda")
+    yb = torch.full((n,), float("nan"), device="cuda")
+    say("diffusion 256^3: products", values)
+    A.mv(xt, ya, values=values)
+    B.mv(xt, yb, values=values)
+    assert A.format == "dia_symmetric" and B.format == "dia"
+    ya, yb = ya.cpu().numpy(), yb.cpu().numpy()
+    assert np.array_equal(ya[:-1].view(np.uint32), yb[:-1].view(np.uint32))
+    if values == "f32":
+        y64 = OS.csr_spmv(M.row_ptr, M.col_ind, M.vals, x)
+        bound = 2 * OS.spmv_error_bound(M.row_ptr, M.col_ind, M.vals, x) + 1e-30
+        assert np.all(np.abs(ya.astype(np.float64) - y64) <= bound)
+
+
 # ---------------------------------------------------------------------------
 # batched Jacobi-PCG, 512 systems x 4 right-hand sides
 def test_batched_512_systems_all_pairs():

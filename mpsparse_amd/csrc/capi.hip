@@ -826,6 +826,62 @@ static msp_status segment_sort(const uint64_t* keys, const float* vals, uint64_t
   return launch_check("segment sort");
 }
 
+// The two-level sort with two phase-1 passes over keys of 33..40 bits, the
+// keys kept in 32 bits throughout: the first pass writes each key without its
+// own digit (recoverable from the position through that pass's digit starts),
+// the second ranks on the segment id's high digit (now at the same shift) and
+// writes the key with that digit replaced by the first pass's, recovered from
+// the input position. The segment bounds come from the second pass's digit
+// starts and the low digit; the segment sort reads only the local bits below
+// the digit. Against 64-bit keys: 4 bytes less per key in the first pass's
+// writes, the second pass's reads and writes, and the segment sort's reads.
+static msp_status segment_sort_split(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n,
+                                     const sortk::KeyMap& km, int sb, uint32_t* row_ptr, uint32_t* col_out,
+                                     float* val_out, uint32_t* rows_out, int* err, int* overflow, hipStream_t s) {
+  using namespace sortk;
+  const uint32_t G = (uint32_t)(((uint64_t)n + (1ull << sb) - 1) >> sb);
+  const int shift = (int)km.cbits + sb;
+  const uint32_t num_tiles = (uint32_t)((nnz + kTile - 1) / kTile);
+  const uint32_t nchunks = (num_tiles + kColChunk - 1) / kColChunk;
+  TmpMem bounds, counts, sums, ka, kb, va, vb, st1, st2;
+  MSPCHK(counts.alloc(sizeof(uint32_t) * (uint64_t)kRadix * num_tiles, s));
+  MSPCHK(sums.alloc(sizeof(uint32_t) * kRadix * nchunks, s));
+  MSPCHK(ka.alloc(sizeof(uint32_t) * nnz, s));
+  MSPCHK(va.alloc(sizeof(float) * nnz, s));
+  MSPCHK(kb.alloc(sizeof(uint32_t) * nnz, s));
+  MSPCHK(vb.alloc(sizeof(float) * nnz, s));
+  MSPCHK(st1.alloc(sizeof(uint32_t) * (kRadix + 1), s));
+  MSPCHK(st2.alloc(sizeof(uint32_t) * (kRadix + 1), s));
+  uint32_t* cnt = counts.as<uint32_t>();
+  auto colscan = [&] {
+    k_colscan_reduce<<<nchunks, kRadix, 0, s>>>(cnt, num_tiles, sums.as<uint32_t>());
+    k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks);
+    k_colscan_down<<<nchunks, kRadix, 0, s>>>(cnt, num_tiles, sums.as<uint32_t>());
+  };
+  // pass 1: the segment id's low digit (compacted keys, bits [shift, shift + 8))
+  k_upsweep<uint64_t, true><<<num_tiles, kBlock, 0, s>>>(keys, nnz, km, shift, num_tiles, cnt, err);
+  colscan();
+  k_digit_starts<<<1, kRadix, 0, s>>>(cnt, nnz, st1.as<uint32_t>());
+  k_downsweep<uint64_t, uint32_t, true, false, kXfDrop><<<num_tiles, kBlock, 0, s>>>(
+      keys, vals, ka.as<uint32_t>(), va.as<float>(), nullptr, nullptr, nnz, km, shift, num_tiles, cnt);
+  // pass 2: the high digit, now at the same shift
+  k_upsweep<uint32_t, false><<<num_tiles, kBlock, 0, s>>>(ka.as<uint32_t>(), nnz, km, shift, num_tiles, cnt, err);
+  colscan();
+  k_digit_starts<<<1, kRadix, 0, s>>>(cnt, nnz, st2.as<uint32_t>());
+  k_downsweep<uint32_t, uint32_t, false, false, kXfSwap><<<num_tiles, kBlock, 0, s>>>(
+      ka.as<uint32_t>(), va.as<float>(), kb.as<uint32_t>(), vb.as<float>(), nullptr, nullptr, nnz, km, shift,
+      num_tiles, cnt, st1.as<uint32_t>());
+  MSPCHK(launch_check("segment phase 1 (32-bit keys)"));
+  MSPCHK(bounds.alloc(sizeof(uint32_t) * ((size_t)G + 1), s));
+  k_seg_bounds_split<<<grid_for((uint64_t)G + 1), kBlock, 0, s>>>(kb.as<uint32_t>(), nnz, shift, G,
+                                                                 st2.as<uint32_t>(), bounds.as<uint32_t>());
+  const uint32_t grid = std::max(1u, std::min<uint32_t>(G, 1u << 20));
+  k_segment_sort<uint32_t, false><<<grid, kSegThreads, 0, s>>>(kb.as<uint32_t>(), vb.as<float>(),
+                                                               bounds.as<uint32_t>(), G, km, sb, n, nnz, col_out,
+                                                               val_out, rows_out, row_ptr, overflow, err);
+  return launch_check("segment sort");
+}
+
 // Segment size for the two-level sort: the largest s (rows per segment 2^s)
 // whose average segment holds at most 60% of sortk::kSegMax keys; -1 when the
 // two-level sort does not apply (local bits plus the position index exceed 64,
@@ -849,12 +905,18 @@ __global__ void k_zero_flags(int* __restrict__ f, int count) {
   if ((int)threadIdx.x < count) f[threadIdx.x] = 0;
 }
 
+// max_row: when not null and the two-level sort completes, receives the
+// longest row (read back with the sort's flags, in the same round trip); it
+// is set to -1 otherwise.
 static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uint64_t nnz, uint32_t n, uint32_t m,
                                     uint32_t* row_ptr, uint32_t* col_out, float* val_out, hipStream_t s,
-                                    uint32_t* rows_out = nullptr, bool cols_in_order = false) {
+                                    uint32_t* rows_out = nullptr, bool cols_in_order = false,
+                                    int64_t* max_row = nullptr) {
   TmpMem err, rows_own;
-  MSPCHK(err.alloc(2 * sizeof(int), s));  // [0] key out of range, [1] segment overflow
-  k_zero_flags<<<1, 64, 0, s>>>(err.as<int>(), 2);
+  if (max_row) *max_row = -1;
+  // [0] key out of range, [1] segment overflow, [2] longest row
+  MSPCHK(err.alloc(3 * sizeof(int), s));
+  k_zero_flags<<<1, 64, 0, s>>>(err.as<int>(), 3);
   if (nnz > 0) {
     sortk::KeyMap km;
     km.cbits = m > 1 ? bitlen(m - 1) : 0;
@@ -873,14 +935,24 @@ static msp_status coo_to_csr_device(const uint64_t* keys, const float* vals, uin
       if (tb <= 32)
         MSPCHK(segment_sort<uint32_t>(keys, vals, nnz, n, km, sb, passes1, row_ptr, col_out, val_out, rows_out,
                                       err.as<int>(), ovf, s));
+      else if (passes1 == 2 && tb <= 40)
+        MSPCHK(segment_sort_split(keys, vals, nnz, n, km, sb, row_ptr, col_out, val_out, rows_out, err.as<int>(),
+                                  ovf, s));
       else
         MSPCHK(segment_sort<uint64_t>(keys, vals, nnz, n, km, sb, passes1, row_ptr, col_out, val_out, rows_out,
                                       err.as<int>(), ovf, s));
-      int hf[2] = {0, 0};
-      HIPCHK(hipMemcpyAsync(hf, err.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+      if (max_row && n > 0) {
+        const uint32_t g = std::min<uint32_t>((n + kBlock - 1) / kBlock, 1024);
+        k_max_row<<<g, kBlock, 0, s>>>(row_ptr, n, reinterpret_cast<uint32_t*>(err.as<int>() + 2));
+      }
+      int hf[3] = {0, 0, 0};
+      HIPCHK(hipMemcpyAsync(hf, err.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (hf[0]) return fail(MSP_ERR_KEY_RANGE, "a COO key lies outside [0,%u) x [0,%u)", n, m);
-      if (!hf[1]) return MSP_OK;
+      if (!hf[1]) {
+        if (max_row) *max_row = (int64_t)(uint32_t)hf[2];
+        return MSP_OK;
+      }
       // a segment too large for one workgroup: the single-level sort below
     }
     if (!rows_out) {
@@ -1083,11 +1155,13 @@ extern "C" msp_status msp_csr_create_coo(msp_csr* out, const uint64_t* keys, con
   InArr k, v;
   if (st == MSP_OK) st = k.set(keys, sizeof(uint64_t) * nnz, s);
   if (st == MSP_OK) st = v.set(vals, sizeof(float) * nnz, s);
+  int64_t max_row = -1;
   if (st == MSP_OK)
     st = coo_to_csr_device(static_cast<const uint64_t*>(k.d), static_cast<const float*>(v.d), nnz, num_rows, num_cols,
-                           A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), s);
+                           A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), s, nullptr, false,
+                           &max_row);
   if (st == MSP_OK) st = zero_pad(A, s);
-  if (st == MSP_OK) st = finalize_matrix(A, s);
+  if (st == MSP_OK) st = finalize_matrix(A, s, max_row);
   if (st == MSP_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(MSP_ERR_HIP, "stream sync failed");
   if (st != MSP_OK) {
     delete A;

@@ -218,19 +218,45 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ dat
   }
 }
 
+// Key transforms of a downsweep (the two-level sort's phase 1 with two
+// passes over keys of 33..40 bits, so that its keys travel as 32 bits):
+//   kXfNone: the key is written as read;
+//   kXfDrop: the pass's own digit (bits [shift, shift + 8)) is removed, the
+//            bits above it move down by 8 -- the digit is recoverable from
+//            the output position through the pass's digit starts;
+//   kXfSwap: the digit at [shift, shift + 8) (the pass's own) is replaced by
+//            the previous pass's digit, recovered from the input position
+//            through that pass's digit starts `prev` (257 entries).
+enum : int { kXfNone = 0, kXfDrop = 1, kXfSwap = 2 };
+
+__device__ __forceinline__ uint32_t digit_at(const uint32_t* __restrict__ starts, uint32_t lo, uint32_t hi,
+                                             uint64_t p) {
+  // the largest d in [lo, hi] with starts[d] <= p
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if ((uint64_t)starts[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Stable rank + scatter of one tile. Wave w owns tile positions
 // [w*64*kItems, (w+1)*64*kItems), item i of lane l at w*64*kItems + i*64 + l,
 // so the (wave, item, lane) order is the input order.
-template <typename InK, typename OutK, bool FIRST, bool LAST>
+template <typename InK, typename OutK, bool FIRST, bool LAST, int XF = kXfNone>
 __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ kin, const float* __restrict__ vin,
                                                       OutK* __restrict__ kout, float* __restrict__ vout,
                                                       uint32_t* __restrict__ col_out, uint32_t* __restrict__ row_out,
                                                       uint64_t nnz, KeyMap km, int shift, uint32_t num_tiles,
-                                                      const uint32_t* __restrict__ offsets) {
+                                                      const uint32_t* __restrict__ offsets,
+                                                      const uint32_t* __restrict__ prev = nullptr) {
+  static_assert(XF == kXfNone || (sizeof(OutK) == 4 && !LAST), "key transforms write 32-bit keys");
   __shared__ uint32_t wcnt[kWavesPerBlock][kRadix];
   __shared__ uint32_t dstart[kRadix];
   __shared__ uint32_t gbase[kRadix];
   __shared__ uint64_t stage[kTile];
+  [[maybe_unused]] __shared__ uint32_t pstart[kRadix + 1];
+  [[maybe_unused]] __shared__ uint32_t prange[2];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   // consecutive tiles on the same XCD: the runs that neighbouring tiles write
   // for one digit are adjacent in the output and then meet in one L2
@@ -240,9 +266,18 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
 
   for (int i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
   for (int d = tid; d < kRadix; d += kBlock) gbase[d] = offsets[(uint64_t)tile * kRadix + d];
+  if constexpr (XF == kXfSwap) {
+    for (int d = tid; d <= kRadix; d += kBlock) pstart[d] = prev[d];
+    __syncthreads();
+    // the previous digits of the tile's first and last positions bound every
+    // search below (a tile usually lies within one or two digits)
+    if (tid < 2) prange[tid] = digit_at(pstart, 0, kRadix - 1, tile_base + (tid ? tile_n - 1 : 0));
+  }
   __syncthreads();
 
-  OutK key[kItems];
+  // with a transform, key[i] holds (digit << 32) | transformed key
+  using HeldK = typename std::conditional<XF == kXfNone, OutK, uint64_t>::type;
+  HeldK key[kItems];
   float val[kItems];
   uint32_t rank[kItems];
   const uint64_t lane_lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
@@ -258,9 +293,21 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
       if (FIRST) k = compact_key(k, km, nullptr);
       v = vin[tile_base + pos];
     }
-    key[i] = (OutK)k;
     val[i] = v;
     uint32_t d = digit_of(k, shift);
+    if constexpr (XF == kXfNone) {
+      key[i] = (OutK)k;
+    } else {
+      const uint64_t low = k & ((1ull << shift) - 1ull);
+      uint64_t t;
+      if constexpr (XF == kXfDrop) {
+        t = ((k >> (shift + kRadixBits)) << shift) | low;
+      } else {
+        const uint64_t pd = valid ? digit_at(pstart, prange[0], prange[1], tile_base + pos) : 0u;
+        t = ((k >> (shift + kRadixBits)) << (shift + kRadixBits)) | (pd << shift) | low;
+      }
+      key[i] = ((uint64_t)d << 32) | (uint32_t)t;
+    }
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < kRadixBits; ++b) {
@@ -300,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
   for (int i = 0; i < kItems; ++i) {
     uint32_t pos = wave_base + (uint32_t)i * kWave + lane;
     if (pos < tile_n) {
-      uint32_t d = digit_of(key[i], shift);
+      uint32_t d = XF == kXfNone ? digit_of(key[i], shift) : (uint32_t)((uint64_t)key[i] >> 32);
       lpos[i] = dstart[d] + wcnt[wid][d] + rank[i];
       stage[lpos[i]] = (uint64_t)key[i];
     }
@@ -310,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
   // of consecutive global positions
   for (uint32_t j = tid; j < tile_n; j += kBlock) {
     uint64_t k = stage[j];
-    uint32_t d = digit_of(k, shift);
+    uint32_t d = XF == kXfNone ? digit_of(k, shift) : (uint32_t)(k >> 32);
     uint32_t out = gbase[d] + (j - dstart[d]);
     if (LAST) {
       col_out[out] = (uint32_t)(k & km.cmask);
@@ -370,6 +417,37 @@ __global__ __launch_bounds__(kBlock) void k_seg_bounds(const K* __restrict__ key
     }
     seg_start[g] = (uint32_t)lo;
   }
+}
+
+// The same bounds after phase-1 passes with key transforms: the keys are
+// grouped by the last pass's digit (the segment id's high digit), whose
+// ranges are hi_starts[0..256], and within a range ordered by the low digit
+// held at bits [shift, shift + 8) of the key. Segment g = (hi << 8) | lo.
+__global__ __launch_bounds__(kBlock) void k_seg_bounds_split(const uint32_t* __restrict__ keys, uint64_t nnz,
+                                                             int shift, uint32_t G,
+                                                             const uint32_t* __restrict__ hi_starts,
+                                                             uint32_t* __restrict__ seg_start) {
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += (uint64_t)gridDim.x * blockDim.x) {
+    if (g == G) {
+      seg_start[g] = (uint32_t)nnz;
+      continue;
+    }
+    const uint32_t hi = (uint32_t)(g >> kRadixBits), lo_d = (uint32_t)(g & (kRadix - 1));
+    uint64_t lo = hi_starts[hi], top = hi_starts[hi + 1];
+    while (lo < top) {
+      const uint64_t mid = lo + ((top - lo) >> 1);
+      if (digit_of(keys[mid], shift) < lo_d) lo = mid + 1;
+      else top = mid;
+    }
+    seg_start[g] = (uint32_t)lo;
+  }
+}
+
+// Digit starts of a pass (the scanned offsets of tile 0) and nnz as entry 256.
+__global__ __launch_bounds__(kRadix) void k_digit_starts(const uint32_t* __restrict__ offsets, uint64_t nnz,
+                                                         uint32_t* __restrict__ starts) {
+  starts[threadIdx.x] = offsets[threadIdx.x];
+  if (threadIdx.x == 0) starts[kRadix] = (uint32_t)nnz;
 }
 
 // One workgroup per segment (grid-stride over segments). FIRST: the input is

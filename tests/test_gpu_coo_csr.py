@@ -5,6 +5,7 @@ import pytest
 from gpu_util import build_csr, say, t_f32, t_keys
 from mpsparse_amd.problems import block_sparse, laplacian_2d, pack_keys
 from oracle import coo_csr as oc
+from oracle import spmv as S
 
 pytestmark = pytest.mark.gpu
 
@@ -50,6 +51,8 @@ CASES = [
     (33, 100000, 4000, None), (5000, 5000, 1000, None), (2, 17, 300, None), (70000, 3, 2000, None),
     (4097, 4097, 4096, None), (4096, 4096, 4097, None), (1 << 16, 1 << 16, 1 << 20, None),
     (1 << 20, 1 << 20, 3 << 20, 64), (300000, 2 ** 31 + 5, 200000, None), (9, 2 ** 32 - 1, 100000, None),
+    # 33..40-bit keys with two segment passes: the keys travel as 32 bits
+    (1 << 18, 1 << 18, 2_600_000, None), (200_000, 1 << 21, 1_500_000, None),
 ]
 
 
@@ -81,6 +84,35 @@ def test_block_sparse_sixteenth_size():
     say("block_sparse 4096 block rows", q.nnz)
     A, rp, ci, v = build_csr(q.keys, q.vals, q.num_rows, q.num_cols)
     _same((rp, ci, v), oc.coo_to_csr(q.keys, q.vals, q.num_rows, q.num_cols))
+
+
+def test_32bit_key_path_duplicates_and_long_row():
+    """38-bit keys (n = 2^19 rows, m = 2^19 columns, ~3 entries per row: two
+    segment passes), with 60,000 repeated keys and one 2,000-entry row, built
+    through csr_tensor (the longest row is read back with the sort's flags):
+    the exported CSR is bit-exact against the stable-sort oracle and y = A x is
+    within 2 gamma_k (|A||x|) of the FP64 product."""
+    import torch
+
+    rng = np.random.default_rng(38)
+    n = m = 1 << 19
+    rows = rng.integers(0, n, 1_500_000)
+    cols = rng.integers(0, m, 1_500_000)
+    rows = np.concatenate([rows, np.full(2000, 123_457), rows[:60_000]])
+    cols = np.concatenate([cols, rng.integers(0, m, 2000), cols[:60_000]])
+    perm = rng.permutation(rows.size)
+    keys = pack_keys(rows[perm], cols[perm])
+    vals = rng.standard_normal(keys.size).astype(np.float32)
+    say("32-bit key path", keys.size)
+    A, rp, ci, v = build_csr(keys, vals, n, m)
+    ref = oc.coo_to_csr(keys, vals, n, m)
+    _same((rp, ci, v), ref)
+    x = rng.standard_normal(m).astype(np.float32)
+    y = torch.empty(n, device="cuda")
+    A.mv(torch.from_numpy(x).cuda(), y)
+    y64 = S.csr_spmv(ref[0], ref[1], ref[2], x)
+    bound = 2 * S.spmv_error_bound(ref[0], ref[1], ref[2], x) + 1e-30
+    assert np.all(np.abs(y.cpu().numpy().astype(np.float64) - y64) <= bound)
 
 
 def test_key_out_of_range_raises():

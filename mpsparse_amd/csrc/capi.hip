@@ -719,9 +719,10 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
   const uint32_t num_tiles = (uint32_t)((nnz + kTile - 1) / kTile);
   const uint64_t N = (uint64_t)kRadix * num_tiles;
   const uint32_t nchunks = (num_tiles + kColChunk - 1) / kColChunk;
-  TmpMem counts, sums, ka, kb, va, vb;
+  TmpMem counts, sums, dst, ka, kb, va, vb;
   MSPCHK(counts.alloc(sizeof(uint32_t) * N, s));
   MSPCHK(sums.alloc(sizeof(uint32_t) * kRadix * nchunks, s));
+  MSPCHK(dst.alloc(sizeof(uint32_t) * (kRadix + 1), s));
   if (passes > 1) {
     MSPCHK(ka.alloc(sizeof(K) * nnz, s));
     MSPCHK(va.alloc(sizeof(float) * nnz, s));
@@ -741,8 +742,9 @@ static msp_status sort_passes(const uint64_t* keys, const float* vals, uint64_t 
     else k_upsweep<K, false><<<num_tiles, kBlock, 0, s>>>(kin, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
     MSPCHK(launch_check("k_upsweep"));
     k_colscan_reduce<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
-    k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks);
-    k_colscan_down<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
+    k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks, dst.as<uint32_t>(), nnz);
+    k_colscan_down<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>(),
+                                              dst.as<uint32_t>());
     MSPCHK(launch_check("k_colscan"));
     K* kout = kbuf[p & 1];
     float* vout = vbuf[p & 1];
@@ -775,7 +777,7 @@ static msp_status segment_sort(const uint64_t* keys, const float* vals, uint64_t
   using namespace sortk;
   const uint32_t G = (uint32_t)(((uint64_t)n + (1ull << sb) - 1) >> sb);
   const int shift0 = (int)km.cbits + sb;
-  TmpMem bounds, counts, sums, ka, kb, va, vb;
+  TmpMem bounds, counts, sums, dst, ka, kb, va, vb;
   const K* kres = nullptr;
   const float* vres = vals;
   if (passes > 0) {
@@ -783,6 +785,7 @@ static msp_status segment_sort(const uint64_t* keys, const float* vals, uint64_t
     const uint32_t nchunks = (num_tiles + kColChunk - 1) / kColChunk;
     MSPCHK(counts.alloc(sizeof(uint32_t) * (uint64_t)kRadix * num_tiles, s));
     MSPCHK(sums.alloc(sizeof(uint32_t) * kRadix * nchunks, s));
+    MSPCHK(dst.alloc(sizeof(uint32_t) * (kRadix + 1), s));
     MSPCHK(ka.alloc(sizeof(K) * nnz, s));
     MSPCHK(va.alloc(sizeof(float) * nnz, s));
     if (passes > 1) {
@@ -798,8 +801,9 @@ static msp_status segment_sort(const uint64_t* keys, const float* vals, uint64_t
       else
         k_upsweep<K, false><<<num_tiles, kBlock, 0, s>>>(kres, nnz, km, shift, num_tiles, counts.as<uint32_t>(), err);
       k_colscan_reduce<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
-      k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks);
-      k_colscan_down<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>());
+      k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks, dst.as<uint32_t>(), nnz);
+      k_colscan_down<<<nchunks, kRadix, 0, s>>>(counts.as<uint32_t>(), num_tiles, sums.as<uint32_t>(),
+                                                dst.as<uint32_t>());
       if (p == 0)
         k_downsweep<uint64_t, K, true, false><<<num_tiles, kBlock, 0, s>>>(keys, vres, kbuf[p & 1], vbuf[p & 1], nullptr,
                                                                            nullptr, nnz, km, shift, num_tiles,
@@ -853,21 +857,20 @@ static msp_status segment_sort_split(const uint64_t* keys, const float* vals, ui
   MSPCHK(st1.alloc(sizeof(uint32_t) * (kRadix + 1), s));
   MSPCHK(st2.alloc(sizeof(uint32_t) * (kRadix + 1), s));
   uint32_t* cnt = counts.as<uint32_t>();
-  auto colscan = [&] {
+  // the scanned offsets, and the pass's digit starts into `st`
+  auto colscan = [&](uint32_t* st) {
     k_colscan_reduce<<<nchunks, kRadix, 0, s>>>(cnt, num_tiles, sums.as<uint32_t>());
-    k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks);
-    k_colscan_down<<<nchunks, kRadix, 0, s>>>(cnt, num_tiles, sums.as<uint32_t>());
+    k_colscan_top<<<1, kRadix, 0, s>>>(sums.as<uint32_t>(), nchunks, st, nnz);
+    k_colscan_down<<<nchunks, kRadix, 0, s>>>(cnt, num_tiles, sums.as<uint32_t>(), st);
   };
   // pass 1: the segment id's low digit (compacted keys, bits [shift, shift + 8))
   k_upsweep<uint64_t, true><<<num_tiles, kBlock, 0, s>>>(keys, nnz, km, shift, num_tiles, cnt, err);
-  colscan();
-  k_digit_starts<<<1, kRadix, 0, s>>>(cnt, nnz, st1.as<uint32_t>());
+  colscan(st1.as<uint32_t>());
   k_downsweep<uint64_t, uint32_t, true, false, kXfDrop><<<num_tiles, kBlock, 0, s>>>(
       keys, vals, ka.as<uint32_t>(), va.as<float>(), nullptr, nullptr, nnz, km, shift, num_tiles, cnt);
   // pass 2: the high digit, now at the same shift
   k_upsweep<uint32_t, false><<<num_tiles, kBlock, 0, s>>>(ka.as<uint32_t>(), nnz, km, shift, num_tiles, cnt, err);
-  colscan();
-  k_digit_starts<<<1, kRadix, 0, s>>>(cnt, nnz, st2.as<uint32_t>());
+  colscan(st2.as<uint32_t>());
   k_downsweep<uint32_t, uint32_t, false, false, kXfSwap><<<num_tiles, kBlock, 0, s>>>(
       ka.as<uint32_t>(), va.as<float>(), kb.as<uint32_t>(), vb.as<float>(), nullptr, nullptr, nnz, km, shift,
       num_tiles, cnt, st1.as<uint32_t>());

@@ -95,7 +95,11 @@ __global__ __launch_bounds__(kRadix) void k_colscan_reduce(const uint32_t* __res
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl);
 
-__global__ __launch_bounds__(kRadix) void k_colscan_top(uint32_t* __restrict__ sums, uint32_t nchunks) {
+// sums[c][d] <- sum of the chunk sums of digit d before chunk c; the digit
+// starts (totals scanned over the digits) go to dstart[0..255] and nnz to
+// dstart[256], for k_colscan_down to add and for passes that need them.
+__global__ __launch_bounds__(kRadix) void k_colscan_top(uint32_t* __restrict__ sums, uint32_t nchunks,
+                                                        uint32_t* __restrict__ dstart, uint64_t nnz) {
   const uint32_t d = threadIdx.x;
   uint32_t run = 0;
   for (uint32_t c0 = 0; c0 < nchunks; c0 += 16) {  // 16 loads in flight per pass
@@ -110,14 +114,16 @@ __global__ __launch_bounds__(kRadix) void k_colscan_top(uint32_t* __restrict__ s
   }
   uint32_t start;
   block_exclusive_scan(run, start);  // digit starts: totals scanned over the digits
-  for (uint32_t c = 0; c < nchunks; ++c) sums[(uint64_t)c * kRadix + d] += start;
+  dstart[d] = start;
+  if (d == 0) dstart[kRadix] = (uint32_t)nnz;
 }
 
 __global__ __launch_bounds__(kRadix) void k_colscan_down(uint32_t* __restrict__ counts, uint32_t num_tiles,
-                                                         const uint32_t* __restrict__ sums) {
+                                                         const uint32_t* __restrict__ sums,
+                                                         const uint32_t* __restrict__ dstart) {
   const uint32_t d = threadIdx.x, t0 = blockIdx.x * kColChunk;
   const uint32_t t1 = t0 + kColChunk < num_tiles ? t0 + kColChunk : num_tiles;
-  uint32_t run = sums[(uint64_t)blockIdx.x * kRadix + d];
+  uint32_t run = sums[(uint64_t)blockIdx.x * kRadix + d] + dstart[d];
   for (uint32_t tb = t0; tb < t1; tb += 16) {
     uint32_t v[16];
 #pragma unroll
@@ -441,13 +447,6 @@ __global__ __launch_bounds__(kBlock) void k_seg_bounds_split(const uint32_t* __r
     }
     seg_start[g] = (uint32_t)lo;
   }
-}
-
-// Digit starts of a pass (the scanned offsets of tile 0) and nnz as entry 256.
-__global__ __launch_bounds__(kRadix) void k_digit_starts(const uint32_t* __restrict__ offsets, uint64_t nnz,
-                                                         uint32_t* __restrict__ starts) {
-  starts[threadIdx.x] = offsets[threadIdx.x];
-  if (threadIdx.x == 0) starts[kRadix] = (uint32_t)nnz;
 }
 
 // One workgroup per segment (grid-stride over segments). FIRST: the input is

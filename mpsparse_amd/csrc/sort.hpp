@@ -47,6 +47,23 @@ __device__ __forceinline__ uint32_t digit_of(K k, int shift) {
   return (uint32_t)(k >> shift) & (kRadix - 1);
 }
 
+// The valid lanes of the wave whose digit equals this lane's: one ballot per
+// digit bit, each XORed with this lane's bit spread over a word (0 or ~0);
+// any set bit of the OR of those is a lane that differs somewhere. Kept in
+// 32-bit halves so that each step is one three-input bitwise operation.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+  uint32_t lo = 0, hi = 0;
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);  // bit b as 0 / ~0
+    const uint64_t bb = __builtin_amdgcn_uicmp(m, 0u, 33);  // lanes with m != 0 (ICMP_NE)
+    lo |= (uint32_t)bb ^ m;
+    hi |= (uint32_t)(bb >> 32) ^ m;
+  }
+  const uint64_t v = __ballot(valid);
+  return v & ~(((uint64_t)hi << 32) | lo);
+}
+
 // Tile digit histogram -> counts[tile * kRadix + d] (tile-major: one
 // coalesced 1 KiB row per tile).
 template <typename InK, bool FIRST>
@@ -314,12 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
       }
       key[i] = ((uint64_t)d << 32) | (uint32_t)t;
     }
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
-      uint64_t bb = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? bb : ~bb;
-    }
+    const uint64_t peers = match_digit(d, valid);
     uint32_t below = (uint32_t)__popcll(peers & lane_lt);
     uint32_t before = valid ? wcnt[wid][d] : 0u;
     rank[i] = before + below;
@@ -516,12 +528,7 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const K* __restric
           const uint32_t j = wave_base + (uint32_t)i * kWave + lane;
           const bool valid = j < cnt;
           const uint32_t d = digit_of(key[i], shift);
-          uint64_t peers = __ballot(valid);
-#pragma unroll
-          for (int b = 0; b < kRadixBits; ++b) {
-            const uint64_t bb = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bb : ~bb;
-          }
+          const uint64_t peers = match_digit(d, valid);
           const uint32_t below = (uint32_t)__popcll(peers & lane_lt);
           const uint32_t before = valid ? wcnt[wid][d] : 0u;
           rank[i] = before + below;

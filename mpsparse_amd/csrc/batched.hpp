@@ -278,7 +278,10 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmm(View V, const float* __restr
 // as 0 and contribute nothing) are all issued before the arithmetic, one
 // memory round trip per row instead of three (row pointers -> columns ->
 // gathers). NDIAG > 0: exactly that many diagonals (5: 2D stencils); 0: kBD.
-template <int RPT, int NDIAG>
+// MID: every system's middle diagonal is the main one (offset 0), whose
+// operands are the row's own (no second load). The planes are read once per
+// product: nontemporal loads.
+template <int RPT, int NDIAG, bool MID = false>
 __global__ __launch_bounds__(kBlock) void k_bt_spmm_dia(View V, const float* __restrict__ r,
                                                         const float* __restrict__ pold, float* __restrict__ pnew,
                                                         float* __restrict__ q, const PairState* __restrict__ ps,
@@ -312,12 +315,18 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmm_dia(View V, const float* __r
         pi0[u] = ld4(pold, i);
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
+          vv[u][d] = __builtin_nontemporal_load(V.planes + (uint64_t)d * V.ld + i);
+          if (MID && d == ND / 2) continue;  // the row's own operands
           int64_t j = (int64_t)i + off[d];
           j = j < lo ? lo : (j >= hi ? hi - 1 : j);
-          vv[u][d] = V.planes[(uint64_t)d * V.ld + i];
           dd[u][d] = V.dinv[j];
           rr[u][d] = ld4(r, (uint32_t)j);
           pp[u][d] = ld4(pold, (uint32_t)j);
+        }
+        if constexpr (MID) {
+          dd[u][ND / 2] = di[u];
+          rr[u][ND / 2] = ri[u];
+          pp[u][ND / 2] = pi0[u];
         }
       }
 #pragma unroll

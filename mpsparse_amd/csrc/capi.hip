@@ -2964,6 +2964,7 @@ struct msp_batch_s {
   // batched diagonal storage (every system on at most batch::kBD diagonals)
   DevMem bd_planes, bd_offs, bd_nds;
   int bd_nd = 0;  // 0: CSR product; else the planes stored (5 when every system has exactly 5)
+  bool bd_mid = false;  // 5 planes and every system's middle offset is 0 (the main diagonal)
   int32_t* h_active = nullptr;  // pinned, 2 * nsys
   hipStream_t cs = nullptr;     // capture stream
   hipEvent_t ev[2] = {nullptr, nullptr}, join = nullptr, t0 = nullptr, t1 = nullptr;
@@ -3050,7 +3051,12 @@ static void launch_spmm(msp_batch B, int rpt, int parity, hipStream_t s) {
   float* pnew = B->v(parity ? kBP0 : kBP1);
   PairState* ps = B->state.as<PairState>();
   CSum* pt = B->partials.as<CSum>();
-  if (B->bd_nd == 5) {  // 2D 5-point systems
+  if (B->bd_nd == 5 && B->bd_mid) {  // 2D 5-point systems
+    if (rpt == 2) batch::k_bt_spmm_dia<2, 5, true><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    else batch::k_bt_spmm_dia<1, 5, true><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    return;
+  }
+  if (B->bd_nd == 5) {
     if (rpt == 2) batch::k_bt_spmm_dia<2, 5><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
     else batch::k_bt_spmm_dia<1, 5><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
     return;
@@ -3156,6 +3162,13 @@ static msp_status batch_diagonal_storage(msp_batch B, hipStream_t s) {
   bool all5 = true;
   for (int32_t c : nds) all5 = all5 && c == 5;
   const int planes = all5 ? 5 : batch::kBD;
+  if (all5) {
+    std::vector<int32_t> offs((size_t)batch::kBD * nsys);
+    HIPCHK(hipMemcpy(offs.data(), B->bd_offs.p, sizeof(int32_t) * offs.size(), hipMemcpyDeviceToHost));
+    bool mid = true;
+    for (uint32_t q = 0; q < nsys && mid; ++q) mid = offs[(size_t)q * batch::kBD + 2] == 0;
+    B->bd_mid = mid;
+  }
   const size_t bytes = sizeof(float) * (size_t)planes * B->n;
   MSPCHK(B->bd_planes.alloc(bytes));
   HIPCHK(hipMemsetAsync(B->bd_planes.p, 0, bytes, s));

@@ -578,6 +578,7 @@ def batched_section(dev, num_systems=512, lo=32, hi=512, sigma=1.5, seed=0):
            "max_iterations": info["max_iterations"], "median_iterations": float(np.median(its)),
            "graph_launches": info["graph_launches"], "tile_rows": info["tile_rows"],
            "rows_per_thread": info["rows_per_thread"], "verifications": info["verifications"],
+           "diagonal_planes": info["diagonal_planes"], "symmetric_storage": info["symmetric_storage"],
            "max_true_relres_fp64": float(rel.max()),
            "row_iterations_per_s": float((pair_rows * its).sum() / min(times)),
            "setup": {"generate_s": t_gen, "coo_to_csr_s": t_csr, "first_solve_s_incl_tuning": t_first}}

@@ -150,6 +150,9 @@ typedef struct msp_batch_info {
   int32_t diagonal_planes;  /* product over batched diagonal storage with this many
                                value planes (every system on at most 8 diagonals);
                                0: the CSR product */
+  int32_t symmetric_storage; /* 1: every system has 5 diagonals (-b, -a, 0, a, b) and
+                                exactly symmetric values, so the product reads the lower
+                                diagonals from the upper planes (3 planes streamed) */
   double seconds;          /* device time of the solve (events) */
   double setup_seconds;    /* host time of staging and tuning */
 } msp_batch_info;

@@ -279,9 +279,12 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmm(View V, const float* __restr
 // memory round trip per row instead of three (row pointers -> columns ->
 // gathers). NDIAG > 0: exactly that many diagonals (5: 2D stencils); 0: kBD.
 // MID: every system's middle diagonal is the main one (offset 0), whose
-// operands are the row's own (no second load). The planes are read once per
-// product: nontemporal loads.
-template <int RPT, int NDIAG, bool MID = false>
+// operands are the row's own (no second load). MIRROR (with MID): every
+// system is exactly symmetric, so a lower diagonal -k reads the upper plane
+// +k at row i - k (0 above the system's first row): 3 planes stream instead of
+// 5, and the two upper planes' second reads come from cache (they stay plain
+// loads); planes read once are nontemporal.
+template <int RPT, int NDIAG, bool MID = false, bool MIRROR = false>
 __global__ __launch_bounds__(kBlock) void k_bt_spmm_dia(View V, const float* __restrict__ r,
                                                         const float* __restrict__ pold, float* __restrict__ pnew,
                                                         float* __restrict__ q, const PairState* __restrict__ ps,
@@ -315,7 +318,19 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmm_dia(View V, const float* __r
         pi0[u] = ld4(pold, i);
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-          vv[u][d] = __builtin_nontemporal_load(V.planes + (uint64_t)d * V.ld + i);
+          if constexpr (MIRROR) {
+            if (d < ND / 2) {
+              const int64_t m = (int64_t)i + off[d];
+              const float v = V.planes[(uint64_t)(ND - 1 - d) * V.ld + (uint64_t)(m < lo ? lo : m)];
+              vv[u][d] = m < lo ? 0.0f : v;
+            } else if (d == ND / 2) {
+              vv[u][d] = __builtin_nontemporal_load(V.planes + (uint64_t)d * V.ld + i);
+            } else {
+              vv[u][d] = V.planes[(uint64_t)d * V.ld + i];
+            }
+          } else {
+            vv[u][d] = __builtin_nontemporal_load(V.planes + (uint64_t)d * V.ld + i);
+          }
           if (MID && d == ND / 2) continue;  // the row's own operands
           int64_t j = (int64_t)i + off[d];
           j = j < lo ? lo : (j >= hi ? hi - 1 : j);
@@ -402,6 +417,31 @@ __global__ __launch_bounds__(kBlock) void k_bd_offsets(const uint32_t* __restric
     }
     __syncthreads();
   }
+}
+
+// Symmetry of the 5-plane batched storage (offsets -b, -a, 0, a, b per
+// system): rows whose lower value differs bitwise from the partner's upper
+// value k rows back (or is not 0 where that row lies before the system).
+__global__ __launch_bounds__(kBlock) void k_bd_symcheck(const float* __restrict__ planes, uint64_t ld,
+                                                        const uint32_t* __restrict__ sys_off, uint32_t nsys,
+                                                        const int32_t* __restrict__ offs,
+                                                        uint32_t* __restrict__ misses) {
+  uint32_t miss = 0;
+  for (uint32_t s = blockIdx.x; s < nsys; s += gridDim.x) {
+    const uint32_t lo = sys_off[s], hi = sys_off[s + 1];
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const int64_t m = (int64_t)i + offs[(size_t)s * kBD + d];
+        const float a = planes[(uint64_t)d * ld + i];
+        const float b = m < (int64_t)lo ? 0.0f : planes[(uint64_t)(4 - d) * ld + (uint64_t)m];
+        miss += __float_as_uint(a) != __float_as_uint(b) ? 1u : 0u;
+      }
+    }
+  }
+#pragma unroll
+  for (int sh = kWave / 2; sh > 0; sh >>= 1) miss += __shfl_xor(miss, sh, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && miss) atomicAdd(misses, miss);
 }
 
 // Values into the batched diagonal storage (planes zero-filled by the

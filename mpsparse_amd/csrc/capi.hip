@@ -2965,6 +2965,7 @@ struct msp_batch_s {
   DevMem bd_planes, bd_offs, bd_nds;
   int bd_nd = 0;  // 0: CSR product; else the planes stored (5 when every system has exactly 5)
   bool bd_mid = false;  // 5 planes and every system's middle offset is 0 (the main diagonal)
+  bool bd_mirror = false;  // bd_mid and every system exactly symmetric: lower diagonals read the upper planes
   int32_t* h_active = nullptr;  // pinned, 2 * nsys
   hipStream_t cs = nullptr;     // capture stream
   hipEvent_t ev[2] = {nullptr, nullptr}, join = nullptr, t0 = nullptr, t1 = nullptr;
@@ -3051,6 +3052,13 @@ static void launch_spmm(msp_batch B, int rpt, int parity, hipStream_t s) {
   float* pnew = B->v(parity ? kBP0 : kBP1);
   PairState* ps = B->state.as<PairState>();
   CSum* pt = B->partials.as<CSum>();
+  if (B->bd_nd == 5 && B->bd_mirror) {  // symmetric 2D 5-point systems
+    if (rpt == 2)
+      batch::k_bt_spmm_dia<2, 5, true, true><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    else
+      batch::k_bt_spmm_dia<1, 5, true, true><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
+    return;
+  }
   if (B->bd_nd == 5 && B->bd_mid) {  // 2D 5-point systems
     if (rpt == 2) batch::k_bt_spmm_dia<2, 5, true><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
     else batch::k_bt_spmm_dia<1, 5, true><<<g, kBlock, 0, s>>>(V, B->v(kBR), pold, pnew, B->v(kBQ), ps, pt);
@@ -3177,6 +3185,24 @@ static msp_status batch_diagonal_storage(msp_batch B, hipStream_t s) {
                                         B->bd_planes.as<float>());
   MSPCHK(launch_check("k_bd_fill"));
   B->bd_nd = planes;
+  if (B->bd_mid) {  // symmetric pattern (-b, -a, 0, a, b) in every system, then the values
+    std::vector<int32_t> offs((size_t)batch::kBD * nsys);
+    HIPCHK(hipMemcpy(offs.data(), B->bd_offs.p, sizeof(int32_t) * offs.size(), hipMemcpyDeviceToHost));
+    bool pat = true;
+    for (uint32_t q = 0; q < nsys && pat; ++q) {
+      const int32_t* o = &offs[(size_t)q * batch::kBD];
+      pat = o[0] == -o[4] && o[1] == -o[3] && o[3] > 0;
+    }
+    if (pat) {
+      HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s));
+      batch::k_bd_symcheck<<<g, kBlock, 0, s>>>(B->bd_planes.as<float>(), B->n, B->sys_off.as<uint32_t>(), nsys,
+                                                B->bd_offs.as<int32_t>(), bad.as<uint32_t>());
+      MSPCHK(launch_check("k_bd_symcheck"));
+      HIPCHK(hipMemcpyAsync(&hb, bad.p, sizeof hb, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      B->bd_mirror = hb == 0;
+    }
+  }
   return MSP_OK;
 }
 
@@ -3432,6 +3458,7 @@ static msp_status batch_pcg_impl(msp_batch B, const float* b, void* x, bool x64,
   inf.verifications = verifications;
   inf.tile_rows = (int32_t)B->T;
   inf.diagonal_planes = B->bd_nd;
+  inf.symmetric_storage = B->bd_mirror ? 1 : 0;
   inf.rows_per_thread = B->rpt;
   inf.seconds = ms * 1e-3;
   inf.setup_seconds = std::chrono::duration<double>(wall1 - wall0).count();

@@ -590,6 +590,7 @@ class BatchTensor {
     d["tile_rows"] = info.tile_rows;
     d["rows_per_thread"] = info.rows_per_thread;
     d["diagonal_planes"] = info.diagonal_planes;
+    d["symmetric_storage"] = info.symmetric_storage;
     d["seconds"] = info.seconds;
     d["setup_seconds"] = info.setup_seconds;
     d["status"] = st.view({(int64_t)nsys_, nrhs});

@@ -213,4 +213,31 @@ def test_batched_stencil_batch_uses_five_planes():
     say("batched five planes", sides.tolist())
     info = B.pcg(torch.from_numpy(b).cuda(), x)
     assert info["diagonal_planes"] == 5
+    assert info["symmetric_storage"] == 1  # harmonic-mean faces: A(i,j) == A(j,i) bit for bit
     _check_pairs(S, off, b, x.cpu().numpy(), info, 1e-6, 4)
+
+
+def test_batched_nonsymmetric_values_keep_five_planes():
+    """The same kind of batch with one coupling of the last system changed
+    (one side of it only): the product reads all five planes, and every pair
+    still meets the FP64 check (Jacobi-PCG on the slightly nonsymmetric
+    system converges here; its iteration bounds are not checked)."""
+    import torch
+
+    from gpu_util import csr_from_arrays
+    from mpsparse_amd import batched
+
+    A0, S0, off, sides = _build(3, 10, 30, seed=12)
+    S = S0.tocsr().astype(np.float32)
+    n = int(off[-1])
+    row = n - 2
+    e = S.indptr[row]  # the first (lower) coupling of that row
+    S.data[e] *= 1.0 + 2.0 ** -10
+    A = csr_from_arrays(S.indptr.astype(np.uint32), S.indices.astype(np.uint32), S.data, n, n)
+    B = batched.batch_tensor(A, torch.from_numpy(off))
+    b = np.random.default_rng(13).standard_normal((n, 4)).astype(np.float32)
+    x = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    say("batched nonsymmetric", sides.tolist())
+    info = B.pcg(torch.from_numpy(b).cuda(), x)
+    assert info["diagonal_planes"] == 5 and info["symmetric_storage"] == 0
+    _check_pairs(S.astype(np.float64), off, b, x.cpu().numpy(), info, 1e-6, 4, its_check=False)

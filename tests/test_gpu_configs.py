@@ -228,6 +228,7 @@ def test_batched_512_systems_all_pairs():
     say("batched 512: solve", n, nnz)
     info = B.pcg(b, x)
     assert info["converged_pairs"] == info["total_pairs"] == 2048, info
+    assert info["diagonal_planes"] == 5 and info["symmetric_storage"] == 1, info
     sys_of_row = torch.repeat_interleave(torch.arange(512, device="cuda"), torch.from_numpy(np.diff(off)).cuda())
     ax = torch.zeros((n, 4), dtype=torch.float64, device="cuda").index_add_(0, rows,
                                                                            vals.double()[:, None] * x[cols])

@@ -437,6 +437,12 @@ struct msp_csr_s : MatBase {
   DiaOffsets dia_sym_o{};
   uint64_t dia_ld = 0;
   DevMem dia, dia16, dia_dlo;
+  // a transpose kept as a view of another matrix's diagonal storage: its
+  // diagonal +k is that matrix's -k plane read k rows on (dia_alias_o holds
+  // the plane / shift of each diagonal); no value planes or CSR arrays of its
+  // own
+  const msp_csr_s* dia_alias = nullptr;
+  DiaOffsets dia_alias_o{};
 };
 
 struct msp_bcsr_s : MatBase {
@@ -599,34 +605,36 @@ static msp_status ensure_dia(msp_csr A, hipStream_t s, bool fill = true) {
   return MSP_OK;
 }
 
-// Diagonal storage of T = G^T from G's: every entry (i, j) of G lies on offset
-// j - i, so (j, i) lies on i - j; the offsets are G's negated (same count and
-// density), no probe needed. The values are set later (refresh_dia).
-static msp_status dia_as_transpose(msp_csr T, const MatBase* G) {
-  const auto* g = static_cast<const msp_csr_s*>(G);
-  if (T->dia_state != 0 || g->dia_state != 1 || T->format != MSP_FORMAT_AUTO || T->n != g->m || T->m != g->n)
-    return MSP_OK;
-  DiaOffsets o{};
+// T = G^T as a view of G's diagonal storage: T's diagonal d (offset +k,
+// ascending) reads G's plane of offset -k, k rows on (G(i + k, i) = T(i, i + k)).
+// G's main diagonal is its last plane, so every such read stays inside G's
+// planes (row i + k < n + k <= the next plane's end). T keeps no values or CSR
+// arrays of its own.
+static void dia_alias_transpose(msp_csr T, const msp_csr_s* g) {
+  DiaOffsets o{}, a{};
   dia_plain(o);
-  o.ndiag = g->dia_o.ndiag;
+  const int nd = g->dia_o.ndiag;
+  o.ndiag = a.ndiag = nd;
   int dd = -1;
-  for (int d = 0; d < o.ndiag; ++d) {
-    o.off[d] = -g->dia_o.off[o.ndiag - 1 - d];  // ascending
+  for (int d = 0; d < nd; ++d) {
+    o.off[d] = -g->dia_o.off[nd - 1 - d];  // ascending
+    a.off[d] = o.off[d];
+    a.plane[d] = nd - 1 - d;
+    a.shift[d] = o.off[d];
     if (o.off[d] == 0) dd = d;
   }
-  T->dia_state = 2;
-  T->dia_ld = ((uint64_t)T->n + 63) / 64 * 64;
-  MSPCHK(T->dia.alloc(sizeof(float) * T->dia_ld * o.ndiag));
   T->dia_o = o;
+  T->dia_alias_o = a;
   T->dia_dd = dd;
-  T->dia_state = 1;
+  T->dia_ld = g->dia_ld;
+  T->dia_alias = g;
   T->dia16_type = -1;
-  return MSP_OK;
+  T->dia_state = 1;
 }
 
 // Refill the FP32 diagonal storage after A's values changed (pattern fixed).
 static msp_status refresh_dia(msp_csr A, hipStream_t s) {
-  if (A->dia_state != 1) return MSP_OK;
+  if (A->dia_state != 1 || A->dia_alias) return MSP_OK;
   A->dia_sym = false;  // new values: symmetry no longer known
   HIPCHK(hipMemsetAsync(A->dia.p, 0, sizeof(float) * A->dia_ld * A->dia_o.ndiag, s));
   k_dia_fill<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(),
@@ -636,6 +644,7 @@ static msp_status refresh_dia(msp_csr A, hipStream_t s) {
 }
 
 static msp_status ensure_dia16(msp_csr A, int32_t vt, hipStream_t s) {
+  if (A->dia_alias) return ensure_dia16(const_cast<msp_csr>(A->dia_alias), vt, s);
   if (A->dia_state != 1 || vt == MSP_VALUES_F32 || A->dia16_type == vt) return MSP_OK;
   const size_t cnt = A->dia_ld * A->dia_o.ndiag;
   MSPCHK(A->dia16.alloc(sizeof(uint16_t) * cnt));
@@ -1325,6 +1334,21 @@ constexpr size_t kDiaMirrorMinBytes = (size_t)128 << 20;
 template <typename V>
 static DiaView<V> dia_view_of(msp_csr A) {
   DiaView<V> v;
+  if (const msp_csr_s* g = A->dia_alias) {  // a transpose read from the planes of g
+    if constexpr (std::is_same<V, ValF32>::value) {
+      v.val = g->dia.as<float>();
+      v.dlo = nullptr;
+    } else {
+      v.val = g->dia16.as<uint16_t>();
+      v.dlo = g->dia_dlo.as<float>();
+    }
+    v.ld = g->dia_ld;
+    v.n = A->n;
+    v.m = A->m;
+    v.mirrored = 1;
+    v.o = A->dia_alias_o;
+    return v;
+  }
   if constexpr (std::is_same<V, ValF32>::value) {
     v.val = A->dia.as<float>();
     v.dlo = nullptr;
@@ -1446,7 +1470,8 @@ static uint32_t dia_launch(msp_csr A, XL xl, Epi epi, GridRed rb, const int* don
 // its values of type V are prepared), the CSR row-block kernel otherwise.
 template <typename V, int ND, typename XL, typename Epi, typename Fin>
 static uint32_t spmv_any(msp_csr A, XL xl, Epi epi, Fin fin, GridRed rb, const int* done, hipStream_t s) {
-  const bool dia = A->dia_state == 1 && (V::kType == MSP_VALUES_F32 || A->dia16_type == V::kType);
+  const msp_csr_s* planes = A->dia_alias ? A->dia_alias : A;
+  const bool dia = A->dia_state == 1 && (V::kType == MSP_VALUES_F32 || planes->dia16_type == V::kType);
   uint32_t g;
   if (dia) {
     // the diagonal kernel is grid-stride over rows: a grid beyond residency
@@ -2422,6 +2447,12 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     MSPCHK(finalize_matrix(&F->G, s, A->max_row));
     // G's values are set later (fsai_commit refills the diagonal storage)
     MSPCHK(ensure_dia(&F->G, s, false));
+    if (F->G.dia_state == 1 && F->G.dia_o.off[F->G.dia_o.ndiag - 1] == 0) {
+      // G^T is read from G's planes (G's last plane is its main diagonal, which
+      // keeps every shifted read inside them): no transposed pattern, no copy
+      dia_alias_transpose(&F->GT, &F->G);
+      F->GT.max_row = (uint32_t)F->G.dia_o.ndiag;
+    } else {
     // transpose pattern by sorting (col << 32 | row) with the edge index as payload
     TmpMem keys, payload;
     MSPCHK(keys.alloc(sizeof(uint64_t) * (ne + 1), s));
@@ -2439,9 +2470,9 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     MSPCHK(zero_pad(&F->GT, s));
     // with G in diagonal storage, G^T's rows hold at most its diagonal count
     // and its storage follows from G's
-    MSPCHK(finalize_matrix(&F->GT, s, F->G.dia_state == 1 ? (int64_t)F->G.dia_o.ndiag : -1));
-    MSPCHK(dia_as_transpose(&F->GT, &F->G));
+    MSPCHK(finalize_matrix(&F->GT, s));
     MSPCHK(ensure_dia(&F->GT, s, false));
+    }
     // features and column scaling
     MSPCHK(F->diag.alloc(sizeof(float) * ((size_t)n + 1)));
     MSPCHK(F->feat.alloc(sizeof(float) * 8 * (ne + 64)));
@@ -2526,9 +2557,11 @@ extern "C" msp_status msp_fsai_get_features(msp_fsai F, int32_t* feature_set) {
 
 // Refresh G^T from G and count non-positive diagonal entries of G.
 static msp_status fsai_commit(msp_fsai F, hipStream_t s) {
-  k_gather_by_payload<<<grid_for(F->ne), kBlock, 0, s>>>(F->GT.val.as<float>(), F->G.val.as<float>(),
-                                                         F->GTmap.as<float>(), F->ne);
-  MSPCHK(launch_check("gather G^T"));
+  if (!F->GT.dia_alias) {
+    k_gather_by_payload<<<grid_for(F->ne), kBlock, 0, s>>>(F->GT.val.as<float>(), F->G.val.as<float>(),
+                                                           F->GTmap.as<float>(), F->ne);
+    MSPCHK(launch_check("gather G^T"));
+  }
   MSPCHK(refresh_dia(&F->G, s));
   MSPCHK(refresh_dia(&F->GT, s));
   F->ready = 1;

@@ -363,7 +363,7 @@ static msp_status copy_out(void* dst, const void* src, size_t bytes, hipStream_t
 }
 
 static msp_status copy_in(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  if (bytes == 0) return MSP_OK;
+  if (bytes == 0 || dst == src) return MSP_OK;
   if (!src) return fail(MSP_ERR_INVALID, "null input array");
   HIPCHK(hipMemcpyAsync(dst, src, bytes, is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
   return MSP_OK;
@@ -386,10 +386,10 @@ struct Workspace {
   Scalars* hsnap = nullptr;  // pinned: the state after each of the last two batches
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
-  static constexpr int kNumVec = 13;
-  // RH and X2 are reserved for the refinement's right-hand side and correction
-  // (no inner solver touches them)
-  enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL, RH };
+  static constexpr int kNumVec = 12;
+  // X2 is reserved for the refinement's correction (no inner solver touches
+  // it); the refinement writes each correction's right-hand side into B
+  enum { X = 0, R, P, Q, R0, S, T, Z, W, B, X2, XL };
   float* v(int i) const { return vecs.as<float>() + (size_t)i * npad; }
   Scalars* S_dev() const { return scal.as<Scalars>(); }
   // staging copy of the state for decisions taken inside a step kernel
@@ -1629,13 +1629,11 @@ static msp_status ensure_dinv(const Op& op, hipStream_t s) {
 // halts the steps until the true-residual check `verify` at the end of the
 // batch (launched only when o.verify).
 // Solver state on the host before a solve (tolerance, limits, criterion).
+// (written by a one-thread kernel: a host-to-device copy cost a copy launch
+// and its gap at the start of every correction of a refined solve)
 static msp_status start_solve(Workspace& w, const msp_solve_opts& o, hipStream_t s) {
-  std::memset(w.hS, 0, sizeof(Scalars));
-  w.hS->tol = (float)o.tol;
-  w.hS->maxiter = o.maxiter;
-  w.hS->crit = o.criterion;
-  w.hS->use_verify = o.verify ? 1 : 0;
-  HIPCHK(hipMemcpyAsync(w.S_dev(), w.hS, sizeof(Scalars), hipMemcpyHostToDevice, s));
+  k_start_solve<<<1, 1, 0, s>>>(w.S_dev(), (float)o.tol, o.maxiter, o.criterion, o.verify ? 1 : 0);
+  MSPCHK(launch_check("k_start_solve"));
   HIPCHK(hipEventRecord(w.t0, s));
   return MSP_OK;
 }
@@ -1760,7 +1758,8 @@ static bool compensate(const msp_solve_opts& o) { return o.tol < 1e-4; }
 
 template <class Op>
 static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solve_opts& o, msp_solve_info* info,
-                          hipStream_t s) {
+                          hipStream_t s,
+                                bool sync_out = true) {
   MatBase* A = op.base();
   MSPCHK(solver_prologue(A, b, x, o, s));
   if (o.precond != MSP_PRECOND_NONE && o.precond != MSP_PRECOND_JACOBI)
@@ -1877,13 +1876,15 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   MSPCHK(st);
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
-  HIPCHK(hipStreamSynchronize(s));
+  // (a caller inside the library that keeps working on s skips the wait)
+  if (sync_out) HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
 }
 
 template <class Op>
 static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const msp_solve_opts& o,
-                                msp_solve_info* info, hipStream_t s) {
+                                msp_solve_info* info, hipStream_t s,
+                                bool sync_out = true) {
   MatBase* A = op.base();
   MSPCHK(solver_prologue(A, b, x, o, s));
   if (o.precond != MSP_PRECOND_NONE) return fail(MSP_ERR_UNSUPPORTED, "BiCGSTAB takes no preconditioner");
@@ -1915,7 +1916,8 @@ static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const ms
   MSPCHK(run_solver(A, o, info, s, init, step, verify));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
-  HIPCHK(hipStreamSynchronize(s));
+  // (a caller inside the library that keeps working on s skips the wait)
+  if (sync_out) HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
 }
 
@@ -2248,7 +2250,9 @@ static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_r
                           is_device_ptr_d(x) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
   else
     k_fill64<<<gv, kBlock, 0, s>>>(X64, 0.0, n);
-  float* RHS = A->ws.v(Workspace::RH);
+  // the scaled residual goes straight into the solvers' b vector (their
+  // copy of b onto itself is skipped)
+  float* RHS = A->ws.v(Workspace::B);
   float* D = A->ws.v(Workspace::X2);
   GridRed rb = A->ws.gr();
   const RowsCsr rows{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()};
@@ -2351,7 +2355,8 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
     auto run = [&](int32_t vt) -> msp_status {
       io.values = vt;
       return with_csr_op(A, vt, s, [&](auto op) {
-        return ro.inner_solver == 0 ? cg_impl(op, rhs, d, io, &ii, s) : bicgstab_impl(op, rhs, d, io, &ii, s);
+        return ro.inner_solver == 0 ? cg_impl(op, rhs, d, io, &ii, s, false)
+                                    : bicgstab_impl(op, rhs, d, io, &ii, s, false);
       });
     };
     MSPCHK(run(ro.values));
@@ -2378,6 +2383,7 @@ struct msp_fsai_s {
   DevMem rowscale;  // stated-5 network: the row normalisation s_i of G
   int features = kFeatCoupled8;  // feature set of the edge network (FsaiFeatures)
   int ready = 0;    // values set
+  int feat_for = -1;  // the feature set whose features (and column scaling) feat holds
   int nonpos = 0;   // diagonal entries of G that are not positive
 };
 
@@ -2395,6 +2401,15 @@ static msp_status fsai_compute_features(msp_fsai F, hipStream_t s) {
         F->G.col.as<uint32_t>(), F->Lsrc.as<uint32_t>(), F->diag.as<float>(), F->feat.as<float>(),
         F->colscale.as<float>());
   return launch_check("features");
+}
+
+// The selected set's features and column scaling, computed on first use (a
+// handle that selects another set before using any costs one pass, not two).
+static msp_status fsai_features_ready(msp_fsai F, hipStream_t s) {
+  if (F->feat_for == F->features) return MSP_OK;
+  MSPCHK(fsai_compute_features(F, s));
+  F->feat_for = F->features;
+  return MSP_OK;
 }
 
 static msp_status exclusive_scan_u32(uint32_t* data, uint64_t N, hipStream_t s) {
@@ -2481,7 +2496,6 @@ extern "C" msp_status msp_fsai_create(msp_fsai* out, msp_csr A, void* stream) {
     HIPCHK(hipMemsetAsync(F->feat.as<float>() + 8 * ne, 0, sizeof(float) * 8 * 64, s));
     k_diag<<<grid_for(n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), n,
                                           F->diag.as<float>());
-    MSPCHK(fsai_compute_features(F, s));
     MSPCHK(ensure_workspace(A, s));
     MSPCHK(F->vec.alloc(sizeof(float) * 6 * A->ws.npad));
     HIPCHK(hipMemsetAsync(F->vec.p, 0, sizeof(float) * 6 * A->ws.npad, s));
@@ -2528,6 +2542,7 @@ extern "C" msp_status msp_fsai_features(msp_fsai F, float* feat, float* colscale
   if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
   HIPCHK(hipSetDevice(F->device));
   hipStream_t s = (hipStream_t)stream;
+  MSPCHK(fsai_features_ready(F, s));
   MSPCHK(copy_out(feat, F->feat.p, sizeof(float) * 8 * F->ne, s));
   MSPCHK(copy_out(colscale, F->colscale.p, sizeof(float) * F->ne, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -2544,8 +2559,7 @@ extern "C" msp_status msp_fsai_select_features(msp_fsai F, int32_t feature_set, 
   F->features = feature_set;
   if (feature_set == MSP_FSAI_FEATURES_STATED5 && !F->rowscale.p)
     MSPCHK(F->rowscale.alloc(sizeof(float) * ((size_t)F->A->n + 1)));
-  MSPCHK(fsai_compute_features(F, s));
-  HIPCHK(hipStreamSynchronize(s));
+  (void)s;  // the features follow on first use
   return MSP_OK;
 }
 
@@ -2573,6 +2587,7 @@ extern "C" msp_status msp_fsai_set_values(msp_fsai F, const float* g, int32_t* n
   HIPCHK(hipSetDevice(F->device));
   hipStream_t s = (hipStream_t)stream;
   MSPCHK(copy_in(F->G.val.p, g, sizeof(float) * F->ne, s));
+  MSPCHK(fsai_features_ready(F, s));  // the column scaling
   // count non-positive diagonal entries with the scaling kernel (its scaled
   // output goes to a throw-away buffer; colscale > 0 keeps the signs)
   TmpMem cnt, scaled;
@@ -2596,6 +2611,7 @@ extern "C" msp_status msp_fsai_set_network_output(msp_fsai F, const float* out, 
   if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
   HIPCHK(hipSetDevice(F->device));
   hipStream_t s = (hipStream_t)stream;
+  MSPCHK(fsai_features_ready(F, s));  // the column (and row) scaling
   InArr o;
   MSPCHK(o.set(out, sizeof(float) * F->ne, s));
   TmpMem cnt;
@@ -2732,7 +2748,8 @@ extern "C" msp_status msp_fsai_probe_loss_grad(msp_fsai F, const float* W, int32
 //   x += alpha p, r -= alpha q, <r,r> -> convergence; [true residual check].
 template <typename V>
 static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, float* x, const msp_solve_opts& o,
-                                msp_solve_info* info, hipStream_t s) {
+                                msp_solve_info* info, hipStream_t s,
+                                bool sync_out = true) {
   MatBase* A = op.base();
   MSPCHK(solver_prologue(A, b, x, o, s));
   // With 16-bit values of A, G's values are BF16 too (each rounded on its
@@ -2810,7 +2827,8 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   MSPCHK(run_solver(A, o, info, s, init, step, verify, flush));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
-  HIPCHK(hipStreamSynchronize(s));
+  // (a caller inside the library that keeps working on s skips the wait)
+  if (sync_out) HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
 }
 
@@ -2858,12 +2876,12 @@ extern "C" msp_status msp_csr_solve_refined_fsai(msp_csr A, msp_fsai F, const do
     io.tol = itol;
     const bool use_fsai = F->nonpos == 0;
     if (use_fsai)
-      MSPCHK(with_csr_op(A, io.values, s, [&](auto op) { return pcg_fsai_impl(op, F, rhs, d, io, &ii, s); }));
+      MSPCHK(with_csr_op(A, io.values, s, [&](auto op) { return pcg_fsai_impl(op, F, rhs, d, io, &ii, s, false); }));
     inf->inner_iterations += ii.iterations;
     const bool reached = ii.status == MSP_SOLVE_CONVERGED;
     if (!use_fsai || !reached) {
       if (use_fsai) inf->fallbacks += 1;
-      MSPCHK(with_csr_op(A, io.values, s, [&](auto op) { return cg_impl(op, rhs, d, io, &ii, s); }));
+      MSPCHK(with_csr_op(A, io.values, s, [&](auto op) { return cg_impl(op, rhs, d, io, &ii, s, false); }));
       inf->inner_iterations += ii.iterations;
     }
     *stop = ii.status == MSP_SOLVE_BREAKDOWN && ii.iterations == 0;
@@ -2944,6 +2962,7 @@ extern "C" msp_status msp_fsai_apply_network(msp_fsai F, const float* params, in
   if (!F) return fail(MSP_ERR_INVALID, "null FSAI handle");
   HIPCHK(hipSetDevice(F->device));
   TmpMem out;
+  MSPCHK(fsai_features_ready(F, (hipStream_t)stream));
   MSPCHK(out.alloc(sizeof(float) * (F->ne + 1), (hipStream_t)stream));
   MSPCHK(msp_mlp_forward(params, F->feat.as<float>(), F->ne, out.as<float>(), stream));
   return msp_fsai_set_network_output(F, out.as<float>(), nonpositive_diag, stream);

@@ -306,6 +306,8 @@ __device__ __forceinline__ float rownorm_v(const uint32_t* __restrict__ row_ptr,
   return v;
 }
 
+constexpr int kRowNormRegs = 8;  // rows of G with at most this many entries are normalised from registers
+
 // Forward: G = s_i g~ (s_i = N_i^{-1/2}; s_i = 1 when N_i is not positive and
 // finite, counted in nonpos together with non-positive diagonal entries);
 // rowscale[i] = s_i.
@@ -319,9 +321,58 @@ __global__ __launch_bounds__(kBlock) void k_fsai_rownorm(const uint32_t* __restr
                                                          float* __restrict__ rowscale, uint32_t* __restrict__ nonpos) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t o = Lrp[i], k = Lrp[i + 1] - o;
-    for (uint32_t p = 0; p < k; ++p) G[o + p] = out[o + p] * colscale[o + p];  // g~, scaled below
     float N = 0.0f;
-    for (uint32_t p = 0; p < k; ++p) N += G[o + p] * rownorm_v(row_ptr, col, val, Lcol, o, k, Lcol[o + p], G, 1.0f);
+    if (k <= (uint32_t)kRowNormRegs) {
+      // the row's pattern and g~ in registers; each entry of row J_p of A is
+      // matched against J by compares, so its loads do not wait on one
+      // another (the merge below walks both lists one dependent load at a
+      // time). The same products are added in the same order.
+      uint32_t J[kRowNormRegs];
+      float gt[kRowNormRegs];
+#pragma unroll
+      for (int p = 0; p < kRowNormRegs; ++p) {
+        J[p] = (uint32_t)p < k ? Lcol[o + p] : 0xFFFFFFFFu;
+        gt[p] = (uint32_t)p < k ? out[o + p] * colscale[o + p] : 0.0f;
+      }
+      uint32_t ra[kRowNormRegs], rb[kRowNormRegs];
+#pragma unroll
+      for (int p = 0; p < kRowNormRegs; ++p) {
+        ra[p] = (uint32_t)p < k ? row_ptr[J[p]] : 0u;
+        rb[p] = (uint32_t)p < k ? row_ptr[J[p] + 1] : 0u;
+      }
+#pragma unroll
+      for (int p = 0; p < kRowNormRegs; ++p) {
+        if ((uint32_t)p >= k) continue;
+        G[o + p] = gt[p];
+        float v = 0.0f;
+        for (uint32_t a0 = ra[p]; a0 < rb[p]; a0 += kRowNormRegs) {  // entries 8 at a time
+          uint32_t cv[kRowNormRegs];
+          float wv[kRowNormRegs];
+#pragma unroll
+          for (int u = 0; u < kRowNormRegs; ++u) {
+            const bool in = a0 + u < rb[p];
+            cv[u] = in ? col[a0 + u] : 0xFFFFFFFFu;
+            wv[u] = in ? val[a0 + u] : 0.0f;
+          }
+#pragma unroll
+          for (int u = 0; u < kRowNormRegs; ++u) {
+            bool hit = false;
+            float g = 0.0f;
+#pragma unroll
+            for (int q = 0; q < kRowNormRegs; ++q) {
+              hit |= cv[u] == J[q];
+              g = cv[u] == J[q] ? gt[q] : g;
+            }
+            v = hit ? v + wv[u] * g : v;
+          }
+        }
+        N += gt[p] * v;
+      }
+    } else {
+      for (uint32_t p = 0; p < k; ++p) G[o + p] = out[o + p] * colscale[o + p];  // g~, scaled below
+      for (uint32_t p = 0; p < k; ++p)
+        N += G[o + p] * rownorm_v(row_ptr, col, val, Lcol, o, k, Lcol[o + p], G, 1.0f);
+    }
     const bool ok = N > 0.0f && N < INFINITY;
     const float sc = ok ? rsqrtf(N) : 1.0f;
     bool diag_ok = false;

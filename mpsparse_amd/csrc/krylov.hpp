@@ -72,6 +72,16 @@ struct Scalars {
   int32_t pending;   // explicit preconditioner: the latest step's <r,r> sums await FinCgStep
 };
 
+// Solver state at the start of a solve: every field zero, then the settings.
+__global__ void k_start_solve(Scalars* S, float tol, int32_t maxiter, int32_t crit, int32_t use_verify) {
+  Scalars z{};
+  z.tol = tol;
+  z.maxiter = maxiter;
+  z.crit = crit;
+  z.use_verify = use_verify;
+  *S = z;
+}
+
 // k_finish hook: after a finisher that owns the solver state, refresh the
 // step kernels' gate.
 __device__ __forceinline__ void refresh_halt(Scalars* S) { S->halt = (S->done != 0 || S->verify != 0) ? 1 : 0; }

@@ -9,7 +9,8 @@ as the ratio plain-CG seconds / learned-PCG seconds, on a held-out 128^3
 generated on the GPU). A "step" is one such learned-PCG solve: `--warmup`
 untimed, then `--steps` timed; ms_per_step is their median. Plain CG is the
 faster of FP32 CG and FP64-refined CG (median of `--reps`). Jacobi-PCG,
-classical FSAI-PCG and the 8-feature network variant are reported beside it.
+classical FSAI-PCG (FP32 and BF16 values of A) and the 8-feature network
+variant are reported beside it.
 
 SpMV part (`spmv`, `roofline`): the block-sparse system, n = 1,048,576 (65,536
 block-rows of 16x16 blocks, diagonal block + 8 random block-columns;
@@ -312,6 +313,7 @@ def main():
         out["time_to_solution_s"] = {
             "learned_fsai_pcg_bf16": hs["median_s"], "plain_cg": plain,
             "jacobi_pcg": fsai["jacobi_pcg_refined"]["wall_s"], "classical_fsai_pcg": fsai["classical_fsai_refined"]["wall_s"],
+            "classical_fsai_pcg_bf16": fsai["classical_fsai_refined_bf16"]["wall_s"],
             "learned_fsai_pcg_bf16_comparison_runs": lp["wall_s"]}
         out["roofline_headline"] = {"bound": "cache", "kernel": "k_dia (7 diagonals, BF16 values): q = A p",
                                     "achieved": fsai["mv_bf16"]["GBps"], "unit": "GB/s",
@@ -697,6 +699,7 @@ def fsai_section(dev, steps, warmup, reps, weights=None, only_headline=False):
     res["weights"] = {"stated5": os.path.relpath(weights or LF.DEFAULT_WEIGHTS, HERE),
                       "coupled8": os.path.relpath(LF.COUPLED8_WEIGHTS, HERE)}
     variants = (("classical_fsai_refined", "classical", "f32"),
+                ("classical_fsai_refined_bf16", "classical", "bf16"),
                 ("learned_fsai_stated5_refined_f32", "stated5", "f32"),
                 ("learned_fsai_stated5_refined_bf16", "stated5", "bf16"),
                 ("learned_fsai_coupled8_refined_bf16", "coupled8", "bf16"))
@@ -775,6 +778,7 @@ def fsai_section(dev, steps, warmup, reps, weights=None, only_headline=False):
     res["speedup_learned_bf16_vs_plain_cg"] = base / head
     res["speedup_learned_bf16_vs_jacobi_pcg"] = res["jacobi_pcg_refined"]["wall_s"] / head
     res["speedup_learned_bf16_vs_classical_fsai"] = res["classical_fsai_refined"]["wall_s"] / head
+    res["speedup_learned_bf16_vs_classical_fsai_bf16"] = res["classical_fsai_refined_bf16"]["wall_s"] / head
     res["_csr"] = (rp, ci, cv, b32)  # for the CPU baseline; removed before printing
     return res
 

@@ -1845,14 +1845,34 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       else k_cg_update_p<kPcNone><<<gv, kBlock, 0, s>>>(pnew, pold, R, D, n, dec);
       g = op.template spmv<1>(pnew, EpiStoreDotX{Q, pnew}, FinNone{}, &S2->halt, s);
     }
+    // (pold holds the previous step's direction for a held update of x; pnew
+    // is buffer (k + 1) & 1, recorded when this step holds its own)
+    const int32_t pbuf = (k + 1) & 1;
     if (jac) {
-      if (comp) k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
-      else k_cg_update_xr<kPcJacobi, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
+      if (comp)
+        k_cg_update_xr<kPcJacobi, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, pold, Q, D, n, S, S2, rb2, rb, g, pbuf);
+      else
+        k_cg_update_xr<kPcJacobi, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, pold, Q, D, n, S, S2, rb2, rb, g, pbuf);
     } else {
-      if (comp) k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
-      else k_cg_update_xr<kPcNone, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, Q, D, n, S, S2, rb2, rb, g);
+      if (comp)
+        k_cg_update_xr<kPcNone, true><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, pold, Q, D, n, S, S2, rb2, rb, g, pbuf);
+      else
+        k_cg_update_xr<kPcNone, false><<<gv, kBlock, 0, s>>>(X, XLo, R, pnew, pold, Q, D, n, S, S2, rb2, rb, g, pbuf);
     }
     ++k;
+  };
+  // a held update of x applied before a true-residual check runs (VER) or
+  // before the iterate is returned
+  auto xflush = [&](bool ver) {
+    if (ver) {
+      if (comp) k_cg_xflush<true, true><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      else k_cg_xflush<false, true><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      k_cg_xflush_clear<true><<<1, 1, 0, s>>>(S);
+    } else {
+      if (comp) k_cg_xflush<true, false><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      else k_cg_xflush<false, false><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      k_cg_xflush_clear<false><<<1, 1, 0, s>>>(S);
+    }
   };
   // the last step's decision, before the host reads the state
   auto flush = [&]() {
@@ -1862,6 +1882,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   };
   // a restart (beta = 0) makes the next direction z whatever P[k & 1] holds
   auto verify = [&]() {
+    xflush(true);
     if (jac) {
       verify_launch<1>(op, gv, X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb, s);
       finish<2>(FinVerify<1>{S}, rb, gv, s);
@@ -1874,6 +1895,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   for (auto& e : tev)
     if (e) (void)hipEventDestroy(e);
   MSPCHK(st);
+  xflush(false);
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   // (a caller inside the library that keeps working on s skips the wait)

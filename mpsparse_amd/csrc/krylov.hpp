@@ -70,6 +70,12 @@ struct Scalars {
   int32_t halt;      // done || verify: the step kernels do nothing (kept by k_finish)
   int32_t skip_g;    // explicit preconditioner: w = G r of the current residual is up to date
   int32_t pending;   // explicit preconditioner: the latest step's <r,r> sums await FinCgStep
+  // CG: the iterate's update of the latest step is held (x += alpha_hold p,
+  // p in direction buffer xpend_buf) and applied with the next step's, or by
+  // k_cg_xflush before a verification and at the end of the solve
+  int32_t xpend;
+  int32_t xpend_buf;
+  float alpha_hold;
 };
 
 // Solver state at the start of a solve: every field zero, then the settings.
@@ -493,16 +499,29 @@ __device__ __forceinline__ int kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, i
 // error of x += alpha p (for targets near FP32's attainable accuracy);
 // otherwise x alone, xl untouched. (The explicitly preconditioned step fuses
 // this update into its factor product: XRNew / EpiCgXrW.)
+// The iterate moves every other step: a step whose predecessor left its
+// update held (S2->xpend; its direction is this step's pold, its alpha
+// S->alpha_hold) applies that update and then its own, in that order, so x
+// takes the same values; otherwise the step holds its own (records alpha,
+// the buffer index pbuf of p) and touches neither x nor xl. A step that must
+// fold xl into x (after a restart) applies at once. The recurrence never
+// reads x, so r and every decision are unchanged; x and xl move through
+// memory once per two steps (compensated: 32 -> 24 B per row and step).
 template <int PC, bool COMP>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, float* __restrict__ xl,
                                                          float* __restrict__ r, const float* __restrict__ p,
+                                                         const float* __restrict__ pold,
                                                          const float* __restrict__ q, const float* __restrict__ dinv,
                                                          uint32_t n, Scalars* S, const Scalars* S2, GridRed rb,
-                                                         GridRed pq, uint32_t npq) {
+                                                         GridRed pq, uint32_t npq, int32_t pbuf) {
   // the gate is tested after the partials are in (their loads overlap its
   // latency); a halted step writes nothing but the commit
   const int32_t halt = S2->halt;
   const float pqv = block_total<1>(pq.partials, npq, 0), rz = S2->rz;
+  const bool fold = S2->collapse != 0, pend = S2->xpend != 0;
+  // (S->alpha_hold is written below only by a step that holds, which reads none)
+  const float ah = pend ? S->alpha_hold : 0.0f;
+  const bool applyx = fold || pend;
   if (blockIdx.x == 0 && threadIdx.x == 0) commit_step_state(S, S2);
   if (halt) return;
   const bool ok = finite_pos(pqv);
@@ -511,34 +530,44 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
     S->pq = ok ? pqv : -1.0f;
     S->alpha = alpha;
     S->pending = 1;
+    S->xpend = applyx ? 0 : 1;
+    if (!applyx) {
+      S->alpha_hold = alpha;
+      S->xpend_buf = pbuf;
+    }
   }
-  [[maybe_unused]] const bool fold = S2->collapse != 0;
   CSum acc[2] = {csum_zero(), csum_zero()};
   const uint32_t nc = vec_chunks(n);
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
-    f4 xv = reinterpret_cast<f4*>(x)[c];
     f4 rv = reinterpret_cast<f4*>(r)[c];
-    f4 pv = reinterpret_cast<const f4*>(p)[c];
     f4 qv = reinterpret_cast<const f4*>(q)[c];
-    if constexpr (COMP) {
-      f4 xlv = reinterpret_cast<f4*>(xl)[c];
-      if (fold) {
-        xv += xlv;
-        xlv = f4{0.f, 0.f, 0.f, 0.f};
-      }
+    if (applyx) {
+      f4 xv = reinterpret_cast<f4*>(x)[c];
+      f4 pv = reinterpret_cast<const f4*>(p)[c];
+      f4 hv = f4{0.f, 0.f, 0.f, 0.f};
+      if (pend) hv = reinterpret_cast<const f4*>(pold)[c];
+      if constexpr (COMP) {
+        f4 xlv = reinterpret_cast<f4*>(xl)[c];
+        if (fold) {
+          xv += xlv;
+          xlv = f4{0.f, 0.f, 0.f, 0.f};
+        }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float xk = xv[k], lk = xlv[k];
-        comp_axpy(xk, lk, alpha, pv[k]);
-        xv[k] = xk;
-        xlv[k] = lk;
+        for (int k = 0; k < 4; ++k) {
+          float xk = xv[k], lk = xlv[k];
+          if (pend) comp_axpy(xk, lk, ah, hv[k]);
+          comp_axpy(xk, lk, alpha, pv[k]);
+          xv[k] = xk;
+          xlv[k] = lk;
+        }
+        reinterpret_cast<f4*>(xl)[c] = xlv;
+      } else {
+        if (pend) xv += ah * hv;
+        xv += alpha * pv;
       }
-      reinterpret_cast<f4*>(xl)[c] = xlv;
-    } else {
-      xv += alpha * pv;
+      reinterpret_cast<f4*>(x)[c] = xv;
     }
     rv -= alpha * qv;
-    reinterpret_cast<f4*>(x)[c] = xv;
     reinterpret_cast<f4*>(r)[c] = rv;
     f4 zv = rv;
     if (PC == kPcJacobi) zv = rv * reinterpret_cast<const f4*>(dinv)[c];
@@ -549,6 +578,42 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(float* __restrict__ x, 
     }
   }
   block_partials<2>(acc, rb);
+}
+
+// A held update of the iterate applied ahead of a verification (VER: only when
+// one is about to run: verify set, not done) or at the end of a solve; the
+// flag is cleared by k_cg_xflush_clear once every workgroup has read it.
+template <bool COMP, bool VER>
+__global__ __launch_bounds__(kBlock) void k_cg_xflush(float* __restrict__ x, float* __restrict__ xl,
+                                                      const float* __restrict__ p0, const float* __restrict__ p1,
+                                                      uint32_t n, const Scalars* S) {
+  if (!S->xpend || (VER && (S->verify == 0 || S->done != 0))) return;
+  const float a = S->alpha_hold;
+  const float* __restrict__ p = S->xpend_buf ? p1 : p0;
+  const uint32_t nc = vec_chunks(n);
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    f4 xv = reinterpret_cast<f4*>(x)[c];
+    const f4 pv = reinterpret_cast<const f4*>(p)[c];
+    if constexpr (COMP) {
+      f4 xlv = reinterpret_cast<f4*>(xl)[c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float xk = xv[k], lk = xlv[k];
+        comp_axpy(xk, lk, a, pv[k]);
+        xv[k] = xk;
+        xlv[k] = lk;
+      }
+      reinterpret_cast<f4*>(xl)[c] = xlv;
+    } else {
+      xv += a * pv;
+    }
+    reinterpret_cast<f4*>(x)[c] = xv;
+  }
+}
+template <bool VER>
+__global__ void k_cg_xflush_clear(Scalars* S) {
+  if (!S->xpend || (VER && (S->verify == 0 || S->done != 0))) return;
+  S->xpend = 0;
 }
 
 // Totals of k_cg_update_xr: <r,r> and <r,z> -> iteration count, convergence
@@ -600,6 +665,7 @@ __device__ __forceinline__ void stage_step_state(Scalars* S2, const Scalars& L) 
   S2->pending = L.pending;
   S2->beta = L.beta;
   S2->rz = L.rz;
+  S2->xpend = L.xpend;
 }
 __device__ __forceinline__ void commit_step_state(Scalars* S, const Scalars* S2) {
   const Scalars L = *S2;

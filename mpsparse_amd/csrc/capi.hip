@@ -2825,15 +2825,33 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     float* rnext = Rb[(k + 1) & 1];
     const uint32_t gz = factor_spmv<1>(g16, &F->GT, XPlain{Wv}, EpiStoreDot{Z, rcur}, zr, &S->halt, s);
     const uint32_t gq = op.pcg_dir(Z, pold, S, S2, zr, gz, rb2, gu, Q, pnew, s);
+    // (pold: the previous direction of a held update of x; pnew is buffer
+    // (k + 1) & 1, recorded when this step holds its own)
+    const int32_t pbuf = (k + 1) & 1;
     if (comp)
       gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
-                          EpiCgXrW<true>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0}, rb2,
-                          &S2->halt, s);
+                          EpiCgXrW<true>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0, pold, pbuf,
+                                         0.0f, 0, 0},
+                          rb2, &S2->halt, s);
     else
       gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
-                          EpiCgXrW<false>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0}, rb2,
-                          &S2->halt, s);
+                          EpiCgXrW<false>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0, pold, pbuf,
+                                          0.0f, 0, 0},
+                          rb2, &S2->halt, s);
     ++k;
+  };
+  // a held update of x applied before a true-residual check runs (ver) or
+  // before the iterate is returned
+  auto xflush = [&](bool ver) {
+    if (ver) {
+      if (comp) k_cg_xflush<true, true><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      else k_cg_xflush<false, true><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      k_cg_xflush_clear<true><<<1, 1, 0, s>>>(S);
+    } else {
+      if (comp) k_cg_xflush<true, false><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      else k_cg_xflush<false, false><<<gv, kBlock, 0, s>>>(X, XLo, Pb[0], Pb[1], n, S);
+      k_cg_xflush_clear<false><<<1, 1, 0, s>>>(S);
+    }
   };
   // the last step's decision, before the host reads the state
   auto flush = [&]() {
@@ -2842,11 +2860,13 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   // a verification rewrites the current residual; after a restart w = G r is
   // recomputed (gated by skip_g, which FinVerify clears only then)
   auto verify = [&]() {
+    xflush(true);
     verify_launch<3>(op, gv, X, XLo, B, Rb[k & 1], P, nullptr, nullptr, n, S, rb, s);
     finish<2>(FinVerify<3>{S}, rb, gv, s);
     g_of_r(&S->skip_g);
   };
   MSPCHK(run_solver(A, o, info, s, init, step, verify, flush));
+  xflush(false);
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
   MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   // (a caller inside the library that keeps working on s skips the wait)

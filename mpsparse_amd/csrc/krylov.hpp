@@ -420,6 +420,10 @@ struct XRNew {
 };
 
 template <bool COMP>
+// The iterate moves every other step, as in k_cg_update_xr: with a held
+// update from the previous step (pend; direction pold, alpha_hold) the rows
+// apply it and then this step's; otherwise the step holds its own (applyx
+// false: x and xl neither loaded nor stored) unless it must fold xl into x.
 struct EpiCgXrW {
   float* __restrict__ x;
   float* __restrict__ xl;
@@ -436,26 +440,41 @@ struct EpiCgXrW {
   uint32_t nz;
   float alpha;
   int fold;
+  const float* __restrict__ pold;  // the previous step's direction
+  int32_t pbuf;                    // direction buffer index of p (recorded when holding)
+  float ah;                        // alpha_hold of a pending update
+  int pend, applyx;
   struct Pre {
-    float x, xl, p, r, q;
+    float x, xl, p, ph, r, q;
   };
   __device__ __forceinline__ Pre pre(uint32_t i) const {
-    return Pre{x[i], COMP ? xl[i] : 0.0f, p[i], rold[i], q[i]};
+    Pre v{0.0f, 0.0f, 0.0f, 0.0f, rold[i], q[i]};
+    if (applyx) {
+      v.x = x[i];
+      if (COMP) v.xl = xl[i];
+      v.p = p[i];
+      if (pend) v.ph = pold[i];
+    }
+    return v;
   }
   __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc, Pre v) const {
-    float xi = v.x;
-    if constexpr (COMP) {
-      float li = v.xl;
-      if (fold) {
-        xi += li;
-        li = 0.0f;
+    if (applyx) {
+      float xi = v.x;
+      if constexpr (COMP) {
+        float li = v.xl;
+        if (fold) {
+          xi += li;
+          li = 0.0f;
+        }
+        if (pend) comp_axpy(xi, li, ah, v.ph);
+        comp_axpy(xi, li, alpha, v.p);
+        xl[i] = li;
+      } else {
+        if (pend) xi += ah * v.ph;
+        xi += alpha * v.p;
       }
-      comp_axpy(xi, li, alpha, v.p);
-      xl[i] = li;
-    } else {
-      xi += alpha * v.p;
+      x[i] = xi;
     }
-    x[i] = xi;
     const float ri = v.r - alpha * v.q;
     rnew[i] = ri;
     w[i] = s;
@@ -474,6 +493,10 @@ __device__ __forceinline__ int kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, i
   xl.alpha = alpha;
   epi.alpha = alpha;
   epi.fold = epi.S2->collapse != 0;
+  epi.pend = epi.S2->xpend != 0;
+  // (S->alpha_hold is written below only by a step that holds, which reads none)
+  epi.ah = epi.pend ? S->alpha_hold : 0.0f;
+  epi.applyx = epi.fold || epi.pend;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     commit_step_state(S, epi.S2);
     if (!halt) {
@@ -483,6 +506,11 @@ __device__ __forceinline__ int kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, i
       S->restart = 0;
       S->skip_g = 1;
       S->pending = 1;
+      S->xpend = epi.applyx ? 0 : 1;
+      if (!epi.applyx) {
+        S->alpha_hold = alpha;
+        S->xpend_buf = epi.pbuf;
+      }
     }
   }
   return halt;

@@ -1513,14 +1513,6 @@ struct OpCsr {
     const XPNew<PC> xl{r, pold, dinv, dec, 0.0f};
     return spmv_any<V, 1>(A, xl, EpiCgQ<XPNew<PC>>{q, pnew, xl}, FinNone{}, red_of(A), done, s);
   }
-  // Explicit preconditioner: p = z + beta p_old in the gather with beta from
-  // the <r,z> partials (zr, nz blocks), q = A p, <p,q> partials.
-  uint32_t pcg_dir(const float* z, const float* pold, const Scalars* S, Scalars* S2, GridRed zr, uint32_t nz,
-                   GridRed rr, uint32_t nrr, float* q, float* pnew, hipStream_t s) const {
-    const XPzLocal xl{z, pold, S, S2, zr, nz, rr, nrr, 0.0f};
-    const int* done = &S->halt;
-    return spmv_any<V, 1>(A, xl, EpiCgQ<XPzLocal>{q, pnew, xl}, FinNone{}, red_of(A), done, s);
-  }
   RowsCsr rows() const { return RowsCsr{A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>()}; }
   void inv_diag(float* dinv, int* bad, hipStream_t s) const {
     uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((A->n + kBlock - 1) / kBlock, 8192));
@@ -2766,8 +2758,9 @@ extern "C" msp_status msp_fsai_probe_loss_grad(msp_fsai F, const float* W, int32
 
 // FSAI-preconditioned CG. Step order puts the preconditioner first, so that a
 // restart after a failed true-residual check is just beta = 0:
-//   w = G r; z = G^T w, <r,z> -> beta; p = z + beta p; q = A p, <p,q> -> alpha;
-//   x += alpha p, r -= alpha q, <r,r> -> convergence; [true residual check].
+//   z = G^T w, <r,z> = ||w||^2 -> beta; p = z + beta p; q = A p, <p,q> -> alpha;
+//   x += alpha p, r -= alpha q, w = G r, ||r||^2 -> convergence, ||w||^2;
+//   [true residual check].
 template <typename V>
 static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, float* x, const msp_solve_opts& o,
                                 msp_solve_info* info, hipStream_t s,
@@ -2784,7 +2777,7 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   }
   Workspace& w = A->ws;
   float *X = w.v(Workspace::X), *R = w.v(Workspace::R), *P = w.v(Workspace::P), *Q = w.v(Workspace::Q);
-  float *Z = w.v(Workspace::Z), *Wv = w.v(Workspace::W), *XLo = w.v(Workspace::XL);
+  float *Wv = w.v(Workspace::W), *XLo = w.v(Workspace::XL);
   const float* B = w.v(Workspace::B);
   Scalars* S = w.S_dev();
   Scalars* S2 = w.S_stage();
@@ -2792,16 +2785,22 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   const uint32_t n = A->n, gv = vec_grid(n);
   const bool comp = compensate(o);
   // ping-pong directions and residuals: step k reads Pb[k & 1], Rb[k & 1]
-  // and writes Pb[(k + 1) & 1], Rb[(k + 1) & 1] (each is formed in a gather,
-  // so the vector being written cannot be the one other workgroups gather)
+  // and writes Pb[(k + 1) & 1], Rb[(k + 1) & 1] (the residual is formed in a
+  // gather, so the vector being written cannot be the one other workgroups
+  // gather; the direction is written while the held update of x may still
+  // need the previous one)
   float* Pb[2] = {P, w.v(Workspace::T)};
   float* Rb[2] = {R, w.v(Workspace::R0)};
   int k = 0;
-  uint32_t gu = 0;  // workgroups of the latest x / r update (its <r,r> partials in rb2)
-  GridRed rb2 = w.gr2(), zr = w.gr3();
-  // w = G r of the current residual (gated: skipped when *gate != 0)
+  uint32_t gu = 0;  // workgroups of the latest x / r update (its decision pending on rb2)
+  // workgroups whose <r,r>, <w,w> partials in rb2 describe the current w
+  uint32_t nw = gv;
+  GridRed rb2 = w.gr2();
+  // w = G r of the current residual and its partials, both gated (skipped
+  // when *gate != 0); the partials take the update's block count once one ran
   auto g_of_r = [&](const int* gate) {
     factor_spmv<0>(g16, &F->G, XPlain{Rb[k & 1]}, EpiStore{Wv}, rb, gate, s);
+    k_pcg_wnorm<<<nw, kBlock, 0, s>>>(Rb[k & 1], Wv, n, rb2, gate);
   };
   auto init = [&]() -> msp_status {
     if (o.use_x0) op.template spmv<1>(X, EpiResidual{R, B}, FinNone{}, nullptr, s);
@@ -2811,33 +2810,33 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     g_of_r(&S->halt);
     return launch_check("pcg init");
   };
-  // one step, three launches: z = G^T w with <r,z> partials (zr); q = A p
-  // with p = z + beta p_old formed in the gather (beta from zr), the previous
-  // step's decision (FinCgStep on rb2) taken by every workgroup and staged in
-  // S2, <p,q> partials (rb); the decision committed to S and the x / r update
-  // with alpha from zr and rb fused with the next step's w = G r (r formed in
-  // the gather, <r,r> partials rb2, decision pending). A z product after the
-  // step that converged still runs; its z and <r,z> are recomputed before use.
+  // one step, three launches: z = G^T w with p = z + beta p_old formed in the
+  // epilogue, the previous step's decision (pending on rb2) taken by every
+  // workgroup and staged in S2 with <r,z> = ||w||^2; q = A p with <p,q>
+  // partials (rb); the decision committed to S and the x / r update with
+  // alpha = <r,z> / <p,q> fused with the next step's w = G r (r formed in the
+  // gather, ||r||^2 and ||w||^2 partials on rb2, decision pending).
   auto step = [&]() {
     const float* pold = Pb[k & 1];
     float* pnew = Pb[(k + 1) & 1];
     const float* rcur = Rb[k & 1];
     float* rnext = Rb[(k + 1) & 1];
-    const uint32_t gz = factor_spmv<1>(g16, &F->GT, XPlain{Wv}, EpiStoreDot{Z, rcur}, zr, &S->halt, s);
-    const uint32_t gq = op.pcg_dir(Z, pold, S, S2, zr, gz, rb2, gu, Q, pnew, s);
+    factor_spmv<0>(g16, &F->GT, XPlain{Wv}, EpiPcgDir{pnew, pold, S, S2, rb2, nw, 0.0f}, rb, &S->halt, s);
+    const uint32_t gq = op.template spmv<1>(pnew, EpiStoreDot{Q, pnew}, FinNone{}, &S2->halt, s);
     // (pold: the previous direction of a held update of x; pnew is buffer
     // (k + 1) & 1, recorded when this step holds its own)
     const int32_t pbuf = (k + 1) & 1;
     if (comp)
       gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
-                          EpiCgXrW<true>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0, pold, pbuf,
-                                         0.0f, 0, 0},
+                          EpiCgXrW<true>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, 0.0f, 0, pold, pbuf, 0.0f,
+                                         0, 0},
                           rb2, &S2->halt, s);
     else
       gu = factor_spmv<2>(g16, &F->G, XRNew{rcur, Q, 0.0f},
-                          EpiCgXrW<false>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, zr, gz, 0.0f, 0, pold, pbuf,
-                                          0.0f, 0, 0},
+                          EpiCgXrW<false>{X, XLo, rcur, rnext, pnew, Q, Wv, S, S2, rb, gq, 0.0f, 0, pold, pbuf, 0.0f,
+                                          0, 0},
                           rb2, &S2->halt, s);
+    nw = gu;
     ++k;
   };
   // a held update of x applied before a true-residual check runs (ver) or
@@ -2857,8 +2856,9 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   auto flush = [&]() {
     if (gu) finish<2>(FinCgStepPending<kPcExplicit>{S}, rb2, gu, s);
   };
-  // a verification rewrites the current residual; after a restart w = G r is
-  // recomputed (gated by skip_g, which FinVerify clears only then)
+  // a verification rewrites the current residual; after a restart w = G r and
+  // its partials are recomputed (gated by skip_g, which FinVerify clears only
+  // then)
   auto verify = [&]() {
     xflush(true);
     verify_launch<3>(op, gv, X, XLo, B, Rb[k & 1], P, nullptr, nullptr, n, S, rb, s);

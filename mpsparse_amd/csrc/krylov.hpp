@@ -337,39 +337,31 @@ __device__ __forceinline__ void block_total2(const CSum* __restrict__ pa, uint32
   __syncthreads();
 }
 
-// Explicitly preconditioned CG direction formed in the gather of q = A p:
-// p = z + beta p_old with beta = <r,z>_new / <r,z>_old, <r,z>_new from the
-// block partials of the z product (zr, nz blocks) reduced by every workgroup
-// in kernel_prologue; beta = 0 on the first step, after a restart, or for a
-// non-positive <r,z>_new (the step's update then records the breakdown).
-// p_old must not alias the p written by the epilogue.
-// The prologue also takes the previous step's pending decision in place of a
-// finisher launch: every workgroup sums that step's <r,r> partials (rr, nrr
-// blocks, value 0 of 2) and applies FinCgStep to a private copy of the state,
-// so all agree on the gate; workgroup 0 stages the decided fields in S2, also
-// when halted (every workgroup of this kernel reads S, so the next kernel
-// commits S2 to it).
-struct XPzLocal {
-  const float* __restrict__ z;
+// The first kernel of an FSAI-preconditioned CG step, z = G^T w (w = G r
+// from the previous step's update), with the new direction formed in its
+// epilogue: p = z + beta p_old, written to the other direction buffer (p_old
+// must not alias it). With M = G^T G, <r, z> = <G r, G r> = ||w||^2, which the
+// kernel that produced w summed beside ||r||^2 (rr: nrr blocks of 2 partials);
+// every workgroup reduces both in kernel_prologue, takes the previous step's
+// pending decision on ||r||^2 in place of a finisher launch (FinCgStep on a
+// private copy of the state, so all agree on the gate) and forms
+// beta = ||w||^2 / <r,z>_old (0 on the first step, after a restart, or for a
+// non-positive ||w||^2; the step's update then records the breakdown).
+// Workgroup 0 stages the decided fields in S2, also when halted (every
+// workgroup of this kernel reads S, so a later kernel commits S2 to it), and
+// stages ||w||^2 as the step's <r,z> in S2->rz for alpha.
+struct EpiPcgDir {
+  float* __restrict__ pnew;
   const float* __restrict__ pold;
   const Scalars* S;
   Scalars* S2;
-  GridRed zr;
-  uint32_t nz;
   GridRed rr;
   uint32_t nrr;
   float beta;
-  struct Raw {
-    float z, p;
-  };
-  __device__ __forceinline__ Raw raw(uint32_t c) const { return Raw{z[c], pold[c]}; }
-  __device__ __forceinline__ float make(Raw w) const { return w.z + beta * w.p; }
-  __device__ __forceinline__ float operator()(uint32_t c) const { return z[c] + beta * pold[c]; }
-  __device__ __forceinline__ f4 load4(u4 c) const {
-    if (quad_run(c))
-      return *reinterpret_cast<const f4*>(z + c.x) + beta * *reinterpret_cast<const f4*>(pold + c.x);
-    return f4{(*this)(c.x), (*this)(c.y), (*this)(c.z), (*this)(c.w)};
-  }
+  typedef float Pre;
+  __device__ __forceinline__ Pre pre(uint32_t i) const { return pold[i]; }
+  __device__ __forceinline__ void row(uint32_t i, float s, CSum*, Pre po) const { pnew[i] = s + beta * po; }
+  __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc) const { row(i, s, acc, pre(i)); }
 };
 
 template <int PC>
@@ -377,31 +369,46 @@ __device__ __forceinline__ int take_pending_step(const Scalars* S, Scalars* S2, 
                                                  float& beta);
 __device__ __forceinline__ void commit_step_state(Scalars* S, const Scalars* S2);
 
-__device__ __forceinline__ int kernel_prologue(XPzLocal& xl, EpiCgQ<XPzLocal>& epi, int halt) {
-  float rz_new, rr;
-  block_total2(xl.zr.partials, xl.nz, xl.rr.partials, xl.nrr, rz_new, rr, 2);
+__device__ __forceinline__ int kernel_prologue(XPlain&, EpiPcgDir& epi, int halt) {
+  float rr, ww;
+  block_total2(epi.rr.partials, epi.nrr, epi.rr.partials + 1, epi.nrr, rr, ww, 2, 2);
   const float tot[2] = {rr, 0.0f};
   float unused;
-  halt = take_pending_step<kPcExplicit>(xl.S, xl.S2, tot, halt, unused);
-  const Scalars* S = xl.S;
-  const float beta = (S->restart || S->rz == 0.0f || !finite_pos(rz_new)) ? 0.0f : rz_new / S->rz;
-  xl.beta = beta;
-  epi.xl.beta = beta;
+  halt = take_pending_step<kPcExplicit>(epi.S, epi.S2, tot, halt, unused);
+  const Scalars* S = epi.S;
+  epi.beta = (S->restart || S->rz == 0.0f || !finite_pos(ww)) ? 0.0f : ww / S->rz;
+  if (blockIdx.x == 0 && threadIdx.x == 0) epi.S2->rz = ww;
   return halt;
+}
+
+// ||r||^2 and ||w||^2 as 2-value block partials of the layout the FSAI-PCG
+// update writes, for a w = G r computed outside a step (the start of a solve,
+// after a restart); gated like that product (skipped when *gate != 0).
+__global__ __launch_bounds__(kBlock) void k_pcg_wnorm(const float* __restrict__ r, const float* __restrict__ w,
+                                                      uint32_t n, GridRed rb, const int* __restrict__ gate) {
+  if (*gate) return;
+  CSum acc[2] = {csum_zero(), csum_zero()};
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float ri = r[i], wi = w[i];
+    acc[0] = csum_add(acc[0], ri * ri);
+    acc[1] = csum_add(acc[1], wi * wi);
+  }
+  block_partials<2>(acc, rb);
 }
 
 // The x / r update of an explicitly preconditioned CG step fused with the
 // next step's first factor product w = G r_new: r_new = r - alpha q is formed
 // in the gather (r_new goes to the other residual buffer, so every workgroup
-// gathers the step's r), alpha = <r,z> / <p,q> from the block partials of the
-// z product (zr, nz blocks) and of q = A p (pq, npq blocks), reduced by every
-// workgroup in kernel_prologue; alpha = 0 when either is not positive (the
-// step's decision then records a breakdown). The kernel's gate is S2->halt
-// (the state as decided by XPzLocal's prologue); workgroup 0 commits S2 to S,
-// also when halted, then records pq, alpha, <r,z>, clears the restart flag,
-// marks w current and the step's decision pending.
-// The epilogue updates the owned rows (x, x_lo
-// when COMP, r_new, w) and accumulates <r_new, r_new> (second partial unused).
+// gathers the step's r), alpha = <r,z> / <p,q> with <r,z> staged in S2 by the
+// step's first kernel (EpiPcgDir) and <p,q> from the block partials of
+// q = A p (pq, npq blocks), reduced by every workgroup in kernel_prologue;
+// alpha = 0 when either is not positive (the step's decision then records a
+// breakdown). The kernel's gate is S2->halt (the state as decided by
+// EpiPcgDir's prologue); workgroup 0 commits S2 to S, also when halted, then
+// records pq, alpha, <r,z>, clears the restart flag, marks w current and the
+// step's decision pending.
+// The epilogue updates the owned rows (x, x_lo when COMP, r_new, w) and
+// accumulates <r_new, r_new> and <w, w> (the next step's <r,z>).
 struct XRNew {
   const float* __restrict__ r;
   const float* __restrict__ q;
@@ -436,8 +443,6 @@ struct EpiCgXrW {
   const Scalars* S2;
   GridRed pq;
   uint32_t npq;
-  GridRed zr;
-  uint32_t nz;
   float alpha;
   int fold;
   const float* __restrict__ pold;  // the previous step's direction
@@ -479,14 +484,14 @@ struct EpiCgXrW {
     rnew[i] = ri;
     w[i] = s;
     acc[0] = csum_add(acc[0], ri * ri);
+    acc[1] = csum_add(acc[1], s * s);
   }
   __device__ __forceinline__ void row(uint32_t i, float s, CSum* acc) const { row(i, s, acc, pre(i)); }
 };
 
 template <bool COMP>
 __device__ __forceinline__ int kernel_prologue(XRNew& xl, EpiCgXrW<COMP>& epi, int halt) {
-  float pqv, rz;
-  block_total2(epi.pq.partials, epi.npq, epi.zr.partials, epi.nz, pqv, rz);
+  const float pqv = block_total<1>(epi.pq.partials, epi.npq, 0), rz = epi.S2->rz;
   Scalars* S = epi.S;
   const bool ok = finite_pos(pqv) && finite_pos(rz);
   const float alpha = ok ? rz / pqv : 0.0f;

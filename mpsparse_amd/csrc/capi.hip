@@ -1382,14 +1382,18 @@ static uint32_t csr_grid(uint32_t nblk) {
 }
 
 // Rows per thread of the diagonal kernel's load batch: 4 while a row needs at
-// most 10 loads (factors, 2D stencils), 2 above (the 7-point operator; 4 there
-// leaves 2-3 waves per SIMD), 1 for the generic diagonal count. Against one
-// row per thread on the 128^3 solves: FP32 FSAI-PCG steps 66.3 -> 60.5 us,
-// the other solvers 0.3-0.7% faster.
+// most 14 loads (factors, 2D stencils, the 7-point operator with a one-vector
+// gather), 2 above (the 7-point operator forming its direction from two
+// vectors: 4 there leaves 2-3 waves per SIMD), 1 for the generic diagonal
+// count. Against one row per thread on the 128^3 solves: FP32 FSAI-PCG steps
+// 66.3 -> 60.5 us, the other solvers 0.3-0.7% faster; 4 instead of 2 rows for
+// 11-14 loads: the BF16 FSAI-PCG step's x / r update with w = G r 14.0 ->
+// 13.3 us (with the 32-bit form for products with one sum, plain CG at 128^3
+// 0.498 -> 0.472 s).
 template <typename XL, int NDIAG>
 constexpr int dia_rows_per_thread() {
   constexpr int loads = NDIAG * (1 + (int)(sizeof(typename XL::Raw) / sizeof(float)));
-  return NDIAG == 0 ? 1 : (loads <= 10 ? 4 : 2);
+  return NDIAG == 0 ? 1 : (loads <= 14 ? 4 : 2);
 }
 
 // The diagonal kernel is grid-stride over rows: a grid beyond residency would

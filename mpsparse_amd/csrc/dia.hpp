@@ -66,6 +66,12 @@ struct DiaView {
 // matrix read a clamped in-range x and are then left out of the sum, so the
 // result equals the guarded form bit for bit; rows past the end of the
 // U-group load row i0's data and store nothing. Requires m > 0.
+// A loader with 32-bit gathers (raw32).
+template <typename XL, typename = void>
+struct HasRaw32 : std::false_type {};
+template <typename XL>
+struct HasRaw32<XL, std::void_t<decltype(std::declval<const XL&>().raw32(0u))>> : std::true_type {};
+
 template <typename V, int ND, typename XL, typename Epi, int NDIAG, int U>
 __device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, GridRed rb,
                                          const int* __restrict__ done) {
@@ -81,13 +87,15 @@ __device__ __forceinline__ void dia_rows(const DiaView<V>& A, XL xl, Epi epi, Gr
 #pragma unroll
   for (int d = 0; d < NA; ++d) acc[d] = csum_zero();
   const int nd = NDIAG > 0 ? NDIAG : A.o.ndiag;
-  // W: the plain product (no sums) from the FP32 vector x. Rows, columns and
-  // byte offsets in 32 bits (n, m < kDiaMaxDim), each load one offset register
-  // against a base in scalar registers: the 7-point product at 256^3 took
-  // 82 -> 74 us with BF16 values, 91 -> 89 us with FP32. The same form made
-  // the products with sums and two-vector gathers 5-15% slower at 128^3, so
-  // they keep 64-bit addresses.
-  constexpr bool W = ND == 0 && std::is_same<XL, XPlain>::value;
+  // W: rows, columns and byte offsets in 32 bits (n, m < kDiaMaxDim), each
+  // load one offset register against a base in scalar registers, for loaders
+  // with 32-bit gathers (raw32) and up to two sums: the 7-point product at
+  // 256^3 took 82 -> 74 us with BF16 values, 91 -> 89 us with FP32; at 128^3
+  // the FSAI-PCG step's products with sums 11.3 -> 10.1 us (q = A p with
+  // <p,q>) and 13.3 -> 13.0 us (the x / r update with w = G r). (Loaders
+  // without raw32, such as those forming the direction from two vectors, were
+  // 5-15% slower in this form when it was tried.)
+  constexpr bool W = HasRaw32<XL>::value && ND <= 2;
   using Row = typename std::conditional<W, uint32_t, uint64_t>::type;
   using Col = typename std::conditional<W, int32_t, int64_t>::type;
   const Col m = (Col)A.m;

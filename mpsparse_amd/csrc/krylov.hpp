@@ -417,6 +417,8 @@ struct XRNew {
     float r, q;
   };
   __device__ __forceinline__ Raw raw(uint32_t c) const { return Raw{r[c], q[c]}; }
+  // raw with 32-bit byte offsets (c < 2^30)
+  __device__ __forceinline__ Raw raw32(uint32_t c) const { return Raw{ld_off32(r, c), ld_off32(q, c)}; }
   __device__ __forceinline__ float make(Raw w) const { return w.r - alpha * w.q; }
   __device__ __forceinline__ float operator()(uint32_t c) const { return r[c] - alpha * q[c]; }
   __device__ __forceinline__ f4 load4(u4 c) const {

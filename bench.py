@@ -62,6 +62,14 @@ def measured_traffic(key):
         return None, None
 
 
+def streamed_planes(A, vb, n):
+    """Value planes a product with the 7-point operator streams: exactly
+    symmetric storage reads the diagonal and the 3 upper planes once its 7
+    planes reach 16 MB (the lower diagonals read the upper planes shifted);
+    otherwise all 7."""
+    return 4 if A.format == "dia_symmetric" and 7 * vb * n >= (16 << 20) else 7
+
+
 def time_launches(fn, steps, warmup):
     import torch
 
@@ -768,7 +776,8 @@ def fsai_section(dev, steps, warmup, reps, weights=None, only_headline=False):
     ya = torch.empty(n, device=dev)
     A.mv(xa, ya, "bf16")
     mv_ms = time_launches(lambda: A.mv(xa, ya, "bf16"), 100, 5)
-    mv_bytes = 7 * 2 * n + 4 * n + 4 * n + 4 * n  # 7 BF16 diagonals, FP32 diagonal remainder, x, y
+    # the BF16 diagonals streamed, FP32 diagonal remainder, x, y
+    mv_bytes = streamed_planes(A, 2, n) * 2 * n + 4 * n + 4 * n + 4 * n
     res["mv_bf16"] = {"ms": mv_ms, "algorithmic_bytes": mv_bytes, "GBps": mv_bytes / (mv_ms * 1e-3) / 1e9,
                       "note": "working set ~%d MB: resident in the 256 MB last-level cache, so not an HBM-bound "
                               "figure" % (mv_bytes // 2 ** 20)}
@@ -884,11 +893,7 @@ def precision_section(dev, empty, spmv_steps=50, only_products=False):
         A.mv(xm, ym, vt)
         res["mv"]["storage"] = A.format  # known once the first product has probed the pattern
         ms = time_launches(lambda: A.mv(xm, ym, vt), spmv_steps, 5)
-        # exactly symmetric storage streams the diagonal and the 3 upper planes
-        # once their 7 planes exceed 128 MB (the lower diagonals read the upper
-        # planes shifted); otherwise all 7 planes
-        planes = 4 if A.format == "dia_symmetric" and 7 * vb * n >= (128 << 20) else 7
-        alg = planes * vb * n + 4 * n + 4 * n + extra
+        alg = streamed_planes(A, vb, n) * vb * n + 4 * n + 4 * n + extra
         res["mv"][vt] = {"ms": ms, "algorithmic_bytes": alg, "GBps": alg / (ms * 1e-3) / 1e9,
                          "launches_timed": spmv_steps}
         log("stencil mv 256^3", vt, res["mv"][vt])

@@ -59,6 +59,23 @@ static msp_status fail(msp_status s, const char* fmt, ...) {
     if (s_ != MSP_OK) return s_;        \
   } while (0)
 
+// Waits for a stream's queued work (or an event) with the calling thread
+// polling the runtime, for the solvers' short waits between launches (the
+// launches queued after such a wait start on an idle GPU, so the host should
+// resume as soon as the work is done).
+static hipError_t spin_sync(hipStream_t s) {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e != hipErrorNotReady) return e;
+  }
+}
+static hipError_t spin_event(hipEvent_t ev) {
+  for (;;) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e != hipErrorNotReady) return e;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // device facts
 static int cu_count() {
@@ -384,6 +401,9 @@ struct Workspace {
   DevMem red;             // grid-reduction partials
   Scalars* hS = nullptr;  // pinned host mirror
   Scalars* hsnap = nullptr;  // pinned: the state after each of the last two batches
+  // pinned: the refinement's residual sums (a copy into pageable memory is
+  // staged through a runtime buffer)
+  double* hsum = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
   static constexpr int kNumVec = 12;
@@ -402,6 +422,7 @@ struct Workspace {
     if (in_teardown()) return;
     if (hS) (void)hipHostFree(hS);
     if (hsnap) (void)hipHostFree(hsnap);
+    if (hsum) (void)hipHostFree(hsum);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (t0) (void)hipEventDestroy(t0);
@@ -473,6 +494,7 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
     HIPCHK(hipMemsetAsync(w.red.p, 0, red_bytes, s));
     HIPCHK(hipHostMalloc((void**)&w.hS, sizeof(Scalars), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&w.hsnap, 2 * sizeof(Scalars), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&w.hsum, 2 * sizeof(double), hipHostMallocDefault));
     for (auto& e : w.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipEventCreate(&w.t0));
     HIPCHK(hipEventCreate(&w.t1));
@@ -1324,12 +1346,13 @@ static void finish(const Fin& fin, GridRed rb, uint32_t nb, hipStream_t s) {
   if constexpr (ND > 0 && !std::is_same<Fin, FinNone>::value) k_finish<ND, Fin><<<1, kBlock, 0, s>>>(rb, nb, fin);
 }
 
-// Symmetric storage pays where the value planes stream from HBM: below this
-// size (half the 256 MB last-level cache) they stay cache-resident, and the
-// second read of an upper plane only adds work (learned-FSAI solve at 128^3,
-// BF16 planes of 29 MB: 17.8 -> 19.7 ms with it; 256^3 FP32 product, 470 MB:
-// 121 -> 94 us).
-constexpr size_t kDiaMirrorMinBytes = (size_t)128 << 20;
+// Symmetric storage (the lower diagonals read from the upper planes) for
+// value planes above this size: 256^3 FP32 product, 470 MB: 121 -> 94 us; at
+// 128^3 (BF16 planes 29 MB, FP32 59 MB) the learned-FSAI solve 13.1 -> 12.9
+// ms and plain CG 0.494 -> 0.450 s. (With the kernels' earlier 64-bit
+// addressing and fewer rows per thread, the same learned-FSAI solve was
+// slower with it, 17.8 -> 19.7 ms, and the threshold was 128 MB.)
+constexpr size_t kDiaMirrorMinBytes = (size_t)16 << 20;
 
 template <typename V>
 static DiaView<V> dia_view_of(msp_csr A) {
@@ -1638,7 +1661,7 @@ static msp_status start_solve(Workspace& w, const msp_solve_opts& o, hipStream_t
 static msp_status finish_solve(Workspace& w, msp_solve_info* info, hipStream_t s) {
   HIPCHK(hipEventRecord(w.t1, s));
   HIPCHK(hipMemcpyAsync(w.hS, w.S_dev(), sizeof(Scalars), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(spin_sync(s));
   if (info) {
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, w.t0, w.t1);
@@ -1693,7 +1716,7 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
     HIPCHK(hipMemcpyAsync(&w.hsnap[slot], w.S_dev(), sizeof(Scalars), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(w.ev[slot], s));
     if (have_prev) {
-      HIPCHK(hipEventSynchronize(w.ev[slot ^ 1]));
+      HIPCHK(spin_event(w.ev[slot ^ 1]));
       const Scalars& p = w.hsnap[slot ^ 1];
       if (p.done != 0) break;
       cap = full;
@@ -2286,9 +2309,9 @@ static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_r
     else k_residual64<<<gv, kBlock, 0, s>>>(rows, X64, B64, R64, n, part);
     k_finish_d<2><<<1, kBlock, 0, s>>>(reinterpret_cast<const double*>(rb.partials), gv, SUMS);
     MSPCHK(launch_check("k_residual64"));
-    double sums[2] = {0.0, 0.0};
-    HIPCHK(hipMemcpyAsync(sums, SUMS, sizeof sums, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    double* sums = A->ws.hsum;
+    HIPCHK(hipMemcpyAsync(sums, SUMS, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(spin_sync(s));
     const double rnorm = std::sqrt(sums[0]), bnorm = std::sqrt(sums[1]);
     inf.relres = bnorm > 0 ? rnorm / bnorm : rnorm;
     inf.outer_iterations = k;
@@ -2323,7 +2346,7 @@ static msp_status refine_impl(msp_csr A, const double* b, double* x, const msp_r
   }
   HIPCHK(hipMemcpyAsync(x, X64, sizeof(double) * n, is_device_ptr_d(x) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         s));
-  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(spin_sync(s));
   inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   if (info) *info = inf;
   return MSP_OK;
@@ -2392,6 +2415,26 @@ extern "C" msp_status msp_csr_solve_refined(msp_csr A, const double* b, double* 
 
 // ---------------------------------------------------------------------------
 // ABI: FSAI preconditioner M = G^T G (G on the lower-triangle pattern of A)
+// A pinned host word for a small device-to-host read that the host waits
+// for (a copy into pageable memory is staged through a runtime buffer).
+struct PinnedU32 {
+  uint32_t* p = nullptr;
+  PinnedU32() = default;
+  PinnedU32(const PinnedU32&) = delete;
+  PinnedU32& operator=(const PinnedU32&) = delete;
+  ~PinnedU32() {
+    if (p && !in_teardown()) (void)hipHostFree(p);
+  }
+  // the device word's value, after the work queued on s
+  msp_status read(const void* dev, hipStream_t s, uint32_t& out) {
+    if (!p) HIPCHK(hipHostMalloc((void**)&p, sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(hipMemcpyAsync(p, dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(spin_sync(s));
+    out = *p;
+    return MSP_OK;
+  }
+};
+
 struct msp_fsai_s {
   int device = 0;
   msp_csr A = nullptr;  // borrowed
@@ -2403,6 +2446,7 @@ struct msp_fsai_s {
   int ready = 0;    // values set
   int feat_for = -1;  // the feature set whose features (and column scaling) feat holds
   int nonpos = 0;   // diagonal entries of G that are not positive
+  PinnedU32 hword;  // host side of the set-up's counts
 };
 
 static msp_status fsai_compute_features(msp_fsai F, hipStream_t s) {
@@ -2617,8 +2661,7 @@ extern "C" msp_status msp_fsai_set_values(msp_fsai F, const float* g, int32_t* n
                                                   scaled.as<float>(), cnt.as<uint32_t>());
   MSPCHK(fsai_commit(F, s));
   uint32_t h = 0;
-  HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  MSPCHK(F->hword.read(cnt.p, s, h));
   F->nonpos = (int)h;
   if (nonpositive_diag) *nonpositive_diag = (int32_t)h;
   return MSP_OK;
@@ -2651,8 +2694,7 @@ extern "C" msp_status msp_fsai_set_network_output(msp_fsai F, const float* out, 
   }
   MSPCHK(fsai_commit(F, s));
   uint32_t h = 0;
-  HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  MSPCHK(F->hword.read(cnt.p, s, h));
   F->nonpos = (int)h;
   if (nonpositive_diag) *nonpositive_diag = (int32_t)h;
   return MSP_OK;
@@ -2673,8 +2715,7 @@ extern "C" msp_status msp_fsai_classical(msp_fsai F, int32_t* fallback_rows, voi
   MSPCHK(launch_check("fsai classical"));
   MSPCHK(fsai_commit(F, s));
   uint32_t h = 0;
-  HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  MSPCHK(F->hword.read(cnt.p, s, h));
   F->nonpos = 0;
   if (fallback_rows) *fallback_rows = (int32_t)h;
   return MSP_OK;

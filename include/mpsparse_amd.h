@@ -229,7 +229,7 @@ msp_status msp_csr_prepare_values(msp_csr A, int32_t value_type, void *stream);
  * reference always streams CSR (spmv.metal:11-49); AUTO adds diagonal storage
  * for stencil-like patterns. format_in_use (optional): MSP_FORMAT_CSR, 2
  * when diagonal storage is in use, 3 when it is and the values are exactly
- * symmetric (products whose value planes exceed 128 MB then read the lower
+ * symmetric (products whose value planes exceed 16 MB then read the lower
  * diagonals from the upper planes; results are bit-identical either way)
  * (known after the first SpMV or solve). */
 msp_status msp_csr_set_format(msp_csr A, int32_t format);

@@ -417,7 +417,6 @@ struct Workspace {
   // two halves: SpMV partials, and those of the kernel that consumes them
   GridRed gr() const { return GridRed{red.as<CSum>()}; }
   GridRed gr2() const { return GridRed{red.as<CSum>() + (size_t)16 * cu_count()}; }
-  GridRed gr3() const { return GridRed{red.as<CSum>() + (size_t)32 * cu_count()}; }
   ~Workspace() {
     if (in_teardown()) return;
     if (hS) (void)hipHostFree(hS);
@@ -489,7 +488,7 @@ static msp_status ensure_workspace(MatBase* A, hipStream_t s) {
     MSPCHK(w.scal.alloc(2 * sizeof(Scalars)));
     HIPCHK(hipMemsetAsync(w.scal.p, 0, 2 * sizeof(Scalars), s));
     // partials of up to 4 values for grids of up to 8 workgroups per CU
-    size_t red_bytes = sizeof(CSum) * 6 * (size_t)cu_count() * 8;  // three regions of 16 CSums per CU
+    size_t red_bytes = sizeof(CSum) * 6 * (size_t)cu_count() * 8;  // two regions of 16 CSums per CU in use (gr, gr2), a third spare
     MSPCHK(w.red.alloc(red_bytes));
     HIPCHK(hipMemsetAsync(w.red.p, 0, red_bytes, s));
     HIPCHK(hipHostMalloc((void**)&w.hS, sizeof(Scalars), hipHostMallocDefault));

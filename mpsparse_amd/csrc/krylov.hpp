@@ -14,14 +14,15 @@
 //   workgroup, x += alpha p, r -= alpha q, sums of <r,r>, <r,z>
 //                                        (k_cg_update_xr)
 // and a finisher (FinCgStepPending) only at the end of a batch of steps.
-// FSAI-preconditioned CG (M = G^T G) keeps z explicit, three launches a step:
-//   z = G^T w, sums of <r,z>
-//   q = A p with p = z + beta p_old formed in the gather, beta from the <r,z>
-//   sums in every workgroup, which also take the previous step's convergence
-//   decision from its <r,r> sums; sums of <p,q> (XPzLocal)
-//   the decision committed; alpha from the <r,z> and <p,q> sums, x += alpha p,
-//   r_new = r - alpha q in the gather of the next w = G r_new, sums of
-//   <r_new,r_new> (XRNew / EpiCgXrW)
+// FSAI-preconditioned CG (M = G^T G, so <r,z> = <G r, G r>), three launches
+// a step:
+//   z = G^T w with p = z + beta p_old formed in the epilogue, beta from the
+//   ||w||^2 sums in every workgroup, which also take the previous step's
+//   convergence decision from its <r,r> sums (EpiPcgDir)
+//   q = A p, sums of <p,q>
+//   the decision committed; alpha = ||w||^2 / <p,q>, x += alpha p (every other
+//   step), r_new = r - alpha q in the gather of the next w = G r_new, sums of
+//   <r_new,r_new> and ||w_new||^2 (XRNew / EpiCgXrW)
 // with a finisher (FinCgStepPending) only at the end of a batch of steps.
 // BiCGSTAB (reference: iterative/BiCGSTAB/bicgstab.metal:12-326, spmv.cpp:699-888):
 //   v = A p, <r0,v> -> alpha             (k_spmv + FinBicgAlpha)

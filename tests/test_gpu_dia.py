@@ -124,7 +124,7 @@ def test_solvers_same_on_dia_and_csr(method):
 
 @pytest.mark.parametrize("values", ["f32", "bf16", "f16"])
 def test_symmetric_storage_is_bit_identical(values):
-    """A symmetric stencil whose value planes exceed 128 MB (216^3: 282 MB in
+    """A symmetric stencil whose value planes exceed 16 MB (216^3: 282 MB in
     FP32, 141 MB in 16 bits) reads its lower diagonals from the upper planes
     (4 value planes instead of 7 for the 7-point operator). Against the same
     matrix with one entry of its last row changed (no longer symmetric: every

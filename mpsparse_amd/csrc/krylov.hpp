@@ -1056,16 +1056,21 @@ struct FinVerify {
 };
 
 // Whole CG / Jacobi-PCG solve in one workgroup for small systems (n <=
-// kSmallN): 1024 threads, thread t owns rows t, t + 1024, ... with x, x_lo, r,
-// p, b in registers; the direction p (or the iterate, during a verification)
-// in LDS for the SpMV's gathers; the scalar bookkeeping of FinCgInit /
-// FinCgStep / FinVerify run by thread 0 on an LDS copy of the state. On a
-// 4096-row Laplacian a step is then a few microseconds of barriers and LDS
-// instead of four kernel launches. Same recurrence, stopping rules,
-// true-residual verification and restarts as the multi-kernel path (the
-// reduction trees differ, so the bits do too).
-constexpr int kSmallThreads = 1024;
-constexpr int kSmallRpt = 4;
+// kSmallN): 512 threads, thread t owns rows t, t + 512, ... with x, x_lo, r,
+// p and the columns of its rows in registers (8 waves, so up to 256 registers
+// each); the values of its rows and the direction p (or the iterate, during a
+// verification) in LDS for the product's gathers; the scalar bookkeeping of
+// FinCgInit / FinCgStep / FinVerify run by thread 0 (the step's on a register
+// copy of the LDS state). On a 4096-row Laplacian a step is 3.8 us of LDS and
+// barriers instead of four kernel launches (9 us while each step read A's
+// rows from memory, one dependent row_ptr -> col -> p chain per row). Same
+// recurrence, stopping rules, true-residual verification and restarts as the
+// multi-kernel path (the reduction trees differ, so the bits do too).
+constexpr int kSmallThreads = 512;
+constexpr int kSmallRpt = 8;
+// entries of a row held on chip for the step's product (longer rows read A
+// from memory each step)
+constexpr int kSmallRowCap = 7;
 constexpr uint32_t kSmallN = (uint32_t)kSmallThreads * kSmallRpt;
 
 template <int NDS>
@@ -1101,6 +1106,31 @@ __global__ __launch_bounds__(kSmallThreads) void k_cg_small(RowsCsr A, const flo
     xl[k] = 0.0f;
     dv[k] = (own && PC == kPcJacobi) ? dinv[i] : 1.0f;
     if (own) sv[i] = x[k];
+  }
+  // the owned rows' entries on chip
+  // (a missing entry holds value 0 and the row's own column, so the product
+  // needs no branch: the row's sum gains +-0 terms, which change it only where
+  // p_i is not finite; a row longer than kSmallRowCap holds kLongRow as its
+  // first column and is read from memory)
+  constexpr uint32_t kLongRow = 0xFFFFFFFFu;
+  // a thread's kSmallRpt x kSmallRowCap values at a stride of one more word
+  // (no bank conflicts; every offset fits a load's immediate)
+  constexpr int kSv = kSmallRpt * kSmallRowCap + 1;
+  __shared__ float sval[kSmallThreads * kSv];
+  float* const my = sval + t * kSv;
+  uint32_t rcol[RPT][kSmallRowCap];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const uint32_t i = t + k * kSmallThreads;
+    const uint32_t a0 = i < n ? A.row_ptr[i] : 0u, a1 = i < n ? A.row_ptr[i + 1] : 0u;
+    const uint32_t len = a1 - a0;
+#pragma unroll
+    for (int e = 0; e < kSmallRowCap; ++e) {
+      const bool in = (uint32_t)e < len;
+      rcol[k][e] = in ? A.col[a0 + e] : (i < n ? i : 0u);
+      my[k * kSmallRowCap + e] = in ? A.val[a0 + e] : 0.0f;
+    }
+    if (len > (uint32_t)kSmallRowCap) rcol[k][0] = kLongRow;
   }
   __syncthreads();
   // r = b - A x0 (plain FP32 row sums, as the multi-kernel path's EpiResidual)
@@ -1141,7 +1171,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_cg_small(RowsCsr A, const flo
       for (int k = 0; k < RPT; ++k) {
         const uint32_t i = t + k * kSmallThreads;
         float s = 0.0f;
-        if (i < n) A.foreach(i, [&](float a, uint32_t c) { s += a * sv[c]; });
+        if (rcol[k][0] != kLongRow) {  // the row's products in row order, then the padding
+#pragma unroll
+          for (int e = 0; e < kSmallRowCap; ++e) s += my[k * kSmallRowCap + e] * sv[rcol[k][e]];
+        } else if (i < n) {
+          A.foreach(i, [&](float a, uint32_t c) { s += a * sv[c]; });
+        }
         q[k] = s;
         acc[0] = csum_add(acc[0], p[k] * s);
       }
@@ -1163,10 +1198,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_cg_small(RowsCsr A, const flo
       }
       float trr[2];
       small_total<2>(acc2, trr);
-      if (t == 0) {
-        S.pq = ok ? tpq[0] : -1.0f;
-        S.alpha = alpha;
-        FinCgStep<PC>{&S}(trr);
+      if (t == 0) {  // on a register copy of the state (LDS accesses would chain)
+        Scalars L = S;
+        L.pq = ok ? tpq[0] : -1.0f;
+        L.alpha = alpha;
+        FinCgStep<PC>{&L}(trr);
+        S = L;
       }
       __syncthreads();
       if (S.done == 0 && S.verify == 0) {
@@ -1192,7 +1229,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_cg_small(RowsCsr A, const flo
 #pragma unroll
       for (int k = 0; k < RPT; ++k) {
         const uint32_t i = t + k * kSmallThreads;
-        CSum s{bv[k], 0.0f};
+        CSum s{i < n ? b[i] : 0.0f, 0.0f};  // (b from memory: a rare path)
         if (i < n)
           A.foreach(i, [&](float a, uint32_t c) {
             const float xv = sv[c];

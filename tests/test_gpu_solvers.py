@@ -152,6 +152,55 @@ def test_small_system_single_workgroup_path(precond):
     assert info["status"] == "converged" and info["iterations"] <= 3, info
 
 
+def _long_row_spd(side=50, rows=(0, 17, 333, 1000, 2499), extra=30, seed=3):
+    """A 2D Laplacian with a few rows coupled to `extra` more columns (and the
+    matching symmetric entries), kept diagonally dominant: rows of 9 to 35
+    entries among rows of 3 to 6."""
+    import scipy.sparse as sp
+
+    L = laplacian_2d_csr(side).to_scipy().tolil()
+    n = L.shape[0]
+    rng = np.random.default_rng(seed)
+    for r in rows:
+        for c in rng.choice(np.setdiff1d(np.arange(n), [r]), extra, replace=False):
+            L[r, c] = L[c, r] = -0.25
+            L[r, r] += 0.25
+            L[c, c] += 0.25
+    return sp.csr_matrix(L, dtype=np.float64)
+
+
+@pytest.mark.parametrize("shape", ["stencil_3d_16", "long_rows"])
+def test_small_path_rows_on_chip_and_from_memory(shape):
+    """The single-workgroup solve keeps rows of up to 7 entries on chip and
+    reads longer rows from memory every step: on a 16^3 7-point stencil (every
+    row on chip) and on a Laplacian with a few rows of 9-35 entries (those
+    from memory) it and the multi-kernel form reach the FP64 true residual
+    bar, each within 15% (+2) of FP64 CG's step count (the long-row system
+    separates the two forms' roundings by a few steps: 131 against 140, FP64
+    127)."""
+    from mpsparse_amd.problems import diffusion_3d
+
+    if shape == "stencil_3d_16":
+        M = diffusion_3d(16, sigma=1.0, seed=5)
+        rp, ci, v, n = M.row_ptr, M.col_ind, M.vals, M.num_rows
+        S = M.to_scipy()
+    else:
+        S = _long_row_spd()
+        rp, ci, v, n = (S.indptr.astype(np.uint32), S.indices.astype(np.uint32), S.data.astype(np.float32),
+                        S.shape[0])
+        S = _scipy(rp, ci, v, n)
+        assert np.diff(rp).max() > 8
+    A = csr_from_arrays(rp, ci, v, n, n)
+    b = np.random.default_rng(11).standard_normal(n).astype(np.float32)
+    _, it64, _, _ = K.cg(S, b.astype(np.float64), tol=1e-6)
+    for form in ("auto", "separate"):
+        say("small path rows", shape, form)
+        x, info = _solve(A, b, tol=1e-6, cg_form=form)
+        assert info["status"] == "converged", info
+        assert K.true_relres(S, x, b) <= 1.05e-6
+        assert abs(info["iterations"] - it64) <= 0.15 * it64 + 2, (form, info, it64)
+
+
 def test_bicgstab_block_sparse():
     q = block_sparse(1024, 16, 8, seed=2)
     say("bicgstab block sparse 1024")

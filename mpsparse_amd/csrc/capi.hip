@@ -1693,9 +1693,12 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
   // a few extra steps beyond maxiter leave room for the final verification
   const int limit = o.maxiter + (o.verify ? 2 * o.check_every : 0);
   const int full = std::max(1, o.check_every);
-  // batches grow 4, 8, ... up to check_every, so that a solve that converges in
-  // a few steps does not queue two full batches of (skipped) step launches
+  // batches of 4, 4, 8, 16, ... up to check_every: the second batch is queued
+  // before the host has read the first batch's state, so a solve that
+  // converges within the first batch pays 4 skipped steps, not 8 (BiCGSTAB on
+  // the block-sparse system: 4 steps)
   int grow = std::min(4, full);
+  int nbatch = 0;
   // Near convergence the batches shrink to the steps the recurrence still
   // needs: the host reads the state after each batch (one batch behind the
   // GPU), takes the residual's rate of decrease over the last two snapshots
@@ -1706,7 +1709,7 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
   float rr_a = 0.0f;
   while (true) {
     const int batch = std::min(std::min(grow, cap), std::max(1, limit - enqueued));
-    grow = std::min(2 * grow, full);
+    if (++nbatch > 1) grow = std::min(2 * grow, full);
     for (int k = 0; k < batch; ++k) step();
     flush();
     if (o.verify) verify();
@@ -1750,22 +1753,49 @@ static msp_status solver_prologue(MatBase* A, const float* b, const float* x, co
 }
 
 // True-residual check (k_verify): lane groups per row for CSR matrices with
-// long rows, one thread per row otherwise.
+// long rows and for block rows (a lane per block column), one thread per row
+// otherwise. Returns the workgroups launched (the finisher's partial count):
+// the lane-group forms take up to 8 workgroups per CU, since a group walks
+// its rows one after another and more groups in flight hide that latency.
 template <int MODE, typename Op>
-static void verify_launch(const Op& op, uint32_t g, const float* X, const float* XLo, const float* B, float* R,
-                          float* P, float* R0, const float* D, uint32_t n, Scalars* S, GridRed rb, hipStream_t s) {
+static uint32_t verify_launch(const Op& op, uint32_t g, const float* X, const float* XLo, const float* B, float* R,
+                              float* P, float* R0, const float* D, uint32_t n, Scalars* S, GridRed rb,
+                              hipStream_t s) {
+  const auto lanes_grid = [&](uint32_t L) {
+    const uint64_t groups = ((uint64_t)n + kBlock / L - 1) / (kBlock / L);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(groups, (uint64_t)cu_count() * 8));
+  };
   if constexpr (std::is_same<decltype(op.rows()), RowsCsr>::value) {
     const uint64_t avg = op.A->n ? op.A->nnz / op.A->n : 0;
     if (avg >= 64) {
-      k_verify_lanes<MODE, 64><<<g, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
-      return;
+      const uint32_t gl = lanes_grid(64);
+      k_verify_lanes<MODE, 64><<<gl, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
+      return gl;
     }
     if (avg >= 16) {
-      k_verify_lanes<MODE, 16><<<g, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
-      return;
+      const uint32_t gl = lanes_grid(16);
+      k_verify_lanes<MODE, 16><<<gl, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
+      return gl;
+    }
+  }
+  if constexpr (std::is_same<decltype(op.rows()), RowsBcsr>::value) {
+    const RowsBcsr rows = op.rows();
+    const uint32_t gl = lanes_grid(rows.B);
+    if (rows.B == 16) {
+      k_verify_bcsr<MODE, 16><<<gl, kBlock, 0, s>>>(rows.A, rows.m, X, XLo, B, R, P, R0, D, n, S, rb);
+      return gl;
+    }
+    if (rows.B == 8) {
+      k_verify_bcsr<MODE, 8><<<gl, kBlock, 0, s>>>(rows.A, rows.m, X, XLo, B, R, P, R0, D, n, S, rb);
+      return gl;
+    }
+    if (rows.B == 4) {
+      k_verify_bcsr<MODE, 4><<<gl, kBlock, 0, s>>>(rows.A, rows.m, X, XLo, B, R, P, R0, D, n, S, rb);
+      return gl;
     }
   }
   k_verify<MODE><<<g, kBlock, 0, s>>>(op.rows(), X, XLo, B, R, P, R0, D, n, S, rb);
+  return g;
 }
 
 // The CG iterate carries its rounding error in a second vector (x + xl) only
@@ -1902,11 +1932,11 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
   auto verify = [&]() {
     xflush(true);
     if (jac) {
-      verify_launch<1>(op, gv, X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb, s);
-      finish<2>(FinVerify<1>{S}, rb, gv, s);
+      const uint32_t gvf = verify_launch<1>(op, gv, X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb, s);
+      finish<2>(FinVerify<1>{S}, rb, gvf, s);
     } else {
-      verify_launch<0>(op, gv, X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb, s);
-      finish<2>(FinVerify<0>{S}, rb, gv, s);
+      const uint32_t gvf = verify_launch<0>(op, gv, X, XLo, B, R, Pb[k & 1], nullptr, D, n, S, rb, s);
+      finish<2>(FinVerify<0>{S}, rb, gvf, s);
     }
   };
   const msp_status st = run_solver(A, o, info, s, init, step, verify, flush);
@@ -1950,8 +1980,8 @@ static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const ms
     k_bicg_p<<<gv, kBlock, 0, s>>>(P, R, Vv, n, S);
   };
   auto verify = [&]() {
-    verify_launch<2>(op, gv, X, XLo, B, R, P, R0, nullptr, n, S, rb, s);
-    finish<2>(FinVerify<2>{S}, rb, gv, s);
+    const uint32_t gvf = verify_launch<2>(op, gv, X, XLo, B, R, P, R0, nullptr, n, S, rb, s);
+    finish<2>(FinVerify<2>{S}, rb, gvf, s);
   };
   MSPCHK(run_solver(A, o, info, s, init, step, verify));
   k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
@@ -2905,8 +2935,8 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
   // then)
   auto verify = [&]() {
     xflush(true);
-    verify_launch<3>(op, gv, X, XLo, B, Rb[k & 1], P, nullptr, nullptr, n, S, rb, s);
-    finish<2>(FinVerify<3>{S}, rb, gv, s);
+    const uint32_t gvf = verify_launch<3>(op, gv, X, XLo, B, Rb[k & 1], P, nullptr, nullptr, n, S, rb, s);
+    finish<2>(FinVerify<3>{S}, rb, gvf, s);
     g_of_r(&S->skip_g);
   };
   MSPCHK(run_solver(A, o, info, s, init, step, verify, flush));

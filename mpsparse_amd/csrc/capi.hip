@@ -2709,11 +2709,19 @@ extern "C" msp_status msp_fsai_set_network_output(msp_fsai F, const float* out, 
   HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
   if (F->features == kFeatStated5) {
     msp_csr A = F->A;
-    k_fsai_rownorm<<<grid_for(A->n), kBlock, 0, s>>>(A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(),
-                                                     A->val.as<float>(), A->n, F->G.row_ptr.as<uint32_t>(),
-                                                     F->G.col.as<uint32_t>(), static_cast<const float*>(o.d),
-                                                     F->colscale.as<float>(), F->G.val.as<float>(),
-                                                     F->rowscale.as<float>(), cnt.as<uint32_t>());
+    // A's diagonal storage (probed and filled here, else at its first product)
+    // turns each A(J_p, J_q) of the normalisation into one load
+    MSPCHK(ensure_dia(A, s));
+    if (A->dia_state == 1)
+      k_fsai_rownorm<true><<<grid_for(A->n), kBlock, 0, s>>>(
+          A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), A->n, F->G.row_ptr.as<uint32_t>(),
+          F->G.col.as<uint32_t>(), static_cast<const float*>(o.d), F->colscale.as<float>(), F->G.val.as<float>(),
+          F->rowscale.as<float>(), cnt.as<uint32_t>(), dia_view_of<ValF32>(A));
+    else
+      k_fsai_rownorm<false><<<grid_for(A->n), kBlock, 0, s>>>(
+          A->row_ptr.as<uint32_t>(), A->col.as<uint32_t>(), A->val.as<float>(), A->n, F->G.row_ptr.as<uint32_t>(),
+          F->G.col.as<uint32_t>(), static_cast<const float*>(o.d), F->colscale.as<float>(), F->G.val.as<float>(),
+          F->rowscale.as<float>(), cnt.as<uint32_t>());
     MSPCHK(launch_check("fsai rownorm"));
   } else {
     k_fsai_scale<<<grid_for(F->ne), kBlock, 0, s>>>(static_cast<const float*>(o.d), F->colscale.as<float>(),

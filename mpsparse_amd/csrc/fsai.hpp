@@ -8,6 +8,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "dia.hpp"
 
 namespace msp {
 
@@ -310,7 +311,12 @@ constexpr int kRowNormRegs = 8;  // rows of G with at most this many entries are
 
 // Forward: G = s_i g~ (s_i = N_i^{-1/2}; s_i = 1 when N_i is not positive and
 // finite, counted in nonpos together with non-positive diagonal entries);
-// rowscale[i] = s_i.
+// rowscale[i] = s_i. DIA: A in diagonal storage (FP32 values, `Ad`), so each
+// A(J_p, J_q) is one load from its diagonal's plane instead of a walk of row
+// J_p (no dependent row_ptr -> col loads; 128^3: 169 -> 137 us); the terms
+// are added in the same order (J ascending), a pair off A's diagonals or
+// outside its pattern adding a stored zero.
+template <bool DIA = false>
 __global__ __launch_bounds__(kBlock) void k_fsai_rownorm(const uint32_t* __restrict__ row_ptr,
                                                          const uint32_t* __restrict__ col,
                                                          const float* __restrict__ val, uint32_t n,
@@ -318,7 +324,8 @@ __global__ __launch_bounds__(kBlock) void k_fsai_rownorm(const uint32_t* __restr
                                                          const uint32_t* __restrict__ Lcol,
                                                          const float* __restrict__ out,
                                                          const float* __restrict__ colscale, float* __restrict__ G,
-                                                         float* __restrict__ rowscale, uint32_t* __restrict__ nonpos) {
+                                                         float* __restrict__ rowscale, uint32_t* __restrict__ nonpos,
+                                                         DiaView<ValF32> Ad = {}) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t o = Lrp[i], k = Lrp[i + 1] - o;
     float N = 0.0f;
@@ -334,6 +341,35 @@ __global__ __launch_bounds__(kBlock) void k_fsai_rownorm(const uint32_t* __restr
         J[p] = (uint32_t)p < k ? Lcol[o + p] : 0xFFFFFFFFu;
         gt[p] = (uint32_t)p < k ? out[o + p] * colscale[o + p] : 0.0f;
       }
+      if constexpr (DIA) {
+        // every A(J_p, J_q) first (all loads in flight), then the sums
+        float av[kRowNormRegs][kRowNormRegs];
+#pragma unroll
+        for (int p = 0; p < kRowNormRegs; ++p)
+#pragma unroll
+          for (int q = 0; q < kRowNormRegs; ++q) {
+            av[p][q] = 0.0f;
+            if ((uint32_t)p < k && (uint32_t)q < k) {
+              const int32_t off = (int32_t)(J[q] - J[p]);
+              int64_t at = -1;
+              for (int d = 0; d < Ad.o.ndiag; ++d)
+                if (Ad.o.off[d] == off)
+                  at = (Ad.mirrored ? (int64_t)Ad.o.plane[d] * (int64_t)Ad.ld + Ad.o.shift[d]
+                                    : (int64_t)d * (int64_t)Ad.ld) + (int64_t)J[p];
+              if (at >= 0) av[p][q] = Ad.val[at];
+            }
+          }
+#pragma unroll
+        for (int p = 0; p < kRowNormRegs; ++p) {
+          if ((uint32_t)p >= k) continue;
+          G[o + p] = gt[p];
+          float v = 0.0f;
+#pragma unroll
+          for (int q = 0; q < kRowNormRegs; ++q)
+            if ((uint32_t)q < k) v += av[p][q] * gt[q];
+          N += gt[p] * v;
+        }
+      } else {
       uint32_t ra[kRowNormRegs], rb[kRowNormRegs];
 #pragma unroll
       for (int p = 0; p < kRowNormRegs; ++p) {
@@ -367,6 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_fsai_rownorm(const uint32_t* __restr
           }
         }
         N += gt[p] * v;
+      }
       }
     } else {
       for (uint32_t p = 0; p < k; ++p) G[o + p] = out[o + p] * colscale[o + p];  // g~, scaled below

@@ -111,6 +111,28 @@ def test_rownorm_of_network_output(name):
         assert np.max(np.abs(g2 - g_cl)) <= 1e-5 * np.max(np.abs(g_cl))
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_rownorm_from_diagonal_storage_matches_csr(name):
+    """With A in diagonal storage the normalisation reads each A(J_p, J_q) from
+    its diagonal's plane instead of walking row J_p: the same G as with A held
+    as CSR only, to the bit (the same terms in the same order; a pair off A's
+    pattern adds a stored zero)."""
+    M = CASES[name]()
+    r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)
+    f, _ = OF.features(M.row_ptr, M.col_ind, M.vals, feature_set="stated5")
+    rng = np.random.default_rng(8)
+    o = np.where(r == c, 1.0, -f[:, 0]) * np.exp(0.3 * rng.standard_normal(M.num_rows))[r]
+    o = o.astype(np.float32)
+    gs = {}
+    for fmt in ("auto", "csr"):
+        say("stated5 rownorm storage", name, fmt)
+        A, F = _handles(M)
+        A.set_format(fmt)
+        assert F.set_network_output(t_f32(o)) == 0
+        gs[fmt] = F.values().cpu().numpy()
+    assert np.array_equal(gs["auto"], gs["csr"])
+
+
 def test_rownorm_counts_bad_rows():
     M = diffusion_2d(16, sigma=1.0, seed=3)
     r, c, _ = OF.lower_pattern(M.row_ptr, M.col_ind)

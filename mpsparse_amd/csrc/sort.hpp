@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_bounds_split(const uint32_t* __r
 // values stay in registers during the passes and are placed in LDS (over the
 // keys' buffer) only once the final order is known.
 template <typename K, bool FIRST>
-__global__ __launch_bounds__(kSegThreads) void k_segment_sort(const K* __restrict__ keys,
+__global__ __launch_bounds__(kSegThreads, 8) void k_segment_sort(const K* __restrict__ keys,
                                                               const float* __restrict__ vals,
                                                               const uint32_t* __restrict__ seg_start, uint32_t G,
                                                               KeyMap km, int s, uint32_t n, uint64_t nnz,
@@ -480,7 +480,10 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const K* __restric
                                                               uint32_t* __restrict__ row_ptr,
                                                               int* __restrict__ overflow, int* __restrict__ err) {
   __shared__ uint64_t skey[kSegMax];
-  __shared__ uint32_t wcnt[kSegWaves][kRadix];
+  // per-wave digit counts in 16 bits (at most kSegMax keys): 37 KB of LDS a
+  // workgroup and at most 64 registers a lane, so 4 segments per CU instead of 3
+  __shared__ uint16_t wcnt[kSegWaves][kRadix];
+  static_assert(kSegMax <= 65535, "16-bit counts");
   __shared__ uint32_t dstart[kRadix];
   float* sval = reinterpret_cast<float*>(skey);  // after the last pass
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
@@ -532,7 +535,7 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const K* __restric
           const uint32_t below = (uint32_t)__popcll(peers & lane_lt);
           const uint32_t before = valid ? wcnt[wid][d] : 0u;
           rank[i] = before + below;
-          if (valid && (peers >> lane) == 1ull) wcnt[wid][d] = before + (uint32_t)__popcll(peers);
+          if (valid && (peers >> lane) == 1ull) wcnt[wid][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
           __builtin_amdgcn_wave_barrier();
         }
       }
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const K* __restric
 #pragma unroll
         for (int w = 0; w < kSegWaves; ++w) {
           const uint32_t c = wcnt[w][tid];
-          wcnt[w][tid] = run;
+          wcnt[w][tid] = (uint16_t)run;
           run += c;
         }
         tot = run;

@@ -19,8 +19,12 @@
 namespace msp {
 namespace sortk {
 
-constexpr int kItems = 12;
-constexpr int kTile = kBlock * kItems;  // 3072 keys per tile (measured best of 2048 / 3072 / 4096 / 6144)
+constexpr int kItems = 10;
+// 2560 keys per tile: the downsweep's LDS (25.6 KB with 16-bit digit counts)
+// then fits 6 workgroups per CU (3072 measured best of 2048 / 3072 / 4096 /
+// 6144 with 32-bit counts, at 5 per CU)
+constexpr int kTile = kBlock * kItems;
+static_assert(kTile <= 65535, "16-bit digit counts");
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kScanItems = 16;
@@ -265,16 +269,20 @@ __device__ __forceinline__ uint32_t digit_at(const uint32_t* __restrict__ starts
 
 // Stable rank + scatter of one tile. Wave w owns tile positions
 // [w*64*kItems, (w+1)*64*kItems), item i of lane l at w*64*kItems + i*64 + l,
-// so the (wave, item, lane) order is the input order.
+// so the (wave, item, lane) order is the input order. Compiled for 6 waves
+// per SIMD (the LDS allows 6 workgroups per CU; 151M keys, the pass that
+// drops its digit: 1027 -> 854 us), the digit-swapping pass for 4 (its search
+// of the previous digit's range spills at 6 and 5; 1000 us at 4 or 5).
 template <typename InK, typename OutK, bool FIRST, bool LAST, int XF = kXfNone>
-__global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ kin, const float* __restrict__ vin,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(XF == kXfSwap ? 4 : 6))) void k_downsweep(
+                                                      const InK* __restrict__ kin, const float* __restrict__ vin,
                                                       OutK* __restrict__ kout, float* __restrict__ vout,
                                                       uint32_t* __restrict__ col_out, uint32_t* __restrict__ row_out,
                                                       uint64_t nnz, KeyMap km, int shift, uint32_t num_tiles,
                                                       const uint32_t* __restrict__ offsets,
                                                       const uint32_t* __restrict__ prev = nullptr) {
   static_assert(XF == kXfNone || (sizeof(OutK) == 4 && !LAST), "key transforms write 32-bit keys");
-  __shared__ uint32_t wcnt[kWavesPerBlock][kRadix];
+  __shared__ uint16_t wcnt[kWavesPerBlock][kRadix];  // (at most kTile per digit)
   __shared__ uint32_t dstart[kRadix];
   __shared__ uint32_t gbase[kRadix];
   __shared__ uint64_t stage[kTile];
@@ -336,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
     uint32_t before = valid ? wcnt[wid][d] : 0u;
     rank[i] = before + below;
     // highest peer lane publishes the running count for this digit
-    if (valid && (peers >> lane) == 1ull) wcnt[wid][d] = before + (uint32_t)__popcll(peers);
+    if (valid && (peers >> lane) == 1ull) wcnt[wid][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
@@ -348,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(const InK* __restrict__ ki
 #pragma unroll
     for (int w = 0; w < kWavesPerBlock; ++w) {
       uint32_t c = wcnt[w][d];
-      wcnt[w][d] = run;
+      wcnt[w][d] = (uint16_t)run;
       run += c;
     }
     tot = run;

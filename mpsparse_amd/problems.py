@@ -8,7 +8,7 @@ COO inputs use the packed-key layout of the reference's harness
 iterative/cgd/benchmark.py:112-115): one ``u64`` key per nonzero plus one
 ``float32`` value, in arbitrary (here: shuffled) order.
 
-Workloads (sizes and distributions are fixed here and restated in DESIGN.md):
+Workloads (sizes and distributions are fixed here):
 
 * ``laplacian_2d(64)``      -- 2D 5-point Dirichlet Laplacian, 64x64 grid
   (n=4096, nnz=20224), keys shuffled with seed 0, b ~ N(0,1).

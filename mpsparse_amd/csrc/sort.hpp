@@ -76,14 +76,22 @@ __global__ __launch_bounds__(kBlock) void k_upsweep(const InK* __restrict__ keys
                                                     int* err) {
   __shared__ uint32_t h[kWavesPerBlock][kRadix];
   const int tid = threadIdx.x, wid = tid / kWave;
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  // every key load of the tile issued before the first count, and before the
+  // counts are cleared (the clearing and its barrier overlap the loads)
+  InK kv[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t idx = base + (uint64_t)i * kBlock + tid;
+    kv[i] = idx < nnz ? keys[idx] : InK(0);
+  }
   for (int i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&h[0][0])[i] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kTile;
-#pragma unroll 4
+#pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    uint64_t idx = base + (uint64_t)i * kBlock + tid;
+    const uint64_t idx = base + (uint64_t)i * kBlock + tid;
     if (idx < nnz) {
-      uint64_t k = (uint64_t)keys[idx];
+      uint64_t k = (uint64_t)kv[i];
       if (FIRST) k = compact_key(k, km, err);
       atomicAdd(&h[wid][digit_of(k, shift)], 1u);
     }

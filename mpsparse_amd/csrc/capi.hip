@@ -1657,9 +1657,19 @@ static msp_status start_solve(Workspace& w, const msp_solve_opts& o, hipStream_t
 }
 
 // Final state back to the host and into the info struct.
-static msp_status finish_solve(Workspace& w, msp_solve_info* info, hipStream_t s) {
+struct NoTail {
+  msp_status operator()() const { return MSP_OK; }
+};
+
+// The solver's final state to the host. `tail` enqueues the work that follows
+// the last step and needs no host decision (the held x update, the collapse
+// of the compensated iterate, the copy out) before the one wait, so the GPU
+// runs it straight after the last batch instead of after a host round trip.
+template <typename Tail = NoTail>
+static msp_status finish_solve(Workspace& w, msp_solve_info* info, hipStream_t s, Tail tail = Tail{}) {
   HIPCHK(hipEventRecord(w.t1, s));
   HIPCHK(hipMemcpyAsync(w.hS, w.S_dev(), sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  MSPCHK(tail());
   HIPCHK(spin_sync(s));
   if (info) {
     float ms = 0.0f;
@@ -1682,9 +1692,9 @@ struct NoFlush {
 // `flush` runs after each batch of steps, before the verification and the
 // host's read of the state (a solver whose steps leave a decision pending
 // takes it there).
-template <typename Init, typename Step, typename Verify, typename Flush = NoFlush>
+template <typename Init, typename Step, typename Verify, typename Flush = NoFlush, typename Tail = NoTail>
 static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info* info, hipStream_t s, Init init,
-                             Step step, Verify verify, Flush flush = Flush{}) {
+                             Step step, Verify verify, Flush flush = Flush{}, Tail tail = Tail{}) {
   Workspace& w = A->ws;
   MSPCHK(start_solve(w, o, s));
   MSPCHK(init());
@@ -1738,7 +1748,7 @@ static msp_status run_solver(MatBase* A, const msp_solve_opts& o, msp_solve_info
     if (enqueued >= limit) break;
     slot ^= 1;
   }
-  return finish_solve(w, info, s);
+  return finish_solve(w, info, s, tail);
 }
 
 static msp_status solver_prologue(MatBase* A, const float* b, const float* x, const msp_solve_opts& o,
@@ -1833,8 +1843,7 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       else if (comp) k_cg_small<kPcNone, true><<<1, kSmallThreads, 0, s>>>(rows, Bv, X, D, A->n, o.use_x0, S);
       else k_cg_small<kPcNone, false><<<1, kSmallThreads, 0, s>>>(rows, Bv, X, D, A->n, o.use_x0, S);
       MSPCHK(launch_check("k_cg_small"));
-      MSPCHK(finish_solve(w, info, s));
-      MSPCHK(copy_out(x, X, sizeof(float) * A->n, s));
+      MSPCHK(finish_solve(w, info, s, [&]() { return copy_out(x, X, sizeof(float) * A->n, s); }));
       HIPCHK(hipStreamSynchronize(s));
       return MSP_OK;
     }
@@ -1939,13 +1948,15 @@ static msp_status cg_impl(const Op& op, const float* b, float* x, const msp_solv
       finish<2>(FinVerify<0>{S}, rb, gvf, s);
     }
   };
-  const msp_status st = run_solver(A, o, info, s, init, step, verify, flush);
+  auto tail = [&]() -> msp_status {
+    xflush(false);
+    k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
+    return copy_out(x, X, sizeof(float) * n, s);
+  };
+  const msp_status st = run_solver(A, o, info, s, init, step, verify, flush, tail);
   for (auto& e : tev)
     if (e) (void)hipEventDestroy(e);
   MSPCHK(st);
-  xflush(false);
-  k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
-  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
   // (a caller inside the library that keeps working on s skips the wait)
   if (sync_out) HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
@@ -1983,9 +1994,11 @@ static msp_status bicgstab_impl(const Op& op, const float* b, float* x, const ms
     const uint32_t gvf = verify_launch<2>(op, gv, X, XLo, B, R, P, R0, nullptr, n, S, rb, s);
     finish<2>(FinVerify<2>{S}, rb, gvf, s);
   };
-  MSPCHK(run_solver(A, o, info, s, init, step, verify));
-  k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
-  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
+  auto tail = [&]() -> msp_status {
+    k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
+    return copy_out(x, X, sizeof(float) * n, s);
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step, verify, NoFlush{}, tail));
   // (a caller inside the library that keeps working on s skips the wait)
   if (sync_out) HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
@@ -2947,10 +2960,12 @@ static msp_status pcg_fsai_impl(const OpCsr<V>& op, msp_fsai F, const float* b, 
     finish<2>(FinVerify<3>{S}, rb, gvf, s);
     g_of_r(&S->skip_g);
   };
-  MSPCHK(run_solver(A, o, info, s, init, step, verify, flush));
-  xflush(false);
-  k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
-  MSPCHK(copy_out(x, X, sizeof(float) * n, s));
+  auto tail = [&]() -> msp_status {
+    xflush(false);
+    k_collapse<<<gv, kBlock, 0, s>>>(X, XLo, n);
+    return copy_out(x, X, sizeof(float) * n, s);
+  };
+  MSPCHK(run_solver(A, o, info, s, init, step, verify, flush, tail));
   // (a caller inside the library that keeps working on s skips the wait)
   if (sync_out) HIPCHK(hipStreamSynchronize(s));
   return MSP_OK;
